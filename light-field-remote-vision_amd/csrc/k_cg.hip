@@ -1,0 +1,469 @@
+// Jacobi-preconditioned CG of the gradient-constraint normal equations
+//   A u = (alpha sum_i S_i^T W_i S_i + c0 I + beta D^T D) u = b
+// per RGB channel, FP32 vectors, FP64 reductions, scalars on the device.
+//
+// One iteration is two launches:
+//   k_cg_op     d = z + beta d_prev (z = Minv r) over the tile and its halo,
+//               q = A d with the binned gather/splat in LDS, d.q reduced;
+//               the last block computes alpha.
+//   k_cg_update x += alpha d, r -= alpha q, r.r and r.z reduced; the last block
+//               tests convergence and computes beta.
+// d is double-buffered because neighbouring tiles read it in their halos.
+#include "tile_common.hpp"
+
+#include <chrono>
+
+namespace lfrv {
+
+namespace {
+
+// q = A * din (mode 0) or, with the fused direction update (mode 1),
+// d = Minv r + beta d_prev, q = A d, written to dout / q, d.q reduced.
+template <int KS>
+__global__ void __launch_bounds__(256) k_cg_op(TileGeom g, TileEntries te, float wscale, float c0,
+                                               float beta_reg, int mode, const float *__restrict__ din,
+                                               const float *__restrict__ r, const float *__restrict__ minv,
+                                               float *__restrict__ dout, float *__restrict__ q,
+                                               CgState *st, double *partials, unsigned int *counter)
+{
+    constexpr int HAL = KS - 1;
+    extern __shared__ float lds[];
+    __shared__ double red[3 * 16];
+    if (mode == 1 && st->done) return;
+    const int t = xcd_tile(blockIdx.x, g.ntiles);
+    const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
+    const int tp = g.tw * g.th;
+    const int lw = g.tw + 2 * HAL, lh = g.th + 2 * HAL, lp = lw * lh;
+    float *dl = lds;            // 3 planes of lw x lh
+    float *acc = lds + 3 * lp;  // 3 planes of tw x th
+    const size_t N = (size_t)g.W * g.H;
+
+    float bc[3] = {0.0f, 0.0f, 0.0f};
+    int first = 1;
+    if (mode == 1) {
+        first = st->first;
+        for (int c = 0; c < 3; c++) bc[c] = (float)st->beta[c];
+    }
+    // stage d (tile + halo); outside the image d = 0
+    for (int i = threadIdx.x; i < lp; i += blockDim.x) {
+        const int lx = i % lw, ly = i / lw;
+        const int x = tx0 - HAL + lx, y = ty0 - HAL + ly;
+        const bool in = (x >= 0 && x < g.W && y >= 0 && y < g.H);
+        const size_t o = in ? (size_t)y * g.W + x : 0;
+        float v[3] = {0.0f, 0.0f, 0.0f};
+        if (in) {
+            if (mode == 0) {
+                for (int c = 0; c < 3; c++) v[c] = din[c * N + o];
+            } else {
+                const float mi = minv[o];
+                for (int c = 0; c < 3; c++) {
+                    const float z = mi * r[c * N + o];
+                    v[c] = first ? z : z + bc[c] * din[c * N + o];
+                }
+                const bool interior = lx >= HAL && lx < HAL + g.tw && ly >= HAL && ly < HAL + g.th;
+                if (interior)
+                    for (int c = 0; c < 3; c++) dout[c * N + o] = v[c];
+            }
+        }
+        for (int c = 0; c < 3; c++) dl[c * lp + i] = v[c];
+    }
+    for (int i = threadIdx.x; i < 3 * tp; i += blockDim.x) acc[i] = 0.0f;
+    __syncthreads();
+
+    // data term: for every binned (view, pixel): s = S d, splat W s
+    if (te.active && wscale != 0.0f) {
+        const int e0 = te.start[t], e1 = te.start[t + 1];
+        for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+            int x0, y0;
+            float wx[KS], wy[KS];
+            if (!footprint<KS>(te.tx[e], te.ty[e], g.W, g.H, x0, y0, wx, wy)) continue;
+            const float om = te.w[e] * wscale;
+            const int bx = x0 - tx0 + HAL, by = y0 - ty0 + HAL;  // in [0, lw-KS] x [0, lh-KS]
+            float s[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int i = 0; i < KS; i++)
+#pragma unroll
+                for (int j = 0; j < KS; j++) {
+                    const float a = wy[i] * wx[j];
+                    const int li = (by + i) * lw + bx + j;
+#pragma unroll
+                    for (int c = 0; c < 3; c++) s[c] += a * dl[c * lp + li];
+                }
+#pragma unroll
+            for (int c = 0; c < 3; c++) s[c] *= om;
+#pragma unroll
+            for (int i = 0; i < KS; i++) {
+                const int ly = y0 + i - ty0;
+                if (ly < 0 || ly >= g.th) continue;
+#pragma unroll
+                for (int j = 0; j < KS; j++) {
+                    const int lx = x0 + j - tx0;
+                    if (lx < 0 || lx >= g.tw) continue;
+                    const float a = wy[i] * wx[j];
+                    if (a == 0.0f) continue;
+#pragma unroll
+                    for (int c = 0; c < 3; c++) atomicAdd(acc + c * tp + ly * g.tw + lx, a * s[c]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    double dq[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < tp; i += blockDim.x) {
+        const int lx = i % g.tw, ly = i / g.tw;
+        const int x = tx0 + lx, y = ty0 + ly;
+        if (x >= g.W || y >= g.H) continue;
+        const size_t o = (size_t)y * g.W + x;
+        const int li = (ly + HAL) * lw + lx + HAL;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float *dp = dl + c * lp;
+            const float dv = dp[li];
+            float lap = 0.0f;
+            if (g.W > 1) {
+                if (x == 0) lap += dv - dp[li + 1];
+                else if (x < g.W - 1) lap += 2.0f * dv - dp[li - 1] - dp[li + 1];
+                else lap += dv - dp[li - 1];
+            }
+            if (g.H > 1) {
+                if (y == 0) lap += dv - dp[li + lw];
+                else if (y < g.H - 1) lap += 2.0f * dv - dp[li - lw] - dp[li + lw];
+                else lap += dv - dp[li - lw];
+            }
+            const float qv = acc[c * tp + i] + c0 * dv + beta_reg * lap;
+            q[c * N + o] = qv;
+            dq[c] += (double)dv * (double)qv;
+        }
+    }
+    if (mode == 0) return;
+    if (grid_reduce<3>(dq, partials, counter, nullptr, red)) {
+        if (threadIdx.x == 0) {
+            for (int c = 0; c < 3; c++) {
+                st->dq[c] = dq[c];
+                if (st->active[c]) {
+                    if (!(dq[c] > 0.0)) {
+                        st->breakdown[c] = 1;
+                        st->active[c] = 0;
+                        st->alpha[c] = 0.0;
+                    } else {
+                        st->alpha[c] = st->rz[c] / dq[c];
+                    }
+                } else {
+                    st->alpha[c] = 0.0;
+                }
+            }
+        }
+    }
+}
+
+// r = b - q (q = A x0), r.r, r.z, b.b; initialises the scalar state.
+__global__ void __launch_bounds__(256) k_cg_init(size_t N, const float *__restrict__ b,
+                                                 const float *__restrict__ q, float *__restrict__ r,
+                                                 const float *__restrict__ minv, double tol2, int maxit,
+                                                 CgState *st, double *partials, unsigned int *counter)
+{
+    __shared__ double red[9 * 16];
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < N; o += (size_t)gridDim.x * blockDim.x) {
+        const float mi = minv[o];
+        for (int c = 0; c < 3; c++) {
+            const float bv = b[c * N + o];
+            const float rv = bv - q[c * N + o];
+            r[c * N + o] = rv;
+            v[c] += (double)rv * rv;
+            v[3 + c] += (double)rv * (double)(mi * rv);
+            v[6 + c] += (double)bv * bv;
+        }
+    }
+    if (grid_reduce<9>(v, partials, counter, nullptr, red)) {
+        if (threadIdx.x == 0) {
+            int any = 0;
+            for (int c = 0; c < 3; c++) {
+                st->rr[c] = v[c];
+                st->rz[c] = v[3 + c];
+                st->bb[c] = v[6 + c];
+                st->alpha[c] = 0.0;
+                st->beta[c] = 0.0;
+                st->iter[c] = 0;
+                st->breakdown[c] = 0;
+                st->active[c] = (v[c] > tol2 * v[6 + c]) && maxit > 0;
+                any |= st->active[c];
+            }
+            st->tol2 = tol2;
+            st->maxit = maxit;
+            st->first = 1;
+            st->done = !any;
+        }
+    }
+}
+
+// x += alpha d; r -= alpha q; reduce r.r and r.(Minv r); convergence + beta.
+__global__ void __launch_bounds__(256) k_cg_update(size_t N, float *__restrict__ x,
+                                                   const float *__restrict__ d, const float *__restrict__ q,
+                                                   float *__restrict__ r, const float *__restrict__ minv,
+                                                   CgState *st, double *partials, unsigned int *counter)
+{
+    __shared__ double red[6 * 16];
+    if (st->done) return;
+    const float a[3] = {(float)st->alpha[0], (float)st->alpha[1], (float)st->alpha[2]};
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < N; o += (size_t)gridDim.x * blockDim.x) {
+        const float mi = minv[o];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const size_t k = c * N + o;
+            x[k] += a[c] * d[k];
+            const float rv = r[k] - a[c] * q[k];
+            r[k] = rv;
+            v[c] += (double)rv * rv;
+            v[3 + c] += (double)rv * (double)(mi * rv);
+        }
+    }
+    if (grid_reduce<6>(v, partials, counter, nullptr, red)) {
+        if (threadIdx.x == 0) {
+            int any = 0;
+            for (int c = 0; c < 3; c++) {
+                if (st->active[c]) {
+                    st->iter[c] += 1;
+                    st->rr[c] = v[c];
+                    if (v[c] <= st->tol2 * st->bb[c]) {
+                        st->active[c] = 0;
+                        st->beta[c] = 0.0;
+                    } else {
+                        st->beta[c] = v[3 + c] / st->rz[c];
+                        st->rz[c] = v[3 + c];
+                        if (st->iter[c] >= st->maxit) st->active[c] = 0;
+                    }
+                }
+                any |= st->active[c];
+            }
+            st->first = 0;
+            st->done = !any;
+        }
+    }
+}
+
+// ||b - q||^2 per channel (true residual check).
+__global__ void __launch_bounds__(256) k_resid(size_t N, const float *__restrict__ b,
+                                               const float *__restrict__ q, double *partials,
+                                               unsigned int *counter, double *out)
+{
+    __shared__ double red[3 * 16];
+    double v[3] = {0, 0, 0};
+    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < N; o += (size_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; c++) {
+            const double t = (double)b[c * N + o] - (double)q[c * N + o];
+            v[c] += t * t;
+        }
+    grid_reduce<3>(v, partials, counter, out, red);
+}
+
+// Energy of the normal equations, data part: sum over binned entries would
+// count duplicates, so this walks the source pixels directly.
+template <int KS>
+__global__ void __launch_bounds__(256) k_gibr_energy_data(int V, int W, int H, int w, int h,
+                                                          const float *__restrict__ tx, const float *__restrict__ ty,
+                                                          const float *__restrict__ omega, float wscale,
+                                                          const float *__restrict__ views,
+                                                          const float *__restrict__ U, double *partials,
+                                                          unsigned int *counter, double *out)
+{
+    __shared__ double red[16];
+    const size_t n = (size_t)w * h, N = (size_t)W * H;
+    double v[1] = {0.0};
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)V * n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const float om = omega[k] * wscale;
+        const float t0 = tx[k], t1 = ty[k];
+        if (om == 0.0f || !cell_valid(t0, t1, W, H)) continue;
+        int x0, y0;
+        float wx[KS], wy[KS];
+        if (!footprint<KS>(t0, t1, W, H, x0, y0, wx, wy)) continue;
+        const size_t vi = k / n, m = k - vi * n;
+        for (int c = 0; c < 3; c++) {
+            double s = 0.0;
+            for (int i = 0; i < KS; i++) {
+                const int y = y0 + i;
+                if (y < 0 || y > H - 1) continue;
+                for (int j = 0; j < KS; j++) {
+                    const int x = x0 + j;
+                    if (x < 0 || x > W - 1) continue;
+                    s += (double)(wy[i] * wx[j]) * U[c * N + (size_t)y * W + x];
+                }
+            }
+            const double rr = s - views[vi * 3 * n + c * n + m];
+            v[0] += (double)om * rr * rr;
+        }
+    }
+    grid_reduce<1>(v, partials, counter, out, red);
+}
+
+__global__ void __launch_bounds__(256) k_gibr_energy_grad(int W, int H, const float *__restrict__ U,
+                                                          const float *__restrict__ gx, const float *__restrict__ gy,
+                                                          double *partials, unsigned int *counter, double *out)
+{
+    __shared__ double red[16];
+    const size_t N = (size_t)W * H;
+    double v[1] = {0.0};
+    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < N; o += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(o % W), y = (int)(o / W);
+        for (int c = 0; c < 3; c++) {
+            const float *u = U + c * N;
+            if (x < W - 1) { const double t = ((double)u[o + 1] - u[o]) - gx[o]; v[0] += t * t; }
+            if (y < H - 1) { const double t = ((double)u[o + W] - u[o]) - gy[o]; v[0] += t * t; }
+        }
+    }
+    grid_reduce<1>(v, partials, counter, out, red);
+}
+
+unsigned int reduce_blocks(size_t N)
+{
+    const size_t b = (N + kBlock - 1) / kBlock;
+    return (unsigned int)(b < 2048 ? (b == 0 ? 1 : b) : 2048);
+}
+
+}  // namespace
+
+// counters: [0] op, [1] init, [2] update, [3] resid, [4..7] misc
+lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use_views, bool use_precond,
+                   bool poisson_rhs, double tol, int maxit, lfrv_cg_stats *st)
+{
+    auto t_start = std::chrono::steady_clock::now();
+    if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "CG before lfrv_prepare");
+    if (use_views) {
+        lfrv_status s = run_refresh_entry_weights(ctx, 1.0f);
+        if (s != LFRV_OK) return s;
+    }
+    const size_t N = ctx->N;
+    const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
+    const float wscale = use_views ? (float)(alpha * s2) : 0.0f;
+    // right-hand side and preconditioner; the Poisson form uses u0 = U
+    float *u0 = ctx->tmp3N;
+    if (poisson_rhs) LFRV_HIP(ctx, hipMemcpyAsync(u0, ctx->U, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    lfrv_status s = run_rhs_and_diag(ctx, alpha, c0, beta, use_views, use_precond, u0);
+    if (s != LFRV_OK) return s;
+
+    const TileGeom g = tile_geom(ctx);
+    TileEntries te = tile_entries(ctx);
+    te.active = use_views ? 1 : 0;
+    const int HAL = ctx->ks - 1;
+    const size_t lds = sizeof(float) * (3 * (size_t)(ctx->tile_w + 2 * HAL) * (ctx->tile_h + 2 * HAL) +
+                                        3 * (size_t)ctx->tile_w * ctx->tile_h);
+    const unsigned int rb = reduce_blocks(N);
+    float *x = ctx->U, *r = ctx->cg_r, *q = ctx->cg_q, *minv = ctx->cg_minv;
+    float *dbuf[2] = {ctx->cg_d, ctx->tmp3N};
+    CgState *stt = ctx->cg_state;
+    double *part = ctx->partials;
+
+#define OPK(K, MODE, DIN, DOUT)                                                                               \
+    k_cg_op<K><<<ctx->ntiles, 256, lds, ctx->stream>>>(g, te, wscale, (float)c0, (float)beta, MODE, DIN, r,  \
+                                                      minv, DOUT, q, stt, part, ctx->counters + 0)
+#define LAUNCH_OP(MODE, DIN, DOUT)                     \
+    switch (ctx->ks) {                                 \
+    case 2: OPK(2, MODE, DIN, DOUT); break;            \
+    case 3: OPK(3, MODE, DIN, DOUT); break;            \
+    case 4: OPK(4, MODE, DIN, DOUT); break;            \
+    case 5: OPK(5, MODE, DIN, DOUT); break;            \
+    default: return fail(ctx, LFRV_E_ARG, "dsf out of range"); \
+    }
+
+    {
+        TimedRegion tr(ctx, 0);
+        LAUNCH_OP(0, x, nullptr);
+    }
+    k_cg_init<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, r, minv, tol * tol, maxit, stt, part, ctx->counters + 1);
+    LFRV_LAUNCH(ctx, "cg init");
+    int it = 0, cur = 0;
+    const int chunk = 8;
+    bool done = false;
+    while (it < maxit && !done) {
+        const int nk = std::min(chunk, maxit - it);
+        for (int k = 0; k < nk; k++) {
+            float *dprev = dbuf[cur], *dnew = dbuf[cur ^ 1];
+            {
+                TimedRegion tr(ctx, 0);
+                LAUNCH_OP(1, dprev, dnew);
+            }
+            {
+                TimedRegion tr(ctx, 1);
+                k_cg_update<<<rb, 256, 0, ctx->stream>>>(N, x, dnew, q, r, minv, stt, part, ctx->counters + 2);
+            }
+            cur ^= 1;
+        }
+        LFRV_LAUNCH(ctx, "cg iteration");
+        it += nk;
+        LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+        LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        done = ctx->cg_state_host->done != 0;
+    }
+#undef LAUNCH_OP
+    // final state and true residual
+    {
+        TimedRegion tr(ctx, 0);
+        switch (ctx->ks) {
+        case 2: OPK(2, 0, x, nullptr); break;
+        case 3: OPK(3, 0, x, nullptr); break;
+        case 4: OPK(4, 0, x, nullptr); break;
+        case 5: OPK(5, 0, x, nullptr); break;
+        }
+    }
+#undef OPK
+    k_resid<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, part, ctx->counters + 3, ctx->scal);
+    LFRV_LAUNCH(ctx, "cg residual");
+    LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+    LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    flush_timing(ctx);
+    const CgState &hs = *ctx->cg_state_host;
+    lfrv_status status = LFRV_OK;
+    for (int c = 0; c < 3; c++) {
+        const bool conv = hs.rr[c] <= hs.tol2 * hs.bb[c];
+        if (st) {
+            st->iterations[c] = hs.iter[c];
+            st->converged[c] = conv ? 1 : 0;
+            st->rel_residual[c] = hs.bb[c] > 0 ? std::sqrt(hs.rr[c] / hs.bb[c]) : std::sqrt(hs.rr[c]);
+            st->true_rel_residual[c] =
+                hs.bb[c] > 0 ? std::sqrt(ctx->scal_host[c] / hs.bb[c]) : std::sqrt(ctx->scal_host[c]);
+        }
+        if (hs.breakdown[c]) status = LFRV_E_BREAKDOWN;
+        else if (!conv && status == LFRV_OK) status = LFRV_E_NONCONVERGED;
+        if (!std::isfinite(hs.rr[c])) status = LFRV_E_NAN;
+    }
+    if (st) {
+        st->status = status;
+        st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    }
+    if (status != LFRV_OK) {
+        const char *msg = status == LFRV_E_BREAKDOWN ? "CG breakdown (d.q <= 0)"
+                          : status == LFRV_E_NAN     ? "CG produced a non-finite residual"
+                                                     : "CG did not reach the tolerance";
+        return fail(ctx, status, msg);
+    }
+    return LFRV_OK;
+}
+
+lfrv_status run_gibr_energy(lfrv_ctx *ctx, double *e)
+{
+    const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
+    const size_t tot = (size_t)ctx->V * ctx->n;
+    const unsigned int rb1 = reduce_blocks(tot), rb2 = reduce_blocks(ctx->N);
+#define EN(K) k_gibr_energy_data<K><<<rb1, 256, 0, ctx->stream>>>(ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, ctx->omega, s2, ctx->views, ctx->U, ctx->partials, ctx->counters + 4, ctx->scal + 4)
+    switch (ctx->ks) {
+    case 2: EN(2); break;
+    case 3: EN(3); break;
+    case 4: EN(4); break;
+    case 5: EN(5); break;
+    default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
+    }
+#undef EN
+    k_gibr_energy_grad<<<rb2, 256, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->u_grad_x, ctx->u_grad_y,
+                                                     ctx->partials, ctx->counters + 5, ctx->scal + 5);
+    LFRV_LAUNCH(ctx, "gibr energy");
+    LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host + 4, ctx->scal + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *e = 0.5 * ((double)ctx->p.dt_alpha * ctx->scal_host[4] + (double)ctx->p.dt_beta * ctx->scal_host[5]);
+    return LFRV_OK;
+}
+
+}  // namespace lfrv

@@ -1,0 +1,430 @@
+// Stencil kernels: hole filling (Poisson diffusion, push-pull), the Gaussian
+// smoothing of the TV weight, and the fused primal-dual ROF step.
+#include "tile_common.hpp"
+
+#include <cmath>
+
+namespace lfrv {
+
+namespace {
+
+// ---- Poisson diffusion in the holes (gradientIBR.cu:87-152) ---------------
+
+__global__ void k_hole_list(size_t N, const float *__restrict__ vis, int *__restrict__ list,
+                            unsigned int *__restrict__ count)
+{
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= N || vis[o] != 0.0f) return;
+    const unsigned int k = atomicAdd(count, 1u);
+    list[k] = (int)o;
+}
+
+struct HoleState {
+    double e_prev;
+    double e;
+    int iters;
+    int done;
+    int maxit;
+    double eps;
+};
+
+// One Jacobi sweep u <- mean of the 4-neighbours (the reference's laplacian
+// then add_to), on the hole pixels only, all three channels; the energy
+// sum(lap * u_new) of the last channel is reduced and compared with the
+// previous sweep's (the stopping rule of gradientIBR.cu:139-143).
+__global__ void __launch_bounds__(256) k_hole_sweep(int W, int H, int nholes, const int *__restrict__ list,
+                                                    const float *__restrict__ uin, float *__restrict__ uout,
+                                                    HoleState *st, double *partials, unsigned int *counter)
+{
+    __shared__ double red[16];
+    if (st->done) return;
+    const size_t N = (size_t)W * H;
+    double v[1] = {0.0};
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nholes; k += gridDim.x * blockDim.x) {
+        const int o = list[k];
+        const int x = o % W, y = o / W;
+        for (int c = 0; c < 3; c++) {
+            const float *u = uin + c * N;
+            float s = 0.0f, wsum = 0.0f;
+            if (x > 0) { s += u[o - 1]; wsum += 1.0f; }
+            if (y > 0) { s += u[o - W]; wsum += 1.0f; }
+            if (x < W - 1) { s += u[o + 1]; wsum += 1.0f; }
+            if (y < H - 1) { s += u[o + W]; wsum += 1.0f; }
+            const float lap = s / wsum - u[o];
+            const float un = u[o] + lap;
+            uout[c * N + o] = un;
+            if (c == 2) v[0] += (double)lap * (double)un;
+        }
+    }
+    if (grid_reduce<1>(v, partials, counter, nullptr, red)) {
+        if (threadIdx.x == 0) {
+            st->e_prev = st->e;
+            st->e = v[0];
+            st->iters += 1;
+            if (fabs(st->e_prev - st->e) < st->eps || st->iters >= st->maxit) st->done = 1;
+        }
+    }
+}
+
+// ---- separable Gaussian, clamp to edge (cuda_convolutions.cu:110-237) -------
+
+struct Taps {
+    float k[15];
+    int r;
+};
+
+__global__ void k_conv_rows(int W, int H, Taps t, const float *__restrict__ in, float *__restrict__ out)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W) return;
+    float s = 0.0f;
+    for (int k = -t.r; k <= t.r; k++) {
+        const int xx = min(max(x + k, 0), W - 1);
+        s += in[(size_t)y * W + xx] * t.k[t.r - k];
+    }
+    out[(size_t)y * W + x] = s;
+}
+
+__global__ void k_conv_cols(int W, int H, Taps t, const float *__restrict__ in, float *__restrict__ out)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W) return;
+    float s = 0.0f;
+    for (int k = -t.r; k <= t.r; k++) {
+        const int yy = min(max(y + k, 0), H - 1);
+        s += in[(size_t)yy * W + x] * t.k[t.r - k];
+    }
+    out[(size_t)y * W + x] = s;
+}
+
+// ---- push-pull (optimization.cu:210-651), three channels per launch -------
+// Images of level s live at img + c*lvl_stride + off[s]; visibility at vis + off[s].
+// The centre tap of expand reads row/column i/2 of the coarser level; for an
+// odd-sized level that index runs one past the coarser image in the
+// reference (optimization.cu:455, 492) and is clamped here.
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__global__ void k_pp_reduce_rows(int Ws, int Hs, int hs, const float *__restrict__ in, float *__restrict__ tmp,
+                                 size_t in_cstride, size_t tmp_cstride)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y, c = blockIdx.z;
+    if (j >= Ws || i >= hs) return;
+    const float a = 0.4f;
+    const float k[5] = {0.25f - 0.5f * a, 0.25f, a, 0.25f, 0.25f - 0.5f * a};
+    const int rows[5] = {max(2 * i - 2, 0), max(2 * i - 1, 0), 2 * i, min(2 * i + 1, Hs - 1), min(2 * i + 2, Hs - 1)};
+    const float *p = in + c * in_cstride;
+    float s = 0.0f, wt = 0.0f;
+    for (int q = 0; q < 5; q++) {
+        const float v = p[(size_t)rows[q] * Ws + j];
+        if (v != 0.0f) { s += k[q] * v; wt += k[q]; }
+    }
+    tmp[c * tmp_cstride + (size_t)i * Ws + j] = (wt == 0.0f) ? 0.0f : s / wt;
+}
+
+__global__ void k_pp_reduce_cols(int Ws, int hs, int ws, const float *__restrict__ tmp, float *__restrict__ out,
+                                 size_t tmp_cstride, size_t out_cstride)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y, c = blockIdx.z;
+    if (j >= ws || i >= hs) return;
+    const float a = 0.4f;
+    const float k[5] = {0.25f - 0.5f * a, 0.25f, a, 0.25f, 0.25f - 0.5f * a};
+    const int cols[5] = {max(2 * j - 2, 0), max(2 * j - 1, 0), 2 * j, min(2 * j + 1, Ws - 1), min(2 * j + 2, Ws - 1)};
+    const float *p = tmp + c * tmp_cstride + (size_t)i * Ws;
+    float s = 0.0f, wt = 0.0f;
+    for (int q = 0; q < 5; q++) {
+        const float v = p[cols[q]];
+        if (v != 0.0f) { s += k[q] * v; wt += k[q]; }
+    }
+    out[c * out_cstride + (size_t)i * ws + j] = (wt == 0.0f) ? 0.0f : s / wt;
+}
+
+// visibility pyramid: OR over the same 5 x 5 support
+__global__ void k_pp_reduce_vis(int Ws, int Hs, int ws, int hs, const uint8_t *__restrict__ in, uint8_t *__restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= ws || i >= hs) return;
+    uint8_t v = 0;
+    const int rows[5] = {max(2 * i - 2, 0), max(2 * i - 1, 0), 2 * i, min(2 * i + 1, Hs - 1), min(2 * i + 2, Hs - 1)};
+    const int cols[5] = {max(2 * j - 2, 0), max(2 * j - 1, 0), 2 * j, min(2 * j + 1, Ws - 1), min(2 * j + 2, Ws - 1)};
+    for (int a = 0; a < 5; a++)
+        for (int b = 0; b < 5; b++) v |= in[(size_t)rows[a] * Ws + cols[b]] ? 1 : 0;
+    out[(size_t)i * ws + j] = v;
+}
+
+__global__ void k_pp_expand_rows(int ws, int hs, int Hs, const float *__restrict__ in, float *__restrict__ tmp,
+                                 size_t in_cstride, size_t tmp_cstride)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y, c = blockIdx.z;
+    if (j >= ws || i >= Hs) return;
+    const float a = 0.4f;
+    const float k[5] = {0.25f - 0.5f * a, 0.25f, a, 0.25f, 0.25f - 0.5f * a};
+    const float *p = in + c * in_cstride;
+    float s = 0.0f;
+    if (i % 2 == 0) {
+        const int r0 = max((i - 2) / 2, 0), r2 = min((i + 2) / 2, hs - 1);
+        s += 2.0f * k[0] * p[(size_t)r0 * ws + j];
+        s += 2.0f * k[2] * p[(size_t)min(i / 2, hs - 1) * ws + j];
+        s += 2.0f * k[4] * p[(size_t)r2 * ws + j];
+    } else {
+        const int r1 = max((i - 1) / 2, 0), r3 = min((i + 1) / 2, hs - 1);
+        s += 2.0f * k[1] * p[(size_t)r1 * ws + j];
+        s += 2.0f * k[3] * p[(size_t)r3 * ws + j];
+    }
+    tmp[c * tmp_cstride + (size_t)i * ws + j] = s;
+}
+
+__global__ void k_pp_expand_cols(int Ws, int Hs, int ws, const float *__restrict__ tmp, float *__restrict__ out,
+                                 const uint8_t *__restrict__ vis, size_t tmp_cstride, size_t out_cstride)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y, c = blockIdx.z;
+    if (j >= Ws || i >= Hs) return;
+    if (vis[(size_t)i * Ws + j]) return;
+    const float a = 0.4f;
+    const float k[5] = {0.25f - 0.5f * a, 0.25f, a, 0.25f, 0.25f - 0.5f * a};
+    const float *p = tmp + c * tmp_cstride + (size_t)i * ws;
+    float s = 0.0f;
+    if (j % 2 == 0) {
+        const int c0 = max((j - 2) / 2, 0), c2 = min((j + 2) / 2, ws - 1);
+        s += 2.0f * k[0] * p[c0];
+        s += 2.0f * k[2] * p[min(j / 2, ws - 1)];
+        s += 2.0f * k[4] * p[c2];
+    } else {
+        const int c1 = max((j - 1) / 2, 0), c3 = min((j + 1) / 2, ws - 1);
+        s += 2.0f * k[1] * p[c1];
+        s += 2.0f * k[3] * p[c3];
+    }
+    out[c * out_cstride + (size_t)i * Ws + j] = s;
+}
+
+__global__ void k_vis_to_u8(size_t N, const float *__restrict__ vis, uint8_t *__restrict__ out)
+{
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o < N) out[o] = vis[o] != 0.0f ? 1 : 0;
+}
+
+// U <- filled - U (optimization.cu:629-646)
+__global__ void k_pp_finish(size_t N, const float *__restrict__ filled, float *__restrict__ U)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < 3 * N) U[k] = filled[k] - U[k];
+}
+
+// ---- fused primal-dual ROF step --------------------------------------------
+// One inner iteration of vtv_sr_ista_step_unstructured (optimization.cu:1817-1828):
+//   dual:   p_c <- p_c + sigma grad+ Uq_c           (vtv_rof.cu:118-147)
+//   proj:   joint TV_F projection onto |p| <= g      (vtv_rof.cu:280-321; radius 1 without g, :237-275)
+//   primal: Uq_c <- (tau F_c + lam (Uq_c + tau div- p_c)) / (tau + lam)   (cuda_kernels.cu:504-532)
+// computed per tile: p_new on the tile plus its left/top halo, then Uq.
+// Inputs and outputs are separate buffers (neighbouring tiles read halos).
+constexpr int PD_TW = 64, PD_TH = 4;
+
+__global__ void __launch_bounds__(256) k_pd_step(int W, int H, float tau, float lam, float sigma,
+                                                 const float *__restrict__ uin, float *__restrict__ uout,
+                                                 const float *__restrict__ pxin, const float *__restrict__ pyin,
+                                                 float *__restrict__ pxout, float *__restrict__ pyout,
+                                                 const float *__restrict__ F, const float *__restrict__ g)
+{
+    constexpr int UW = PD_TW + 2, UH = PD_TH + 2;  // Uq region [x0-1, x0+TW] x [y0-1, y0+TH]
+    constexpr int PW = PD_TW + 1, PH = PD_TH + 1;  // p region  [x0-1, x0+TW-1] x [y0-1, y0+TH-1]
+    __shared__ float us[3][UH][UW];
+    __shared__ float ps[6][PH][PW];
+    const size_t N = (size_t)W * H;
+    const int x0 = blockIdx.x * PD_TW, y0 = blockIdx.y * PD_TH;
+    for (int i = threadIdx.x; i < UW * UH; i += blockDim.x) {
+        const int lx = i % UW, ly = i / UW;
+        const int x = x0 - 1 + lx, y = y0 - 1 + ly;
+        const bool in = x >= 0 && x < W && y >= 0 && y < H;
+        const size_t o = in ? (size_t)y * W + x : 0;
+        for (int c = 0; c < 3; c++) us[c][ly][lx] = in ? uin[c * N + o] : 0.0f;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < PW * PH; i += blockDim.x) {
+        const int lx = i % PW, ly = i / PW;
+        const int x = x0 - 1 + lx, y = y0 - 1 + ly;
+        float a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+        if (x >= 0 && x < W && y >= 0 && y < H) {
+            const size_t o = (size_t)y * W + x;
+            float nn = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float uc = us[c][ly + 1][lx + 1];
+                const float gx = (x < W - 1) ? us[c][ly + 1][lx + 2] - uc : 0.0f;
+                const float gy = (y < H - 1) ? us[c][ly + 2][lx + 1] - uc : 0.0f;
+                a[c] = pxin[c * N + o] + sigma * gx;
+                b[c] = pyin[c * N + o] + sigma * gy;
+                nn += a[c] * a[c] + b[c] * b[c];
+            }
+            nn = sqrtf(nn);
+            const float rad = g ? g[o] : 1.0f;
+            if (nn > rad) {
+                const float s = rad / nn;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    a[c] *= s;
+                    b[c] *= s;
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            ps[c][ly][lx] = a[c];
+            ps[3 + c][ly][lx] = b[c];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < PD_TW * PD_TH; i += blockDim.x) {
+        const int lx = i % PD_TW, ly = i / PD_TW;
+        const int x = x0 + lx, y = y0 + ly;
+        if (x >= W || y >= H) continue;
+        const size_t o = (size_t)y * W + x;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float pxv = ps[c][ly + 1][lx + 1], pyv = ps[3 + c][ly + 1][lx + 1];
+            pxout[c * N + o] = pxv;
+            pyout[c * N + o] = pyv;
+            float step = pxv + pyv;
+            if (x > 0) step -= ps[c][ly + 1][lx];
+            if (y > 0) step -= ps[3 + c][ly][lx + 1];
+            const float unew = us[c][ly + 1][lx + 1] + tau * step;
+            uout[c * N + o] = (tau * F[c * N + o] + lam * unew) / (tau + lam);
+        }
+    }
+}
+
+}  // namespace
+
+lfrv_status run_hole_filling(lfrv_ctx *ctx, int iters, double eps, int *ran)
+{
+    const size_t N = ctx->N;
+    unsigned int *cnt = ctx->counters + 8;
+    LFRV_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(unsigned int), ctx->stream));
+    k_hole_list<<<blocks_for(N), kBlock, 0, ctx->stream>>>(N, ctx->visibility, ctx->hole_list, cnt);
+    unsigned int nh = 0;
+    LFRV_HIP(ctx, hipMemcpyAsync(&nh, cnt, sizeof nh, hipMemcpyDeviceToHost, ctx->stream));
+    LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->n_holes = (int)nh;
+    if (ran) *ran = 0;
+    if (nh == 0 || iters <= 0) return LFRV_OK;
+    HoleState hs{};
+    hs.maxit = iters;
+    hs.eps = eps;
+    HoleState *dst = reinterpret_cast<HoleState *>(ctx->scal + 16);
+    LFRV_HIP(ctx, hipMemcpyAsync(dst, &hs, sizeof hs, hipMemcpyHostToDevice, ctx->stream));
+    float *bufs[2] = {ctx->U, ctx->tmp3N};
+    LFRV_HIP(ctx, hipMemcpyAsync(ctx->tmp3N, ctx->U, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    unsigned int nb = blocks_for(nh);
+    if (nb > 1024) nb = 1024;
+    int launched = 0;
+    HoleState host{};
+    while (launched < iters) {
+        const int chunk = std::min(16, iters - launched);
+        for (int k = 0; k < chunk; k++) {
+            const int cur = (launched + k) & 1;
+            k_hole_sweep<<<nb, 256, 0, ctx->stream>>>(ctx->W, ctx->H, (int)nh, ctx->hole_list, bufs[cur],
+                                                      bufs[cur ^ 1], dst, ctx->partials, ctx->counters + 9);
+        }
+        launched += chunk;
+        LFRV_LAUNCH(ctx, "hole filling");
+        LFRV_HIP(ctx, hipMemcpyAsync(&host, dst, sizeof host, hipMemcpyDeviceToHost, ctx->stream));
+        LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (host.done) break;
+    }
+    // the result of sweep k (1-based) is in bufs[k & 1]
+    if (host.iters & 1)
+        LFRV_HIP(ctx, hipMemcpyAsync(ctx->U, ctx->tmp3N, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    if (ran) *ran = host.iters;
+    return LFRV_OK;
+}
+
+lfrv_status run_gauss_smooth(lfrv_ctx *ctx, const float *in, float *out, float *tmp)
+{
+    // gsl_kernel_gauss_1xn(11, 2.0) (gsl_matrix_convolutions.cpp:42-63)
+    Taps t{};
+    t.r = 5;
+    double s = 0.0, k[11];
+    for (int x = -5; x <= 5; x++) {
+        k[x + 5] = std::exp(-0.5 * x * x / 4.0);
+        s += k[x + 5];
+    }
+    for (int i = 0; i < 11; i++) t.k[i] = (float)(k[i] / s);
+    dim3 bs(256), gs((ctx->W + 255) / 256, ctx->H);
+    k_conv_rows<<<gs, bs, 0, ctx->stream>>>(ctx->W, ctx->H, t, in, tmp);
+    k_conv_cols<<<gs, bs, 0, ctx->stream>>>(ctx->W, ctx->H, t, tmp, out);
+    LFRV_LAUNCH(ctx, "gaussian");
+    return LFRV_OK;
+}
+
+lfrv_status run_push_pull(lfrv_ctx *ctx, int levels)
+{
+    const int W = ctx->W, H = ctx->H;
+    const size_t N = ctx->N;
+    // level offsets inside the pyramid scratch: level 0 is a copy of U
+    std::vector<size_t> off(levels + 1, 0);
+    std::vector<int> ws(levels + 1, 0), hs(levels + 1, 0);
+    size_t tot = 0;
+    int top = 0;
+    ws[0] = W;
+    hs[0] = H;
+    off[0] = 0;
+    tot = N;
+    for (int s = 1; s <= levels; s++) {
+        const int wsn = W >> s, hsn = H >> s;
+        if (wsn <= 0 || hsn <= 0) break;
+        ws[s] = wsn;
+        hs[s] = hsn;
+        off[s] = tot;
+        tot += (size_t)wsn * hsn;
+        top = s;
+    }
+    // scratch: pyramid images (3 * tot floats) + row temp (3 * N floats) + vis pyramid (tot bytes)
+    const size_t need = 3 * tot + 3 * N + (tot + 3) / 4 + 1;
+    if (need > ctx->pp_cap) return fail(ctx, LFRV_E_ARG, "push-pull scratch too small");
+    float *img = ctx->pp_scratch, *tmp = img + 3 * tot;
+    uint8_t *vis = reinterpret_cast<uint8_t *>(tmp + 3 * N);
+    for (int c = 0; c < 3; c++)
+        LFRV_HIP(ctx, hipMemcpyAsync(img + c * tot, ctx->U + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    k_vis_to_u8<<<blocks_for(N), kBlock, 0, ctx->stream>>>(N, ctx->visibility, vis);
+    for (int s = 1; s <= top; s++) {
+        const int Ws = ws[s - 1], Hs = hs[s - 1], wsn = ws[s], hsn = hs[s];
+        dim3 g1((Ws + 255) / 256, hsn, 3), g2((wsn + 255) / 256, hsn, 3), g3((wsn + 255) / 256, hsn, 1);
+        k_pp_reduce_rows<<<g1, 256, 0, ctx->stream>>>(Ws, Hs, hsn, img + off[s - 1], tmp, tot, N);
+        k_pp_reduce_cols<<<g2, 256, 0, ctx->stream>>>(Ws, hsn, wsn, tmp, img + off[s], N, tot);
+        k_pp_reduce_vis<<<g3, 256, 0, ctx->stream>>>(Ws, Hs, wsn, hsn, vis + off[s - 1], vis + off[s]);
+    }
+    for (int s = top; s > 0; s--) {
+        const int Ws = ws[s - 1], Hs = hs[s - 1], wsn = ws[s], hsn = hs[s];
+        dim3 g1((wsn + 255) / 256, Hs, 3), g2((Ws + 255) / 256, Hs, 3);
+        k_pp_expand_rows<<<g1, 256, 0, ctx->stream>>>(wsn, hsn, Hs, img + off[s], tmp, tot, N);
+        k_pp_expand_cols<<<g2, 256, 0, ctx->stream>>>(Ws, Hs, wsn, tmp, img + off[s - 1], vis + off[s - 1], N, tot);
+    }
+    // gather the three level-0 planes contiguously, then U <- filled - U
+    for (int c = 0; c < 3; c++)
+        LFRV_HIP(ctx, hipMemcpyAsync(tmp + c * N, img + c * tot, N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    k_pp_finish<<<blocks_for(3 * N), kBlock, 0, ctx->stream>>>(N, tmp, ctx->U);
+    LFRV_LAUNCH(ctx, "push-pull");
+    return LFRV_OK;
+}
+
+lfrv_status run_pd_steps(lfrv_ctx *ctx, int iters, const float *F)
+{
+    const int W = ctx->W, H = ctx->H;
+    const float tau = (float)(0.3 / std::sqrt(8.0)), sigma = (float)(0.3 / std::sqrt(8.0));
+    const float L = (float)(1.0 / (double)ctx->p.lambda);
+    const float lam = (float)(1.0 / (double)L);
+    dim3 gs((W + PD_TW - 1) / PD_TW, (H + PD_TH - 1) / PD_TH);
+    const float *g = ctx->has_reg_g ? ctx->reg_g : nullptr;
+    TimedRegion tr(ctx, 2);
+    for (int k = 0; k < iters; k++) {
+        float *uin = ctx->Uq, *uout = ctx->U2;
+        float *pxin = ctx->PX, *pyin = ctx->PY, *pxout = ctx->P2X, *pyout = ctx->P2Y;
+        k_pd_step<<<gs, 256, 0, ctx->stream>>>(W, H, tau, lam, sigma, uin, uout, pxin, pyin, pxout, pyout, F, g);
+        std::swap(ctx->Uq, ctx->U2);
+        std::swap(ctx->PX, ctx->P2X);
+        std::swap(ctx->PY, ctx->P2Y);
+    }
+    LFRV_LAUNCH(ctx, "pd step");
+    return LFRV_OK;
+}
+
+}  // namespace lfrv

@@ -1,0 +1,509 @@
+// Warps from depth, gather (warp) and per-view weight kernels.
+#include "lfrv_internal.hpp"
+
+#include <cstring>
+
+namespace lfrv {
+
+namespace {
+
+constexpr float kInvalidDepth = 1000.0f;  // import.cpp:23
+
+// Order-preserving map float -> int for atomicMin on signed floats.
+__device__ __forceinline__ int f2key(float f)
+{
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float key2f(int k) { return __int_as_float(k >= 0 ? k : k ^ 0x7FFFFFFF); }
+
+// Pass 1 of import_warps_from_depth (import.cpp:288-339) for every view:
+// tau = pi(d M x + c), partial tau (tauPartial.frag:38-47, import.cpp:658-680)
+// and the target z-buffer over a dsf x dsf footprint.
+__global__ void k_tau_from_depth(int V, int w, int h, int dsf, const float *__restrict__ depth,
+                                 const WarpCam *__restrict__ cams, float sigma_z,
+                                 float *__restrict__ tx, float *__restrict__ ty,
+                                 float *__restrict__ dpx, float *__restrict__ dpy,
+                                 int *__restrict__ zkey)
+{
+    const size_t n = (size_t)w * h;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)V * n) return;
+    const int v = (int)(k / n);
+    const int m = (int)(k - (size_t)v * n);
+    const int W = w * dsf, H = h * dsf;
+    const WarpCam &cam = cams[v];
+    const float d = depth[k];
+    if (d < 1.0f / kInvalidDepth || kInvalidDepth < d) {
+        tx[k] = -(float)w;
+        ty[k] = -(float)h;
+        dpx[k] = 0.0f;
+        dpy[k] = 0.0f;
+        return;
+    }
+    const float xx = (float)(m % w) + 0.5f, yy = (float)(m / w) + 0.5f;
+    const float Mx0 = cam.M[0] * xx + cam.M[1] * yy + cam.M[2];
+    const float Mx1 = cam.M[3] * xx + cam.M[4] * yy + cam.M[5];
+    const float Mx2 = cam.M[6] * xx + cam.M[7] * yy + cam.M[8];
+    const float X0 = d * Mx0 + cam.c[0], X1 = d * Mx1 + cam.c[1], X2 = d * Mx2 + cam.c[2];
+    const float tux = X0 / X2, tuy = X1 / X2;
+    tx[k] = tux;
+    ty[k] = tuy;
+    dpx[k] = sigma_z * (Mx0 - tux * Mx2) / X2;
+    dpy[k] = sigma_z * (Mx1 - tuy * Mx2) / X2;
+    const int key = f2key(X2);
+    for (int i = 0; i < dsf; i++)
+        for (int j = 0; j < dsf; j++) {
+            const int px = (int)tux - dsf / 2 + j;
+            const int py = (int)tuy - dsf / 2 + i;
+            if (0 <= px && px <= W - 1 && 0 <= py && py <= H - 1) atomicMin(zkey + (size_t)py * W + px, key);
+        }
+}
+
+__global__ void k_fill_key(int *__restrict__ zkey, size_t N, int key)
+{
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o < N) zkey[o] = key;
+}
+
+__global__ void k_key_to_depth(const int *__restrict__ zkey, float *__restrict__ depth, size_t N)
+{
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o < N) depth[o] = key2f(zkey[o]);
+}
+
+// Pass 2 (import.cpp:351-442): footprint-averaged target depth at tau, round
+// trip of the target pixel int(tau)+.5 into view v; a miss by more than tol
+// pixels marks the source pixel occluded.
+__global__ void k_tau_visibility(int V, int w, int h, int dsf, const float *__restrict__ ud,
+                                 const WarpCam *__restrict__ cams, float tol,
+                                 float *__restrict__ tx, float *__restrict__ ty)
+{
+    const size_t n = (size_t)w * h;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)V * n) return;
+    const int v = (int)(k / n);
+    const int m = (int)(k - (size_t)v * n);
+    const int W = w * dsf, H = h * dsf;
+    const WarpCam &cam = cams[v];
+    const float tux = tx[k], tuy = ty[k];
+    float u_d = 0.0f, u_norm = 0.0f;
+    const float hd = 0.5f * (float)dsf;
+    for (int i = 0; i < dsf + 1; i++)
+        for (int j = 0; j < dsf + 1; j++) {
+            const int px = (int)floorf(tux - hd + 0.5f) + j;
+            const int py = (int)floorf(tuy - hd + 0.5f) + i;
+            if (0 <= px && px <= W - 1 && 0 <= py && py <= H - 1) {
+                const float z = ud[(size_t)py * W + px];
+                if (z == kInvalidDepth) continue;
+                float dx = 1.0f, dy = 1.0f;
+                if (j == 0) dx = ((float)px + 0.5f) - (tux - hd);
+                else if (j == dsf) dx = (tux + hd) - ((float)px - 0.5f);
+                if (i == 0) dy = ((float)py + 0.5f) - (tuy - hd);
+                else if (i == dsf) dy = (tuy + hd) - ((float)py - 0.5f);
+                u_d += dx * dy * z;
+                u_norm += dx * dy;
+            }
+        }
+    if (u_norm == 0.0f) {
+        tx[k] = -(float)w;
+        ty[k] = -(float)h;
+        return;
+    }
+    u_d /= u_norm;
+    const float xx = (float)(int)tux + 0.5f, yy = (float)(int)tuy + 0.5f;
+    const float X0 = u_d * (cam.Nb[0] * xx + cam.Nb[1] * yy + cam.Nb[2]) + cam.e[0];
+    const float X1 = u_d * (cam.Nb[3] * xx + cam.Nb[4] * yy + cam.Nb[5]) + cam.e[1];
+    const float X2 = u_d * (cam.Nb[6] * xx + cam.Nb[7] * yy + cam.Nb[8]) + cam.e[2];
+    const int ox = (int)(X0 / X2), oy = (int)(X1 / X2);
+    const int mx = m % w, my = m / w;
+    const float ddx = (float)(ox - mx), ddy = (float)(oy - my);
+    if (ddx < -tol || tol < ddx || ddy < -tol || tol < ddy) {
+        tx[k] = -(float)w * 2.0f;
+        ty[k] = -(float)h * 2.0f;
+    }
+}
+
+// v = S u (gradientIBR_kernels.cuh:196-290), C <= 4 planes.
+template <int KS>
+__global__ void k_gather(int W, int H, int w, int h, const float *__restrict__ tx,
+                         const float *__restrict__ ty, const float *__restrict__ u, int C,
+                         float *__restrict__ out)
+{
+    const size_t n = (size_t)w * h, N = (size_t)W * H;
+    const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const float t0 = tx[m], t1 = ty[m];
+    int x0, y0;
+    float wx[KS], wy[KS];
+    if (gather_valid(t0, t1, W, H) && footprint<KS>(t0, t1, W, H, x0, y0, wx, wy)) {
+#pragma unroll
+        for (int i = 0; i < KS; i++) {
+            const int y = y0 + i;
+            if (y < 0 || y > H - 1) continue;
+#pragma unroll
+            for (int j = 0; j < KS; j++) {
+                const int x = x0 + j;
+                if (x < 0 || x > W - 1) continue;
+                const float a = wy[i] * wx[j];
+                for (int c = 0; c < C; c++) acc[c] += a * u[c * N + (size_t)y * W + x];
+            }
+        }
+    }
+    for (int c = 0; c < C; c++) out[c * n + m] = acc[c];
+}
+
+// out += S^T (wgt * in), wsum += S^T wgt with global float atomics; the
+// reference's deconvolution_nonsep_param (gradientIBR_kernels.cuh:293-444) in
+// atomic form.  Only cell-valid source pixels contribute.
+template <int KS>
+__global__ void k_splat_atomic(int W, int H, int w, int h, const float *__restrict__ tx,
+                               const float *__restrict__ ty, const float *__restrict__ in, int C,
+                               const float *__restrict__ wgt, float *out, float *wsum)
+{
+    const size_t n = (size_t)w * h, N = (size_t)W * H;
+    const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const float t0 = tx[m], t1 = ty[m];
+    if (!cell_valid(t0, t1, W, H)) return;
+    int x0, y0;
+    float wx[KS], wy[KS];
+    if (!footprint<KS>(t0, t1, W, H, x0, y0, wx, wy)) return;
+    const float g = wgt ? wgt[m] : 1.0f;
+    float val[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (out)
+        for (int c = 0; c < C; c++) val[c] = in[c * n + m];
+#pragma unroll
+    for (int i = 0; i < KS; i++) {
+        const int y = y0 + i;
+        if (y < 0 || y > H - 1) continue;
+#pragma unroll
+        for (int j = 0; j < KS; j++) {
+            const int x = x0 + j;
+            if (x < 0 || x > W - 1) continue;
+            const float a = wy[i] * wx[j] * g;
+            const size_t o = (size_t)y * W + x;
+            if (wsum) atomicAdd(wsum + o, a);
+            if (out)
+                for (int c = 0; c < C; c++) atomicAdd(out + c * N + o, a * val[c]);
+        }
+    }
+}
+
+// Sobel/8 of the Rec.709 luminance (gradientIBR_kernels.cuh:57-99); with a
+// visibility plane, pixels with an invisible 3x3 neighbour are left unwritten
+// (vtv_sr_kernels_unstructured.cu:120-126).
+__global__ void k_sobel(int W, int H, const float *__restrict__ r, const float *__restrict__ g,
+                        const float *__restrict__ b, float *gx, float *gy,
+                        const float *__restrict__ visf, const uint8_t *__restrict__ visb)
+{
+    const int ox = blockIdx.x * blockDim.x + threadIdx.x;
+    const int oy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ox >= W || oy >= H) return;
+    const size_t o = (size_t)oy * W + ox;
+    float GX = 0.0f, GY = 0.0f;
+    if (0 < ox && ox < W - 1 && 0 < oy && oy < H - 1) {
+        if (visf || visb) {
+            bool ok = true;
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    const size_t q = o + (long)dy * W + dx;
+                    const bool vv = visf ? (visf[q] != 0.0f) : (visb[q] != 0);
+                    ok = ok && vv;
+                }
+            if (!ok) return;
+        }
+        auto Y = [&](long d) { return 0.2126f * r[o + d] + 0.7152f * g[o + d] + 0.0722f * b[o + d]; };
+        const float Ypp = Y(-W - 1), Ypc = Y(-W), Ypn = Y(-W + 1);
+        const float Ycp = Y(-1), Ycn = Y(1);
+        const float Ynp = Y(W - 1), Ync = Y(W), Ynn = Y(W + 1);
+        GX = (Ypn + 2.0f * Ycn + Ynn - Ypp - 2.0f * Ycp - Ynp) * 0.125f;
+        GY = (Ynp + 2.0f * Ync + Ynn - Ypp - 2.0f * Ypc - Ypn) * 0.125f;
+    }
+    gx[o] = GX;
+    gy[o] = GY;
+}
+
+// Per-view Sobel of the luminance (V images of w x h) and per-view Laplacian
+// of each channel (cuda_laplacian_device, vtv_sr_kernels_unstructured.cu:720-761).
+__global__ void k_view_stencils(int V, int w, int h, const float *__restrict__ views,
+                                float *__restrict__ vgrad, float *__restrict__ vlap)
+{
+    const size_t n = (size_t)w * h;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)V * n) return;
+    const int v = (int)(k / n);
+    const size_t m = k - (size_t)v * n;
+    const int ox = (int)(m % w), oy = (int)(m / w);
+    const float *r = views + (size_t)v * 3 * n, *g = r + n, *b = g + n;
+    float GX = 0.0f, GY = 0.0f;
+    if (0 < ox && ox < w - 1 && 0 < oy && oy < h - 1) {
+        auto Y = [&](long d) { return 0.2126f * r[m + d] + 0.7152f * g[m + d] + 0.0722f * b[m + d]; };
+        const float Ypp = Y(-w - 1), Ypc = Y(-w), Ypn = Y(-w + 1);
+        const float Ycp = Y(-1), Ycn = Y(1);
+        const float Ynp = Y(w - 1), Ync = Y(w), Ynn = Y(w + 1);
+        GX = (Ypn + 2.0f * Ycn + Ynn - Ypp - 2.0f * Ycp - Ynp) * 0.125f;
+        GY = (Ynp + 2.0f * Ync + Ynn - Ypp - 2.0f * Ypc - Ypn) * 0.125f;
+    }
+    vgrad[(size_t)v * 2 * n + m] = GX;
+    vgrad[(size_t)v * 2 * n + n + m] = GY;
+    for (int c = 0; c < 3; c++) {
+        const float *u = views + (size_t)v * 3 * n + c * n;
+        float s = 0.0f, wsum = 0.0f;
+        if (ox > 0) { s += u[m - 1]; wsum += 1.0f; }
+        if (oy > 0) { s += u[m - w]; wsum += 1.0f; }
+        if (ox < w - 1) { s += u[m + 1]; wsum += 1.0f; }
+        if (oy < h - 1) { s += u[m + w]; wsum += 1.0f; }
+        vlap[(size_t)v * 3 * n + c * n + m] = s / wsum - u[m];
+    }
+}
+
+// omega = angular * deformation weight for every view pixel
+// (compute_weights, gradientIBR.cu:558-648; angular_weights and
+// gold_deform_weights, gradientIBR_kernels.cuh:102-194).  The warped target
+// gradient is gathered from (gxs, gys) at the footprint of tau.
+template <int KS>
+__global__ void k_weights(int V, int W, int H, int w, int h, const float *__restrict__ tx,
+                          const float *__restrict__ ty, const float *__restrict__ gxs,
+                          const float *__restrict__ gys, const float *__restrict__ dpx,
+                          const float *__restrict__ dpy, float sigma, float thr,
+                          float *__restrict__ omega)
+{
+    const size_t n = (size_t)w * h;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)V * n) return;
+    const int v = (int)(k / n);
+    const size_t m = k - (size_t)v * n;
+    const float *vtx = tx + (size_t)v * n, *vty = ty + (size_t)v * n;
+    // warped u gradient (convolution_nonsep_param)
+    float gxv = 0.0f, gyv = 0.0f;
+    const float t0 = vtx[m], t1 = vty[m];
+    int x0, y0;
+    float wx[KS], wy[KS];
+    if (gather_valid(t0, t1, W, H) && footprint<KS>(t0, t1, W, H, x0, y0, wx, wy)) {
+#pragma unroll
+        for (int i = 0; i < KS; i++) {
+            const int y = y0 + i;
+            if (y < 0 || y > H - 1) continue;
+#pragma unroll
+            for (int j = 0; j < KS; j++) {
+                const int x = x0 + j;
+                if (x < 0 || x > W - 1) continue;
+                const float a = wy[i] * wx[j];
+                gxv += a * gxs[(size_t)y * W + x];
+                gyv += a * gys[(size_t)y * W + x];
+            }
+        }
+    }
+    // angular weight
+    const float nrm = sqrtf(gxv * gxv + gyv * gyv);
+    if (nrm > thr) {
+        gxv = gxv / nrm * thr;
+        gyv = gyv / nrm * thr;
+    }
+    float sg = gxv * dpx[k] + gyv * dpy[k];
+    if (sg > 0.5f) sg = 0.5f;
+    const float ang = 1.0f / (sg * sg + sigma * sigma);
+    // deformation weight
+    const int mx = (int)(m % w), my = (int)(m / w);
+    auto inval = [&](size_t q) {
+        return vtx[q] < 0.0f || vtx[q] > (float)W || vty[q] < 0.0f || vty[q] > (float)H;
+    };
+    float dw;
+    if (inval(m)) dw = 0.0f;
+    else if (mx == 0 || my == 0 || mx == w - 1 || my == h - 1) dw = 1.0f;
+    else if (inval(m - 1) || inval(m + 1) || inval(m - w) || inval(m + w)) dw = 1.0f;
+    else {
+        const float Dxx = 0.5f * (vtx[m + 1] - vtx[m - 1]);
+        const float Dyx = 0.5f * (vty[m + 1] - vty[m - 1]);
+        const float Dxy = 0.5f * (vtx[m + w] - vtx[m - w]);
+        const float Dyy = 0.5f * (vty[m + w] - vty[m - w]);
+        const float Nd = fabsf(Dxx * Dyy - Dxy * Dyx);
+        dw = (Nd > 1.0f) ? 1.0f / Nd : 1.0f;
+    }
+    omega[k] = ang * dw;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+static void mat3_mul(const double *A, const double *B, double *C)
+{
+    double T[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++) s += A[3 * i + k] * B[3 * k + j];
+            T[3 * i + j] = s;
+        }
+    for (int i = 0; i < 9; i++) C[i] = T[i];
+}
+
+void decompose_P(const double *P, double *K, double *R, double *t, double *C);  // camera.cpp
+
+static void kinv_ref(const double *K, double *Ki)
+{
+    for (int i = 0; i < 9; i++) Ki[i] = 0.0;
+    Ki[0] = 1.0 / K[0];
+    Ki[4] = 1.0 / K[4];
+    Ki[2] = -K[2] / K[0];
+    Ki[5] = -K[5] / K[4];
+    Ki[8] = 1.0;
+}
+
+lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
+{
+    if (!ctx->have_target_cam) return fail(ctx, LFRV_E_STATE, "lfrv_warps_from_depth: target camera not set");
+    for (int v = 0; v < ctx->V; v++)
+        if (!ctx->have_depth[v]) return fail(ctx, LFRV_E_STATE, "lfrv_warps_from_depth: depth of a view not set");
+    // camera algebra in FP64 on the host (import.cpp:301-320, 412-430)
+    std::vector<WarpCam> hc(ctx->V);
+    double Ku[9], Ru[9], tu[3], Cu[3], Kui[9];
+    decompose_P(ctx->P_target, Ku, Ru, tu, Cu);
+    kinv_ref(Ku, Kui);
+    const double *Pu = ctx->P_target;
+    for (int v = 0; v < ctx->V; v++) {
+        const double *Pv = ctx->P_views.data() + 12 * v;
+        double Kv[9], Rv[9], tv[3], Cv[3], Kvi[9], RvT[9], Pu3[9], M[9], Pv3[9], RbT[9], Nb[9];
+        decompose_P(Pv, Kv, Rv, tv, Cv);
+        kinv_ref(Kv, Kvi);
+        const double *Rb = ctx->p.backproject_source_cam ? Rv : Ru;
+        const double *Cb = ctx->p.backproject_source_cam ? Cv : Cu;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                RvT[3 * i + j] = Rv[3 * j + i];
+                RbT[3 * i + j] = Rb[3 * j + i];
+                Pu3[3 * i + j] = Pu[4 * i + j];
+                Pv3[3 * i + j] = Pv[4 * i + j];
+            }
+        mat3_mul(RvT, Kvi, M);
+        mat3_mul(Pu3, M, M);
+        mat3_mul(RbT, Kui, Nb);
+        mat3_mul(Pv3, Nb, Nb);
+        WarpCam &c = hc[v];
+        for (int i = 0; i < 9; i++) {
+            c.M[i] = (float)M[i];
+            c.Nb[i] = (float)Nb[i];
+        }
+        for (int i = 0; i < 3; i++) {
+            c.c[i] = (float)(Pu[4 * i] * Cv[0] + Pu[4 * i + 1] * Cv[1] + Pu[4 * i + 2] * Cv[2] + Pu[4 * i + 3]);
+            c.e[i] = (float)(Pv[4 * i] * Cb[0] + Pv[4 * i + 1] * Cb[1] + Pv[4 * i + 2] * Cb[2] + Pv[4 * i + 3]);
+        }
+    }
+    WarpCam *dcams = reinterpret_cast<WarpCam *>(ctx->partials);  // scratch, >= V*sizeof(WarpCam)
+    if (ctx->partials_cap * sizeof(double) < hc.size() * sizeof(WarpCam))
+        return fail(ctx, LFRV_E_ARG, "too many views for the camera scratch");
+    LFRV_HIP(ctx, hipMemcpyAsync(dcams, hc.data(), hc.size() * sizeof(WarpCam), hipMemcpyHostToDevice, ctx->stream));
+    int *zkey = reinterpret_cast<int *>(ctx->tmpN);
+    TimedRegion tr(ctx, 4);
+    int inv_key;
+    {
+        const float f = kInvalidDepth;
+        std::memcpy(&inv_key, &f, sizeof inv_key);  // positive float: key == bit pattern
+    }
+    k_fill_key<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(zkey, ctx->N, inv_key);
+    const size_t tot = (size_t)ctx->V * ctx->n;
+    k_tau_from_depth<<<blocks_for(tot), kBlock, 0, ctx->stream>>>(
+        ctx->V, ctx->w, ctx->h, ctx->dsf, ctx->depth, dcams, ctx->p.sigma_z, ctx->tau_x, ctx->tau_y,
+        ctx->dpart_x, ctx->dpart_y, zkey);
+    k_key_to_depth<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(zkey, ctx->target_depth, ctx->N);
+    k_tau_visibility<<<blocks_for(tot), kBlock, 0, ctx->stream>>>(
+        ctx->V, ctx->w, ctx->h, ctx->dsf, ctx->target_depth, dcams, ctx->p.occlusion_tol_px,
+        ctx->tau_x, ctx->tau_y);
+    LFRV_LAUNCH(ctx, "warps_from_depth kernels");
+    ctx->prepared = false;
+    return LFRV_OK;
+}
+
+template <int KS>
+static void launch_weights_ks(lfrv_ctx *ctx, const float *gxs, const float *gys)
+{
+    const size_t tot = (size_t)ctx->V * ctx->n;
+    k_weights<KS><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(
+        ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, gxs, gys, ctx->dpart_x,
+        ctx->dpart_y, ctx->p.sigma_sensor, ctx->p.ugrad_threshold, ctx->omega);
+}
+
+lfrv_status run_compute_weights(lfrv_ctx *ctx)
+{
+    const float *gxs, *gys;
+    dim3 bs(32, 8), gs((ctx->W + 31) / 32, (ctx->H + 7) / 8);
+    const size_t N = ctx->N;
+    if (ctx->p.mode == LFRV_MODE_COLOR_SR) {
+        // vtv_sr_compute_gradient_device writes the gated Sobel of U into the
+        // dual variable of channel 0 (optimization.cu:1090-1091); kept as is.
+        float *px0 = ctx->pd_swapped ? ctx->P2X : ctx->PX;
+        float *py0 = ctx->pd_swapped ? ctx->P2Y : ctx->PY;
+        k_sobel<<<gs, bs, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->U + N, ctx->U + 2 * N, px0, py0,
+                                            ctx->visibility, nullptr);
+        gxs = px0;
+        gys = py0;
+    } else {
+        // compute_gradient over the whole target (gradientIBR.cu:578-580 launched
+        // it on the low-res grid only; the build covers W x H).
+        k_sobel<<<gs, bs, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->U + N, ctx->U + 2 * N,
+                                            ctx->u_grad_x, ctx->u_grad_y, nullptr, nullptr);
+        gxs = ctx->u_grad_x;
+        gys = ctx->u_grad_y;
+    }
+    switch (ctx->ks) {
+    case 2: launch_weights_ks<2>(ctx, gxs, gys); break;
+    case 3: launch_weights_ks<3>(ctx, gxs, gys); break;
+    case 4: launch_weights_ks<4>(ctx, gxs, gys); break;
+    case 5: launch_weights_ks<5>(ctx, gxs, gys); break;
+    default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
+    }
+    LFRV_LAUNCH(ctx, "compute_weights kernels");
+    ctx->entry_weights_valid = false;
+    return LFRV_OK;
+}
+
+lfrv_status run_view_stencils(lfrv_ctx *ctx)
+{
+    const size_t tot = (size_t)ctx->V * ctx->n;
+    k_view_stencils<<<blocks_for(tot), kBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->views,
+                                                                 ctx->view_grad, ctx->view_lap);
+    LFRV_LAUNCH(ctx, "view stencils");
+    return LFRV_OK;
+}
+
+hipError_t launch_gather(int W, int H, int w, int h, int dsf, const float *tx, const float *ty,
+                         const float *u, int C, float *out, hipStream_t s)
+{
+    const size_t n = (size_t)w * h;
+    switch (dsf + 1) {
+    case 2: k_gather<2><<<blocks_for(n), kBlock, 0, s>>>(W, H, w, h, tx, ty, u, C, out); break;
+    case 3: k_gather<3><<<blocks_for(n), kBlock, 0, s>>>(W, H, w, h, tx, ty, u, C, out); break;
+    case 4: k_gather<4><<<blocks_for(n), kBlock, 0, s>>>(W, H, w, h, tx, ty, u, C, out); break;
+    case 5: k_gather<5><<<blocks_for(n), kBlock, 0, s>>>(W, H, w, h, tx, ty, u, C, out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_splat_atomic(int W, int H, int w, int h, int dsf, const float *tx,
+                               const float *ty, const float *in, int C, const float *wgt,
+                               float *out, float *wsum, hipStream_t s)
+{
+    const size_t n = (size_t)w * h;
+    switch (dsf + 1) {
+    case 2: k_splat_atomic<2><<<blocks_for(n), kBlock, 0, s>>>(W, H, w, h, tx, ty, in, C, wgt, out, wsum); break;
+    case 3: k_splat_atomic<3><<<blocks_for(n), kBlock, 0, s>>>(W, H, w, h, tx, ty, in, C, wgt, out, wsum); break;
+    case 4: k_splat_atomic<4><<<blocks_for(n), kBlock, 0, s>>>(W, H, w, h, tx, ty, in, C, wgt, out, wsum); break;
+    case 5: k_splat_atomic<5><<<blocks_for(n), kBlock, 0, s>>>(W, H, w, h, tx, ty, in, C, wgt, out, wsum); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sobel(int W, int H, const float *r, const float *g, const float *b, float *gx,
+                        float *gy, const float *visf, const uint8_t *visb, hipStream_t s)
+{
+    dim3 bs(32, 8), gs((W + 31) / 32, (H + 7) / 8);
+    k_sobel<<<gs, bs, 0, s>>>(W, H, r, g, b, gx, gy, visf, visb);
+    return hipGetLastError();
+}
+
+}  // namespace lfrv

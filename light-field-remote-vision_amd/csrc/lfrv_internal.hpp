@@ -1,0 +1,324 @@
+// Internal declarations of the lfrv HIP library (gfx950 / CDNA4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lfrv.h"
+
+namespace lfrv {
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+constexpr int kMaxKs = 5;       // dsf <= 4
+constexpr int kBlock = 256;     // threads per block of the elementwise kernels
+constexpr int kNumXcd = 8;      // MI355X: 8 XCDs, blocks are dealt round-robin
+
+// Map a linear block id to a tile id so that each XCD receives a contiguous
+// run of tiles (neighbouring tiles then share that XCD's L2 for their halos).
+__device__ __forceinline__ int xcd_tile(int bid, int ntiles)
+{
+    const int q = ntiles / kNumXcd, r = ntiles % kNumXcd;
+    const int x = bid % kNumXcd, k = bid / kNumXcd;
+    return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// Gather validity (gradientIBR_kernels.cuh:216-219).
+__host__ __device__ __forceinline__ bool gather_valid(float tx, float ty, int W, int H)
+{
+    return !(tx < 0.0f || ty < 0.0f || tx > (float)W || ty > (float)H);
+}
+
+// Splat validity = membership of the reference's cell list
+// (gradientIBR.cu:404-426): gather-valid and floor(tau - .5) inside the target.
+__host__ __device__ __forceinline__ bool cell_valid(float tx, float ty, int W, int H)
+{
+    if (!gather_valid(tx, ty, W, H)) return false;
+    const int px = (int)floorf(tx - 0.5f), py = (int)floorf(ty - 0.5f);
+    return !(px < 0 || py < 0 || px > W - 1 || py > H - 1);
+}
+
+// Footprint weights of one warped pixel (set_B_bilinear,
+// gradientIBR_kernels.cuh:447-507).  The 2-D weights are wy[i]*wx[j]; both
+// factors come out normalised over the in-image taps (the 2-D normaliser of
+// the reference factorises).  Out-of-image taps get weight 0.  Returns false
+// when every in-image tap has zero weight.
+template <int KS>
+__device__ __forceinline__ bool footprint(float tx, float ty, int W, int H, int &x0, int &y0,
+                                          float (&wx)[KS], float (&wy)[KS])
+{
+    constexpr float hd = 0.5f * (float)(KS - 1);
+    const float px = tx - 0.5f, py = ty - 0.5f;
+    x0 = (int)floorf(px - hd + 0.5f);
+    y0 = (int)floorf(py - hd + 0.5f);
+    float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+    for (int j = 0; j < KS; j++) {
+        const int x = x0 + j;
+        float d = 1.0f;
+        if (j == 0) d = ((float)x + 0.5f) - (px - hd);
+        else if (j == KS - 1) d = (px + hd) - ((float)x - 0.5f);
+        d = (x >= 0 && x <= W - 1) ? d : 0.0f;
+        wx[j] = d;
+        sx += d;
+    }
+#pragma unroll
+    for (int i = 0; i < KS; i++) {
+        const int y = y0 + i;
+        float d = 1.0f;
+        if (i == 0) d = ((float)y + 0.5f) - (py - hd);
+        else if (i == KS - 1) d = (py + hd) - ((float)y - 0.5f);
+        d = (y >= 0 && y <= H - 1) ? d : 0.0f;
+        wy[i] = d;
+        sy += d;
+    }
+    const float nrm = sx * sy;
+    if (nrm == 0.0f) return false;
+    const float isx = 1.0f / sx, isy = 1.0f / sy;
+#pragma unroll
+    for (int j = 0; j < KS; j++) wx[j] *= isx;
+#pragma unroll
+    for (int i = 0; i < KS; i++) wy[i] *= isy;
+    return true;
+}
+
+// Wave-level FP64 sum (wave64) followed by a block-level sum through LDS.
+// Every thread must call it; the result is valid in thread 0.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+template <int NQ>
+__device__ __forceinline__ void block_sum(double (&v)[NQ], double *lds /* >= NQ * 16 */)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const double s = wave_sum(v[q]);
+        if (lane == 0) lds[q * 16 + wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            double s = 0.0;
+            for (int k = 0; k < nw; k++) s += lds[q * 16 + k];
+            v[q] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Grid-wide reduction of NQ doubles per block: each block stores its partial
+// row, the last block to arrive sums all rows in block order (deterministic)
+// into v[] of thread 0 (and into `out` when non-null) and resets the arrival
+// counter.  Returns true in the last block (all threads), so the caller can
+// finalise scalars there from thread 0.
+template <int NQ>
+__device__ bool grid_reduce(double (&v)[NQ], double *partials, unsigned int *counter, double *out,
+                            double *lds)
+{
+    block_sum<NQ>(v, lds);
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) partials[(size_t)blockIdx.x * NQ + q] = v[q];
+        __threadfence();
+        const unsigned int t = atomicAdd(counter, 1u);
+        s_last = (t == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) acc[q] = 0.0;
+    // Thread t sums blocks t, t+blockDim, ... in order; then a fixed-order block sum.
+    for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+        const volatile double *pr = partials + (size_t)b * NQ;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) acc[q] += pr[q];
+    }
+    block_sum<NQ>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            v[q] = acc[q];
+            if (out) out[q] = acc[q];
+        }
+        *counter = 0u;
+    }
+    __syncthreads();
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+// Per-view camera data for the depth warp, in FP32 (built in FP64 on the host).
+struct WarpCam {
+    float M[9];   // K_u R_u R_v^T K_v^-1 (import.cpp:301-320 composed)
+    float c[3];   // P_u (C_v, 1)
+    float Nb[9];  // P_v3 R_b^T K_u^-1 (back-projection of a target pixel, import.cpp:412-430)
+    float e[3];   // P_v (C_b, 1)
+};
+
+// CG scalars, device resident (one problem, three RGB channels).
+struct CgState {
+    double rz[3], dq[3], rr[3], bb[3];
+    double alpha[3], beta[3];
+    double tol2;
+    int active[3];
+    int iter[3];
+    int first;    // the next operator launch builds d = z (no previous direction)
+    int done;
+    int maxit;
+    int breakdown[3];
+};
+
+struct TimingSlot {
+    double ms = 0.0;
+    int64_t launches = 0;
+};
+
+}  // namespace lfrv
+
+struct lfrv_ctx {
+    lfrv_params p{};
+    int V = 0, w = 0, h = 0, dsf = 1, ks = 2, W = 0, H = 0;
+    size_t n = 0, N = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // views (contiguous over views)
+    float *views = nullptr;    // V*3n
+    float *tau_x = nullptr, *tau_y = nullptr;      // V*n
+    float *dpart_x = nullptr, *dpart_y = nullptr;  // V*n
+    float *omega = nullptr;    // V*n
+    float *depth = nullptr;    // V*n
+    float *view_lap = nullptr; // V*3n Laplacian of the views (IBR_color gradient term)
+    float *view_grad = nullptr;// V*2n Sobel of the views' luminance
+    std::vector<double> P_views;  // V*12
+    double P_target[12] = {0};
+    bool have_target_cam = false;
+    std::vector<int> have_depth;
+
+    // binned splat structure
+    int tile_w = 32, tile_h = 16, ntx = 0, nty = 0, ntiles = 0;
+    int *tile_count = nullptr, *tile_start = nullptr, *tile_cursor = nullptr;  // ntiles(+1)
+    int *e_src = nullptr;       // view * n + m
+    float *e_tx = nullptr, *e_ty = nullptr, *e_w = nullptr;
+    size_t e_cap = 0;
+    int64_t e_total = 0;
+    bool prepared = false;
+    bool entry_weights_valid = false;
+
+    // target side (3 planes each unless noted)
+    float *U = nullptr, *Uq = nullptr, *G = nullptr;
+    float *PX = nullptr, *PY = nullptr;      // dual variables
+    float *U2 = nullptr, *P2X = nullptr, *P2Y = nullptr;  // ping-pong partners of the PD step
+    float *reg_g = nullptr;        // N (valid once has_reg_g)
+    bool has_reg_g = false;
+    float *norm_mask = nullptr;    // N
+    float *coverage = nullptr;     // N
+    float *u_grad_x = nullptr, *u_grad_y = nullptr, *target_mask = nullptr;  // N
+    float *visibility = nullptr;   // N (0/1)
+    float *target_depth = nullptr; // N
+    float *tmpN = nullptr;         // N scratch
+    float *tmp3N = nullptr;        // 3N scratch
+    double fista_alpha = 1.0;
+
+    // CG
+    float *cg_r = nullptr, *cg_d = nullptr, *cg_q = nullptr, *cg_b = nullptr;  // 3N
+    float *cg_minv = nullptr;      // N
+    lfrv::CgState *cg_state = nullptr;
+    lfrv::CgState *cg_state_host = nullptr;   // pinned
+    double *partials = nullptr;    // per-block partial sums
+    size_t partials_cap = 0;       // doubles
+    unsigned int *counters = nullptr;  // arrival counters (one per reduction site)
+    double *scal = nullptr;        // device scalars (reduction outputs)
+    double *scal_host = nullptr;   // pinned mirror
+    int *hole_list = nullptr;      // N
+    int n_holes = 0;
+    int hole_iters = 0;            // sweeps run by the last hole filling
+    float *pp_scratch = nullptr;   // push-pull pyramid scratch
+    size_t pp_cap = 0;             // floats
+    bool vis_ready = false;
+
+    // PD ping-pong orientation: which buffers currently hold Uq / p
+    bool pd_swapped = false;
+
+    // timing
+    bool timing = false;
+    lfrv::TimingSlot tslot[6];
+    std::vector<hipEvent_t> ev_pool;
+};
+
+namespace lfrv {
+
+// error helpers
+lfrv_status fail(lfrv_ctx *ctx, lfrv_status st, const std::string &msg);
+lfrv_status hip_fail(lfrv_ctx *ctx, hipError_t e, const char *what);
+
+#define LFRV_HIP(ctx, call)                                       \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return lfrv::hip_fail(ctx, e_, #call); \
+    } while (0)
+
+#define LFRV_LAUNCH(ctx, what)                                    \
+    do {                                                          \
+        hipError_t e_ = hipGetLastError();                        \
+        if (e_ != hipSuccess) return lfrv::hip_fail(ctx, e_, what); \
+    } while (0)
+
+// Timed launch bracket (class k); no-ops unless timing is on.
+struct TimedRegion {
+    lfrv_ctx *ctx;
+    int k;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedRegion(lfrv_ctx *c, int kk);
+    ~TimedRegion();
+};
+void flush_timing(lfrv_ctx *ctx);
+
+inline unsigned int blocks_for(size_t n, int bs = kBlock) { return (unsigned int)((n + bs - 1) / bs); }
+
+// kernels-side entry points (defined in the .hip files)
+lfrv_status run_warps_from_depth(lfrv_ctx *ctx);
+lfrv_status run_prepare(lfrv_ctx *ctx);
+lfrv_status run_refresh_entry_weights(lfrv_ctx *ctx, float scale);
+lfrv_status run_visibility(lfrv_ctx *ctx);
+lfrv_status run_compute_weights(lfrv_ctx *ctx);
+lfrv_status run_initial_image(lfrv_ctx *ctx);
+lfrv_status run_hole_filling(lfrv_ctx *ctx, int iters, double eps, int *ran);
+lfrv_status run_push_pull(lfrv_ctx *ctx, int levels);
+lfrv_status run_u_gradient(lfrv_ctx *ctx);
+lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use_views,
+                   bool use_precond, bool poisson_rhs, double tol, int maxit, lfrv_cg_stats *st);
+lfrv_status run_gibr_energy(lfrv_ctx *ctx, double *e);
+lfrv_status run_regularizer_weight(lfrv_ctx *ctx);
+lfrv_status run_fista(lfrv_ctx *ctx);
+lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts);
+lfrv_status run_count_moving(lfrv_ctx *ctx, const float *prev_dev, float max_diff, int64_t *moving);
+
+// stateless kernels
+hipError_t launch_gather(int W, int H, int w, int h, int dsf, const float *tx, const float *ty,
+                         const float *u, int C, float *out, hipStream_t s);
+hipError_t launch_splat_atomic(int W, int H, int w, int h, int dsf, const float *tx,
+                               const float *ty, const float *in, int C, const float *wgt,
+                               float *out, float *wsum, hipStream_t s);
+hipError_t launch_sobel(int W, int H, const float *r, const float *g, const float *b, float *gx,
+                        float *gy, const float *visf, const uint8_t *visb, hipStream_t s);
+
+}  // namespace lfrv

@@ -1,0 +1,133 @@
+"""GPU parity of the gradient-domain IBR path (IBR_gradient) against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import lfrv
+import oracle
+from lfrv import pipelines
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(scene, **kw):
+    ctx = lfrv.Context(pipelines.gradient_ibr_params(scene, **kw))
+    pipelines.load_scene(ctx, scene)
+    return ctx
+
+
+def test_warps_from_depth_matches_oracle(small_scene):
+    s = small_scene
+    with _ctx(s) as ctx:
+        ctx.warps_from_depth()
+        tx = ctx.get_buffer("TAU_X").reshape(s["V"], s["h"], s["w"])
+        ty = ctx.get_buffer("TAU_Y").reshape(s["V"], s["h"], s["w"])
+        dpx = ctx.get_buffer("DPART_X").reshape(s["V"], s["h"], s["w"])
+        zb = ctx.get_buffer("TARGET_DEPTH").reshape(s["H"], s["W"])
+    otx, oty, odx, ody, oz = oracle.warps_from_depth(s["depths"], s["P_views"], s["P_target"], s["dsf"])
+    valid = (otx >= 0) & (tx >= 0)
+    # visibility decisions (int truncation of tau) may flip on a few boundary pixels
+    assert np.mean((otx >= 0) != (tx >= 0)) < 2e-3
+    assert np.max(np.abs(tx[valid] - otx[valid])) < 2e-3
+    assert np.max(np.abs(ty[valid] - oty[valid])) < 2e-3
+    assert np.max(np.abs(dpx - odx)) < 1e-3 * max(1.0, np.abs(odx).max())
+    fin = oz < 1000
+    assert np.mean(np.abs(zb[fin] - oz[fin]) > 1e-3) < 2e-3
+
+
+def test_pipeline_stages_match_oracle(small_scene):
+    """Every stage fed with the GPU's own warps, compared with the oracle stage."""
+    s = small_scene
+    sig = 1.0 / 255.0
+    with _ctx(s) as ctx:
+        ctx.warps_from_depth()
+        ctx.prepare()
+        ctx.setup_visibility()
+        ctx.compute_weights()
+        V, h, w, H, W = s["V"], s["h"], s["w"], s["H"], s["W"]
+        tx = ctx.get_buffer("TAU_X").reshape(V, h, w)
+        ty = ctx.get_buffer("TAU_Y").reshape(V, h, w)
+        dpx = ctx.get_buffer("DPART_X").reshape(V, h, w)
+        dpy = ctx.get_buffer("DPART_Y").reshape(V, h, w)
+        om = ctx.get_buffer("OMEGA").reshape(V, h, w)
+        vis = ctx.get_buffer("VISIBILITY").reshape(H, W)
+        cov = ctx.get_buffer("COVERAGE").reshape(H, W)
+        o_om = oracle.gibr_weights(W, H, s["dsf"], tx, ty, dpx, dpy, np.zeros((3, H, W)), sig, 1.0)
+        assert rel_err(om, o_om) < 1e-6
+        o_vis = oracle.target_visibility(W, H, s["dsf"], list(zip(tx, ty)))
+        assert np.array_equal(vis.astype(np.uint8), o_vis)
+        _, o_cov = oracle.splat(W, H, s["dsf"], tx[0], ty[0], None)
+        for v in range(1, V):
+            oracle.splat(W, H, s["dsf"], tx[v], ty[v], None, wsum=o_cov)
+        assert np.max(np.abs(cov - o_cov)) < 1e-5
+        ctx.compute_initial_image()
+        U0 = ctx.get_solution()
+        oU, _ = oracle.initial_image(W, H, s["dsf"], tx, ty, s["views"], om)
+        oU, _ = oracle.hole_filling(oU, o_vis)
+        assert np.max(np.abs(U0 - oU)) < 1e-5
+        ctx.init_u_gradient()
+        gx = ctx.get_buffer("U_GRAD_X").reshape(H, W)
+        gy = ctx.get_buffer("U_GRAD_Y").reshape(H, W)
+        tm = ctx.get_buffer("TARGET_MASK").reshape(H, W)
+        ogx, ogy, otm, _ = oracle.u_gradient(W, H, s["dsf"], tx, ty, s["views"], om, sig)
+        assert np.max(np.abs(gx - ogx)) < 1e-6
+        assert np.max(np.abs(gy - ogy)) < 1e-6
+        assert np.max(np.abs(tm - otm)) < 1e-6
+        st = ctx.cg_solve(1e-5, 2000)
+        U = ctx.get_solution()
+        b = ctx.get_buffer("CG_B").reshape(3, H, W)
+        wd = om.astype(np.float64) * sig * sig
+        ob = oracle.rhs(W, H, s["dsf"], tx, ty, s["views"], wd, 1.0, 0.0, None, 0.5, gx, gy)
+        assert rel_err(b, ob) < 1e-5
+        oX, ost = oracle.pcg(W, H, s["dsf"], tx, ty, wd, 1.0, 0.0, 0.5, ob, U0, 1e-5, 2000)
+        assert all(st["converged"]) and all(ost["converged"])
+        assert max(st["true_rel_residual"]) < 2e-5
+        assert rel_err(U, oX) < 1e-4
+        E = ctx.gibr_energy()
+        oE = oracle.gibr_energy(W, H, s["dsf"], tx, ty, s["views"], wd, 1.0, 0.0, None, 0.5, gx, gy, oX)
+        assert abs(E - oE) / oE < 1e-4
+        assert abs(oracle.psnr(U, s["target"]) - oracle.psnr(oX, s["target"])) < 0.01
+
+
+def test_end_to_end_matches_oracle(small_scene):
+    s = small_scene
+    with _ctx(s) as ctx:
+        st = pipelines.gradient_ibr_solve(ctx, 1e-5, 2000)
+        U = ctx.get_solution()
+    r = oracle.gibr_pipeline(s["views"], s["depths"], s["P_views"], s["P_target"], s["dsf"], 1 / 255.0, 1.0,
+                             1.0, 0.5, 1e-5, 2000)
+    assert all(st["converged"])
+    assert rel_err(U, r["U"]) < 1e-3
+    assert abs(oracle.psnr(U, s["target"]) - oracle.psnr(r["U"], s["target"])) < 0.01
+
+
+def test_stateless_gather_splat_adjoint(small_scene):
+    """<S u, r> == <u, S^T r> on the GPU's stateless operators (FP64 check of FP32 kernels)."""
+    import torch
+    s = small_scene
+    dev = torch.device("cuda:0")
+    W, H, w, h, dsf = s["W"], s["H"], s["w"], s["h"], s["dsf"]
+    rng = np.random.default_rng(0)
+    tx = (np.arange(w)[None, :] + 0.5 + rng.uniform(0.5, 3.0, (h, w))).astype(np.float32)
+    ty = (np.arange(h)[:, None] + 0.5 + rng.uniform(-2.0, 1.0, (h, w))).astype(np.float32)
+    u = rng.standard_normal((3, H, W)).astype(np.float32)
+    r = rng.standard_normal((3, h, w)).astype(np.float32)
+    # restrict the gather to cell-valid pixels, like the splat
+    cv = np.vectorize(lambda a, b: oracle.cell_valid(a, b, W, H))(tx, ty)
+    txg = np.where(cv, tx, -1.0).astype(np.float32)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    dtx, dty, dtxg, du, dr = T(tx), T(ty), T(txg), T(u), T(r)
+    su = torch.zeros((3, h, w), device=dev)
+    st = torch.zeros((3, H, W), device=dev)
+    L = lfrv.lib()
+    assert L.lfrv_op_gather(W, H, w, h, dsf, dtxg.data_ptr(), dty.data_ptr(), du.data_ptr(), 3, su.data_ptr(), None) == 0
+    assert L.lfrv_op_splat_atomic(W, H, w, h, dsf, dtx.data_ptr(), dty.data_ptr(), dr.data_ptr(), 3, None,
+                                  st.data_ptr(), None, None) == 0
+    torch.cuda.synchronize()
+    lhs = float((su.double() * dr.double()).sum())
+    rhs = float((du.double() * st.double()).sum())
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+    osu = oracle.gather(W, H, dsf, txg, ty, u)
+    assert np.max(np.abs(su.cpu().numpy() - osu)) < 1e-5
+    ost, _ = oracle.splat(W, H, dsf, tx, ty, r)
+    assert np.max(np.abs(st.cpu().numpy() - ost)) < 1e-4
