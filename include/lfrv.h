@@ -165,6 +165,10 @@ lfrv_status lfrv_get_solution(lfrv_ctx *ctx, float *rgb);
 /* Overwrite the current solution (planar RGB W*H). */
 lfrv_status lfrv_set_solution(lfrv_ctx *ctx, const float *rgb);
 
+/* Zero the current solution on the device (the cleared U of init_data,
+ * gradientIBR.cu:281-287, and of compute_initial_image, :732-735). */
+lfrv_status lfrv_clear_solution(lfrv_ctx *ctx);
+
 /* ---- gradient-domain IBR --------------------------------------------------- */
 
 /* Target gradient g and target mask by splatting the views' Sobel gradients.
@@ -280,6 +284,23 @@ lfrv_status lfrv_op_splat_atomic(int W, int H, int w, int h, int dsf, const floa
  * (gated pixels are left unwritten).  Device pointers. */
 lfrv_status lfrv_op_sobel(int W, int H, const float *rgb, float *gx, float *gy,
                           const uint8_t *vis, void *stream);
+
+/* ---- device memory helpers ------------------------------------------------- */
+/* Plumbing for callers that hold no device allocator of their own (no
+ * counterpart in the reference, whose buffers are private to Data /
+ * coco_vtv_workspace): allocate, free and copy device memory on the
+ * library's HIP runtime, so the stateless operators above can be fed without
+ * a second runtime in the process. */
+lfrv_status lfrv_dev_alloc(size_t bytes, void **ptr);
+lfrv_status lfrv_dev_free(void *ptr);
+/* Synchronous copy between any host / device pointers. */
+lfrv_status lfrv_memcpy(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte vector loads and
+ * stores; bytes and both pointers 16-byte aligned).  Its memory-counter
+ * readings calibrate the HBM-traffic figures of the other kernels. */
+lfrv_status lfrv_dev_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
+/* Wait for the default stream (the stateless operators' NULL stream). */
+lfrv_status lfrv_device_sync(void);
 
 /* ---- quality metric (host) ------------------------------------------------- */
 

@@ -140,10 +140,16 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     ctx->n = (size_t)ctx->w * ctx->h;
     ctx->N = (size_t)ctx->W * ctx->H;
     ctx->tile_w = p->tile_w > 0 ? p->tile_w : 32;
-    ctx->tile_h = p->tile_h > 0 ? p->tile_h : 16;
-    if (ctx->tile_w < ctx->ks || ctx->tile_h < ctx->ks || ctx->tile_w * ctx->tile_h > 4096) {
+    ctx->tile_h = p->tile_h > 0 ? p->tile_h : 8;
+    if (ctx->tile_w < ctx->ks || ctx->tile_h < ctx->ks || ctx->tile_w * ctx->tile_h > 1024 ||
+        (ctx->tile_w + ctx->ks - 1) * (ctx->tile_h + ctx->ks - 1) > 1024) {
         delete ctx;
         return LFRV_E_ARG;
+    }
+    {
+        const int cells = (ctx->tile_w + ctx->ks - 1) * (ctx->tile_h + ctx->ks - 1);
+        const int need = std::max(cells, ctx->tile_w * ctx->tile_h);
+        ctx->pull_bs = std::min(1024, (need + 63) / 64 * 64);
     }
     ctx->ntx = (ctx->W + ctx->tile_w - 1) / ctx->tile_w;
     ctx->nty = (ctx->H + ctx->tile_h - 1) / ctx->tile_h;
@@ -165,10 +171,12 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     A(&ctx->depth, V * n);
     A(&ctx->view_lap, 3 * V * n);
     A(&ctx->view_grad, 2 * V * n);
-    A(&ctx->tile_count, (size_t)ctx->ntiles + 1);
-    A(&ctx->tile_start, (size_t)ctx->ntiles + 1);
-    A(&ctx->tile_cursor, (size_t)ctx->ntiles + 1);
-    ctx->e_cap = 4 * V * n;
+    ctx->ncells = (ctx->W + ctx->ks - 1) * (ctx->H + ctx->ks - 1);
+    A(&ctx->cell_count, (size_t)ctx->ncells + 1);
+    A(&ctx->cell_start, (size_t)ctx->ncells + 1);
+    A(&ctx->cell_cursor, (size_t)ctx->ncells);
+    A(&ctx->scan_tmp, (size_t)ctx->ncells / 2048 + 16);
+    ctx->e_cap = V * n;
     A(&ctx->e_src, ctx->e_cap);
     A(&ctx->e_tx, ctx->e_cap);
     A(&ctx->e_ty, ctx->e_cap);
@@ -207,7 +215,7 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
     g_pending.erase(ctx);
     for (void *p : std::initializer_list<void *>{
              ctx->views, ctx->tau_x, ctx->tau_y, ctx->dpart_x, ctx->dpart_y, ctx->omega, ctx->depth, ctx->view_lap,
-             ctx->view_grad, ctx->tile_count, ctx->tile_start, ctx->tile_cursor, ctx->e_src, ctx->e_tx, ctx->e_ty,
+             ctx->view_grad, ctx->cell_count, ctx->cell_start, ctx->cell_cursor, ctx->scan_tmp, ctx->e_src, ctx->e_tx, ctx->e_ty,
              ctx->e_w, ctx->U, ctx->Uq, ctx->G, ctx->PX, ctx->PY, ctx->U2, ctx->P2X, ctx->P2Y, ctx->tmp3N, ctx->cg_r,
              ctx->cg_d, ctx->cg_q, ctx->cg_b, ctx->reg_g, ctx->norm_mask, ctx->coverage, ctx->u_grad_x,
              ctx->u_grad_y, ctx->target_mask, ctx->visibility, ctx->target_depth, ctx->tmpN, ctx->cg_minv,
@@ -354,6 +362,20 @@ extern "C" lfrv_status lfrv_set_solution(lfrv_ctx *ctx, const float *rgb)
 {
     if (!ctx) return LFRV_E_ARG;
     return copy_in(ctx, ctx->U, rgb, 3 * ctx->N);
+}
+
+__global__ static void k_zero(size_t n, float *__restrict__ p)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0.0f;
+}
+
+extern "C" lfrv_status lfrv_clear_solution(lfrv_ctx *ctx)
+{
+    if (!ctx) return LFRV_E_ARG;
+    k_zero<<<blocks_for(3 * ctx->N), kBlock, 0, ctx->stream>>>(3 * ctx->N, ctx->U);
+    LFRV_LAUNCH(ctx, "clear solution");
+    return LFRV_OK;
 }
 
 extern "C" lfrv_status lfrv_init_u_gradient(lfrv_ctx *ctx)
@@ -522,6 +544,44 @@ extern "C" lfrv_status lfrv_op_sobel(int W, int H, const float *rgb, float *gx, 
                ? LFRV_OK
                : LFRV_E_HIP;
 }
+
+// ---- device memory helpers -------------------------------------------------------------
+
+extern "C" lfrv_status lfrv_dev_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr) return LFRV_E_ARG;
+    *ptr = nullptr;
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) return LFRV_E_HIP;
+    return hipMemset(*ptr, 0, bytes ? bytes : 1) == hipSuccess ? LFRV_OK : LFRV_E_HIP;
+}
+
+extern "C" lfrv_status lfrv_dev_free(void *ptr) { return hipFree(ptr) == hipSuccess ? LFRV_OK : LFRV_E_HIP; }
+
+extern "C" lfrv_status lfrv_memcpy(void *dst, const void *src, size_t bytes)
+{
+    if ((!dst || !src) && bytes) return LFRV_E_ARG;
+    return hipMemcpy(dst, src, bytes, hipMemcpyDefault) == hipSuccess ? LFRV_OK : LFRV_E_HIP;
+}
+
+// Plain streaming copy: 16-byte loads and stores, grid-stride.
+__global__ static void __launch_bounds__(256) k_copy16(size_t n16, const uint4 *__restrict__ src, uint4 *__restrict__ dst)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" lfrv_status lfrv_dev_copy_kernel(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (!dst || !src || bytes % 16 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 16 != 0) return LFRV_E_ARG;
+    const size_t n16 = bytes / 16;
+    if (n16 == 0) return LFRV_OK;
+    const size_t want = (n16 + 255) / 256;
+    const unsigned int nb = (unsigned int)(want < 8192 ? want : 8192);
+    k_copy16<<<nb, 256, 0, (hipStream_t)stream>>>(n16, (const uint4 *)src, (uint4 *)dst);
+    return hipGetLastError() == hipSuccess ? LFRV_OK : LFRV_E_HIP;
+}
+
+extern "C" lfrv_status lfrv_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? LFRV_OK : LFRV_E_HIP; }
 
 // ---- quality metric ------------------------------------------------------------------
 

@@ -1,13 +1,11 @@
-// Splat structure: source pixels binned by target tile, and the tile kernels
-// that accumulate splats in LDS (no global atomics, one store per target pixel).
+// Splat structure (entries sorted by footprint cell) and the pull kernels.
 //
 // The reference makes its scatter race-free by colouring source pixels on the
 // host into segments whose targets are >= 1 + ks/2 pixels apart and launching
 // one kernel per segment (gradientIBR.cu:383-521, optimization.cu:882-1025).
-// Here every cell-valid source pixel of every view is appended to the list of
-// each target tile its footprint touches (at most 2 x 2 tiles); a workgroup
-// owns one tile, walks its list, and accumulates the footprint taps that fall
-// inside the tile with LDS float atomics.
+// Here the source pixels are counting-sorted on the GPU by the target cell of
+// their footprint's top-left tap and every splat becomes a pull (see
+// tile_common.hpp): no atomics in the splats, deterministic sums.
 #include "tile_common.hpp"
 
 namespace lfrv {
@@ -15,79 +13,121 @@ namespace lfrv {
 namespace {
 
 template <int KS>
-__device__ __forceinline__ void tile_range(float t0, float t1, const TileGeom &g, int &cx0,
-                                           int &cx1, int &cy0, int &cy1)
+__device__ __forceinline__ int cell_of(float t0, float t1, int Wc)
 {
     constexpr float hd = 0.5f * (float)(KS - 1);
     const int x0 = (int)floorf(t0 - 0.5f - hd + 0.5f);
     const int y0 = (int)floorf(t1 - 0.5f - hd + 0.5f);
-    const int xa = max(x0, 0), xb = min(x0 + KS - 1, g.W - 1);
-    const int ya = max(y0, 0), yb = min(y0 + KS - 1, g.H - 1);
-    cx0 = xa / g.tw;
-    cx1 = xb / g.tw;
-    cy0 = ya / g.th;
-    cy1 = yb / g.th;
+    return (y0 + KS - 1) * Wc + (x0 + KS - 1);
 }
 
 template <int KS>
-__global__ void k_bin_count(int V, size_t n, TileGeom g, const float *__restrict__ tx,
-                            const float *__restrict__ ty, int *__restrict__ count)
+__global__ void k_cell_count(size_t tot, int W, int H, int Wc, const float *__restrict__ tx,
+                             const float *__restrict__ ty, int *__restrict__ count)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (size_t)V * n) return;
+    if (k >= tot) return;
     const float t0 = tx[k], t1 = ty[k];
-    if (!cell_valid(t0, t1, g.W, g.H)) return;
-    int cx0, cx1, cy0, cy1;
-    tile_range<KS>(t0, t1, g, cx0, cx1, cy0, cy1);
-    for (int cy = cy0; cy <= cy1; cy++)
-        for (int cx = cx0; cx <= cx1; cx++) atomicAdd(count + cy * g.ntx + cx, 1);
+    if (!cell_valid(t0, t1, W, H)) return;
+    atomicAdd(count + cell_of<KS>(t0, t1, Wc), 1);
 }
 
 template <int KS>
-__global__ void k_bin_fill(int V, size_t n, TileGeom g, const float *__restrict__ tx,
-                           const float *__restrict__ ty, int *__restrict__ cursor,
-                           int *__restrict__ e_src, float *__restrict__ e_tx, float *__restrict__ e_ty)
+__global__ void k_cell_fill(size_t tot, int W, int H, int Wc, const float *__restrict__ tx,
+                            const float *__restrict__ ty, const int *__restrict__ start,
+                            int *__restrict__ cursor, int *__restrict__ e_src)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (size_t)V * n) return;
+    if (k >= tot) return;
     const float t0 = tx[k], t1 = ty[k];
-    if (!cell_valid(t0, t1, g.W, g.H)) return;
-    int cx0, cx1, cy0, cy1;
-    tile_range<KS>(t0, t1, g, cx0, cx1, cy0, cy1);
-    for (int cy = cy0; cy <= cy1; cy++)
-        for (int cx = cx0; cx <= cx1; cx++) {
-            const int pos = atomicAdd(cursor + cy * g.ntx + cx, 1);
-            e_src[pos] = (int)k;
-            e_tx[pos] = t0;
-            e_ty[pos] = t1;
+    if (!cell_valid(t0, t1, W, H)) return;
+    const int c = cell_of<KS>(t0, t1, Wc);
+    const int pos = start[c] + atomicAdd(cursor + c, 1);
+    e_src[pos] = (int)k;
+}
+
+// Order every cell's entries by source index (insertion sort; cells hold a
+// handful of entries) and gather the per-entry tau.
+__global__ void k_cell_sort(int ncells, const int *__restrict__ start, int *__restrict__ e_src,
+                            const float *__restrict__ tx, const float *__restrict__ ty,
+                            float *__restrict__ e_tx, float *__restrict__ e_ty)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const int a = start[c], b = start[c + 1];
+    for (int i = a + 1; i < b; i++) {
+        const int key = e_src[i];
+        int j = i - 1;
+        while (j >= a && e_src[j] > key) {
+            e_src[j + 1] = e_src[j];
+            j--;
         }
+        e_src[j + 1] = key;
+    }
+    for (int i = a; i < b; i++) {
+        const int k = e_src[i];
+        e_tx[i] = tx[k];
+        e_ty[i] = ty[k];
+    }
 }
 
-// Exclusive scan of the tile counts in one workgroup of 1024 threads.
-__global__ void k_scan_tiles(int ntiles, const int *__restrict__ count, int *__restrict__ start,
-                             int *__restrict__ cursor)
+// Device-wide exclusive scan of `in` (n ints) into `out` (n + 1 ints):
+// per-block scan with block sums, a single-block scan of the sums, offsets.
+constexpr int kScanPer = 8;  // elements per thread
+
+__global__ void k_scan_blocks(int n, const int *__restrict__ in, int *__restrict__ out, int *__restrict__ bsum)
 {
-    __shared__ int part[1024];
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int per = (ntiles + nt - 1) / nt;
-    const int a = tid * per, b = min(a + per, ntiles);
-    int s = 0;
-    for (int i = a; i < b; i++) s += count[i];
-    part[tid] = s;
+    __shared__ int tmp[256];
+    const int base = blockIdx.x * 256 * kScanPer + threadIdx.x * kScanPer;
+    int v[kScanPer], s = 0;
+    for (int i = 0; i < kScanPer; i++) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        s += v[i];
+    }
+    tmp[threadIdx.x] = s;
     __syncthreads();
-    for (int off = 1; off < nt; off <<= 1) {
-        const int v = (tid >= off) ? part[tid - off] : 0;
+    for (int off = 1; off < 256; off <<= 1) {
+        const int t = (threadIdx.x >= off) ? tmp[threadIdx.x - off] : 0;
         __syncthreads();
-        part[tid] += v;
+        tmp[threadIdx.x] += t;
         __syncthreads();
     }
-    int run = part[tid] - s;
-    for (int i = a; i < b; i++) {
-        start[i] = run;
-        cursor[i] = run;
-        run += count[i];
+    int run = tmp[threadIdx.x] - s;
+    for (int i = 0; i < kScanPer; i++) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
     }
-    if (tid == nt - 1) start[ntiles] = part[nt - 1];
+    if (threadIdx.x == 255) bsum[blockIdx.x] = tmp[255];
+}
+
+__global__ void k_scan_sums(int nb, int *__restrict__ bsum, int *__restrict__ total_out)
+{
+    __shared__ int tmp[1024];
+    const int per = (nb + 1023) / 1024;
+    const int b0 = threadIdx.x * per, b1 = min(b0 + per, nb);
+    int s = 0;
+    for (int i = b0; i < b1; i++) s += bsum[i];
+    tmp[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int t = (threadIdx.x >= off) ? tmp[threadIdx.x - off] : 0;
+        __syncthreads();
+        tmp[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int run = tmp[threadIdx.x] - s;
+    for (int i = b0; i < b1; i++) {
+        const int c = bsum[i];
+        bsum[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) *total_out = tmp[1023];
+}
+
+__global__ void k_scan_add(int n, int *__restrict__ out, const int *__restrict__ bsum)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += bsum[i / (256 * kScanPer)];
 }
 
 __global__ void k_refresh_weights(int64_t ne, const int *__restrict__ e_src,
@@ -97,78 +137,40 @@ __global__ void k_refresh_weights(int64_t ne, const int *__restrict__ e_src,
     if (e < ne) e_w[e] = omega[e_src[e]];
 }
 
-// ---- generic tile splat ---------------------------------------------------
-// For every entry of the tile, vals(e, src, out[NC]) produces NC values that
-// are splatted with the footprint weights (times `scale`) into NC LDS planes;
-// the epilogue then sees the accumulated planes per pixel.
+// ---- pull kernels -----------------------------------------------------------
 
-template <int KS, int NC, class Vals, class Epi>
-__device__ __forceinline__ void tile_splat(const TileGeom &g, const TileEntries &te, Vals vals,
-                                           Epi epi, float *acc)
-{
+struct PullCfg {
+    int cap;  // LDS slots
+};
+
+#define PULL_PROLOGUE(NV)                     \
+    extern __shared__ int smem[];             \
     const int t = xcd_tile(blockIdx.x, g.ntiles);
-    const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
-    const int tp = g.tw * g.th;
-    for (int i = threadIdx.x; i < NC * tp; i += blockDim.x) acc[i] = 0.0f;
-    __syncthreads();
-    const int e0 = te.active ? te.start[t] : 0, e1 = te.active ? te.start[t + 1] : 0;
-    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-        const float t0 = te.tx[e], t1 = te.ty[e];
-        int x0, y0;
-        float wx[KS], wy[KS];
-        if (!footprint<KS>(t0, t1, g.W, g.H, x0, y0, wx, wy)) continue;
-        float v[NC];
-        vals(e, v);
-#pragma unroll
-        for (int i = 0; i < KS; i++) {
-            const int ly = y0 + i - ty0;
-            if (ly < 0 || ly >= g.th) continue;
-#pragma unroll
-            for (int j = 0; j < KS; j++) {
-                const int lx = x0 + j - tx0;
-                if (lx < 0 || lx >= g.tw) continue;
-                const float a = wy[i] * wx[j];
-                if (a == 0.0f) continue;
-#pragma unroll
-                for (int c = 0; c < NC; c++) atomicAdd(acc + c * tp + ly * g.tw + lx, a * v[c]);
-            }
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < tp; i += blockDim.x) {
-        const int lx = i % g.tw, ly = i / g.tw;
-        const int x = tx0 + lx, y = ty0 + ly;
-        if (x >= g.W || y >= g.H) continue;
-        float a[NC];
-#pragma unroll
-        for (int c = 0; c < NC; c++) a[c] = acc[c * tp + i];
-        epi((size_t)y * g.W + x, x, y, a);
-    }
-}
 
 // coverage = sum_i B_i^T 1
 template <int KS>
-__global__ void k_tile_coverage(TileGeom g, TileEntries te, float *__restrict__ coverage)
+__global__ void __launch_bounds__(1024) k_pull_coverage(TileGeom g, TileEntries te, PullCfg pc, float *__restrict__ coverage)
 {
-    extern __shared__ float acc[];
-    tile_splat<KS, 1>(
-        g, te, [&](int, float *v) { v[0] = 1.0f; },
-        [&](size_t o, int, int, const float *a) { coverage[o] = a[0]; }, acc);
+    PULL_PROLOGUE(1)
+    any_pull<KS, 1, false>(
+        g, te, t, smem, pc.cap,
+        [&](int, int, int, const float(&)[KS], const float(&)[KS], float(&v)[1]) { v[0] = 1.0f; },
+        [&](size_t o, int, int, const float(&a)[1]) { coverage[o] = a[0]; });
 }
 
 // Weighted initial image (gradientIBR.cu:764-811, optimization.cu:1303-1349):
 // U = sum B^T (omega v) / sum B^T omega where the weight is non-zero, else 0;
 // norm_mask = sum B^T omega.
 template <int KS>
-__global__ void k_tile_init_image(TileGeom g, TileEntries te, size_t n,
-                                  const float *__restrict__ views, float *__restrict__ U,
-                                  float *__restrict__ norm_mask)
+__global__ void __launch_bounds__(1024) k_pull_init_image(TileGeom g, TileEntries te, PullCfg pc, size_t n,
+                                                         const float *__restrict__ views, float *__restrict__ U,
+                                                         float *__restrict__ norm_mask)
 {
-    extern __shared__ float acc[];
+    PULL_PROLOGUE(4)
     const size_t N = (size_t)g.W * g.H;
-    tile_splat<KS, 4>(
-        g, te,
-        [&](int e, float *v) {
+    any_pull<KS, 4, false>(
+        g, te, t, smem, pc.cap,
+        [&](int e, int, int, const float(&)[KS], const float(&)[KS], float(&v)[4]) {
             const int k = te.src[e];
             const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
             const float om = te.w[e];
@@ -178,25 +180,25 @@ __global__ void k_tile_init_image(TileGeom g, TileEntries te, size_t n,
             v[2] = om * vp[2 * n];
             v[3] = om;
         },
-        [&](size_t o, int, int, const float *a) {
+        [&](size_t o, int, int, const float(&a)[4]) {
             const float wsum = a[3];
             for (int c = 0; c < 3; c++) U[c * N + o] = (wsum != 0.0f) ? a[c] / wsum : a[c];
             norm_mask[o] = wsum;
-        },
-        acc);
+        });
 }
 
-// Target gradient (init_u_gradient, gradientIBR.cu:858-913): splat (gx, gy,
+// Target gradient (init_u_gradient, gradientIBR.cu:858-913): pull (gx, gy,
 // omega, 1) with the footprint weights, normalise by the last plane.
 template <int KS>
-__global__ void k_tile_u_gradient(TileGeom g, TileEntries te, size_t n, const float *__restrict__ vgrad,
-                                  float sigma2, float *__restrict__ ugx, float *__restrict__ ugy,
-                                  float *__restrict__ tmask, float *__restrict__ norm_mask)
+__global__ void __launch_bounds__(1024) k_pull_u_gradient(TileGeom g, TileEntries te, PullCfg pc, size_t n,
+                                                         const float *__restrict__ vgrad, float sigma2,
+                                                         float *__restrict__ ugx, float *__restrict__ ugy,
+                                                         float *__restrict__ tmask, float *__restrict__ norm_mask)
 {
-    extern __shared__ float acc[];
-    tile_splat<KS, 4>(
-        g, te,
-        [&](int e, float *v) {
+    PULL_PROLOGUE(4)
+    any_pull<KS, 4, false>(
+        g, te, t, smem, pc.cap,
+        [&](int e, int, int, const float(&)[KS], const float(&)[KS], float(&v)[4]) {
             const int k = te.src[e];
             const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
             v[0] = vgrad[vi * 2 * n + m];
@@ -204,7 +206,7 @@ __global__ void k_tile_u_gradient(TileGeom g, TileEntries te, size_t n, const fl
             v[2] = te.w[e];
             v[3] = 1.0f;
         },
-        [&](size_t o, int, int, const float *a) {
+        [&](size_t o, int, int, const float(&a)[4]) {
             const float nm = a[3];
             float gx = a[0], gy = a[1], tm = a[2];
             if (nm != 0.0f) {
@@ -216,44 +218,45 @@ __global__ void k_tile_u_gradient(TileGeom g, TileEntries te, size_t n, const fl
             ugy[o] = gy;
             tmask[o] = tm * sigma2;
             norm_mask[o] = nm;
-        },
-        acc);
+        });
 }
 
 // Regularizer weight, splat part (optimization.cu:2264-2324 with
 // vtv_sr_kernels.cu:1172-1194): the reference accumulates sum B^T 1 on top of
 // whatever its normalisation buffer held, so norm_mask += coverage here.
 template <int KS>
-__global__ void k_tile_reg_weight(TileGeom g, TileEntries te, const float *__restrict__ coverage,
-                                  float sigma2, float lmax, float lmin, float nviews,
-                                  float *__restrict__ norm_mask, float *__restrict__ graw)
+__global__ void __launch_bounds__(1024) k_pull_reg_weight(TileGeom g, TileEntries te, PullCfg pc,
+                                                         const float *__restrict__ coverage, float sigma2,
+                                                         float lmax, float lmin, float nviews,
+                                                         float *__restrict__ norm_mask, float *__restrict__ graw)
 {
-    extern __shared__ float acc[];
-    tile_splat<KS, 1>(
-        g, te, [&](int e, float *v) { v[0] = te.w[e]; },
-        [&](size_t o, int, int, const float *a) {
+    PULL_PROLOGUE(1)
+    any_pull<KS, 1, false>(
+        g, te, t, smem, pc.cap,
+        [&](int e, int, int, const float(&)[KS], const float(&)[KS], float(&v)[1]) { v[0] = te.w[e]; },
+        [&](size_t o, int, int, const float(&a)[1]) {
             const float nm = norm_mask[o] + coverage[o];
             norm_mask[o] = nm;
             float m = a[0];
             if (nm != 0.0f) m /= nm;
             m *= sigma2;
             graw[o] = (m == 0.0f) ? lmax : fmaxf(lmin, lmax - m / nviews);
-        },
-        acc);
+        });
 }
 
 // CG right-hand side: b = alpha sum B^T (omega sigma^2 v) + c0 u0 + beta D^T g.
 template <int KS>
-__global__ void k_tile_rhs(TileGeom g, TileEntries te, size_t n, const float *__restrict__ views,
-                           float wscale, float c0, const float *__restrict__ u0, float beta,
-                           const float *__restrict__ gx, const float *__restrict__ gy,
-                           float *__restrict__ b)
+__global__ void __launch_bounds__(1024) k_pull_rhs(TileGeom g, TileEntries te, PullCfg pc, size_t n,
+                                                  const float *__restrict__ views, float wscale, float c0,
+                                                  const float *__restrict__ u0, float beta,
+                                                  const float *__restrict__ gx, const float *__restrict__ gy,
+                                                  float *__restrict__ b)
 {
-    extern __shared__ float acc[];
+    PULL_PROLOGUE(3)
     const size_t N = (size_t)g.W * g.H;
-    tile_splat<KS, 3>(
-        g, te,
-        [&](int e, float *v) {
+    any_pull<KS, 3, false>(
+        g, te, t, smem, pc.cap,
+        [&](int e, int, int, const float(&)[KS], const float(&)[KS], float(&v)[3]) {
             const int k = te.src[e];
             const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
             const float om = te.w[e] * wscale;
@@ -262,7 +265,7 @@ __global__ void k_tile_rhs(TileGeom g, TileEntries te, size_t n, const float *__
             v[1] = om * vp[n];
             v[2] = om * vp[2 * n];
         },
-        [&](size_t o, int x, int y, const float *a) {
+        [&](size_t o, int x, int y, const float(&a)[3]) {
             float dtg = 0.0f;
             if (beta != 0.0f) {
                 if (x > 0) dtg += gx[o - 1];
@@ -272,48 +275,27 @@ __global__ void k_tile_rhs(TileGeom g, TileEntries te, size_t n, const float *__
                 dtg *= beta;
             }
             for (int c = 0; c < 3; c++) b[c * N + o] = a[c] + (c0 != 0.0f ? c0 * u0[c * N + o] : 0.0f) + dtg;
-        },
-        acc);
+        });
 }
 
 // Jacobi preconditioner 1 / diag(A), diag = alpha sum omega sigma^2 B^2 + c0 + beta deg.
 template <int KS>
-__global__ void k_tile_diag(TileGeom g, TileEntries te, float wscale, float c0, float beta,
-                            float *__restrict__ minv)
+__global__ void __launch_bounds__(1024) k_pull_diag(TileGeom g, TileEntries te, PullCfg pc, float wscale, float c0,
+                                                   float beta, float *__restrict__ minv)
 {
-    extern __shared__ float acc[];
-    const int t = xcd_tile(blockIdx.x, g.ntiles);
-    const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
-    const int tp = g.tw * g.th;
-    for (int i = threadIdx.x; i < tp; i += blockDim.x) acc[i] = 0.0f;
-    __syncthreads();
-    if (wscale != 0.0f && te.active)
-        for (int e = te.start[t] + threadIdx.x; e < te.start[t + 1]; e += blockDim.x) {
-            int x0, y0;
-            float wx[KS], wy[KS];
-            if (!footprint<KS>(te.tx[e], te.ty[e], g.W, g.H, x0, y0, wx, wy)) continue;
-            const float om = te.w[e] * wscale;
-            for (int i = 0; i < KS; i++) {
-                const int ly = y0 + i - ty0;
-                if (ly < 0 || ly >= g.th) continue;
-                for (int j = 0; j < KS; j++) {
-                    const int lx = x0 + j - tx0;
-                    if (lx < 0 || lx >= g.tw) continue;
-                    const float a = wy[i] * wx[j];
-                    if (a != 0.0f) atomicAdd(acc + ly * g.tw + lx, om * a * a);
-                }
-            }
-        }
-    __syncthreads();
-    for (int i = threadIdx.x; i < tp; i += blockDim.x) {
-        const int x = tx0 + i % g.tw, y = ty0 + i / g.tw;
-        if (x >= g.W || y >= g.H) continue;
-        const int deg = (g.W > 1 ? ((x == 0 || x == g.W - 1) ? 1 : 2) : 0) +
-                        (g.H > 1 ? ((y == 0 || y == g.H - 1) ? 1 : 2) : 0);
-        const float d = acc[i] + c0 + beta * (float)deg;
-        minv[(size_t)y * g.W + x] = d > 0.0f ? 1.0f / d : 1.0f;
-    }
+    PULL_PROLOGUE(1)
+    any_pull<KS, 1, true>(
+        g, te, t, smem, pc.cap,
+        [&](int e, int, int, const float(&)[KS], const float(&)[KS], float(&v)[1]) { v[0] = te.w[e] * wscale; },
+        [&](size_t o, int x, int y, const float(&a)[1]) {
+            const int deg = (g.W > 1 ? ((x == 0 || x == g.W - 1) ? 1 : 2) : 0) +
+                            (g.H > 1 ? ((y == 0 || y == g.H - 1) ? 1 : 2) : 0);
+            const float d = a[0] + c0 + beta * (float)deg;
+            minv[o] = d > 0.0f ? 1.0f / d : 1.0f;
+        });
 }
+
+#undef PULL_PROLOGUE
 
 __global__ void k_fill(float *__restrict__ p, size_t N, float v)
 {
@@ -341,8 +323,6 @@ __global__ void k_add_to(size_t N, const float *__restrict__ src, float *__restr
 
 }  // namespace
 
-size_t tile_lds_bytes(const lfrv_ctx *ctx, int planes) { return sizeof(float) * (size_t)planes * ctx->tile_w * ctx->tile_h; }
-
 TileGeom tile_geom(const lfrv_ctx *ctx)
 {
     TileGeom g;
@@ -352,19 +332,39 @@ TileGeom tile_geom(const lfrv_ctx *ctx)
     g.th = ctx->tile_h;
     g.ntx = ctx->ntx;
     g.ntiles = ctx->ntiles;
+    g.Wc = ctx->W + ctx->ks - 1;
+    g.Hc = ctx->H + ctx->ks - 1;
     return g;
 }
 
 TileEntries tile_entries(const lfrv_ctx *ctx)
 {
     TileEntries t;
-    t.start = ctx->tile_start;
+    t.cell_start = ctx->cell_start;
     t.src = ctx->e_src;
     t.tx = ctx->e_tx;
     t.ty = ctx->e_ty;
     t.w = ctx->e_w;
     t.active = 1;
     return t;
+}
+
+// Largest slot count whose LDS fits `budget` bytes besides `extra` bytes.
+int pull_cap(const lfrv_ctx *ctx, int nv, size_t extra, size_t budget)
+{
+    const size_t meta = sizeof(int) * (size_t)pull_meta_ints(ctx->tile_w, ctx->tile_h, ctx->ks, ctx->pull_bs);
+    const long avail = (long)budget - (long)meta - (long)extra;
+    const long cap = avail / (long)(sizeof(float) * (nv + 2));
+    return (int)std::max(64L, std::min(cap, 16384L));
+}
+
+size_t pull_lds(const lfrv_ctx *ctx, int nv, int cap)
+{
+    if (ctx->ks <= 3) {
+        const size_t cells = (size_t)(ctx->tile_w + ctx->ks - 1) * (ctx->tile_h + ctx->ks - 1);
+        return sizeof(float) * (size_t)ctx->ks * ctx->ks * nv * cells;
+    }
+    return pull_lds_bytes(ctx->tile_w, ctx->tile_h, ctx->ks, ctx->pull_bs, nv, cap);
 }
 
 #define LFRV_KS_DISPATCH(ks, MACRO) \
@@ -376,30 +376,37 @@ TileEntries tile_entries(const lfrv_ctx *ctx)
     default: return fail(ctx, LFRV_E_ARG, "dsf out of range"); \
     }
 
-lfrv_status run_view_stencils(lfrv_ctx *ctx);  // k_warp.hip
+static constexpr size_t kPullBudget = 64 * 1024;
 
 lfrv_status run_prepare(lfrv_ctx *ctx)
 {
     const TileGeom g = tile_geom(ctx);
     const size_t tot = (size_t)ctx->V * ctx->n;
+    const int nc = ctx->ncells;
     TimedRegion tr(ctx, 5);
-    LFRV_HIP(ctx, hipMemsetAsync(ctx->tile_count, 0, sizeof(int) * ctx->ntiles, ctx->stream));
-#define BIN_COUNT(K) k_bin_count<K><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(ctx->V, ctx->n, g, ctx->tau_x, ctx->tau_y, ctx->tile_count)
-    LFRV_KS_DISPATCH(ctx->ks, BIN_COUNT);
-#undef BIN_COUNT
-    k_scan_tiles<<<1, 1024, 0, ctx->stream>>>(ctx->ntiles, ctx->tile_count, ctx->tile_start, ctx->tile_cursor);
+    LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_count, 0, sizeof(int) * (nc + 1), ctx->stream));
+    LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_cursor, 0, sizeof(int) * nc, ctx->stream));
+#define CC(K) k_cell_count<K><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(tot, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_count)
+    LFRV_KS_DISPATCH(ctx->ks, CC);
+#undef CC
+    const int nb = (nc + 256 * kScanPer - 1) / (256 * kScanPer);
+    k_scan_blocks<<<nb, 256, 0, ctx->stream>>>(nc, ctx->cell_count, ctx->cell_start, ctx->scan_tmp);
+    k_scan_sums<<<1, 1024, 0, ctx->stream>>>(nb, ctx->scan_tmp, ctx->cell_start + nc);
+    k_scan_add<<<blocks_for(nc), kBlock, 0, ctx->stream>>>(nc, ctx->cell_start, ctx->scan_tmp);
+#define CF(K) k_cell_fill<K><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(tot, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_start, ctx->cell_cursor, ctx->e_src)
+    LFRV_KS_DISPATCH(ctx->ks, CF);
+#undef CF
+    k_cell_sort<<<blocks_for(nc), kBlock, 0, ctx->stream>>>(nc, ctx->cell_start, ctx->e_src, ctx->tau_x, ctx->tau_y,
+                                                           ctx->e_tx, ctx->e_ty);
+    LFRV_LAUNCH(ctx, "splat structure");
     int total = 0;
-    LFRV_HIP(ctx, hipMemcpyAsync(&total, ctx->tile_start + ctx->ntiles, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    LFRV_HIP(ctx, hipMemcpyAsync(&total, ctx->cell_start + nc, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if ((size_t)total > ctx->e_cap) return fail(ctx, LFRV_E_ARG, "splat structure overflow");
     ctx->e_total = total;
-#define BIN_FILL(K) k_bin_fill<K><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(ctx->V, ctx->n, g, ctx->tau_x, ctx->tau_y, ctx->tile_cursor, ctx->e_src, ctx->e_tx, ctx->e_ty)
-    LFRV_KS_DISPATCH(ctx->ks, BIN_FILL);
-#undef BIN_FILL
-    LFRV_LAUNCH(ctx, "binning");
     const TileEntries te = tile_entries(ctx);
-    const size_t lds = tile_lds_bytes(ctx, 1);
-#define COV(K) k_tile_coverage<K><<<ctx->ntiles, kBlock, lds, ctx->stream>>>(g, te, ctx->coverage)
+    PullCfg pc{pull_cap(ctx, 1, 0, kPullBudget)};
+#define COV(K) k_pull_coverage<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap), ctx->stream>>>(g, te, pc, ctx->coverage)
     LFRV_KS_DISPATCH(ctx->ks, COV);
 #undef COV
     LFRV_LAUNCH(ctx, "coverage");
@@ -439,9 +446,9 @@ lfrv_status run_initial_image(lfrv_ctx *ctx)
     if (st != LFRV_OK) return st;
     const TileGeom g = tile_geom(ctx);
     const TileEntries te = tile_entries(ctx);
-    const size_t lds = tile_lds_bytes(ctx, 4);
+    PullCfg pc{pull_cap(ctx, 4, 0, kPullBudget)};
     TimedRegion tr(ctx, 5);
-#define INIT(K) k_tile_init_image<K><<<ctx->ntiles, kBlock, lds, ctx->stream>>>(g, te, ctx->n, ctx->views, ctx->U, ctx->norm_mask)
+#define INIT(K) k_pull_init_image<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 4, pc.cap), ctx->stream>>>(g, te, pc, ctx->n, ctx->views, ctx->U, ctx->norm_mask)
     LFRV_KS_DISPATCH(ctx->ks, INIT);
 #undef INIT
     LFRV_LAUNCH(ctx, "initial image");
@@ -455,17 +462,15 @@ lfrv_status run_u_gradient(lfrv_ctx *ctx)
     if (st != LFRV_OK) return st;
     const TileGeom g = tile_geom(ctx);
     const TileEntries te = tile_entries(ctx);
-    const size_t lds = tile_lds_bytes(ctx, 4);
+    PullCfg pc{pull_cap(ctx, 4, 0, kPullBudget)};
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
     TimedRegion tr(ctx, 5);
-#define UG(K) k_tile_u_gradient<K><<<ctx->ntiles, kBlock, lds, ctx->stream>>>(g, te, ctx->n, ctx->view_grad, s2, ctx->u_grad_x, ctx->u_grad_y, ctx->target_mask, ctx->norm_mask)
+#define UG(K) k_pull_u_gradient<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 4, pc.cap), ctx->stream>>>(g, te, pc, ctx->n, ctx->view_grad, s2, ctx->u_grad_x, ctx->u_grad_y, ctx->target_mask, ctx->norm_mask)
     LFRV_KS_DISPATCH(ctx->ks, UG);
 #undef UG
     LFRV_LAUNCH(ctx, "u gradient");
     return LFRV_OK;
 }
-
-lfrv_status run_gauss_smooth(lfrv_ctx *ctx, const float *in, float *out, float *tmp);  // k_stencil.hip
 
 lfrv_status run_regularizer_weight(lfrv_ctx *ctx)
 {
@@ -474,11 +479,11 @@ lfrv_status run_regularizer_weight(lfrv_ctx *ctx)
     if (st != LFRV_OK) return st;
     const TileGeom g = tile_geom(ctx);
     const TileEntries te = tile_entries(ctx);
-    const size_t lds = tile_lds_bytes(ctx, 1);
+    PullCfg pc{pull_cap(ctx, 1, 0, kPullBudget)};
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
     const float lmax = ctx->p.lambda_max_factor * ctx->p.lambda, lmin = ctx->p.lambda;
     float *graw = ctx->tmpN;
-#define RW(K) k_tile_reg_weight<K><<<ctx->ntiles, kBlock, lds, ctx->stream>>>(g, te, ctx->coverage, s2, lmax, lmin, (float)ctx->V, ctx->norm_mask, graw)
+#define RW(K) k_pull_reg_weight<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap), ctx->stream>>>(g, te, pc, ctx->coverage, s2, lmax, lmin, (float)ctx->V, ctx->norm_mask, graw)
     LFRV_KS_DISPATCH(ctx->ks, RW);
 #undef RW
     LFRV_LAUNCH(ctx, "regularizer weight");
@@ -492,26 +497,20 @@ lfrv_status run_rhs_and_diag(lfrv_ctx *ctx, double alpha, double c0, double beta
                              bool use_precond, const float *u0)
 {
     const TileGeom g = tile_geom(ctx);
-    const TileEntries te = tile_entries(ctx);
+    TileEntries te = tile_entries(ctx);
+    te.active = use_views ? 1 : 0;
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
     const float wscale = use_views ? (float)(alpha * s2) : 0.0f;
     TimedRegion tr(ctx, 5);
-    if (use_views) {
-        const size_t lds = tile_lds_bytes(ctx, 3);
-#define RHS(K) k_tile_rhs<K><<<ctx->ntiles, kBlock, lds, ctx->stream>>>(g, te, ctx->n, ctx->views, wscale, (float)c0, u0, (float)beta, ctx->u_grad_x, ctx->u_grad_y, ctx->cg_b)
+    {
+        PullCfg pc{pull_cap(ctx, 3, 0, kPullBudget)};
+#define RHS(K) k_pull_rhs<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 3, pc.cap), ctx->stream>>>(g, te, pc, ctx->n, ctx->views, wscale, (float)c0, u0, (float)beta, ctx->u_grad_x, ctx->u_grad_y, ctx->cg_b)
         LFRV_KS_DISPATCH(ctx->ks, RHS);
 #undef RHS
-    } else {
-        TileEntries none = te;
-        none.active = 0;
-        const size_t lds = tile_lds_bytes(ctx, 3);
-#define RHS0(K) k_tile_rhs<K><<<ctx->ntiles, kBlock, lds, ctx->stream>>>(g, none, ctx->n, ctx->views, 0.0f, (float)c0, u0, (float)beta, ctx->u_grad_x, ctx->u_grad_y, ctx->cg_b)
-        LFRV_KS_DISPATCH(ctx->ks, RHS0);
-#undef RHS0
     }
     if (use_precond) {
-        const size_t lds = tile_lds_bytes(ctx, 1);
-#define DG(K) k_tile_diag<K><<<ctx->ntiles, kBlock, lds, ctx->stream>>>(g, te, wscale, (float)c0, (float)beta, ctx->cg_minv)
+        PullCfg pc{pull_cap(ctx, 1, 0, kPullBudget)};
+#define DG(K) k_pull_diag<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap), ctx->stream>>>(g, te, pc, wscale, (float)c0, (float)beta, ctx->cg_minv)
         LFRV_KS_DISPATCH(ctx->ks, DG);
 #undef DG
     } else {

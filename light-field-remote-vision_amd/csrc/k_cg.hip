@@ -2,12 +2,12 @@
 //   A u = (alpha sum_i S_i^T W_i S_i + c0 I + beta D^T D) u = b
 // per RGB channel, FP32 vectors, FP64 reductions, scalars on the device.
 //
-// One iteration is two launches:
+// One iteration is two streaming launches and two one-block launches:
 //   k_cg_op     d = z + beta d_prev (z = Minv r) over the tile and its halo,
-//               q = A d with the binned gather/splat in LDS, d.q reduced;
-//               the last block computes alpha.
-//   k_cg_update x += alpha d, r -= alpha q, r.r and r.z reduced; the last block
-//               tests convergence and computes beta.
+//               q = A d with the binned gather/splat in LDS, per-block d.q;
+//               a one-block kernel sums them and computes alpha.
+//   k_cg_update x += alpha d, r -= alpha q, per-block r.r and r.z; a one-block
+//               kernel sums them, tests convergence and computes beta.
 // d is double-buffered because neighbouring tiles read it in their halos.
 #include "tile_common.hpp"
 
@@ -19,23 +19,23 @@ namespace {
 
 // q = A * din (mode 0) or, with the fused direction update (mode 1),
 // d = Minv r + beta d_prev, q = A d, written to dout / q, d.q reduced.
+// LDS: d tile with halo ks-1 (3 planes), then the pull's meta and slots.
 template <int KS>
-__global__ void __launch_bounds__(256) k_cg_op(TileGeom g, TileEntries te, float wscale, float c0,
+__global__ void __launch_bounds__(1024) k_cg_op(TileGeom g, TileEntries te, int cap, float wscale, float c0,
                                                float beta_reg, int mode, const float *__restrict__ din,
                                                const float *__restrict__ r, const float *__restrict__ minv,
                                                float *__restrict__ dout, float *__restrict__ q,
-                                               CgState *st, double *partials, unsigned int *counter)
+                                               const CgState *st, double *partials)
 {
     constexpr int HAL = KS - 1;
-    extern __shared__ float lds[];
+    extern __shared__ int smem[];
     __shared__ double red[3 * 16];
     if (mode == 1 && st->done) return;
     const int t = xcd_tile(blockIdx.x, g.ntiles);
     const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
-    const int tp = g.tw * g.th;
     const int lw = g.tw + 2 * HAL, lh = g.th + 2 * HAL, lp = lw * lh;
-    float *dl = lds;            // 3 planes of lw x lh
-    float *acc = lds + 3 * lp;  // 3 planes of tw x th
+    float *dl = reinterpret_cast<float *>(smem);  // 3 planes of lw x lh
+    int *plds = smem + 3 * lp;
     const size_t N = (size_t)g.W * g.H;
 
     float bc[3] = {0.0f, 0.0f, 0.0f};
@@ -67,91 +67,76 @@ __global__ void __launch_bounds__(256) k_cg_op(TileGeom g, TileEntries te, float
         }
         for (int c = 0; c < 3; c++) dl[c * lp + i] = v[c];
     }
-    for (int i = threadIdx.x; i < 3 * tp; i += blockDim.x) acc[i] = 0.0f;
-    __syncthreads();
-
-    // data term: for every binned (view, pixel): s = S d, splat W s
-    if (te.active && wscale != 0.0f) {
-        const int e0 = te.start[t], e1 = te.start[t + 1];
-        for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-            int x0, y0;
-            float wx[KS], wy[KS];
-            if (!footprint<KS>(te.tx[e], te.ty[e], g.W, g.H, x0, y0, wx, wy)) continue;
-            const float om = te.w[e] * wscale;
-            const int bx = x0 - tx0 + HAL, by = y0 - ty0 + HAL;  // in [0, lw-KS] x [0, lh-KS]
-            float s[3] = {0.0f, 0.0f, 0.0f};
+    // (tile_pull synchronises before it reads the staged tile)
+    TileEntries tea = te;
+    tea.active = te.active && wscale != 0.0f;
+    double dq[3] = {0.0, 0.0, 0.0};
+    any_pull<KS, 3, false>(
+        g, tea, t, plds, cap,
+        [&](int e, int x0, int y0, const float(&wx)[KS], const float(&wy)[KS], float(&v)[3]) {
+            const int bx = x0 - tx0 + HAL, by = y0 - ty0 + HAL;
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
             for (int i = 0; i < KS; i++)
 #pragma unroll
                 for (int j = 0; j < KS; j++) {
                     const float a = wy[i] * wx[j];
                     const int li = (by + i) * lw + bx + j;
-#pragma unroll
-                    for (int c = 0; c < 3; c++) s[c] += a * dl[c * lp + li];
+                    s0 += a * dl[li];
+                    s1 += a * dl[lp + li];
+                    s2 += a * dl[2 * lp + li];
                 }
+            const float om = te.w[e] * wscale;
+            v[0] = om * s0;
+            v[1] = om * s1;
+            v[2] = om * s2;
+        },
+        [&](size_t o, int x, int y, const float(&a)[3]) {
+            const int li = (y - ty0 + HAL) * lw + (x - tx0) + HAL;
 #pragma unroll
-            for (int c = 0; c < 3; c++) s[c] *= om;
-#pragma unroll
-            for (int i = 0; i < KS; i++) {
-                const int ly = y0 + i - ty0;
-                if (ly < 0 || ly >= g.th) continue;
-#pragma unroll
-                for (int j = 0; j < KS; j++) {
-                    const int lx = x0 + j - tx0;
-                    if (lx < 0 || lx >= g.tw) continue;
-                    const float a = wy[i] * wx[j];
-                    if (a == 0.0f) continue;
-#pragma unroll
-                    for (int c = 0; c < 3; c++) atomicAdd(acc + c * tp + ly * g.tw + lx, a * s[c]);
-                }
-            }
-        }
-    }
-    __syncthreads();
-
-    double dq[3] = {0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < tp; i += blockDim.x) {
-        const int lx = i % g.tw, ly = i / g.tw;
-        const int x = tx0 + lx, y = ty0 + ly;
-        if (x >= g.W || y >= g.H) continue;
-        const size_t o = (size_t)y * g.W + x;
-        const int li = (ly + HAL) * lw + lx + HAL;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float *dp = dl + c * lp;
-            const float dv = dp[li];
-            float lap = 0.0f;
-            if (g.W > 1) {
-                if (x == 0) lap += dv - dp[li + 1];
-                else if (x < g.W - 1) lap += 2.0f * dv - dp[li - 1] - dp[li + 1];
-                else lap += dv - dp[li - 1];
-            }
-            if (g.H > 1) {
-                if (y == 0) lap += dv - dp[li + lw];
-                else if (y < g.H - 1) lap += 2.0f * dv - dp[li - lw] - dp[li + lw];
-                else lap += dv - dp[li - lw];
-            }
-            const float qv = acc[c * tp + i] + c0 * dv + beta_reg * lap;
-            q[c * N + o] = qv;
-            dq[c] += (double)dv * (double)qv;
-        }
-    }
-    if (mode == 0) return;
-    if (grid_reduce<3>(dq, partials, counter, nullptr, red)) {
-        if (threadIdx.x == 0) {
             for (int c = 0; c < 3; c++) {
-                st->dq[c] = dq[c];
-                if (st->active[c]) {
-                    if (!(dq[c] > 0.0)) {
-                        st->breakdown[c] = 1;
-                        st->active[c] = 0;
-                        st->alpha[c] = 0.0;
-                    } else {
-                        st->alpha[c] = st->rz[c] / dq[c];
-                    }
-                } else {
-                    st->alpha[c] = 0.0;
+                const float *dp = dl + c * lp;
+                const float dv = dp[li];
+                float lap = 0.0f;
+                if (g.W > 1) {
+                    if (x == 0) lap += dv - dp[li + 1];
+                    else if (x < g.W - 1) lap += 2.0f * dv - dp[li - 1] - dp[li + 1];
+                    else lap += dv - dp[li - 1];
                 }
+                if (g.H > 1) {
+                    if (y == 0) lap += dv - dp[li + lw];
+                    else if (y < g.H - 1) lap += 2.0f * dv - dp[li - lw] - dp[li + lw];
+                    else lap += dv - dp[li - lw];
+                }
+                const float qv = a[c] + c0 * dv + beta_reg * lap;
+                q[c * N + o] = qv;
+                dq[c] += (double)dv * (double)qv;
+            }
+        });
+    if (mode == 0) return;
+    block_partials<3>(dq, partials, red);
+}
+
+// alpha = r.z / d.q from the operator's partial sums (one block).
+__global__ void __launch_bounds__(256) k_cg_alpha(CgState *st, const double *__restrict__ partials, int G)
+{
+    __shared__ double red[3 * 16];
+    if (st->done) return;
+    double dq[3];
+    reduce_partials<3>(partials, G, dq, red);
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < 3; c++) {
+            st->dq[c] = dq[c];
+            if (st->active[c]) {
+                if (!(dq[c] > 0.0)) {
+                    st->breakdown[c] = 1;
+                    st->active[c] = 0;
+                    st->alpha[c] = 0.0;
+                } else {
+                    st->alpha[c] = st->rz[c] / dq[c];
+                }
+            } else {
+                st->alpha[c] = 0.0;
             }
         }
     }
@@ -160,8 +145,7 @@ __global__ void __launch_bounds__(256) k_cg_op(TileGeom g, TileEntries te, float
 // r = b - q (q = A x0), r.r, r.z, b.b; initialises the scalar state.
 __global__ void __launch_bounds__(256) k_cg_init(size_t N, const float *__restrict__ b,
                                                  const float *__restrict__ q, float *__restrict__ r,
-                                                 const float *__restrict__ minv, double tol2, int maxit,
-                                                 CgState *st, double *partials, unsigned int *counter)
+                                                 const float *__restrict__ minv, double *partials)
 {
     __shared__ double red[9 * 16];
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -176,25 +160,33 @@ __global__ void __launch_bounds__(256) k_cg_init(size_t N, const float *__restri
             v[6 + c] += (double)bv * bv;
         }
     }
-    if (grid_reduce<9>(v, partials, counter, nullptr, red)) {
-        if (threadIdx.x == 0) {
-            int any = 0;
-            for (int c = 0; c < 3; c++) {
-                st->rr[c] = v[c];
-                st->rz[c] = v[3 + c];
-                st->bb[c] = v[6 + c];
-                st->alpha[c] = 0.0;
-                st->beta[c] = 0.0;
-                st->iter[c] = 0;
-                st->breakdown[c] = 0;
-                st->active[c] = (v[c] > tol2 * v[6 + c]) && maxit > 0;
-                any |= st->active[c];
-            }
-            st->tol2 = tol2;
-            st->maxit = maxit;
-            st->first = 1;
-            st->done = !any;
+    block_partials<9>(v, partials, red);
+}
+
+// Scalar state of a new solve from the partial sums of k_cg_init (one block).
+__global__ void __launch_bounds__(256) k_cg_init_state(CgState *st, const double *__restrict__ partials, int G,
+                                                       double tol2, int maxit)
+{
+    __shared__ double red[9 * 16];
+    double v[9];
+    reduce_partials<9>(partials, G, v, red);
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int c = 0; c < 3; c++) {
+            st->rr[c] = v[c];
+            st->rz[c] = v[3 + c];
+            st->bb[c] = v[6 + c];
+            st->alpha[c] = 0.0;
+            st->beta[c] = 0.0;
+            st->iter[c] = 0;
+            st->breakdown[c] = 0;
+            st->active[c] = (v[c] > tol2 * v[6 + c]) && maxit > 0;
+            any |= st->active[c];
         }
+        st->tol2 = tol2;
+        st->maxit = maxit;
+        st->first = 1;
+        st->done = !any;
     }
 }
 
@@ -202,7 +194,7 @@ __global__ void __launch_bounds__(256) k_cg_init(size_t N, const float *__restri
 __global__ void __launch_bounds__(256) k_cg_update(size_t N, float *__restrict__ x,
                                                    const float *__restrict__ d, const float *__restrict__ q,
                                                    float *__restrict__ r, const float *__restrict__ minv,
-                                                   CgState *st, double *partials, unsigned int *counter)
+                                                   const CgState *st, double *partials)
 {
     __shared__ double red[6 * 16];
     if (st->done) return;
@@ -220,34 +212,41 @@ __global__ void __launch_bounds__(256) k_cg_update(size_t N, float *__restrict__
             v[3 + c] += (double)rv * (double)(mi * rv);
         }
     }
-    if (grid_reduce<6>(v, partials, counter, nullptr, red)) {
-        if (threadIdx.x == 0) {
-            int any = 0;
-            for (int c = 0; c < 3; c++) {
-                if (st->active[c]) {
-                    st->iter[c] += 1;
-                    st->rr[c] = v[c];
-                    if (v[c] <= st->tol2 * st->bb[c]) {
-                        st->active[c] = 0;
-                        st->beta[c] = 0.0;
-                    } else {
-                        st->beta[c] = v[3 + c] / st->rz[c];
-                        st->rz[c] = v[3 + c];
-                        if (st->iter[c] >= st->maxit) st->active[c] = 0;
-                    }
+    block_partials<6>(v, partials, red);
+}
+
+// Convergence test and beta = r.z_new / r.z_old (one block).
+__global__ void __launch_bounds__(256) k_cg_beta(CgState *st, const double *__restrict__ partials, int G)
+{
+    __shared__ double red[6 * 16];
+    if (st->done) return;
+    double v[6];
+    reduce_partials<6>(partials, G, v, red);
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int c = 0; c < 3; c++) {
+            if (st->active[c]) {
+                st->iter[c] += 1;
+                st->rr[c] = v[c];
+                if (v[c] <= st->tol2 * st->bb[c]) {
+                    st->active[c] = 0;
+                    st->beta[c] = 0.0;
+                } else {
+                    st->beta[c] = v[3 + c] / st->rz[c];
+                    st->rz[c] = v[3 + c];
+                    if (st->iter[c] >= st->maxit) st->active[c] = 0;
                 }
-                any |= st->active[c];
             }
-            st->first = 0;
-            st->done = !any;
+            any |= st->active[c];
         }
+        st->first = 0;
+        st->done = !any;
     }
 }
 
 // ||b - q||^2 per channel (true residual check).
 __global__ void __launch_bounds__(256) k_resid(size_t N, const float *__restrict__ b,
-                                               const float *__restrict__ q, double *partials,
-                                               unsigned int *counter, double *out)
+                                               const float *__restrict__ q, double *partials)
 {
     __shared__ double red[3 * 16];
     double v[3] = {0, 0, 0};
@@ -256,7 +255,7 @@ __global__ void __launch_bounds__(256) k_resid(size_t N, const float *__restrict
             const double t = (double)b[c * N + o] - (double)q[c * N + o];
             v[c] += t * t;
         }
-    grid_reduce<3>(v, partials, counter, out, red);
+    block_partials<3>(v, partials, red);
 }
 
 // Energy of the normal equations, data part: sum over binned entries would
@@ -266,8 +265,7 @@ __global__ void __launch_bounds__(256) k_gibr_energy_data(int V, int W, int H, i
                                                           const float *__restrict__ tx, const float *__restrict__ ty,
                                                           const float *__restrict__ omega, float wscale,
                                                           const float *__restrict__ views,
-                                                          const float *__restrict__ U, double *partials,
-                                                          unsigned int *counter, double *out)
+                                                          const float *__restrict__ U, double *partials)
 {
     __shared__ double red[16];
     const size_t n = (size_t)w * h, N = (size_t)W * H;
@@ -296,12 +294,12 @@ __global__ void __launch_bounds__(256) k_gibr_energy_data(int V, int W, int H, i
             v[0] += (double)om * rr * rr;
         }
     }
-    grid_reduce<1>(v, partials, counter, out, red);
+    block_partials<1>(v, partials, red);
 }
 
 __global__ void __launch_bounds__(256) k_gibr_energy_grad(int W, int H, const float *__restrict__ U,
                                                           const float *__restrict__ gx, const float *__restrict__ gy,
-                                                          double *partials, unsigned int *counter, double *out)
+                                                          double *partials)
 {
     __shared__ double red[16];
     const size_t N = (size_t)W * H;
@@ -314,7 +312,7 @@ __global__ void __launch_bounds__(256) k_gibr_energy_grad(int W, int H, const fl
             if (y < H - 1) { const double t = ((double)u[o + W] - u[o]) - gy[o]; v[0] += t * t; }
         }
     }
-    grid_reduce<1>(v, partials, counter, out, red);
+    block_partials<1>(v, partials, red);
 }
 
 unsigned int reduce_blocks(size_t N)
@@ -325,7 +323,6 @@ unsigned int reduce_blocks(size_t N)
 
 }  // namespace
 
-// counters: [0] op, [1] init, [2] update, [3] resid, [4..7] misc
 lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use_views, bool use_precond,
                    bool poisson_rhs, double tol, int maxit, lfrv_cg_stats *st)
 {
@@ -348,8 +345,9 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     TileEntries te = tile_entries(ctx);
     te.active = use_views ? 1 : 0;
     const int HAL = ctx->ks - 1;
-    const size_t lds = sizeof(float) * (3 * (size_t)(ctx->tile_w + 2 * HAL) * (ctx->tile_h + 2 * HAL) +
-                                        3 * (size_t)ctx->tile_w * ctx->tile_h);
+    const size_t dtile = sizeof(float) * 3 * (size_t)(ctx->tile_w + 2 * HAL) * (ctx->tile_h + 2 * HAL);
+    const int cap = pull_cap(ctx, 3, dtile, 64 * 1024);
+    const size_t lds = dtile + pull_lds(ctx, 3, cap);
     const unsigned int rb = reduce_blocks(N);
     float *x = ctx->U, *r = ctx->cg_r, *q = ctx->cg_q, *minv = ctx->cg_minv;
     float *dbuf[2] = {ctx->cg_d, ctx->tmp3N};
@@ -357,8 +355,8 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     double *part = ctx->partials;
 
 #define OPK(K, MODE, DIN, DOUT)                                                                               \
-    k_cg_op<K><<<ctx->ntiles, 256, lds, ctx->stream>>>(g, te, wscale, (float)c0, (float)beta, MODE, DIN, r,  \
-                                                      minv, DOUT, q, stt, part, ctx->counters + 0)
+    k_cg_op<K><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, wscale, (float)c0, (float)beta, MODE, DIN, r, \
+                                                      minv, DOUT, q, stt, part)
 #define LAUNCH_OP(MODE, DIN, DOUT)                     \
     switch (ctx->ks) {                                 \
     case 2: OPK(2, MODE, DIN, DOUT); break;            \
@@ -372,7 +370,8 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         TimedRegion tr(ctx, 0);
         LAUNCH_OP(0, x, nullptr);
     }
-    k_cg_init<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, r, minv, tol * tol, maxit, stt, part, ctx->counters + 1);
+    k_cg_init<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, r, minv, part);
+    k_cg_init_state<<<1, 256, 0, ctx->stream>>>(stt, part, (int)rb, tol * tol, maxit);
     LFRV_LAUNCH(ctx, "cg init");
     int it = 0, cur = 0;
     const int chunk = 8;
@@ -385,10 +384,12 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
                 TimedRegion tr(ctx, 0);
                 LAUNCH_OP(1, dprev, dnew);
             }
+            k_cg_alpha<<<1, 256, 0, ctx->stream>>>(stt, part, ctx->ntiles);
             {
                 TimedRegion tr(ctx, 1);
-                k_cg_update<<<rb, 256, 0, ctx->stream>>>(N, x, dnew, q, r, minv, stt, part, ctx->counters + 2);
+                k_cg_update<<<rb, 256, 0, ctx->stream>>>(N, x, dnew, q, r, minv, stt, part);
             }
+            k_cg_beta<<<1, 256, 0, ctx->stream>>>(stt, part, (int)rb);
             cur ^= 1;
         }
         LFRV_LAUNCH(ctx, "cg iteration");
@@ -409,7 +410,8 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         }
     }
 #undef OPK
-    k_resid<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, part, ctx->counters + 3, ctx->scal);
+    k_resid<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, part);
+    k_reduce_out<3><<<1, 256, 0, ctx->stream>>>(part, (int)rb, ctx->scal);
     LFRV_LAUNCH(ctx, "cg residual");
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -448,7 +450,7 @@ lfrv_status run_gibr_energy(lfrv_ctx *ctx, double *e)
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
     const size_t tot = (size_t)ctx->V * ctx->n;
     const unsigned int rb1 = reduce_blocks(tot), rb2 = reduce_blocks(ctx->N);
-#define EN(K) k_gibr_energy_data<K><<<rb1, 256, 0, ctx->stream>>>(ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, ctx->omega, s2, ctx->views, ctx->U, ctx->partials, ctx->counters + 4, ctx->scal + 4)
+#define EN(K) k_gibr_energy_data<K><<<rb1, 256, 0, ctx->stream>>>(ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, ctx->omega, s2, ctx->views, ctx->U, ctx->partials)
     switch (ctx->ks) {
     case 2: EN(2); break;
     case 3: EN(3); break;
@@ -457,8 +459,10 @@ lfrv_status run_gibr_energy(lfrv_ctx *ctx, double *e)
     default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
     }
 #undef EN
+    k_reduce_out<1><<<1, 256, 0, ctx->stream>>>(ctx->partials, (int)rb1, ctx->scal + 4);
     k_gibr_energy_grad<<<rb2, 256, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->u_grad_x, ctx->u_grad_y,
-                                                     ctx->partials, ctx->counters + 5, ctx->scal + 5);
+                                                     ctx->partials);
+    k_reduce_out<1><<<1, 256, 0, ctx->stream>>>(ctx->partials, (int)rb2, ctx->scal + 5);
     LFRV_LAUNCH(ctx, "gibr energy");
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host + 4, ctx->scal + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));

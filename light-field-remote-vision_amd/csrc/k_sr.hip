@@ -25,22 +25,21 @@ struct DataTermArgs {
 // Writes F (into G), Uq = U and norm_mask = cov (the reference's buffer ends
 // holding the coverage).
 template <int KS>
-__global__ void __launch_bounds__(256) k_sr_dataterm(TileGeom g, TileEntries te, size_t n, DataTermArgs da,
+__global__ void __launch_bounds__(1024) k_sr_dataterm(TileGeom g, TileEntries te, int cap, size_t n, DataTermArgs da,
                                                      const float *__restrict__ U, const float *__restrict__ views,
                                                      const float *__restrict__ vlap, const float *__restrict__ coverage,
                                                      float *__restrict__ Gout, float *__restrict__ Uq,
                                                      float *__restrict__ norm_mask)
 {
     constexpr int HAL = KS - 1;
-    extern __shared__ float lds[];
+    extern __shared__ int smem[];
     const int t = xcd_tile(blockIdx.x, g.ntiles);
     const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
-    const int tp = g.tw * g.th;
     const int uw = g.tw + 2 * HAL + 2, uh = g.th + 2 * HAL + 2, up = uw * uh;  // U: halo HAL+1
     const int lw = g.tw + 2 * HAL, lh = g.th + 2 * HAL, lpp = lw * lh;         // lap U: halo HAL
-    float *ul = lds;              // 3 * up
-    float *ll = ul + 3 * up;      // 3 * lpp
-    float *acc = ll + 3 * lpp;    // 6 * tp
+    float *ul = reinterpret_cast<float *>(smem);  // 3 * up
+    float *ll = ul + 3 * up;                      // 3 * lpp
+    int *plds = reinterpret_cast<int *>(ll + 3 * lpp);
     const size_t N = (size_t)g.W * g.H;
     for (int i = threadIdx.x; i < up; i += blockDim.x) {
         const int x = tx0 - HAL - 1 + i % uw, y = ty0 - HAL - 1 + i / uw;
@@ -48,7 +47,6 @@ __global__ void __launch_bounds__(256) k_sr_dataterm(TileGeom g, TileEntries te,
         const size_t o = in ? (size_t)y * g.W + x : 0;
         for (int c = 0; c < 3; c++) ul[c * up + i] = in ? U[c * N + o] : 0.0f;
     }
-    for (int i = threadIdx.x; i < 6 * tp; i += blockDim.x) acc[i] = 0.0f;
     __syncthreads();
     const bool need_lap = da.beta > 0.0f;
     if (need_lap) {
@@ -71,98 +69,71 @@ __global__ void __launch_bounds__(256) k_sr_dataterm(TileGeom g, TileEntries te,
                 ll[c * lpp + i] = v;
             }
         }
-        __syncthreads();
     }
-    const int e0 = te.start[t], e1 = te.start[t + 1];
-    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-        int x0, y0;
-        float wx[KS], wy[KS];
-        if (!footprint<KS>(te.tx[e], te.ty[e], g.W, g.H, x0, y0, wx, wy)) continue;
-        const int k = te.src[e];
-        const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
-        const float om = te.w[e];
-        const int bx = x0 - tx0 + HAL, by = y0 - ty0 + HAL;
-        float su[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
+    // (tile_pull synchronises before phase 1)
+    any_pull<KS, 6, false>(
+        g, te, t, plds, cap,
+        [&](int e, int x0, int y0, const float(&wx)[KS], const float(&wy)[KS], float(&r)[6]) {
+            const int k = te.src[e];
+            const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
+            const float om = te.w[e];
+            const int bx = x0 - tx0 + HAL, by = y0 - ty0 + HAL;
+            float su[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
 #pragma unroll
-        for (int i = 0; i < KS; i++)
+            for (int i = 0; i < KS; i++)
 #pragma unroll
-            for (int j = 0; j < KS; j++) {
-                const float a = wy[i] * wx[j];
-                const int uidx = (by + i + 1) * uw + bx + j + 1;
-                const int lidx = (by + i) * lw + bx + j;
+                for (int j = 0; j < KS; j++) {
+                    const float a = wy[i] * wx[j];
+                    const int uidx = (by + i + 1) * uw + bx + j + 1;
+                    const int lidx = (by + i) * lw + bx + j;
 #pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    su[c] += a * ul[c * up + uidx];
-                    if (need_lap) sl[c] += a * ll[c * lpp + lidx];
+                    for (int c = 0; c < 3; c++) {
+                        su[c] += a * ul[c * up + uidx];
+                        if (need_lap) sl[c] += a * ll[c * lpp + lidx];
+                    }
                 }
+            const float *vp = views + vi * 3 * n + m;
+            const float *lp = vlap + vi * 3 * n + m;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                r[c] = (da.alpha > 0.0f) ? (su[c] - vp[c * n]) * om : 0.0f;
+                float rg = 0.0f;
+                if (need_lap) {
+                    rg = sl[c] * da.dsf2 - lp[c * n];
+                    if (da.gw_type != 0) rg *= om;
+                }
+                r[3 + c] = rg;
             }
-        float r[6];
-        const float *vp = views + vi * 3 * n + m;
-        const float *lp = vlap + vi * 3 * n + m;
+        },
+        [&](size_t o, int x, int y, const float(&acc)[6]) {
+            const float cov = coverage[o];
+            const int ui = (y - ty0 + HAL + 1) * uw + (x - tx0) + HAL + 1;
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            r[c] = (da.alpha > 0.0f) ? (su[c] - vp[c * n]) * om : 0.0f;
-            float rg = 0.0f;
-            if (need_lap) {
-                rg = sl[c] * da.dsf2 - lp[c * n];
-                if (da.gw_type != 0) rg *= om;
+            for (int c = 0; c < 3; c++) {
+                float G = 0.0f;
+                if (da.alpha > 0.0f) {
+                    float gi = acc[c];
+                    if (cov != 0.0f) gi /= cov;
+                    gi *= da.sigma2;
+                    G += da.alpha * gi;
+                }
+                if (need_lap) {
+                    float gg = acc[3 + c];
+                    if (cov != 0.0f) gg /= cov;
+                    if (da.gw_type != 0) gg *= da.sigma2;
+                    gg *= -da.grad_step;
+                    G += da.beta * gg;
+                }
+                G = fminf(1.0f, fmaxf(-1.0f, G));
+                const float uv = ul[c * up + ui];
+                G *= da.prox_factor;
+                G += uv;
+                G = fminf(1.0f, fmaxf(0.0f, G));
+                Gout[c * N + o] = G;
+                Uq[c * N + o] = uv;
             }
-            r[3 + c] = rg;
-        }
-#pragma unroll
-        for (int i = 0; i < KS; i++) {
-            const int ly = y0 + i - ty0;
-            if (ly < 0 || ly >= g.th) continue;
-#pragma unroll
-            for (int j = 0; j < KS; j++) {
-                const int lx = x0 + j - tx0;
-                if (lx < 0 || lx >= g.tw) continue;
-                const float a = wy[i] * wx[j];
-                if (a == 0.0f) continue;
-                const int li = ly * g.tw + lx;
-                if (da.alpha > 0.0f)
-#pragma unroll
-                    for (int c = 0; c < 3; c++) atomicAdd(acc + c * tp + li, a * r[c]);
-                if (need_lap)
-#pragma unroll
-                    for (int c = 0; c < 3; c++) atomicAdd(acc + (3 + c) * tp + li, a * r[3 + c]);
-            }
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < tp; i += blockDim.x) {
-        const int lx = i % g.tw, ly = i / g.tw;
-        const int x = tx0 + lx, y = ty0 + ly;
-        if (x >= g.W || y >= g.H) continue;
-        const size_t o = (size_t)y * g.W + x;
-        const float cov = coverage[o];
-        const int ui = (ly + HAL + 1) * uw + lx + HAL + 1;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            float G = 0.0f;
-            if (da.alpha > 0.0f) {
-                float gi = acc[c * tp + i];
-                if (cov != 0.0f) gi /= cov;
-                gi *= da.sigma2;
-                G += da.alpha * gi;
-            }
-            if (need_lap) {
-                float gg = acc[(3 + c) * tp + i];
-                if (cov != 0.0f) gg /= cov;
-                if (da.gw_type != 0) gg *= da.sigma2;
-                gg *= -da.grad_step;
-                G += da.beta * gg;
-            }
-            G = fminf(1.0f, fmaxf(-1.0f, G));
-            const float uv = ul[c * up + ui];
-            G *= da.prox_factor;
-            G += uv;
-            G = fminf(1.0f, fmaxf(0.0f, G));
-            Gout[c * N + o] = G;
-            Uq[c * N + o] = uv;
-        }
-        if (da.alpha > 0.0f || need_lap) norm_mask[o] = cov;
-    }
+            if (da.alpha > 0.0f || need_lap) norm_mask[o] = cov;
+        });
 }
 
 // theta over-relaxation (vtv_rof.cu:28-45): uq <- u + theta (uq - u); u <- uq_old
@@ -179,8 +150,7 @@ __global__ void k_overrelax(size_t n3, float theta, float *__restrict__ uq, floa
 // of Y(U) for the gradient term (optimization.cu:1505-1534, 1622-1625).
 __global__ void __launch_bounds__(256) k_sr_energy_tv(int W, int H, const float *__restrict__ U,
                                                       const float *__restrict__ vis, float *__restrict__ ugx,
-                                                      float *__restrict__ ugy, double *partials,
-                                                      unsigned int *counter, double *out)
+                                                      float *__restrict__ ugy, double *partials)
 {
     __shared__ double red[16];
     const size_t N = (size_t)W * H;
@@ -206,7 +176,7 @@ __global__ void __launch_bounds__(256) k_sr_energy_tv(int W, int H, const float 
         ugy[o] = GY;
         if (ok) v[0] += fabs((double)GX) + fabs((double)GY);
     }
-    grid_reduce<1>(v, partials, counter, out, red);
+    block_partials<1>(v, partials, red);
 }
 
 // E_int and E_grad over every (view, pixel) with gather validity
@@ -217,8 +187,7 @@ __global__ void __launch_bounds__(256) k_sr_energy_views(int V, int W, int H, in
                                                          const float *__restrict__ omega, const float *__restrict__ views,
                                                          const float *__restrict__ vgrad, const float *__restrict__ U,
                                                          const float *__restrict__ ugx, const float *__restrict__ ugy,
-                                                         float sigma2, int gw_type, double *partials,
-                                                         unsigned int *counter, double *out)
+                                                         float sigma2, int gw_type, double *partials)
 {
     __shared__ double red[2 * 16];
     const size_t n = (size_t)w * h, N = (size_t)W * H;
@@ -261,12 +230,12 @@ __global__ void __launch_bounds__(256) k_sr_energy_views(int V, int W, int H, in
         }
         v[1] += (double)a + (double)b;
     }
-    grid_reduce<2>(v, partials, counter, out, red);
+    block_partials<2>(v, partials, red);
 }
 
 __global__ void __launch_bounds__(256) k_count_moving(size_t N, const float *__restrict__ U,
                                                       const float *__restrict__ prev, float thr,
-                                                      double *partials, unsigned int *counter, double *out)
+                                                      double *partials)
 {
     __shared__ double red[16];
     double v[1] = {0.0};
@@ -275,7 +244,7 @@ __global__ void __launch_bounds__(256) k_count_moving(size_t N, const float *__r
         for (int c = 0; c < 3; c++) mx = fmaxf(mx, fabsf(U[c * N + o] - prev[c * N + o]));
         if (!(mx < thr)) v[0] += 1.0;
     }
-    grid_reduce<1>(v, partials, counter, out, red);
+    block_partials<1>(v, partials, red);
 }
 
 unsigned int rblocks(size_t N)
@@ -305,10 +274,12 @@ lfrv_status run_fista(lfrv_ctx *ctx)
     const int HAL = ctx->ks - 1;
     const size_t up = (size_t)(ctx->tile_w + 2 * HAL + 2) * (ctx->tile_h + 2 * HAL + 2);
     const size_t lpp = (size_t)(ctx->tile_w + 2 * HAL) * (ctx->tile_h + 2 * HAL);
-    const size_t lds = sizeof(float) * (3 * up + 3 * lpp + 6 * (size_t)ctx->tile_w * ctx->tile_h);
+    const size_t tiles = sizeof(float) * (3 * up + 3 * lpp);
+    const int cap = pull_cap(ctx, 6, tiles, 64 * 1024);
+    const size_t lds = tiles + pull_lds(ctx, 6, cap);
     {
         TimedRegion tr(ctx, 3);
-#define DT(K) k_sr_dataterm<K><<<ctx->ntiles, 256, lds, ctx->stream>>>(g, te, ctx->n, da, ctx->U, ctx->views, ctx->view_lap, ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask)
+#define DT(K) k_sr_dataterm<K><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, ctx->n, da, ctx->U, ctx->views, ctx->view_lap, ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask)
         switch (ctx->ks) {
         case 2: DT(2); break;
         case 3: DT(3); break;
@@ -335,10 +306,11 @@ lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts)
     const size_t N = ctx->N;
     float *ugx = ctx->tmp3N, *ugy = ctx->tmp3N + N;
     k_sr_energy_tv<<<rblocks(N), 256, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->visibility, ugx, ugy,
-                                                        ctx->partials, ctx->counters + 10, ctx->scal + 8);
+                                                        ctx->partials);
+    k_reduce_out<1><<<1, 256, 0, ctx->stream>>>(ctx->partials, (int)rblocks(N), ctx->scal + 8);
     const size_t tot = (size_t)ctx->V * ctx->n;
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
-#define EV(K) k_sr_energy_views<K><<<rblocks(tot), 256, 0, ctx->stream>>>(ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, ctx->omega, ctx->views, ctx->view_grad, ctx->U, ugx, ugy, s2, ctx->p.gw_type, ctx->partials, ctx->counters + 11, ctx->scal + 9)
+#define EV(K) k_sr_energy_views<K><<<rblocks(tot), 256, 0, ctx->stream>>>(ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, ctx->omega, ctx->views, ctx->view_grad, ctx->U, ugx, ugy, s2, ctx->p.gw_type, ctx->partials)
     switch (ctx->ks) {
     case 2: EV(2); break;
     case 3: EV(3); break;
@@ -347,6 +319,7 @@ lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts)
     default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
     }
 #undef EV
+    k_reduce_out<2><<<1, 256, 0, ctx->stream>>>(ctx->partials, (int)rblocks(tot), ctx->scal + 9);
     LFRV_LAUNCH(ctx, "sr energy");
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host + 8, ctx->scal + 8, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -362,8 +335,8 @@ lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts)
 
 lfrv_status run_count_moving(lfrv_ctx *ctx, const float *prev_dev, float max_diff, int64_t *moving)
 {
-    k_count_moving<<<rblocks(ctx->N), 256, 0, ctx->stream>>>(ctx->N, ctx->U, prev_dev, max_diff, ctx->partials,
-                                                             ctx->counters + 12, ctx->scal + 12);
+    k_count_moving<<<rblocks(ctx->N), 256, 0, ctx->stream>>>(ctx->N, ctx->U, prev_dev, max_diff, ctx->partials);
+    k_reduce_out<1><<<1, 256, 0, ctx->stream>>>(ctx->partials, (int)rblocks(ctx->N), ctx->scal + 12);
     LFRV_LAUNCH(ctx, "count moving");
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host + 12, ctx->scal + 12, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));

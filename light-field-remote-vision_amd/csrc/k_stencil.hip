@@ -34,7 +34,7 @@ struct HoleState {
 // previous sweep's (the stopping rule of gradientIBR.cu:139-143).
 __global__ void __launch_bounds__(256) k_hole_sweep(int W, int H, int nholes, const int *__restrict__ list,
                                                     const float *__restrict__ uin, float *__restrict__ uout,
-                                                    HoleState *st, double *partials, unsigned int *counter)
+                                                    const HoleState *st, double *partials)
 {
     __shared__ double red[16];
     if (st->done) return;
@@ -56,13 +56,21 @@ __global__ void __launch_bounds__(256) k_hole_sweep(int W, int H, int nholes, co
             if (c == 2) v[0] += (double)lap * (double)un;
         }
     }
-    if (grid_reduce<1>(v, partials, counter, nullptr, red)) {
-        if (threadIdx.x == 0) {
-            st->e_prev = st->e;
-            st->e = v[0];
-            st->iters += 1;
-            if (fabs(st->e_prev - st->e) < st->eps || st->iters >= st->maxit) st->done = 1;
-        }
+    block_partials<1>(v, partials, red);
+}
+
+// Energy bookkeeping and stopping rule of one sweep (one block).
+__global__ void __launch_bounds__(256) k_hole_state(HoleState *st, const double *__restrict__ partials, int G)
+{
+    __shared__ double red[16];
+    if (st->done) return;
+    double v[1];
+    reduce_partials<1>(partials, G, v, red);
+    if (threadIdx.x == 0) {
+        st->e_prev = st->e;
+        st->e = v[0];
+        st->iters += 1;
+        if (fabs(st->e_prev - st->e) < st->eps || st->iters >= st->maxit) st->done = 1;
     }
 }
 
@@ -248,9 +256,10 @@ __global__ void __launch_bounds__(256) k_pd_step(int W, int H, float tau, float 
             float nn = 0.0f;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                const float uc = us[c][ly + 1][lx + 1];
-                const float gx = (x < W - 1) ? us[c][ly + 1][lx + 2] - uc : 0.0f;
-                const float gy = (y < H - 1) ? us[c][ly + 2][lx + 1] - uc : 0.0f;
+                // the p region and the Uq region share the origin (x0-1, y0-1)
+                const float uc = us[c][ly][lx];
+                const float gx = (x < W - 1) ? us[c][ly][lx + 1] - uc : 0.0f;
+                const float gy = (y < H - 1) ? us[c][ly + 1][lx] - uc : 0.0f;
                 a[c] = pxin[c * N + o] + sigma * gx;
                 b[c] = pyin[c * N + o] + sigma * gy;
                 nn += a[c] * a[c] + b[c] * b[c];
@@ -322,7 +331,8 @@ lfrv_status run_hole_filling(lfrv_ctx *ctx, int iters, double eps, int *ran)
         for (int k = 0; k < chunk; k++) {
             const int cur = (launched + k) & 1;
             k_hole_sweep<<<nb, 256, 0, ctx->stream>>>(ctx->W, ctx->H, (int)nh, ctx->hole_list, bufs[cur],
-                                                      bufs[cur ^ 1], dst, ctx->partials, ctx->counters + 9);
+                                                      bufs[cur ^ 1], dst, ctx->partials);
+            k_hole_state<<<1, 256, 0, ctx->stream>>>(dst, ctx->partials, (int)nb);
         }
         launched += chunk;
         LFRV_LAUNCH(ctx, "hole filling");

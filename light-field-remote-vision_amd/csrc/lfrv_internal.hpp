@@ -118,47 +118,48 @@ __device__ __forceinline__ void block_sum(double (&v)[NQ], double *lds /* >= NQ 
     __syncthreads();
 }
 
-// Grid-wide reduction of NQ doubles per block: each block stores its partial
-// row, the last block to arrive sums all rows in block order (deterministic)
-// into v[] of thread 0 (and into `out` when non-null) and resets the arrival
-// counter.  Returns true in the last block (all threads), so the caller can
-// finalise scalars there from thread 0.
+// Grid-wide reductions without device-scope fences.  The L2 caches of the
+// XCDs are not coherent with each other, so a device-scope release after
+// every block (the classic last-block pattern) writes back an L2 per block.
+// Instead each block stores its NQ partial sums (layout [q][block], plain
+// stores) and a one-block finalize kernel launched after it on the same
+// stream (the kernel boundary orders the stores) reduces them in block order.
 template <int NQ>
-__device__ bool grid_reduce(double (&v)[NQ], double *partials, unsigned int *counter, double *out,
-                            double *lds)
+__device__ __forceinline__ void block_partials(double (&v)[NQ], double *partials, double *lds)
 {
     block_sum<NQ>(v, lds);
-    __shared__ int s_last;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NQ; q++) partials[(size_t)blockIdx.x * NQ + q] = v[q];
-        __threadfence();
-        const unsigned int t = atomicAdd(counter, 1u);
-        s_last = (t == gridDim.x - 1) ? 1 : 0;
+        for (int q = 0; q < NQ; q++) partials[(size_t)q * gridDim.x + blockIdx.x] = v[q];
     }
-    __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
-    double acc[NQ];
+}
+
+// Deterministic sum of G block partials (layout [q][block]) by one block;
+// the result is valid in thread 0.
+template <int NQ>
+__device__ __forceinline__ void reduce_partials(const double *partials, int G, double (&out)[NQ], double *lds)
+{
 #pragma unroll
-    for (int q = 0; q < NQ; q++) acc[q] = 0.0;
-    // Thread t sums blocks t, t+blockDim, ... in order; then a fixed-order block sum.
-    for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-        const volatile double *pr = partials + (size_t)b * NQ;
-#pragma unroll
-        for (int q = 0; q < NQ; q++) acc[q] += pr[q];
+    for (int q = 0; q < NQ; q++) {
+        double a = 0.0;
+        const double *pq = partials + (size_t)q * G;
+        for (int b = threadIdx.x; b < G; b += blockDim.x) a += pq[b];
+        out[q] = a;
     }
-    block_sum<NQ>(acc, lds);
+    block_sum<NQ>(out, lds);
+}
+
+// Finalize kernel of the plain sums: one block of 256 threads.
+template <int NQ>
+__global__ void __launch_bounds__(256) k_reduce_out(const double *__restrict__ partials, int G, double *out)
+{
+    __shared__ double red[NQ * 16];
+    double v[NQ];
+    reduce_partials<NQ>(partials, G, v, red);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            v[q] = acc[q];
-            if (out) out[q] = acc[q];
-        }
-        *counter = 0u;
+        for (int q = 0; q < NQ; q++) out[q] = v[q];
     }
-    __syncthreads();
-    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -213,9 +214,12 @@ struct lfrv_ctx {
     bool have_target_cam = false;
     std::vector<int> have_depth;
 
-    // binned splat structure
-    int tile_w = 32, tile_h = 16, ntx = 0, nty = 0, ntiles = 0;
-    int *tile_count = nullptr, *tile_start = nullptr, *tile_cursor = nullptr;  // ntiles(+1)
+    // splat structure: entries sorted by footprint cell, target tiling of the pulls
+    int tile_w = 32, tile_h = 8, ntx = 0, nty = 0, ntiles = 0;
+    int ncells = 0;                       // (W + ks - 1) * (H + ks - 1)
+    int pull_bs = 256;                    // threads of the pull kernels: >= window cells and tile pixels
+    int *cell_count = nullptr, *cell_start = nullptr, *cell_cursor = nullptr;  // ncells(+1)
+    int *scan_tmp = nullptr;              // block sums of the cell scan
     int *e_src = nullptr;       // view * n + m
     float *e_tx = nullptr, *e_ty = nullptr, *e_w = nullptr;
     size_t e_cap = 0;
@@ -245,7 +249,7 @@ struct lfrv_ctx {
     lfrv::CgState *cg_state_host = nullptr;   // pinned
     double *partials = nullptr;    // per-block partial sums
     size_t partials_cap = 0;       // doubles
-    unsigned int *counters = nullptr;  // arrival counters (one per reduction site)
+    unsigned int *counters = nullptr;  // small device counters (hole-list length)
     double *scal = nullptr;        // device scalars (reduction outputs)
     double *scal_host = nullptr;   // pinned mirror
     int *hole_list = nullptr;      // N

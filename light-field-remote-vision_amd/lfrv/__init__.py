@@ -102,6 +102,7 @@ def lib():
         "lfrv_compute_initial_image": (st, [_vp, _vp]),
         "lfrv_get_solution": (st, [_vp, _vp]),
         "lfrv_set_solution": (st, [_vp, _vp]),
+        "lfrv_clear_solution": (st, [_vp]),
         "lfrv_init_u_gradient": (st, [_vp]),
         "lfrv_cg_solve": (st, [_vp, ctypes.c_double, st, ctypes.POINTER(CgStats)]),
         "lfrv_poisson_cg": (st, [_vp, ctypes.c_double, ctypes.c_double, st, st, ctypes.POINTER(CgStats)]),
@@ -119,6 +120,11 @@ def lib():
         "lfrv_op_splat_atomic": (st, [st, st, st, st, st, _vp, _vp, _vp, st, _vp, _vp, _vp, _vp]),
         "lfrv_op_sobel": (st, [st, st, _vp, _vp, _vp, _vp, _vp]),
         "lfrv_psnr": (ctypes.c_double, [st, st, _vp, _vp]),
+        "lfrv_dev_alloc": (st, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
+        "lfrv_dev_free": (st, [_vp]),
+        "lfrv_dev_copy_kernel": (st, [_vp, _vp, ctypes.c_size_t, _vp]),
+        "lfrv_memcpy": (st, [_vp, _vp, ctypes.c_size_t]),
+        "lfrv_device_sync": (st, []),
         "lfrv_synth_preset": (None, [ctypes.POINTER(SynthDesc), st, ctypes.c_uint64]),
         "lfrv_synth_nviews": (st, [ctypes.POINTER(SynthDesc)]),
         "lfrv_synth_generate": (st, [ctypes.POINTER(SynthDesc), _vp, _vp, _dp, _dp, _vp, _vp]),
@@ -265,6 +271,9 @@ class Context:
     def set_solution(self, rgb):
         self._check(lib().lfrv_set_solution(self._h, _ptr(_f32(rgb))))
 
+    def clear_solution(self):
+        self._check(lib().lfrv_clear_solution(self._h))
+
     def get_buffer(self, name):
         which = BUF[name] if isinstance(name, str) else name
         cnt = ctypes.c_size_t()
@@ -292,6 +301,59 @@ class Context:
         t = ctypes.c_int()
         self._check(lib().lfrv_get_bin_stats(self._h, ctypes.byref(e), ctypes.byref(t)))
         return e.value, t.value
+
+
+class DeviceArray:
+    """A float32 device buffer owned by the library's HIP runtime."""
+
+    def __init__(self, shape_or_array):
+        if isinstance(shape_or_array, np.ndarray):
+            self.shape = shape_or_array.shape
+        else:
+            self.shape = tuple(shape_or_array)
+        self.nbytes = int(np.prod(self.shape)) * 4
+        self.ptr = ctypes.c_void_p()
+        if lib().lfrv_dev_alloc(self.nbytes, ctypes.byref(self.ptr)) != OK:
+            raise LfrvError(E_HIP, "lfrv_dev_alloc failed")
+        if isinstance(shape_or_array, np.ndarray):
+            self.upload(shape_or_array)
+
+    def upload(self, a):
+        a = _f32(a)
+        assert a.nbytes == self.nbytes
+        if lib().lfrv_memcpy(self.ptr, _ptr(a), self.nbytes) != OK:
+            raise LfrvError(E_HIP, "lfrv_memcpy failed")
+
+    def numpy(self):
+        out = np.empty(self.shape, np.float32)
+        if lib().lfrv_memcpy(_ptr(out), self.ptr, self.nbytes) != OK:
+            raise LfrvError(E_HIP, "lfrv_memcpy failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().lfrv_dev_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_sync():
+    if lib().lfrv_device_sync() != OK:
+        raise LfrvError(E_HIP, "lfrv_device_sync failed")
+
+
+def copy_kernel(dst, src):
+    """Device-to-device copy of src into dst (DeviceArrays of equal size) by
+    the library's plain streaming copy kernel, on the default stream."""
+    if dst.nbytes != src.nbytes:
+        raise LfrvError(E_ARG, "copy_kernel: size mismatch")
+    if lib().lfrv_dev_copy_kernel(dst.ptr, src.ptr, src.nbytes, None) != OK:
+        raise LfrvError(E_HIP, "lfrv_dev_copy_kernel failed")
 
 
 def psnr(a, b):

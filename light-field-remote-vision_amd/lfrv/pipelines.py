@@ -28,6 +28,7 @@ def gradient_ibr_params(scene, **kw):
 def gradient_ibr_solve(ctx, tol=1e-5, maxit=2000, check=True):
     """Device-resident part of IBR_gradient: from the views, depths and cameras
     already in the context to the solution in ctx.U."""
+    ctx.clear_solution()
     ctx.warps_from_depth()
     ctx.prepare()
     ctx.setup_visibility()

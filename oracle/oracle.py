@@ -423,3 +423,25 @@ def gibr_pipeline(views, depths, P_views, P_target, dsf, sigma, thr, alpha, beta
     E = gibr_energy(W, H, dsf, tx, ty, views, wdata, alpha, 0.0, None, beta, gx, gy, X)
     return dict(U=X, U0=U0, stats=st, energy=E, tx=tx, ty=ty, dpx=dpx, dpy=dpy, omega=omega, vis=vis, gx=gx,
                 gy=gy, mask=mask, b=b, hole_iters=hole_iters)
+
+
+def sr_weights(W, H, dsf, tx, ty, dpx, dpy, U, vis, PX0, PY0, sigma, thr):
+    """coco_vtv_sr_compute_weights_unstructured (optimization.cu:1066-1167): the
+    gated Sobel of U is written into the channel-0 dual variables (the
+    reference's scratch choice, :1090-1091) and gathered from there.
+    Returns omega and the updated (PX0, PY0)."""
+    gx, gy = sobel(U, vis, PX0, PY0)
+    V = tx.shape[0]
+    om = np.zeros(tx.shape)
+    for v in range(V):
+        g = gather(W, H, dsf, tx[v], ty[v], np.stack([gx, gy]))
+        ang = angular_weights(g[0], g[1], sigma, dpx[v], dpy[v], thr)
+        om[v] = ang * deform_weights(W, H, tx[v], ty[v])
+    return om, gx, gy
+
+
+def coverage(W, H, dsf, tx, ty):
+    cov = np.zeros((H, W))
+    for v in range(tx.shape[0]):
+        splat(W, H, dsf, tx[v], ty[v], None, wsum=cov)
+    return cov

@@ -1,0 +1,97 @@
+"""GPU parity of the IBR_color path (TV primal-dual with warped data terms)
+against the FP64 oracle, step by step through sr_synthesize_view."""
+import numpy as np
+import pytest
+
+import lfrv
+import oracle
+from lfrv import pipelines
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(dsf=2, kind=1, w=48):
+    d = lfrv.synth_desc(1, 5)
+    d.w = d.h = w
+    d.dsf = dsf
+    d.render_scale = dsf
+    d.focal = float(w)
+    d.baseline = 0.7 * 4.0 / (w * dsf)
+    if kind == 2:
+        d.kind = 2
+        d.baseline = 2.0 * 4.0 / (w * dsf)
+    return lfrv.synth(d)
+
+
+@pytest.mark.parametrize("dsf,beta,gw", [(2, 0.0, 0), (2, 0.5, 1), (3, 0.3, 0), (1, 0.5, 1)])
+def test_sr_steps_match_oracle(dsf, beta, gw):
+    s = _scene(dsf=dsf, kind=2 if dsf == 1 else 1)
+    V, h, w, H, W = s["V"], s["h"], s["w"], s["H"], s["W"]
+    sig, lam, lmf, inner = 1.0 / 255.0, 0.05, 100.0, 10
+    prm = pipelines.color_sr_params(s, lambda_=lam, lambda_max_factor=lmf, inner_iterations=inner, dt_beta=beta,
+                                    gw_type=gw, gradient_step=1.0)
+    with lfrv.Context(prm) as ctx:
+        pipelines.load_scene(ctx, s)
+        ctx.warps_from_depth()
+        ctx.prepare()
+        tx = ctx.get_buffer("TAU_X").reshape(V, h, w)
+        ty = ctx.get_buffer("TAU_Y").reshape(V, h, w)
+        dpx = ctx.get_buffer("DPART_X").reshape(V, h, w)
+        dpy = ctx.get_buffer("DPART_Y").reshape(V, h, w)
+        # oracle state mirrors coco_vtv_workspace
+        st = oracle.SrState(W, H, np.zeros((3, H, W)))
+        vis0 = np.zeros((H, W), np.uint8)
+        ctx.compute_weights()
+        om, gx, gy = oracle.sr_weights(W, H, dsf, tx, ty, dpx, dpy, st.U, vis0, st.PX[0], st.PY[0], sig, 1.0)
+        st.PX[0], st.PY[0] = gx, gy
+        assert rel_err(ctx.get_buffer("OMEGA").reshape(V, h, w), om) < 1e-6
+        ctx.setup_visibility()
+        cov = oracle.coverage(W, H, dsf, tx, ty)
+        vis = (cov >= 1.0).astype(np.uint8)
+        st.norm_mask += cov
+        gvis = ctx.get_buffer("VISIBILITY").reshape(H, W)
+        near = np.abs(cov - 1.0) < 1e-5
+        assert np.array_equal(gvis[~near].astype(np.uint8), vis[~near])
+        vis = np.where(near, gvis.astype(np.uint8), vis)
+        ctx.compute_initial_image()
+        U0, nm = oracle.initial_image(W, H, dsf, tx, ty, s["views"], om)
+        st.norm_mask = nm
+        U0 = oracle.push_pull(U0, vis)
+        st.U = U0.copy()
+        st.Uq = U0.copy()
+        assert np.max(np.abs(ctx.get_solution() - U0)) < 1e-5
+        for meta in range(2):
+            ctx.compute_weights()
+            om, gx, gy = oracle.sr_weights(W, H, dsf, tx, ty, dpx, dpy, st.U, vis, st.PX[0], st.PY[0], sig, 1.0)
+            st.PX[0], st.PY[0] = gx, gy
+            assert rel_err(ctx.get_buffer("OMEGA").reshape(V, h, w), om) < 1e-5
+            ctx.sr_init_regularizer_weight()
+            g, st.norm_mask = oracle.regularizer_weight(W, H, dsf, tx, ty, om, sig, lmf * lam, lam, st.norm_mask)
+            assert np.max(np.abs(ctx.get_buffer("REG_G").reshape(H, W) - g)) < 1e-5 * lmf * lam
+            assert rel_err(ctx.get_buffer("NORM_MASK").reshape(H, W), st.norm_mask) < 1e-5
+            e, parts = ctx.sr_primal_energy()
+            oe, oparts = oracle.sr_energy(W, H, dsf, tx, ty, s["views"], om, sig, lam, 1.0, beta, gw, vis, st.U)
+            assert abs(e - oe) / oe < 1e-4
+            for k in range(3):
+                ctx.sr_iteration_fista()
+                oracle.fista_step(W, H, dsf, tx, ty, s["views"], om, sig, 1.0, beta, 1.0, gw, lam, inner, g, st)
+            U = ctx.get_solution()
+            assert np.max(np.abs(U - st.U)) < 2e-4, np.max(np.abs(U - st.U))
+            assert np.max(np.abs(ctx.get_buffer("PX").reshape(3, H, W) - st.PX)) < 2e-4
+            assert rel_err(ctx.get_buffer("NORM_MASK").reshape(H, W), st.norm_mask) < 1e-5
+
+
+def test_sr_count_moving():
+    s = _scene()
+    prm = pipelines.color_sr_params(s)
+    with lfrv.Context(prm) as ctx:
+        pipelines.load_scene(ctx, s)
+        pipelines.color_sr_setup(ctx)
+        U = ctx.get_solution()
+        prev = U.copy()
+        rng = np.random.default_rng(0)
+        idx = rng.choice(s["H"] * s["W"], 100, replace=False)
+        prev.reshape(3, -1)[1, idx] += 2e-3
+        prev.reshape(3, -1)[0, idx[:10]] += 5e-4
+        assert ctx.sr_count_moving(prev, 1e-3) == 100

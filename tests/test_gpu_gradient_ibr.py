@@ -103,9 +103,7 @@ def test_end_to_end_matches_oracle(small_scene):
 
 def test_stateless_gather_splat_adjoint(small_scene):
     """<S u, r> == <u, S^T r> on the GPU's stateless operators (FP64 check of FP32 kernels)."""
-    import torch
     s = small_scene
-    dev = torch.device("cuda:0")
     W, H, w, h, dsf = s["W"], s["H"], s["w"], s["h"], s["dsf"]
     rng = np.random.default_rng(0)
     tx = (np.arange(w)[None, :] + 0.5 + rng.uniform(0.5, 3.0, (h, w))).astype(np.float32)
@@ -115,19 +113,38 @@ def test_stateless_gather_splat_adjoint(small_scene):
     # restrict the gather to cell-valid pixels, like the splat
     cv = np.vectorize(lambda a, b: oracle.cell_valid(a, b, W, H))(tx, ty)
     txg = np.where(cv, tx, -1.0).astype(np.float32)
-    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    dtx, dty, dtxg, du, dr = T(tx), T(ty), T(txg), T(u), T(r)
-    su = torch.zeros((3, h, w), device=dev)
-    st = torch.zeros((3, H, W), device=dev)
+    D = lfrv.DeviceArray
+    dtx, dty, dtxg, du, dr = D(tx), D(ty), D(txg), D(u), D(r)
+    su, st = D((3, h, w)), D((3, H, W))
     L = lfrv.lib()
-    assert L.lfrv_op_gather(W, H, w, h, dsf, dtxg.data_ptr(), dty.data_ptr(), du.data_ptr(), 3, su.data_ptr(), None) == 0
-    assert L.lfrv_op_splat_atomic(W, H, w, h, dsf, dtx.data_ptr(), dty.data_ptr(), dr.data_ptr(), 3, None,
-                                  st.data_ptr(), None, None) == 0
-    torch.cuda.synchronize()
-    lhs = float((su.double() * dr.double()).sum())
-    rhs = float((du.double() * st.double()).sum())
+    assert L.lfrv_op_gather(W, H, w, h, dsf, dtxg.ptr, dty.ptr, du.ptr, 3, su.ptr, None) == 0
+    assert L.lfrv_op_splat_atomic(W, H, w, h, dsf, dtx.ptr, dty.ptr, dr.ptr, 3, None, st.ptr, None, None) == 0
+    lfrv.device_sync()
+    su_h, st_h = su.numpy().astype(np.float64), st.numpy().astype(np.float64)
+    lhs = float((su_h * r.astype(np.float64)).sum())
+    rhs = float((u.astype(np.float64) * st_h).sum())
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
     osu = oracle.gather(W, H, dsf, txg, ty, u)
-    assert np.max(np.abs(su.cpu().numpy() - osu)) < 1e-5
+    assert np.max(np.abs(su_h - osu)) < 1e-5
     ost, _ = oracle.splat(W, H, dsf, tx, ty, r)
-    assert np.max(np.abs(st.cpu().numpy() - ost)) < 1e-4
+    assert np.max(np.abs(st_h - ost)) < 1e-4
+
+
+def test_sobel_gate_matches_oracle(small_scene):
+    s = small_scene
+    W, H = s["W"], s["H"]
+    rng = np.random.default_rng(1)
+    img = rng.uniform(0, 1, (3, H, W)).astype(np.float32)
+    vis = (rng.uniform(0, 1, (H, W)) > 0.05).astype(np.uint8)
+    prev = rng.standard_normal((2, H, W)).astype(np.float32)
+    D = lfrv.DeviceArray
+    dimg, dgx, dgy = D(img), D(prev[0]), D(prev[1])
+    dvis = lfrv.DeviceArray((H * W // 4 + 1,))
+    vbytes = np.zeros(dvis.nbytes, np.uint8)
+    vbytes[: H * W] = vis.ravel()
+    assert lfrv.lib().lfrv_memcpy(dvis.ptr, vbytes.ctypes.data, dvis.nbytes) == 0
+    assert lfrv.lib().lfrv_op_sobel(W, H, dimg.ptr, dgx.ptr, dgy.ptr, dvis.ptr, None) == 0
+    lfrv.device_sync()
+    ogx, ogy = oracle.sobel(img, vis, prev[0], prev[1])
+    assert np.max(np.abs(dgx.numpy() - ogx)) < 1e-6
+    assert np.max(np.abs(dgy.numpy() - ogy)) < 1e-6
