@@ -1,0 +1,112 @@
+"""CPU checks of the C-ABI library: it loads, exports every function the
+public headers declare, and its host-only entry points (defaults, PSNR,
+synthetic scenes, camera algebra) behave as documented.  No device compute
+is called here."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import lfrv
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        text = re.sub(r"typedef[^;]*;", "", text)
+        names |= set(re.findall(r"\b(lfrv_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_headers_declare_the_boundary():
+    names = declared_functions()
+    for must in ("lfrv_create", "lfrv_destroy", "lfrv_warps_from_depth", "lfrv_prepare", "lfrv_cg_solve",
+                 "lfrv_sr_iteration_fista", "lfrv_get_solution", "lfrv_last_error"):
+        assert must in names
+    assert len(names) >= 40
+
+
+def test_library_exports_every_declared_function():
+    L = lfrv.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_library_is_built_for_gfx950_only():
+    so = os.path.join(ROOT, "light-field-remote-vision_amd", "liblfrv.so")
+    blob = open(so, "rb").read()
+    assert b"gfx950" in blob
+    for other in (b"gfx942", b"gfx90a", b"gfx1100"):
+        assert other not in blob
+
+
+def test_default_params_follow_reference_defaults():
+    p = lfrv.Params()
+    lfrv.lib().lfrv_default_params(ctypes.byref(p), lfrv.MODE_GRADIENT_IBR, 9, 64, 48, 2)
+    assert (p.nviews, p.w, p.h, p.dsf) == (9, 64, 48, 2)
+    assert abs(p.sigma_sensor - 1.0 / 255.0) < 1e-9     # config.cpp:118-119 divides by 255
+    assert p.ugrad_threshold == 1.0
+    assert abs(p.gradient_step - 0.1) < 1e-7              # config.cpp default
+    assert p.dt_alpha == 1.0 and p.dt_beta == 0.0
+    assert p.occlusion_tol_px == 10.0 and p.sigma_z == 10.0
+
+
+def test_psnr_host_entry_matches_oracle():
+    rng = np.random.default_rng(5)
+    a = rng.random((3, 20, 30)).astype(np.float32)
+    b = (a + rng.normal(0, 0.02, a.shape)).astype(np.float32)
+    assert abs(lfrv.psnr(a, b) - oracle.psnr(a, b)) < 1e-9
+    assert lfrv.psnr(a, a) == 1e308
+
+
+@pytest.mark.parametrize("config", [0, 1, 2])
+def test_synth_presets_are_deterministic_and_well_formed(config):
+    s1 = lfrv.synth(config, seed=11, w=40, h=24)
+    s2 = lfrv.synth(config, seed=11, w=40, h=24)
+    s3 = lfrv.synth(config, seed=12, w=40, h=24)
+    for k in ("views", "depths", "P_views", "P_target", "target"):
+        assert np.array_equal(s1[k], s2[k])
+    assert not np.array_equal(s1["views"], s3["views"])
+    v = s1["views"]
+    assert v.dtype == np.float32 and np.isfinite(v).all()
+    assert v.min() >= -0.2 and v.max() <= 1.2
+    assert s1["W"] == s1["w"] * s1["dsf"] and s1["H"] == s1["h"] * s1["dsf"]
+
+
+def test_synth_config2_cameras_and_depths():
+    s = lfrv.synth(2, seed=7, w=96, h=64)
+    d = s["desc"]
+    z = s["depths"]
+    assert z.min() >= d.zmin - 1e-4 and z.max() <= d.zmax + 1e-4
+    centres = []
+    for P in s["P_views"]:
+        K, R, t, C = oracle.decompose_P(P)
+        np.testing.assert_allclose(R, np.eye(3), atol=1e-9)
+        assert abs(K[0, 0] - d.focal) < 1e-6 * d.focal and abs(K[1, 1] - d.focal) < 1e-6 * d.focal
+        assert abs(C[2]) < 1e-9
+        centres.append(C[:2])
+    centres = np.array(centres) / d.baseline
+    grid = sorted(map(tuple, np.round(centres).astype(int)))
+    assert grid == sorted((sx, sy) for sx in (-1, 0, 1) for sy in (-1, 0, 1))
+    np.testing.assert_allclose(centres, np.round(centres), atol=1e-9)
+    _, _, _, Ct = oracle.decompose_P(s["P_target"])
+    assert np.all(np.abs(Ct[:2]) <= d.baseline + 1e-12)
+
+
+def test_product_path_fails_loudly_without_a_device():
+    # no GPU in this container: creating a context must raise, never fall back
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU device node is present")
+    s = lfrv.synth(0, seed=1, w=32, h=32)
+    from lfrv import pipelines
+    with pytest.raises(lfrv.LfrvError):
+        lfrv.Context(pipelines.gradient_ibr_params(s))
