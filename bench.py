@@ -106,10 +106,14 @@ def main():
 
     ctx.set_timing(True)
     iters = []
+    per_step = []
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        ts = time.perf_counter()
         st = pipelines.gradient_ibr_solve(ctx, TOL, 5000)
+        ctx.sync()
+        per_step.append(1e3 * (time.perf_counter() - ts))
         iters.append(st["iterations"])
     ctx.sync()
     t1 = time.perf_counter()
@@ -125,9 +129,16 @@ def main():
     ty = ctx.get_buffer("TAU_Y")
     e_unique = cell_valid_count(tx, ty, scene["W"], scene["H"])
     e_binned, ntiles = ctx.bin_stats()
-    # compulsory bytes of one fused operator launch: r, Minv, d_prev, d, q planes
-    # (52 B per target pixel) + tau_x, tau_y, omega of every splatted source pixel (12 B)
-    op_bytes = 52 * N + 12 * e_unique
+    if scene["dsf"] == 1:
+        # assembled 9-point stencil operator: 5 coefficient planes (20 B), r (12),
+        # Minv (4), d_prev (12) read; d and q (24) written, per target pixel
+        op_kernel = "k_cg_op_st2 (fused PCG operator, assembled 9-point stencil)"
+        op_bytes = 72 * N
+    else:
+        # entry-walk operator: r, Minv, d_prev, d, q planes (52 B per target pixel)
+        # + tau_x, tau_y, omega of every splatted source pixel (12 B)
+        op_kernel = "k_cg_op (fused PCG operator, entry walk)"
+        op_bytes = 52 * N + 12 * e_unique
     op_avg_s = op_ms / max(op_n, 1) / 1e3
     achieved = op_bytes / op_avg_s / 1e9
     up_bytes = 76 * N
@@ -161,11 +172,14 @@ def main():
                                "RGB, 8 tilted planes at depth 2-10, 3x3 cameras baseline 0.05 f=1000",
                    "views": scene["V"], "width": scene["W"], "height": scene["H"], "tol": TOL,
                    "seed": args.seed, "tile": tile},
-        "roofline": {"bound": "hbm", "kernel": "k_cg_op (fused PCG operator)",
+        "roofline": {"bound": "hbm", "kernel": op_kernel,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": op_bytes, "avg_launch_us": round(op_avg_s * 1e6, 2),
                      "launches": op_n},
+        "ms_per_step_median": round(float(np.median(per_step)), 3),
+        "ms_per_step_p10": round(float(np.percentile(per_step, 10)), 3),
+        "ms_per_step_p90": round(float(np.percentile(per_step, 90)), 3),
         "cg_iterations": list(st_last["iterations"]),
         "true_rel_residual": [float("%.3g" % r) for r in st_last["true_rel_residual"]],
         "psnr_vs_ground_truth_db": round(psnr, 3),

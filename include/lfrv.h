@@ -71,7 +71,8 @@ typedef struct {
     float occlusion_tol_px;  /* round-trip tolerance of the depth visibility test (import.cpp:436-441: 10) */
     float sigma_z;           /* depth std-dev for partial tau (tauPartial.frag:25: 10) */
     int backproject_source_cam; /* 1: import.cpp:423-424 literal (view R,C); 0: target camera */
-    int tile_w, tile_h;      /* target tile of the binned splat kernels; 0 = default */
+    int tile_w, tile_h;      /* target tile of the splat (pull) kernels; 0 = default 32 x 8 */
+    int pull_lds_kb;         /* LDS budget of a pull's entry cache in KiB; 0 = default 48 */
 } lfrv_params;
 
 /* Fill *p with the reference defaults (config.cpp:11-64) for a problem of the

@@ -108,6 +108,7 @@ extern "C" void lfrv_default_params(lfrv_params *p, int mode, int nviews, int w,
     p->backproject_source_cam = 0;
     p->tile_w = 0;
     p->tile_h = 0;
+    p->pull_lds_kb = 0;
 }
 
 template <class T>
@@ -141,6 +142,11 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     ctx->N = (size_t)ctx->W * ctx->H;
     ctx->tile_w = p->tile_w > 0 ? p->tile_w : 32;
     ctx->tile_h = p->tile_h > 0 ? p->tile_h : 8;
+    if (p->pull_lds_kb < 0 || p->pull_lds_kb > 160) {
+        delete ctx;
+        return LFRV_E_ARG;
+    }
+    ctx->pull_budget = (p->pull_lds_kb > 0 ? p->pull_lds_kb : 48) * 1024;
     if (ctx->tile_w < ctx->ks || ctx->tile_h < ctx->ks || ctx->tile_w * ctx->tile_h > 1024 ||
         (ctx->tile_w + ctx->ks - 1) * (ctx->tile_h + ctx->ks - 1) > 1024) {
         delete ctx;
@@ -182,19 +188,22 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     A(&ctx->e_ty, ctx->e_cap);
     A(&ctx->e_w, ctx->e_cap);
     for (float **b : {&ctx->U, &ctx->Uq, &ctx->G, &ctx->PX, &ctx->PY, &ctx->U2, &ctx->P2X, &ctx->P2Y, &ctx->tmp3N,
-                      &ctx->cg_r, &ctx->cg_d, &ctx->cg_q, &ctx->cg_b})
+                      &ctx->cg_r, &ctx->cg_d, &ctx->cg_q, &ctx->cg_b, &ctx->cache_num})
         A(b, 3 * N);
     for (float **b : {&ctx->reg_g, &ctx->norm_mask, &ctx->coverage, &ctx->u_grad_x, &ctx->u_grad_y,
-                      &ctx->target_mask, &ctx->visibility, &ctx->target_depth, &ctx->tmpN, &ctx->cg_minv})
+                      &ctx->target_mask, &ctx->visibility, &ctx->target_depth, &ctx->tmpN, &ctx->cg_minv,
+                      &ctx->cache_sq})
         A(b, N);
     A(&ctx->hole_list, N);
+    if (ctx->ks == 2) A(&ctx->gram, 5 * N);
     ctx->pp_cap = 3 * (N + N / 2) + 3 * N + N / 4 + 64;
     A(&ctx->pp_scratch, ctx->pp_cap);
     A(&ctx->cg_state, 1);
     A(&ctx->counters, 32);
     A(&ctx->scal, 64);
     const size_t cam_doubles = (V * sizeof(WarpCam) + 7) / 8;
-    ctx->partials_cap = std::max<size_t>({(size_t)ctx->ntiles * 9, 2048 * 9, cam_doubles});
+    const size_t st2_tiles = (size_t)((ctx->W + 63) / 64) * ((ctx->H + 3) / 4);  // stencil operator tiles
+    ctx->partials_cap = std::max<size_t>({(size_t)ctx->ntiles * 9, st2_tiles * 9, 2048 * 9, cam_doubles});
     A(&ctx->partials, ctx->partials_cap);
     if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->cg_state_host, sizeof(CgState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->scal_host, 64 * sizeof(double), hipHostMallocDefault);
@@ -219,7 +228,8 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
              ctx->e_w, ctx->U, ctx->Uq, ctx->G, ctx->PX, ctx->PY, ctx->U2, ctx->P2X, ctx->P2Y, ctx->tmp3N, ctx->cg_r,
              ctx->cg_d, ctx->cg_q, ctx->cg_b, ctx->reg_g, ctx->norm_mask, ctx->coverage, ctx->u_grad_x,
              ctx->u_grad_y, ctx->target_mask, ctx->visibility, ctx->target_depth, ctx->tmpN, ctx->cg_minv,
-             ctx->hole_list, ctx->pp_scratch, ctx->cg_state, ctx->counters, ctx->scal, ctx->partials})
+             ctx->hole_list, ctx->pp_scratch, ctx->cg_state, ctx->counters, ctx->scal, ctx->partials,
+             ctx->cache_num, ctx->cache_sq, ctx->gram})
         if (p) hipFree(p);
     if (ctx->cg_state_host) hipHostFree(ctx->cg_state_host);
     if (ctx->scal_host) hipHostFree(ctx->scal_host);
@@ -253,6 +263,7 @@ extern "C" lfrv_status lfrv_set_view(lfrv_ctx *ctx, int i, const float *rgb)
     lfrv_status st = copy_in(ctx, ctx->views + (size_t)i * 3 * ctx->n, rgb, 3 * ctx->n);
     if (st != LFRV_OK) return st;
     ctx->prepared = false;  // view stencils are rebuilt by lfrv_prepare
+    ctx->epoch++;
     return LFRV_OK;
 }
 
@@ -262,6 +273,7 @@ extern "C" lfrv_status lfrv_set_tau(lfrv_ctx *ctx, int i, const float *tau_x, co
     lfrv_status st = copy_in(ctx, ctx->tau_x + (size_t)i * ctx->n, tau_x, ctx->n);
     if (st == LFRV_OK) st = copy_in(ctx, ctx->tau_y + (size_t)i * ctx->n, tau_y, ctx->n);
     ctx->prepared = false;
+    ctx->epoch++;
     return st;
 }
 
@@ -337,6 +349,7 @@ extern "C" lfrv_status lfrv_compute_weights(lfrv_ctx *ctx)
 extern "C" lfrv_status lfrv_compute_initial_image(lfrv_ctx *ctx, const float *u0)
 {
     if (!ctx) return LFRV_E_ARG;
+    ctx->u_zero = false;
     if (u0) return copy_in(ctx, ctx->U, u0, 3 * ctx->N);
     if (!ctx->vis_ready) return fail(ctx, LFRV_E_STATE, "lfrv_compute_initial_image before lfrv_setup_visibility");
     lfrv_status st = run_initial_image(ctx);
@@ -361,6 +374,7 @@ extern "C" lfrv_status lfrv_get_solution(lfrv_ctx *ctx, float *rgb)
 extern "C" lfrv_status lfrv_set_solution(lfrv_ctx *ctx, const float *rgb)
 {
     if (!ctx) return LFRV_E_ARG;
+    ctx->u_zero = false;
     return copy_in(ctx, ctx->U, rgb, 3 * ctx->N);
 }
 
@@ -375,6 +389,7 @@ extern "C" lfrv_status lfrv_clear_solution(lfrv_ctx *ctx)
     if (!ctx) return LFRV_E_ARG;
     k_zero<<<blocks_for(3 * ctx->N), kBlock, 0, ctx->stream>>>(3 * ctx->N, ctx->U);
     LFRV_LAUNCH(ctx, "clear solution");
+    ctx->u_zero = true;
     return LFRV_OK;
 }
 
@@ -387,6 +402,7 @@ extern "C" lfrv_status lfrv_init_u_gradient(lfrv_ctx *ctx)
 extern "C" lfrv_status lfrv_cg_solve(lfrv_ctx *ctx, double tol, int maxit, lfrv_cg_stats *st)
 {
     if (!ctx || !(tol > 0.0) || maxit < 0) return ctx ? fail(ctx, LFRV_E_ARG, "bad CG arguments") : LFRV_E_ARG;
+    ctx->u_zero = false;
     return run_cg(ctx, ctx->p.dt_alpha, 0.0, ctx->p.dt_beta, true, true, false, tol, maxit, st);
 }
 
@@ -394,6 +410,7 @@ extern "C" lfrv_status lfrv_poisson_cg(lfrv_ctx *ctx, double lambda, double tol,
                                        lfrv_cg_stats *st)
 {
     if (!ctx || !(tol > 0.0) || maxit < 0 || lambda < 0.0) return ctx ? fail(ctx, LFRV_E_ARG, "bad CG arguments") : LFRV_E_ARG;
+    ctx->u_zero = false;
     return run_cg(ctx, 0.0, lambda, 1.0, false, use_precond != 0, true, tol, maxit, st);
 }
 
@@ -413,6 +430,7 @@ extern "C" lfrv_status lfrv_sr_iteration_fista(lfrv_ctx *ctx)
 {
     if (!ctx) return LFRV_E_ARG;
     if (ctx->p.mode != LFRV_MODE_COLOR_SR) return fail(ctx, LFRV_E_STATE, "FISTA step needs LFRV_MODE_COLOR_SR");
+    ctx->u_zero = false;
     return run_fista(ctx);
 }
 
@@ -478,10 +496,12 @@ extern "C" lfrv_status lfrv_set_buffer(lfrv_ctx *ctx, int which, const float *in
     size_t count = 0;
     float *p = buffer_ptr(ctx, which, &count);
     if (!p) return fail(ctx, LFRV_E_ARG, "unknown buffer");
+    if (which == LFRV_BUF_U) ctx->u_zero = false;
     lfrv_status st = copy_in(ctx, p, in, count);
     if (st != LFRV_OK) return st;
     if (which == LFRV_BUF_TAU_X || which == LFRV_BUF_TAU_Y) ctx->prepared = false;
     if (which == LFRV_BUF_OMEGA) ctx->entry_weights_valid = false;
+    ctx->epoch++;  // any overwritten input invalidates the set-up sums
     if (which == LFRV_BUF_REG_G) ctx->has_reg_g = true;
     if (which == LFRV_BUF_VISIBILITY) ctx->vis_ready = true;
     return LFRV_OK;

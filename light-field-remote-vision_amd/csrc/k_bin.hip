@@ -21,24 +21,35 @@ __device__ __forceinline__ int cell_of(float t0, float t1, int Wc)
     return (y0 + KS - 1) * Wc + (x0 + KS - 1);
 }
 
+// Source index of thread t: the views are interleaved (t = m * V + v), so a
+// wave touches the same target neighbourhood from every view at once and the
+// atomics and scattered stores of the binning stay within few cache lines.
+__device__ __forceinline__ size_t interleaved_source(size_t t, int V, size_t n)
+{
+    const size_t m = t / (size_t)V, v = t - m * (size_t)V;
+    return v * n + m;
+}
+
 template <int KS>
-__global__ void k_cell_count(size_t tot, int W, int H, int Wc, const float *__restrict__ tx,
+__global__ void k_cell_count(size_t tot, int V, size_t n, int W, int H, int Wc, const float *__restrict__ tx,
                              const float *__restrict__ ty, int *__restrict__ count)
 {
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= tot) return;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tot) return;
+    const size_t k = interleaved_source(t, V, n);
     const float t0 = tx[k], t1 = ty[k];
     if (!cell_valid(t0, t1, W, H)) return;
     atomicAdd(count + cell_of<KS>(t0, t1, Wc), 1);
 }
 
 template <int KS>
-__global__ void k_cell_fill(size_t tot, int W, int H, int Wc, const float *__restrict__ tx,
+__global__ void k_cell_fill(size_t tot, int V, size_t n, int W, int H, int Wc, const float *__restrict__ tx,
                             const float *__restrict__ ty, const int *__restrict__ start,
                             int *__restrict__ cursor, int *__restrict__ e_src)
 {
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= tot) return;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tot) return;
+    const size_t k = interleaved_source(t, V, n);
     const float t0 = tx[k], t1 = ty[k];
     if (!cell_valid(t0, t1, W, H)) return;
     const int c = cell_of<KS>(t0, t1, Wc);
@@ -46,14 +57,50 @@ __global__ void k_cell_fill(size_t tot, int W, int H, int Wc, const float *__res
     e_src[pos] = (int)k;
 }
 
-// Order every cell's entries by source index (insertion sort; cells hold a
-// handful of entries) and gather the per-entry tau.
-__global__ void k_cell_sort(int ncells, const int *__restrict__ start, int *__restrict__ e_src,
-                            const float *__restrict__ tx, const float *__restrict__ ty,
-                            float *__restrict__ e_tx, float *__restrict__ e_ty)
+// Order every cell's entries by source index and gather their tau.  A block
+// owns kSortCells consecutive cells, whose entries are one contiguous run:
+// the run is staged in LDS with coalesced loads, each thread insertion-sorts
+// its cell's segment there (cells hold a handful of entries), and the sorted
+// run is written back coalesced.  Runs longer than kSortCap are sorted in
+// place in global memory.
+constexpr int kSortCells = 256;
+constexpr int kSortCap = 4096;
+
+__global__ void __launch_bounds__(kSortCells) k_cell_sort(int ncells, const int *__restrict__ start,
+                                                         int *__restrict__ e_src, const float *__restrict__ tx,
+                                                         const float *__restrict__ ty, float *__restrict__ e_tx,
+                                                         float *__restrict__ e_ty)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
+    __shared__ int keys[kSortCap];
+    const int c0 = blockIdx.x * kSortCells;
+    const int c1 = min(c0 + kSortCells, ncells);
+    const int base = start[c0], total = start[c1] - base;
+    const int c = c0 + (int)threadIdx.x;
+    if (total <= kSortCap) {
+        for (int i = threadIdx.x; i < total; i += blockDim.x) keys[i] = e_src[base + i];
+        __syncthreads();
+        if (c < c1) {
+            const int a = start[c] - base, b = start[c + 1] - base;
+            for (int i = a + 1; i < b; i++) {
+                const int key = keys[i];
+                int j = i - 1;
+                while (j >= a && keys[j] > key) {
+                    keys[j + 1] = keys[j];
+                    j--;
+                }
+                keys[j + 1] = key;
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < total; i += blockDim.x) {
+            const int k = keys[i];
+            e_src[base + i] = k;
+            e_tx[base + i] = tx[k];
+            e_ty[base + i] = ty[k];
+        }
+        return;
+    }
+    if (c >= c1) return;
     const int a = start[c], b = start[c + 1];
     for (int i = a + 1; i < b; i++) {
         const int key = e_src[i];
@@ -140,21 +187,22 @@ __global__ void k_refresh_weights(int64_t ne, const int *__restrict__ e_src,
 // ---- pull kernels -----------------------------------------------------------
 
 struct PullCfg {
-    int cap;  // LDS slots
+    int cap;  // staged entries (ks <= 3) or LDS slots (ks >= 4)
 };
 
-#define PULL_PROLOGUE(NV)                     \
-    extern __shared__ int smem[];             \
+#define PULL_PROLOGUE()               \
+    extern __shared__ int smem[];     \
     const int t = xcd_tile(blockIdx.x, g.ntiles);
+
+#define PULL_VALS(NV) [&](int e, int x0, int y0, const float(&wx)[KS], const float(&wy)[KS], float om, float(&v)[NV])
 
 // coverage = sum_i B_i^T 1
 template <int KS>
 __global__ void __launch_bounds__(1024) k_pull_coverage(TileGeom g, TileEntries te, PullCfg pc, float *__restrict__ coverage)
 {
-    PULL_PROLOGUE(1)
-    any_pull<KS, 1, false>(
-        g, te, t, smem, pc.cap,
-        [&](int, int, int, const float(&)[KS], const float(&)[KS], float(&v)[1]) { v[0] = 1.0f; },
+    PULL_PROLOGUE()
+    any_pull<KS, 1, 0>(
+        g, te, t, smem, pc.cap, PULL_VALS(1) { v[0] = 1.0f; },
         [&](size_t o, int, int, const float(&a)[1]) { coverage[o] = a[0]; });
 }
 
@@ -166,14 +214,13 @@ __global__ void __launch_bounds__(1024) k_pull_init_image(TileGeom g, TileEntrie
                                                          const float *__restrict__ views, float *__restrict__ U,
                                                          float *__restrict__ norm_mask)
 {
-    PULL_PROLOGUE(4)
+    PULL_PROLOGUE()
     const size_t N = (size_t)g.W * g.H;
-    any_pull<KS, 4, false>(
+    any_pull<KS, 4, 0>(
         g, te, t, smem, pc.cap,
-        [&](int e, int, int, const float(&)[KS], const float(&)[KS], float(&v)[4]) {
+        PULL_VALS(4) {
             const int k = te.src[e];
             const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
-            const float om = te.w[e];
             const float *vp = views + vi * 3 * n + m;
             v[0] = om * vp[0];
             v[1] = om * vp[n];
@@ -187,6 +234,63 @@ __global__ void __launch_bounds__(1024) k_pull_init_image(TileGeom g, TileEntrie
         });
 }
 
+// The gradient-IBR set-up in one pull: every sum the set-up needs from the
+// same entries, read once.
+//   initial image   U = sum B^T(omega v) / sum B^T omega, norm_mask = sum B^T omega
+//                   (gradientIBR.cu:764-811)
+//   target gradient g = sum B^T Sobel(Y v) / sum B^T 1, target mask
+//                   sigma^2 sum B^T omega / sum B^T 1 (gradientIBR.cu:858-913)
+//   and, kept for the CG right-hand side and preconditioner, the raw sums
+//   num = sum B^T(omega v) (3 planes) and sq = sum (B^2)^T omega.
+// sum B^T 1 is the coverage plane of lfrv_prepare (the same sums in the same
+// order as the u-gradient pull's own normaliser).
+template <int KS>
+__global__ void __launch_bounds__(1024) k_pull_gibr_setup(TileGeom g, TileEntries te, PullCfg pc, size_t n,
+                                                         const float *__restrict__ views,
+                                                         const float *__restrict__ vgrad,
+                                                         const float *__restrict__ coverage, float sigma2,
+                                                         float *__restrict__ U, float *__restrict__ norm_mask,
+                                                         float *__restrict__ ugx, float *__restrict__ ugy,
+                                                         float *__restrict__ tmask, float *__restrict__ num,
+                                                         float *__restrict__ sq)
+{
+    PULL_PROLOGUE()
+    const size_t N = (size_t)g.W * g.H;
+    any_pull<KS, 7, (1 << 6)>(
+        g, te, t, smem, pc.cap,
+        PULL_VALS(7) {
+            const int k = te.src[e];
+            const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
+            const float *vp = views + vi * 3 * n + m;
+            v[0] = om * vp[0];
+            v[1] = om * vp[n];
+            v[2] = om * vp[2 * n];
+            v[3] = om;
+            v[4] = vgrad[vi * 2 * n + m];
+            v[5] = vgrad[vi * 2 * n + n + m];
+            v[6] = om;
+        },
+        [&](size_t o, int, int, const float(&a)[7]) {
+            const float wsum = a[3];
+            for (int c = 0; c < 3; c++) {
+                U[c * N + o] = (wsum != 0.0f) ? a[c] / wsum : a[c];
+                num[c * N + o] = a[c];
+            }
+            norm_mask[o] = wsum;
+            const float nm = coverage[o];
+            float gx = a[4], gy = a[5], tm = a[3];
+            if (nm != 0.0f) {
+                gx /= nm;
+                gy /= nm;
+                tm /= nm;
+            }
+            ugx[o] = gx;
+            ugy[o] = gy;
+            tmask[o] = tm * sigma2;
+            sq[o] = a[6];
+        });
+}
+
 // Target gradient (init_u_gradient, gradientIBR.cu:858-913): pull (gx, gy,
 // omega, 1) with the footprint weights, normalise by the last plane.
 template <int KS>
@@ -195,15 +299,15 @@ __global__ void __launch_bounds__(1024) k_pull_u_gradient(TileGeom g, TileEntrie
                                                          float *__restrict__ ugx, float *__restrict__ ugy,
                                                          float *__restrict__ tmask, float *__restrict__ norm_mask)
 {
-    PULL_PROLOGUE(4)
-    any_pull<KS, 4, false>(
+    PULL_PROLOGUE()
+    any_pull<KS, 4, 0>(
         g, te, t, smem, pc.cap,
-        [&](int e, int, int, const float(&)[KS], const float(&)[KS], float(&v)[4]) {
+        PULL_VALS(4) {
             const int k = te.src[e];
             const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
             v[0] = vgrad[vi * 2 * n + m];
             v[1] = vgrad[vi * 2 * n + n + m];
-            v[2] = te.w[e];
+            v[2] = om;
             v[3] = 1.0f;
         },
         [&](size_t o, int, int, const float(&a)[4]) {
@@ -230,10 +334,9 @@ __global__ void __launch_bounds__(1024) k_pull_reg_weight(TileGeom g, TileEntrie
                                                          float lmax, float lmin, float nviews,
                                                          float *__restrict__ norm_mask, float *__restrict__ graw)
 {
-    PULL_PROLOGUE(1)
-    any_pull<KS, 1, false>(
-        g, te, t, smem, pc.cap,
-        [&](int e, int, int, const float(&)[KS], const float(&)[KS], float(&v)[1]) { v[0] = te.w[e]; },
+    PULL_PROLOGUE()
+    any_pull<KS, 1, 0>(
+        g, te, t, smem, pc.cap, PULL_VALS(1) { v[0] = om; },
         [&](size_t o, int, int, const float(&a)[1]) {
             const float nm = norm_mask[o] + coverage[o];
             norm_mask[o] = nm;
@@ -244,6 +347,23 @@ __global__ void __launch_bounds__(1024) k_pull_reg_weight(TileGeom g, TileEntrie
         });
 }
 
+// D^T g at pixel o (initConjGrad, solver.cu:384-467).
+__device__ __forceinline__ float dtg_at(const float *__restrict__ gx, const float *__restrict__ gy, size_t o, int x,
+                                        int y, int W, int H)
+{
+    float v = 0.0f;
+    if (x > 0) v += gx[o - 1];
+    if (x < W - 1) v -= gx[o];
+    if (y > 0) v += gy[o - W];
+    if (y < H - 1) v -= gy[o];
+    return v;
+}
+
+__device__ __forceinline__ int dtd_degree(int x, int y, int W, int H)
+{
+    return (W > 1 ? ((x == 0 || x == W - 1) ? 1 : 2) : 0) + (H > 1 ? ((y == 0 || y == H - 1) ? 1 : 2) : 0);
+}
+
 // CG right-hand side: b = alpha sum B^T (omega sigma^2 v) + c0 u0 + beta D^T g.
 template <int KS>
 __global__ void __launch_bounds__(1024) k_pull_rhs(TileGeom g, TileEntries te, PullCfg pc, size_t n,
@@ -252,28 +372,21 @@ __global__ void __launch_bounds__(1024) k_pull_rhs(TileGeom g, TileEntries te, P
                                                   const float *__restrict__ gx, const float *__restrict__ gy,
                                                   float *__restrict__ b)
 {
-    PULL_PROLOGUE(3)
+    PULL_PROLOGUE()
     const size_t N = (size_t)g.W * g.H;
-    any_pull<KS, 3, false>(
+    any_pull<KS, 3, 0>(
         g, te, t, smem, pc.cap,
-        [&](int e, int, int, const float(&)[KS], const float(&)[KS], float(&v)[3]) {
+        PULL_VALS(3) {
             const int k = te.src[e];
             const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
-            const float om = te.w[e] * wscale;
+            const float ow = om * wscale;
             const float *vp = views + vi * 3 * n + m;
-            v[0] = om * vp[0];
-            v[1] = om * vp[n];
-            v[2] = om * vp[2 * n];
+            v[0] = ow * vp[0];
+            v[1] = ow * vp[n];
+            v[2] = ow * vp[2 * n];
         },
         [&](size_t o, int x, int y, const float(&a)[3]) {
-            float dtg = 0.0f;
-            if (beta != 0.0f) {
-                if (x > 0) dtg += gx[o - 1];
-                if (x < g.W - 1) dtg -= gx[o];
-                if (y > 0) dtg += gy[o - g.W];
-                if (y < g.H - 1) dtg -= gy[o];
-                dtg *= beta;
-            }
+            const float dtg = beta != 0.0f ? beta * dtg_at(gx, gy, o, x, y, g.W, g.H) : 0.0f;
             for (int c = 0; c < 3; c++) b[c * N + o] = a[c] + (c0 != 0.0f ? c0 * u0[c * N + o] : 0.0f) + dtg;
         });
 }
@@ -283,19 +396,38 @@ template <int KS>
 __global__ void __launch_bounds__(1024) k_pull_diag(TileGeom g, TileEntries te, PullCfg pc, float wscale, float c0,
                                                    float beta, float *__restrict__ minv)
 {
-    PULL_PROLOGUE(1)
-    any_pull<KS, 1, true>(
-        g, te, t, smem, pc.cap,
-        [&](int e, int, int, const float(&)[KS], const float(&)[KS], float(&v)[1]) { v[0] = te.w[e] * wscale; },
+    PULL_PROLOGUE()
+    any_pull<KS, 1, 1>(
+        g, te, t, smem, pc.cap, PULL_VALS(1) { v[0] = om * wscale; },
         [&](size_t o, int x, int y, const float(&a)[1]) {
-            const int deg = (g.W > 1 ? ((x == 0 || x == g.W - 1) ? 1 : 2) : 0) +
-                            (g.H > 1 ? ((y == 0 || y == g.H - 1) ? 1 : 2) : 0);
-            const float d = a[0] + c0 + beta * (float)deg;
+            const float d = a[0] + c0 + beta * (float)dtd_degree(x, y, g.W, g.H);
             minv[o] = d > 0.0f ? 1.0f / d : 1.0f;
         });
 }
 
+// Right-hand side and preconditioner from the sums kept by the set-up pull:
+// b = wscale num + c0 u0 + beta D^T g, Minv = 1 / (wscale sq + c0 + beta deg).
+__global__ void k_rhs_diag_cached(int W, int H, float wscale, float c0, const float *__restrict__ u0, float beta,
+                                  const float *__restrict__ gx, const float *__restrict__ gy,
+                                  const float *__restrict__ num, const float *__restrict__ sq, int precond,
+                                  float *__restrict__ b, float *__restrict__ minv)
+{
+    const size_t N = (size_t)W * H;
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= N) return;
+    const int x = (int)(o % W), y = (int)(o / W);
+    const float dtg = beta != 0.0f ? beta * dtg_at(gx, gy, o, x, y, W, H) : 0.0f;
+    for (int c = 0; c < 3; c++) b[c * N + o] = wscale * num[c * N + o] + (c0 != 0.0f ? c0 * u0[c * N + o] : 0.0f) + dtg;
+    if (precond) {
+        const float d = sq[o] * wscale + c0 + beta * (float)dtd_degree(x, y, W, H);
+        minv[o] = d > 0.0f ? 1.0f / d : 1.0f;
+    } else {
+        minv[o] = 1.0f;
+    }
+}
+
 #undef PULL_PROLOGUE
+#undef PULL_VALS
 
 __global__ void k_fill(float *__restrict__ p, size_t N, float v)
 {
@@ -349,9 +481,16 @@ TileEntries tile_entries(const lfrv_ctx *ctx)
     return t;
 }
 
-// Largest slot count whose LDS fits `budget` bytes besides `extra` bytes.
+// Capacity of a pull's LDS cache within `budget` bytes besides `extra` bytes:
+// staged entries for the cell pull (ks <= 3), slots for the slot pull.
 int pull_cap(const lfrv_ctx *ctx, int nv, size_t extra, size_t budget)
 {
+    if (ctx->ks <= 3) {
+        const int wh = ctx->tile_h + ctx->ks - 1;
+        const long avail = (long)budget - (long)extra - (long)sizeof(int) * (2 * wh + 2);
+        const long ecap = avail / (long)(3 * sizeof(float));
+        return (int)std::max(0L, std::min(ecap, (long)kStagePer * ctx->pull_bs));
+    }
     const size_t meta = sizeof(int) * (size_t)pull_meta_ints(ctx->tile_w, ctx->tile_h, ctx->ks, ctx->pull_bs);
     const long avail = (long)budget - (long)meta - (long)extra;
     const long cap = avail / (long)(sizeof(float) * (nv + 2));
@@ -361,8 +500,10 @@ int pull_cap(const lfrv_ctx *ctx, int nv, size_t extra, size_t budget)
 size_t pull_lds(const lfrv_ctx *ctx, int nv, int cap)
 {
     if (ctx->ks <= 3) {
-        const size_t cells = (size_t)(ctx->tile_w + ctx->ks - 1) * (ctx->tile_h + ctx->ks - 1);
-        return sizeof(float) * (size_t)ctx->ks * ctx->ks * nv * cells;
+        const int wh = ctx->tile_h + ctx->ks - 1;
+        const size_t cells = (size_t)(ctx->tile_w + ctx->ks - 1) * wh;
+        const size_t part = (size_t)ctx->ks * ctx->ks * nv * cells;
+        return sizeof(int) * (size_t)(2 * wh + 2) + sizeof(float) * std::max(part, (size_t)3 * cap);
     }
     return pull_lds_bytes(ctx->tile_w, ctx->tile_h, ctx->ks, ctx->pull_bs, nv, cap);
 }
@@ -376,7 +517,6 @@ size_t pull_lds(const lfrv_ctx *ctx, int nv, int cap)
     default: return fail(ctx, LFRV_E_ARG, "dsf out of range"); \
     }
 
-static constexpr size_t kPullBudget = 64 * 1024;
 
 lfrv_status run_prepare(lfrv_ctx *ctx)
 {
@@ -386,18 +526,18 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
     TimedRegion tr(ctx, 5);
     LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_count, 0, sizeof(int) * (nc + 1), ctx->stream));
     LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_cursor, 0, sizeof(int) * nc, ctx->stream));
-#define CC(K) k_cell_count<K><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(tot, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_count)
+#define CC(K) k_cell_count<K><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(tot, ctx->V, ctx->n, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_count)
     LFRV_KS_DISPATCH(ctx->ks, CC);
 #undef CC
     const int nb = (nc + 256 * kScanPer - 1) / (256 * kScanPer);
     k_scan_blocks<<<nb, 256, 0, ctx->stream>>>(nc, ctx->cell_count, ctx->cell_start, ctx->scan_tmp);
     k_scan_sums<<<1, 1024, 0, ctx->stream>>>(nb, ctx->scan_tmp, ctx->cell_start + nc);
     k_scan_add<<<blocks_for(nc), kBlock, 0, ctx->stream>>>(nc, ctx->cell_start, ctx->scan_tmp);
-#define CF(K) k_cell_fill<K><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(tot, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_start, ctx->cell_cursor, ctx->e_src)
+#define CF(K) k_cell_fill<K><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(tot, ctx->V, ctx->n, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_start, ctx->cell_cursor, ctx->e_src)
     LFRV_KS_DISPATCH(ctx->ks, CF);
 #undef CF
-    k_cell_sort<<<blocks_for(nc), kBlock, 0, ctx->stream>>>(nc, ctx->cell_start, ctx->e_src, ctx->tau_x, ctx->tau_y,
-                                                           ctx->e_tx, ctx->e_ty);
+    k_cell_sort<<<(nc + kSortCells - 1) / kSortCells, kSortCells, 0, ctx->stream>>>(
+        nc, ctx->cell_start, ctx->e_src, ctx->tau_x, ctx->tau_y, ctx->e_tx, ctx->e_ty);
     LFRV_LAUNCH(ctx, "splat structure");
     int total = 0;
     LFRV_HIP(ctx, hipMemcpyAsync(&total, ctx->cell_start + nc, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -405,7 +545,7 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
     if ((size_t)total > ctx->e_cap) return fail(ctx, LFRV_E_ARG, "splat structure overflow");
     ctx->e_total = total;
     const TileEntries te = tile_entries(ctx);
-    PullCfg pc{pull_cap(ctx, 1, 0, kPullBudget)};
+    PullCfg pc{pull_cap(ctx, 1, 0, ctx->pull_budget)};
 #define COV(K) k_pull_coverage<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap), ctx->stream>>>(g, te, pc, ctx->coverage)
     LFRV_KS_DISPATCH(ctx->ks, COV);
 #undef COV
@@ -414,6 +554,7 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
     if (st != LFRV_OK) return st;
     ctx->prepared = true;
     ctx->entry_weights_valid = false;
+    ctx->epoch++;
     return LFRV_OK;
 }
 
@@ -446,8 +587,19 @@ lfrv_status run_initial_image(lfrv_ctx *ctx)
     if (st != LFRV_OK) return st;
     const TileGeom g = tile_geom(ctx);
     const TileEntries te = tile_entries(ctx);
-    PullCfg pc{pull_cap(ctx, 4, 0, kPullBudget)};
     TimedRegion tr(ctx, 5);
+    if (ctx->p.mode == LFRV_MODE_GRADIENT_IBR) {
+        // one pull for the initial image, the target gradient and the CG sums
+        PullCfg pc{pull_cap(ctx, 7, 0, ctx->pull_budget)};
+        const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
+#define SETUP(K) k_pull_gibr_setup<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 7, pc.cap), ctx->stream>>>(g, te, pc, ctx->n, ctx->views, ctx->view_grad, ctx->coverage, s2, ctx->U, ctx->norm_mask, ctx->u_grad_x, ctx->u_grad_y, ctx->target_mask, ctx->cache_num, ctx->cache_sq)
+        LFRV_KS_DISPATCH(ctx->ks, SETUP);
+#undef SETUP
+        LFRV_LAUNCH(ctx, "set-up pull");
+        ctx->setup_epoch = ctx->epoch;
+        return LFRV_OK;
+    }
+    PullCfg pc{pull_cap(ctx, 4, 0, ctx->pull_budget)};
 #define INIT(K) k_pull_init_image<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 4, pc.cap), ctx->stream>>>(g, te, pc, ctx->n, ctx->views, ctx->U, ctx->norm_mask)
     LFRV_KS_DISPATCH(ctx->ks, INIT);
 #undef INIT
@@ -455,14 +607,27 @@ lfrv_status run_initial_image(lfrv_ctx *ctx)
     return LFRV_OK;
 }
 
+// True when the set-up pull ran on the current entries, weights and views.
+static bool setup_cache_valid(const lfrv_ctx *ctx)
+{
+    return ctx->setup_epoch == ctx->epoch && ctx->entry_weights_valid && ctx->prepared;
+}
+
 lfrv_status run_u_gradient(lfrv_ctx *ctx)
 {
     if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "lfrv_init_u_gradient before lfrv_prepare");
     lfrv_status st = run_refresh_entry_weights(ctx, 1.0f);
     if (st != LFRV_OK) return st;
+    if (setup_cache_valid(ctx)) {
+        // g and the target mask came out of the set-up pull; the reference
+        // leaves sum B^T 1 in its normalisation buffer here
+        LFRV_HIP(ctx, hipMemcpyAsync(ctx->norm_mask, ctx->coverage, ctx->N * sizeof(float), hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+        return LFRV_OK;
+    }
     const TileGeom g = tile_geom(ctx);
     const TileEntries te = tile_entries(ctx);
-    PullCfg pc{pull_cap(ctx, 4, 0, kPullBudget)};
+    PullCfg pc{pull_cap(ctx, 4, 0, ctx->pull_budget)};
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
     TimedRegion tr(ctx, 5);
 #define UG(K) k_pull_u_gradient<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 4, pc.cap), ctx->stream>>>(g, te, pc, ctx->n, ctx->view_grad, s2, ctx->u_grad_x, ctx->u_grad_y, ctx->target_mask, ctx->norm_mask)
@@ -479,7 +644,7 @@ lfrv_status run_regularizer_weight(lfrv_ctx *ctx)
     if (st != LFRV_OK) return st;
     const TileGeom g = tile_geom(ctx);
     const TileEntries te = tile_entries(ctx);
-    PullCfg pc{pull_cap(ctx, 1, 0, kPullBudget)};
+    PullCfg pc{pull_cap(ctx, 1, 0, ctx->pull_budget)};
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
     const float lmax = ctx->p.lambda_max_factor * ctx->p.lambda, lmin = ctx->p.lambda;
     float *graw = ctx->tmpN;
@@ -502,14 +667,21 @@ lfrv_status run_rhs_and_diag(lfrv_ctx *ctx, double alpha, double c0, double beta
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
     const float wscale = use_views ? (float)(alpha * s2) : 0.0f;
     TimedRegion tr(ctx, 5);
+    if (use_views && setup_cache_valid(ctx)) {
+        k_rhs_diag_cached<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(
+            ctx->W, ctx->H, wscale, (float)c0, u0, (float)beta, ctx->u_grad_x, ctx->u_grad_y, ctx->cache_num,
+            ctx->cache_sq, use_precond ? 1 : 0, ctx->cg_b, ctx->cg_minv);
+        LFRV_LAUNCH(ctx, "rhs / diag");
+        return LFRV_OK;
+    }
     {
-        PullCfg pc{pull_cap(ctx, 3, 0, kPullBudget)};
+        PullCfg pc{pull_cap(ctx, 3, 0, ctx->pull_budget)};
 #define RHS(K) k_pull_rhs<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 3, pc.cap), ctx->stream>>>(g, te, pc, ctx->n, ctx->views, wscale, (float)c0, u0, (float)beta, ctx->u_grad_x, ctx->u_grad_y, ctx->cg_b)
         LFRV_KS_DISPATCH(ctx->ks, RHS);
 #undef RHS
     }
     if (use_precond) {
-        PullCfg pc{pull_cap(ctx, 1, 0, kPullBudget)};
+        PullCfg pc{pull_cap(ctx, 1, 0, ctx->pull_budget)};
 #define DG(K) k_pull_diag<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap), ctx->stream>>>(g, te, pc, wscale, (float)c0, (float)beta, ctx->cg_minv)
         LFRV_KS_DISPATCH(ctx->ks, DG);
 #undef DG

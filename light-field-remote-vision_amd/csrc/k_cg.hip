@@ -67,13 +67,13 @@ __global__ void __launch_bounds__(1024) k_cg_op(TileGeom g, TileEntries te, int 
         }
         for (int c = 0; c < 3; c++) dl[c * lp + i] = v[c];
     }
-    // (tile_pull synchronises before it reads the staged tile)
+    // (the pull synchronises before it reads the staged tile)
     TileEntries tea = te;
     tea.active = te.active && wscale != 0.0f;
     double dq[3] = {0.0, 0.0, 0.0};
-    any_pull<KS, 3, false>(
+    any_pull<KS, 3, 0>(
         g, tea, t, plds, cap,
-        [&](int e, int x0, int y0, const float(&wx)[KS], const float(&wy)[KS], float(&v)[3]) {
+        [&](int, int x0, int y0, const float(&wx)[KS], const float(&wy)[KS], float om_e, float(&v)[3]) {
             const int bx = x0 - tx0 + HAL, by = y0 - ty0 + HAL;
             float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(1024) k_cg_op(TileGeom g, TileEntries te, int 
                     s1 += a * dl[lp + li];
                     s2 += a * dl[2 * lp + li];
                 }
-            const float om = te.w[e] * wscale;
+            const float om = om_e * wscale;
             v[0] = om * s0;
             v[1] = om * s1;
             v[2] = om * s2;
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(1024) k_cg_op(TileGeom g, TileEntries te, int 
 }
 
 // alpha = r.z / d.q from the operator's partial sums (one block).
-__global__ void __launch_bounds__(256) k_cg_alpha(CgState *st, const double *__restrict__ partials, int G)
+__global__ void __launch_bounds__(kFinalizeThreads) k_cg_alpha(CgState *st, const double *__restrict__ partials, int G)
 {
     __shared__ double red[3 * 16];
     if (st->done) return;
@@ -140,6 +140,151 @@ __global__ void __launch_bounds__(256) k_cg_alpha(CgState *st, const double *__r
             }
         }
     }
+}
+
+// ---- dsf = 1: the data operator assembled into a 9-point stencil -----------
+//
+// With 2 x 2 footprints, sum_i S_i^T W_i S_i couples a pixel with its eight
+// neighbours only.  Its raw coefficients (without alpha sigma^2) are the Gram
+// sums sum_e omega_e a_e(p) a_e(p') over the entries whose footprint holds
+// both p and p'; five planes per pixel by symmetry: D (p, p), E (p, p+(1,0)),
+// S (p, p+(0,1)), SE (p, p+(1,1)), SW (p, p+(-1,1)).  They are fixed during a
+// solve, so the CG operator becomes a dense stencil pass.
+
+// Assembly: each window cell sums its 4 squared and 6 cross tap products,
+// each pixel then adds the cells that hold it (cell (x0, y0) has taps
+// TL = (x0, y0), TR, BL, BR).
+__global__ void __launch_bounds__(1024) k_gram2(TileGeom g, TileEntries te, int cap, float *__restrict__ gD,
+                                               float *__restrict__ gE, float *__restrict__ gS,
+                                               float *__restrict__ gSE, float *__restrict__ gSW)
+{
+    extern __shared__ int smem[];
+    const int t = xcd_tile(blockIdx.x, g.ntiles);
+    const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
+    const int ww = g.tw + 1, wh = g.th + 1, ncw = ww * wh;
+    float acc[10];
+    cell_accumulate<2, 10>(g, te, t, smem, cap, acc,
+                           [&](int, int, int, const float(&wx)[2], const float(&wy)[2], float om, float(&a)[10]) {
+                               const float tl = wy[0] * wx[0], tr = wy[0] * wx[1];
+                               const float bl = wy[1] * wx[0], br = wy[1] * wx[1];
+                               a[0] += om * tl * tl;
+                               a[1] += om * tr * tr;
+                               a[2] += om * bl * bl;
+                               a[3] += om * br * br;
+                               a[4] += om * tl * tr;  // E of TL
+                               a[5] += om * bl * br;  // E of BL
+                               a[6] += om * tl * bl;  // S of TL
+                               a[7] += om * tr * br;  // S of TR
+                               a[8] += om * tl * br;  // SE of TL
+                               a[9] += om * tr * bl;  // SW of TR
+                           });
+    float *part = reinterpret_cast<float *>(smem + 2 * wh + 2);
+    const int c = threadIdx.x;
+    if (c < ncw) {
+#pragma unroll
+        for (int k = 0; k < 10; k++) part[k * ncw + c] = acc[k];
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < g.tw * g.th; p += blockDim.x) {
+        const int lx = p % g.tw, ly = p / g.tw;
+        const int x = tx0 + lx, y = ty0 + ly;
+        if (x >= g.W || y >= g.H) continue;
+        // the cell whose top-left tap is (x - dx, y - dy) sits at local (lx + 1 - dx, ly + 1 - dy)
+        const int c00 = (ly + 1) * ww + lx + 1, c10 = c00 - 1, c01 = c00 - ww, c11 = c00 - ww - 1;
+        const size_t o = (size_t)y * g.W + x;
+        gD[o] = part[0 * ncw + c00] + part[1 * ncw + c10] + part[2 * ncw + c01] + part[3 * ncw + c11];
+        gE[o] = part[4 * ncw + c00] + part[5 * ncw + c01];
+        gS[o] = part[6 * ncw + c00] + part[7 * ncw + c10];
+        gSE[o] = part[8 * ncw + c00];
+        gSW[o] = part[9 * ncw + c10];
+    }
+}
+
+constexpr int kSt2W = 64, kSt2H = 4;  // tile of the stencil operator (256 threads)
+
+// q = A din (mode 0) or, fused with the direction update (mode 1),
+// d = Minv r + beta d_prev, q = A d, d.q partials;
+// A d = wscale * stencil(d) + c0 d + beta_reg D^T D d.
+__global__ void __launch_bounds__(256) k_cg_op_st2(int W, int H, int ntx, int ntiles, float wscale, float c0,
+                                                  float beta_reg, int mode, const float *__restrict__ din,
+                                                  const float *__restrict__ r, const float *__restrict__ minv,
+                                                  float *__restrict__ dout, float *__restrict__ q,
+                                                  const float *__restrict__ gD, const float *__restrict__ gE,
+                                                  const float *__restrict__ gS, const float *__restrict__ gSE,
+                                                  const float *__restrict__ gSW, const CgState *st,
+                                                  double *partials)
+{
+    constexpr int LW = kSt2W + 2, LH = kSt2H + 2, LP = LW * LH;
+    __shared__ float dl[3 * LP];
+    __shared__ double red[3 * 16];
+    if (mode == 1 && st->done) return;
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const int tx0 = (t % ntx) * kSt2W, ty0 = (t / ntx) * kSt2H;
+    const size_t N = (size_t)W * H;
+    float bc[3] = {0.0f, 0.0f, 0.0f};
+    int first = 1;
+    if (mode == 1) {
+        first = st->first;
+        for (int c = 0; c < 3; c++) bc[c] = (float)st->beta[c];
+    }
+    for (int i = threadIdx.x; i < LP; i += blockDim.x) {
+        const int lx = i % LW, ly = i / LW;
+        const int x = tx0 - 1 + lx, y = ty0 - 1 + ly;
+        const bool in = (x >= 0 && x < W && y >= 0 && y < H);
+        const size_t o = in ? (size_t)y * W + x : 0;
+        float v[3] = {0.0f, 0.0f, 0.0f};
+        if (in) {
+            if (mode == 0) {
+                for (int c = 0; c < 3; c++) v[c] = din[c * N + o];
+            } else {
+                const float mi = minv[o];
+                for (int c = 0; c < 3; c++) {
+                    const float z = mi * r[c * N + o];
+                    v[c] = first ? z : z + bc[c] * din[c * N + o];
+                }
+                if (lx >= 1 && lx <= kSt2W && ly >= 1 && ly <= kSt2H)
+                    for (int c = 0; c < 3; c++) dout[c * N + o] = v[c];
+            }
+        }
+        for (int c = 0; c < 3; c++) dl[c * LP + i] = v[c];
+    }
+    __syncthreads();
+    double dq[3] = {0.0, 0.0, 0.0};
+    const int lx = threadIdx.x % kSt2W, ly = threadIdx.x / kSt2W;
+    const int x = tx0 + lx, y = ty0 + ly;
+    if (x < W && y < H) {
+        const size_t o = (size_t)y * W + x;
+        const float cD = gD[o], cE = gE[o], cS = gS[o], cSE = gSE[o], cSW = gSW[o];
+        const float cEm = x > 0 ? gE[o - 1] : 0.0f;
+        const float cSm = y > 0 ? gS[o - W] : 0.0f;
+        const float cSEm = (x > 0 && y > 0) ? gSE[o - W - 1] : 0.0f;
+        const float cSWm = (x < W - 1 && y > 0) ? gSW[o - W + 1] : 0.0f;
+        const int li = (ly + 1) * LW + lx + 1;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float *dp = dl + c * LP;
+            const float dv = dp[li];
+            const float s = cD * dv + cE * dp[li + 1] + cEm * dp[li - 1] + cS * dp[li + LW] + cSm * dp[li - LW] +
+                            cSE * dp[li + LW + 1] + cSEm * dp[li - LW - 1] + cSW * dp[li + LW - 1] +
+                            cSWm * dp[li - LW + 1];
+            float lap = 0.0f;
+            if (W > 1) {
+                if (x == 0) lap += dv - dp[li + 1];
+                else if (x < W - 1) lap += 2.0f * dv - dp[li - 1] - dp[li + 1];
+                else lap += dv - dp[li - 1];
+            }
+            if (H > 1) {
+                if (y == 0) lap += dv - dp[li + LW];
+                else if (y < H - 1) lap += 2.0f * dv - dp[li - LW] - dp[li + LW];
+                else lap += dv - dp[li - LW];
+            }
+            const float qv = wscale * s + c0 * dv + beta_reg * lap;
+            q[c * N + o] = qv;
+            dq[c] += (double)dv * (double)qv;
+        }
+    }
+    if (mode == 0) return;
+    block_partials<3>(dq, partials, red);
 }
 
 // r = b - q (q = A x0), r.r, r.z, b.b; initialises the scalar state.
@@ -164,7 +309,7 @@ __global__ void __launch_bounds__(256) k_cg_init(size_t N, const float *__restri
 }
 
 // Scalar state of a new solve from the partial sums of k_cg_init (one block).
-__global__ void __launch_bounds__(256) k_cg_init_state(CgState *st, const double *__restrict__ partials, int G,
+__global__ void __launch_bounds__(kFinalizeThreads) k_cg_init_state(CgState *st, const double *__restrict__ partials, int G,
                                                        double tol2, int maxit)
 {
     __shared__ double red[9 * 16];
@@ -216,7 +361,7 @@ __global__ void __launch_bounds__(256) k_cg_update(size_t N, float *__restrict__
 }
 
 // Convergence test and beta = r.z_new / r.z_old (one block).
-__global__ void __launch_bounds__(256) k_cg_beta(CgState *st, const double *__restrict__ partials, int G)
+__global__ void __launch_bounds__(kFinalizeThreads) k_cg_beta(CgState *st, const double *__restrict__ partials, int G)
 {
     __shared__ double red[6 * 16];
     if (st->done) return;
@@ -346,7 +491,7 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     te.active = use_views ? 1 : 0;
     const int HAL = ctx->ks - 1;
     const size_t dtile = sizeof(float) * 3 * (size_t)(ctx->tile_w + 2 * HAL) * (ctx->tile_h + 2 * HAL);
-    const int cap = pull_cap(ctx, 3, dtile, 64 * 1024);
+    const int cap = pull_cap(ctx, 3, dtile, ctx->pull_budget);
     const size_t lds = dtile + pull_lds(ctx, 3, cap);
     const unsigned int rb = reduce_blocks(N);
     float *x = ctx->U, *r = ctx->cg_r, *q = ctx->cg_q, *minv = ctx->cg_minv;
@@ -354,9 +499,28 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     CgState *stt = ctx->cg_state;
     double *part = ctx->partials;
 
+    // dsf = 1 with views: assemble the 9-point stencil once per entries / weights
+    const bool st2 = ctx->ks == 2 && use_views && wscale != 0.0f && ctx->gram != nullptr;
+    const int st2_ntx = (ctx->W + kSt2W - 1) / kSt2W, st2_tiles = st2_ntx * ((ctx->H + kSt2H - 1) / kSt2H);
+    float *gD = ctx->gram, *gE = gD + N, *gS = gE + N, *gSE = gS + N, *gSW = gSE + N;
+    if (st2 && ctx->gram_epoch != ctx->epoch) {
+        TimedRegion tr(ctx, 5);
+        const int gcap = pull_cap(ctx, 3, 0, ctx->pull_budget);
+        const size_t glds = pull_lds(ctx, 3, gcap);  // 12 floats per window cell >= the 10 Gram sums
+        k_gram2<<<ctx->ntiles, ctx->pull_bs, glds, ctx->stream>>>(g, te, gcap, gD, gE, gS, gSE, gSW);
+        LFRV_LAUNCH(ctx, "stencil assembly");
+        ctx->gram_epoch = ctx->epoch;
+    }
+    const int op_blocks = st2 ? st2_tiles : ctx->ntiles;
+
 #define OPK(K, MODE, DIN, DOUT)                                                                               \
-    k_cg_op<K><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, wscale, (float)c0, (float)beta, MODE, DIN, r, \
-                                                      minv, DOUT, q, stt, part)
+    if (st2)                                                                                                  \
+        k_cg_op_st2<<<st2_tiles, 256, 0, ctx->stream>>>(ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, \
+                                                        (float)beta, MODE, DIN, r, minv, DOUT, q, gD, gE, gS,  \
+                                                        gSE, gSW, stt, part);                                 \
+    else                                                                                                      \
+        k_cg_op<K><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, wscale, (float)c0, (float)beta, \
+                                                                    MODE, DIN, r, minv, DOUT, q, stt, part)
 #define LAUNCH_OP(MODE, DIN, DOUT)                     \
     switch (ctx->ks) {                                 \
     case 2: OPK(2, MODE, DIN, DOUT); break;            \
@@ -371,7 +535,7 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         LAUNCH_OP(0, x, nullptr);
     }
     k_cg_init<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, r, minv, part);
-    k_cg_init_state<<<1, 256, 0, ctx->stream>>>(stt, part, (int)rb, tol * tol, maxit);
+    k_cg_init_state<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, (int)rb, tol * tol, maxit);
     LFRV_LAUNCH(ctx, "cg init");
     int it = 0, cur = 0;
     const int chunk = 8;
@@ -384,12 +548,12 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
                 TimedRegion tr(ctx, 0);
                 LAUNCH_OP(1, dprev, dnew);
             }
-            k_cg_alpha<<<1, 256, 0, ctx->stream>>>(stt, part, ctx->ntiles);
+            k_cg_alpha<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, op_blocks);
             {
                 TimedRegion tr(ctx, 1);
                 k_cg_update<<<rb, 256, 0, ctx->stream>>>(N, x, dnew, q, r, minv, stt, part);
             }
-            k_cg_beta<<<1, 256, 0, ctx->stream>>>(stt, part, (int)rb);
+            k_cg_beta<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, (int)rb);
             cur ^= 1;
         }
         LFRV_LAUNCH(ctx, "cg iteration");
@@ -411,7 +575,7 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     }
 #undef OPK
     k_resid<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, part);
-    k_reduce_out<3><<<1, 256, 0, ctx->stream>>>(part, (int)rb, ctx->scal);
+    k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, (int)rb, ctx->scal);
     LFRV_LAUNCH(ctx, "cg residual");
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -459,10 +623,10 @@ lfrv_status run_gibr_energy(lfrv_ctx *ctx, double *e)
     default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
     }
 #undef EN
-    k_reduce_out<1><<<1, 256, 0, ctx->stream>>>(ctx->partials, (int)rb1, ctx->scal + 4);
+    k_reduce_out<1><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->partials, (int)rb1, ctx->scal + 4);
     k_gibr_energy_grad<<<rb2, 256, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->u_grad_x, ctx->u_grad_y,
                                                      ctx->partials);
-    k_reduce_out<1><<<1, 256, 0, ctx->stream>>>(ctx->partials, (int)rb2, ctx->scal + 5);
+    k_reduce_out<1><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->partials, (int)rb2, ctx->scal + 5);
     LFRV_LAUNCH(ctx, "gibr energy");
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host + 4, ctx->scal + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -70,13 +70,12 @@ __global__ void __launch_bounds__(1024) k_sr_dataterm(TileGeom g, TileEntries te
             }
         }
     }
-    // (tile_pull synchronises before phase 1)
-    any_pull<KS, 6, false>(
+    // (the pull synchronises before it reads the staged tiles)
+    any_pull<KS, 6, 0>(
         g, te, t, plds, cap,
-        [&](int e, int x0, int y0, const float(&wx)[KS], const float(&wy)[KS], float(&r)[6]) {
+        [&](int e, int x0, int y0, const float(&wx)[KS], const float(&wy)[KS], float om, float(&r)[6]) {
             const int k = te.src[e];
             const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
-            const float om = te.w[e];
             const int bx = x0 - tx0 + HAL, by = y0 - ty0 + HAL;
             float su[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
 #pragma unroll
@@ -275,7 +274,7 @@ lfrv_status run_fista(lfrv_ctx *ctx)
     const size_t up = (size_t)(ctx->tile_w + 2 * HAL + 2) * (ctx->tile_h + 2 * HAL + 2);
     const size_t lpp = (size_t)(ctx->tile_w + 2 * HAL) * (ctx->tile_h + 2 * HAL);
     const size_t tiles = sizeof(float) * (3 * up + 3 * lpp);
-    const int cap = pull_cap(ctx, 6, tiles, 64 * 1024);
+    const int cap = pull_cap(ctx, 6, tiles, ctx->pull_budget);
     const size_t lds = tiles + pull_lds(ctx, 6, cap);
     {
         TimedRegion tr(ctx, 3);
@@ -307,7 +306,7 @@ lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts)
     float *ugx = ctx->tmp3N, *ugy = ctx->tmp3N + N;
     k_sr_energy_tv<<<rblocks(N), 256, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->visibility, ugx, ugy,
                                                         ctx->partials);
-    k_reduce_out<1><<<1, 256, 0, ctx->stream>>>(ctx->partials, (int)rblocks(N), ctx->scal + 8);
+    k_reduce_out<1><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->partials, (int)rblocks(N), ctx->scal + 8);
     const size_t tot = (size_t)ctx->V * ctx->n;
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
 #define EV(K) k_sr_energy_views<K><<<rblocks(tot), 256, 0, ctx->stream>>>(ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, ctx->omega, ctx->views, ctx->view_grad, ctx->U, ugx, ugy, s2, ctx->p.gw_type, ctx->partials)
@@ -319,7 +318,7 @@ lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts)
     default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
     }
 #undef EV
-    k_reduce_out<2><<<1, 256, 0, ctx->stream>>>(ctx->partials, (int)rblocks(tot), ctx->scal + 9);
+    k_reduce_out<2><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->partials, (int)rblocks(tot), ctx->scal + 9);
     LFRV_LAUNCH(ctx, "sr energy");
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host + 8, ctx->scal + 8, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -336,7 +335,7 @@ lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts)
 lfrv_status run_count_moving(lfrv_ctx *ctx, const float *prev_dev, float max_diff, int64_t *moving)
 {
     k_count_moving<<<rblocks(ctx->N), 256, 0, ctx->stream>>>(ctx->N, ctx->U, prev_dev, max_diff, ctx->partials);
-    k_reduce_out<1><<<1, 256, 0, ctx->stream>>>(ctx->partials, (int)rblocks(ctx->N), ctx->scal + 12);
+    k_reduce_out<1><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->partials, (int)rblocks(ctx->N), ctx->scal + 12);
     LFRV_LAUNCH(ctx, "count moving");
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host + 12, ctx->scal + 12, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(256) k_hole_sweep(int W, int H, int nholes, co
 }
 
 // Energy bookkeeping and stopping rule of one sweep (one block).
-__global__ void __launch_bounds__(256) k_hole_state(HoleState *st, const double *__restrict__ partials, int G)
+__global__ void __launch_bounds__(kFinalizeThreads) k_hole_state(HoleState *st, const double *__restrict__ partials, int G)
 {
     __shared__ double red[16];
     if (st->done) return;
@@ -332,7 +332,7 @@ lfrv_status run_hole_filling(lfrv_ctx *ctx, int iters, double eps, int *ran)
             const int cur = (launched + k) & 1;
             k_hole_sweep<<<nb, 256, 0, ctx->stream>>>(ctx->W, ctx->H, (int)nh, ctx->hole_list, bufs[cur],
                                                       bufs[cur ^ 1], dst, ctx->partials);
-            k_hole_state<<<1, 256, 0, ctx->stream>>>(dst, ctx->partials, (int)nb);
+            k_hole_state<<<1, kFinalizeThreads, 0, ctx->stream>>>(dst, ctx->partials, (int)nb);
         }
         launched += chunk;
         LFRV_LAUNCH(ctx, "hole filling");

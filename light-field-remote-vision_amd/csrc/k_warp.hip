@@ -259,11 +259,52 @@ __global__ void k_view_stencils(int V, int w, int h, const float *__restrict__ v
     }
 }
 
+// Sobel/8 of the luminance of every view (gradientIBR_kernels.cuh:57-99),
+// LDS-tiled: a 64 x 4 tile computes Y once per pixel of the tile and its
+// 1-pixel halo, then the 3 x 3 stencil reads LDS.  grid = (tiles x, tiles y, V).
+constexpr int kVsW = 64, kVsH = 4;
+
+__global__ void __launch_bounds__(kVsW * kVsH) k_view_sobel(int w, int h, const float *__restrict__ views,
+                                                           float *__restrict__ vgrad)
+{
+    constexpr int LW = kVsW + 2, LH = kVsH + 2;
+    __shared__ float Yl[LH * LW];
+    const size_t n = (size_t)w * h;
+    const int v = blockIdx.z;
+    const int x0 = blockIdx.x * kVsW, y0 = blockIdx.y * kVsH;
+    const float *r = views + (size_t)v * 3 * n, *g = r + n, *b = g + n;
+    for (int i = threadIdx.x; i < LH * LW; i += blockDim.x) {
+        const int x = x0 - 1 + i % LW, y = y0 - 1 + i / LW;
+        float Y = 0.0f;
+        if (x >= 0 && x < w && y >= 0 && y < h) {
+            const size_t m = (size_t)y * w + x;
+            Y = 0.2126f * r[m] + 0.7152f * g[m] + 0.0722f * b[m];
+        }
+        Yl[i] = Y;
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % kVsW, ly = threadIdx.x / kVsW;
+    const int ox = x0 + lx, oy = y0 + ly;
+    if (ox >= w || oy >= h) return;
+    float GX = 0.0f, GY = 0.0f;
+    if (0 < ox && ox < w - 1 && 0 < oy && oy < h - 1) {
+        const int c = (ly + 1) * LW + lx + 1;
+        const float Ypp = Yl[c - LW - 1], Ypc = Yl[c - LW], Ypn = Yl[c - LW + 1];
+        const float Ycp = Yl[c - 1], Ycn = Yl[c + 1];
+        const float Ynp = Yl[c + LW - 1], Ync = Yl[c + LW], Ynn = Yl[c + LW + 1];
+        GX = (Ypn + 2.0f * Ycn + Ynn - Ypp - 2.0f * Ycp - Ynp) * 0.125f;
+        GY = (Ynp + 2.0f * Ync + Ynn - Ypp - 2.0f * Ypc - Ypn) * 0.125f;
+    }
+    const size_t m = (size_t)oy * w + ox;
+    vgrad[(size_t)v * 2 * n + m] = GX;
+    vgrad[(size_t)v * 2 * n + n + m] = GY;
+}
+
 // omega = angular * deformation weight for every view pixel
 // (compute_weights, gradientIBR.cu:558-648; angular_weights and
 // gold_deform_weights, gradientIBR_kernels.cuh:102-194).  The warped target
 // gradient is gathered from (gxs, gys) at the footprint of tau.
-template <int KS>
+template <int KS, bool ZERO_GRAD>
 __global__ void k_weights(int V, int W, int H, int w, int h, const float *__restrict__ tx,
                           const float *__restrict__ ty, const float *__restrict__ gxs,
                           const float *__restrict__ gys, const float *__restrict__ dpx,
@@ -276,12 +317,13 @@ __global__ void k_weights(int V, int W, int H, int w, int h, const float *__rest
     const int v = (int)(k / n);
     const size_t m = k - (size_t)v * n;
     const float *vtx = tx + (size_t)v * n, *vty = ty + (size_t)v * n;
-    // warped u gradient (convolution_nonsep_param)
+    // warped u gradient (convolution_nonsep_param); identically zero, and not
+    // gathered, when the solution is the cleared image
     float gxv = 0.0f, gyv = 0.0f;
     const float t0 = vtx[m], t1 = vty[m];
     int x0, y0;
     float wx[KS], wy[KS];
-    if (gather_valid(t0, t1, W, H) && footprint<KS>(t0, t1, W, H, x0, y0, wx, wy)) {
+    if (!ZERO_GRAD && gather_valid(t0, t1, W, H) && footprint<KS>(t0, t1, W, H, x0, y0, wx, wy)) {
 #pragma unroll
         for (int i = 0; i < KS; i++) {
             const int y = y0 + i;
@@ -302,7 +344,7 @@ __global__ void k_weights(int V, int W, int H, int w, int h, const float *__rest
         gxv = gxv / nrm * thr;
         gyv = gyv / nrm * thr;
     }
-    float sg = gxv * dpx[k] + gyv * dpy[k];
+    float sg = ZERO_GRAD ? 0.0f : gxv * dpx[k] + gyv * dpy[k];
     if (sg > 0.5f) sg = 0.5f;
     const float ang = 1.0f / (sg * sg + sigma * sigma);
     // deformation weight
@@ -414,16 +456,22 @@ lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
         ctx->tau_x, ctx->tau_y);
     LFRV_LAUNCH(ctx, "warps_from_depth kernels");
     ctx->prepared = false;
+    ctx->epoch++;
     return LFRV_OK;
 }
 
 template <int KS>
-static void launch_weights_ks(lfrv_ctx *ctx, const float *gxs, const float *gys)
+static void launch_weights_ks(lfrv_ctx *ctx, const float *gxs, const float *gys, bool zero_grad)
 {
     const size_t tot = (size_t)ctx->V * ctx->n;
-    k_weights<KS><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(
-        ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, gxs, gys, ctx->dpart_x,
-        ctx->dpart_y, ctx->p.sigma_sensor, ctx->p.ugrad_threshold, ctx->omega);
+    if (zero_grad)
+        k_weights<KS, true><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(
+            ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, gxs, gys, ctx->dpart_x,
+            ctx->dpart_y, ctx->p.sigma_sensor, ctx->p.ugrad_threshold, ctx->omega);
+    else
+        k_weights<KS, false><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(
+            ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, gxs, gys, ctx->dpart_x,
+            ctx->dpart_y, ctx->p.sigma_sensor, ctx->p.ugrad_threshold, ctx->omega);
 }
 
 lfrv_status run_compute_weights(lfrv_ctx *ctx)
@@ -448,20 +496,30 @@ lfrv_status run_compute_weights(lfrv_ctx *ctx)
         gxs = ctx->u_grad_x;
         gys = ctx->u_grad_y;
     }
+    // the ungated Sobel of the cleared image is zero everywhere, so is its warp
+    const bool zero_grad = ctx->p.mode == LFRV_MODE_GRADIENT_IBR && ctx->u_zero;
     switch (ctx->ks) {
-    case 2: launch_weights_ks<2>(ctx, gxs, gys); break;
-    case 3: launch_weights_ks<3>(ctx, gxs, gys); break;
-    case 4: launch_weights_ks<4>(ctx, gxs, gys); break;
-    case 5: launch_weights_ks<5>(ctx, gxs, gys); break;
+    case 2: launch_weights_ks<2>(ctx, gxs, gys, zero_grad); break;
+    case 3: launch_weights_ks<3>(ctx, gxs, gys, zero_grad); break;
+    case 4: launch_weights_ks<4>(ctx, gxs, gys, zero_grad); break;
+    case 5: launch_weights_ks<5>(ctx, gxs, gys, zero_grad); break;
     default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
     }
     LFRV_LAUNCH(ctx, "compute_weights kernels");
     ctx->entry_weights_valid = false;
+    ctx->epoch++;
     return LFRV_OK;
 }
 
 lfrv_status run_view_stencils(lfrv_ctx *ctx)
 {
+    if (ctx->p.mode == LFRV_MODE_GRADIENT_IBR) {
+        // IBR_gradient needs the views' Sobel only (init_u_gradient)
+        dim3 gs((ctx->w + kVsW - 1) / kVsW, (ctx->h + kVsH - 1) / kVsH, ctx->V);
+        k_view_sobel<<<gs, kVsW * kVsH, 0, ctx->stream>>>(ctx->w, ctx->h, ctx->views, ctx->view_grad);
+        LFRV_LAUNCH(ctx, "view stencils");
+        return LFRV_OK;
+    }
     const size_t tot = (size_t)ctx->V * ctx->n;
     k_view_stencils<<<blocks_for(tot), kBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->views,
                                                                  ctx->view_grad, ctx->view_lap);

@@ -135,23 +135,35 @@ __device__ __forceinline__ void block_partials(double (&v)[NQ], double *partials
 }
 
 // Deterministic sum of G block partials (layout [q][block]) by one block;
-// the result is valid in thread 0.
+// the result is valid in thread 0.  Four independent loads per thread and
+// value are in flight at a time (the partials come from other XCDs' L2s, so
+// a dependent load chain would pay the full memory latency per element).
 template <int NQ>
 __device__ __forceinline__ void reduce_partials(const double *partials, int G, double (&out)[NQ], double *lds)
 {
+    const int bs = blockDim.x;
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-        double a = 0.0;
         const double *pq = partials + (size_t)q * G;
-        for (int b = threadIdx.x; b < G; b += blockDim.x) a += pq[b];
-        out[q] = a;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int b = threadIdx.x;
+        for (; b + 3 * bs < G; b += 4 * bs) {
+            a0 += pq[b];
+            a1 += pq[b + bs];
+            a2 += pq[b + 2 * bs];
+            a3 += pq[b + 3 * bs];
+        }
+        for (; b < G; b += bs) a0 += pq[b];
+        out[q] = (a0 + a1) + (a2 + a3);
     }
     block_sum<NQ>(out, lds);
 }
 
-// Finalize kernel of the plain sums: one block of 256 threads.
+constexpr int kFinalizeThreads = 1024;  // threads of the one-block finalize kernels
+
+// Finalize kernel of the plain sums.
 template <int NQ>
-__global__ void __launch_bounds__(256) k_reduce_out(const double *__restrict__ partials, int G, double *out)
+__global__ void __launch_bounds__(kFinalizeThreads) k_reduce_out(const double *__restrict__ partials, int G, double *out)
 {
     __shared__ double red[NQ * 16];
     double v[NQ];
@@ -226,6 +238,12 @@ struct lfrv_ctx {
     int64_t e_total = 0;
     bool prepared = false;
     bool entry_weights_valid = false;
+    uint64_t epoch = 1;        // bumped whenever entries, weights or views change
+    uint64_t setup_epoch = 0;  // epoch of the last gradient-IBR set-up pull
+    float *cache_num = nullptr, *cache_sq = nullptr;  // its raw sums: 3N and N
+    float *gram = nullptr;     // dsf = 1: 5N raw stencil coefficients of sum_i S_i^T omega_i S_i
+    uint64_t gram_epoch = 0;
+    int pull_budget = 48 * 1024;  // LDS bytes of a pull's entry cache
 
     // target side (3 planes each unless noted)
     float *U = nullptr, *Uq = nullptr, *G = nullptr;
@@ -258,6 +276,10 @@ struct lfrv_ctx {
     float *pp_scratch = nullptr;   // push-pull pyramid scratch
     size_t pp_cap = 0;             // floats
     bool vis_ready = false;
+
+    // U holds the cleared (all-zero) image: set by lfrv_clear_solution, reset by
+    // every entry point that writes U
+    bool u_zero = false;
 
     // PD ping-pong orientation: which buffers currently hold Uq / p
     bool pd_swapped = false;

@@ -77,19 +77,30 @@ __device__ __forceinline__ void block_exclusive_scan(int *a, int n, int *tmp /* 
     __syncthreads();
 }
 
-// LDS words the pull needs besides the slots: lstart (ncw + 1), row bases
-// (wh), scan scratch (blockDim).
+// LDS words the slot pull needs besides the slots: lstart (ncw + 1), row
+// bases (wh), scan scratch (blockDim).
 __host__ __device__ __forceinline__ int pull_window_w(int tw, int ks) { return tw + ks - 1; }
 __host__ __device__ __forceinline__ int pull_meta_ints(int tw, int th, int ks, int bs)
 {
     return (tw + ks - 1) * (th + ks - 1) + 1 + (th + ks - 1) + bs;
 }
 
-// The pull.  `vals(e, x0, y0, wx, wy, v)` produces the NV values of entry e;
-// `epi(o, x, y, acc)` receives the pulled sums of pixel (x, y).  With
-// SQUARED the footprint weight enters squared (a^2 v, the Jacobi diagonal).
-// Slots beyond `cap` are not cached: phase 2 recomputes their values.
-template <int KS, int NV, bool SQUARED, class Vals, class Epi>
+// acc += a v, with a squared for the values whose bit is set in SQ (the
+// Jacobi diagonal sum_e a_e^2 w_e).
+template <int NV, int SQ>
+__device__ __forceinline__ void tap_accumulate(float (&acc)[NV], float a, const float (&v)[NV])
+{
+#pragma unroll
+    for (int q = 0; q < NV; q++) acc[q] += (((SQ >> q) & 1) ? a * a : a) * v[q];
+}
+
+// Slot pull (ks >= 4: few entries per cell).  `vals(e, x0, y0, wx, wy, om, v)`
+// produces the NV values of entry e (om = its omega); `epi(o, x, y, acc)`
+// receives the pulled sums of pixel (x, y).  Phase 1 stages the tile's entries
+// and their values in LDS slots (coalesced over the window rows); phase 2 lets
+// each thread pull its pixel's cells.  Slots beyond `cap` are not cached:
+// phase 2 recomputes their values.
+template <int KS, int NV, int SQ, class Vals, class Epi>
 __device__ __forceinline__ void tile_pull(const TileGeom &g, const TileEntries &te, int t, int *meta,
                                           float *slots, int cap, Vals vals, Epi epi)
 {
@@ -100,6 +111,7 @@ __device__ __forceinline__ void tile_pull(const TileGeom &g, const TileEntries &
     int *scan_tmp = grow + wh;       // blockDim
     constexpr int SW = NV + 2;       // slot: tx, ty, values
 
+    __syncthreads();  // the caller's staged tiles are complete
     // phase 0: window cell counts -> local starts
     for (int c = threadIdx.x; c < ncw; c += blockDim.x) {
         const int r = c / ww, col = c - r * ww;
@@ -130,7 +142,7 @@ __device__ __forceinline__ void tile_pull(const TileGeom &g, const TileEntries &
         float wx[KS], wy[KS];
         float v[NV];
         if (footprint<KS>(t0, t1, g.W, g.H, x0, y0, wx, wy)) {
-            vals(e, x0, y0, wx, wy, v);
+            vals(e, x0, y0, wx, wy, te.w[e], v);
         } else {
 #pragma unroll
             for (int q = 0; q < NV; q++) v[q] = 0.0f;
@@ -176,7 +188,7 @@ __device__ __forceinline__ void tile_pull(const TileGeom &g, const TileEntries &
                         int xx0, yy0;
                         float wx2[KS], wy2[KS];
                         if (footprint<KS>(t0, t1, g.W, g.H, xx0, yy0, wx2, wy2)) {
-                            vals(e, xx0, yy0, wx2, wy2, v);
+                            vals(e, xx0, yy0, wx2, wy2, te.w[e], v);
                         } else {
 #pragma unroll
                             for (int q = 0; q < NV; q++) v[q] = 0.0f;
@@ -185,10 +197,7 @@ __device__ __forceinline__ void tile_pull(const TileGeom &g, const TileEntries &
                     int x0, y0;
                     float wx[KS], wy[KS];
                     if (!footprint<KS>(t0, t1, g.W, g.H, x0, y0, wx, wy)) continue;
-                    float a2 = wx[j] * wy[i];
-                    if (SQUARED) a2 *= a2;
-#pragma unroll
-                    for (int q = 0; q < NV; q++) acc[q] += a2 * v[q];
+                    tap_accumulate<NV, SQ>(acc, wx[j] * wy[i], v);
                 }
             }
         }
@@ -196,100 +205,194 @@ __device__ __forceinline__ void tile_pull(const TileGeom &g, const TileEntries &
     }
 }
 
-// Cell-owner pull for small footprints (ks <= 3).  Thread c owns window cell
-// c: it walks the cell's entries, forms their values and accumulates the
-// per-tap sums sum_e a_e(i,j) v_e in registers, then publishes them to LDS
-// part[(i*KS+j)*NV + q][cell]; each pixel finally adds the KS*KS tap sums of
-// the cells that cover it.  LDS: KS*KS*NV*ncw floats + ncw+1 ints.
-template <int KS>
-__host__ __device__ __forceinline__ int cell_pull_cells(int tw, int th)
+// Cell-owner pull for small footprints (ks <= 3).
+//
+// Stage: the window's entries (tau_x, tau_y, omega) are copied into LDS with
+// coalesced loads -- the cells of one window row are contiguous in the CSR,
+// so the window is wh contiguous runs.  Walk: thread c owns window cell c,
+// walks the cell's staged entries, forms their values and accumulates the
+// per-tap sums sum_e a_e(i,j) v_e in registers.  Publish: after a barrier
+// the same LDS holds part[(i*KS+j)*NV + q][cell]; each pixel finally adds the
+// KS*KS tap sums of the cells that cover it.  A window holding more than
+// `ecap` entries walks them from global memory instead (same order, same
+// sums).  Requires blockDim.x >= ncw.
+constexpr int kStagePer = 16;  // staged entries per thread at most
+
+template <int KS, int NV>
+__host__ __device__ __forceinline__ size_t cell_pull_region_floats(int tw, int th, int ecap)
 {
-    return (tw + KS - 1) * (th + KS - 1);
+    const size_t part = (size_t)KS * KS * NV * (tw + KS - 1) * (th + KS - 1);
+    const size_t stage = 3 * (size_t)ecap;
+    return part > stage ? part : stage;
 }
 
-template <int KS, int NV, bool SQUARED, class Vals, class Epi>
-__device__ __forceinline__ void cell_pull(const TileGeom &g, const TileEntries &te, int t, float *part, Vals vals,
-                                          Epi epi)
+// Stage + walk of the cell pull: returns, in acc, the sums that `entry_fn(e,
+// x0, y0, wx, wy, om, acc)` accumulates over the entries of the window cell
+// owned by this thread (threadIdx.x < ncw; other threads get zeros).  On
+// return every thread has finished reading the staged entries, so the region
+// may be overwritten.  Requires blockDim.x >= ncw and the caller's LDS tiles
+// complete (the first barrier below orders them).
+template <int KS, int NC, class EntryFn>
+__device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEntries &te, int t, int *meta, int ecap,
+                                                float (&acc)[NC], EntryFn entry_fn)
 {
     const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
     const int ww = g.tw + KS - 1, wh = g.th + KS - 1, ncw = ww * wh;
-    __syncthreads();  // the caller's staged tiles are complete
-    for (int c = threadIdx.x; c < ncw; c += blockDim.x) {
-        const int r = c / ww, col = c - r * ww;
-        const int cx = tx0 + col, cy = ty0 + r;
-        float acc[KS * KS][NV];
+    int *rbeg = meta;       // wh: CSR index of the row's first entry
+    int *roff = meta + wh;  // wh + 1: staged offset of the row
+    float *region = reinterpret_cast<float *>(meta + 2 * wh + 2);
+    float *stx = region, *sty = region + ecap, *sw = region + 2 * ecap;
+
+    __syncthreads();
+    if ((int)threadIdx.x < wh) {
+        const int r = threadIdx.x, cy = ty0 + r;
+        int b = 0, cnt = 0;
+        if (te.active && cy < g.Hc) {
+            const int c0 = cy * g.Wc + tx0;
+            const int c1 = cy * g.Wc + min(tx0 + ww, g.Wc);
+            b = te.cell_start[c0];
+            cnt = te.cell_start[c1] - b;
+        }
+        rbeg[r] = b;
+        roff[r] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int r = 0; r < wh; r++) {
+            const int c = roff[r];
+            roff[r] = s;
+            s += c;
+        }
+        roff[wh] = s;
+    }
+    __syncthreads();
+    const int total = roff[wh];
+    const bool staged = total <= ecap;
+    if (staged) {
+        float a[kStagePer], b[kStagePer], w[kStagePer];
+        int r = 0;
 #pragma unroll
-        for (int k = 0; k < KS * KS; k++)
-#pragma unroll
-            for (int q = 0; q < NV; q++) acc[k][q] = 0.0f;
-        if (te.active && cx < g.Wc && cy < g.Hc) {
-            const int gc = cy * g.Wc + cx;
-            const int e0 = te.cell_start[gc], e1 = te.cell_start[gc + 1];
-            for (int e = e0; e < e1; e++) {
-                int x0, y0;
-                float wx[KS], wy[KS];
-                if (!footprint<KS>(te.tx[e], te.ty[e], g.W, g.H, x0, y0, wx, wy)) continue;
-                float v[NV];
-                vals(e, x0, y0, wx, wy, v);
-#pragma unroll
-                for (int i = 0; i < KS; i++)
-#pragma unroll
-                    for (int j = 0; j < KS; j++) {
-                        float a = wy[i] * wx[j];
-                        if (SQUARED) a *= a;
-#pragma unroll
-                        for (int q = 0; q < NV; q++) acc[i * KS + j][q] += a * v[q];
-                    }
+        for (int k = 0; k < kStagePer; k++) {
+            const int i = threadIdx.x + k * blockDim.x;
+            if (i < total) {
+                while (roff[r + 1] <= i) r++;
+                const int e = rbeg[r] + (i - roff[r]);
+                a[k] = te.tx[e];
+                b[k] = te.ty[e];
+                w[k] = te.w[e];
             }
         }
 #pragma unroll
-        for (int k = 0; k < KS * KS; k++)
+        for (int k = 0; k < kStagePer; k++) {
+            const int i = threadIdx.x + k * blockDim.x;
+            if (i < total) {
+                stx[i] = a[k];
+                sty[i] = b[k];
+                sw[i] = w[k];
+            }
+        }
+    }
+    __syncthreads();
+
 #pragma unroll
-            for (int q = 0; q < NV; q++) part[(k * NV + q) * ncw + c] = acc[k][q];
+    for (int k = 0; k < NC; k++) acc[k] = 0.0f;
+    const int c = threadIdx.x;
+    if (c < ncw) {
+        const int r = c / ww, col = c - r * ww;
+        const int cx = tx0 + col, cy = ty0 + r;
+        if (te.active && cx < g.Wc && cy < g.Hc) {
+            const int gc = cy * g.Wc + cx;
+            const int e0 = te.cell_start[gc], e1 = te.cell_start[gc + 1];
+            const int lbase = roff[r] - rbeg[r];
+            for (int e = e0; e < e1; e++) {
+                float t0, t1, om;
+                if (staged) {
+                    const int li = lbase + e;
+                    t0 = stx[li];
+                    t1 = sty[li];
+                    om = sw[li];
+                } else {
+                    t0 = te.tx[e];
+                    t1 = te.ty[e];
+                    om = te.w[e];
+                }
+                int x0, y0;
+                float wx[KS], wy[KS];
+                if (!footprint<KS>(t0, t1, g.W, g.H, x0, y0, wx, wy)) continue;
+                entry_fn(e, x0, y0, wx, wy, om, acc);
+            }
+        }
+    }
+    __syncthreads();  // every walk is done with the staged entries
+}
+
+template <int KS, int NV, int SQ, class Vals, class Epi>
+__device__ __forceinline__ void cell_pull(const TileGeom &g, const TileEntries &te, int t, int *meta,
+                                          int ecap, Vals vals, Epi epi)
+{
+    const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
+    const int ww = g.tw + KS - 1, wh = g.th + KS - 1, ncw = ww * wh;
+    float acc[KS * KS * NV];
+    cell_accumulate<KS, KS * KS * NV>(
+        g, te, t, meta, ecap, acc,
+        [&](int e, int x0, int y0, const float(&wx)[KS], const float(&wy)[KS], float om,
+            float(&a)[KS * KS * NV]) {
+            float v[NV];
+            vals(e, x0, y0, wx, wy, om, v);
+#pragma unroll
+            for (int i = 0; i < KS; i++)
+#pragma unroll
+                for (int j = 0; j < KS; j++) {
+                    const float w = wy[i] * wx[j];
+#pragma unroll
+                    for (int q = 0; q < NV; q++) a[(i * KS + j) * NV + q] += (((SQ >> q) & 1) ? w * w : w) * v[q];
+                }
+        });
+    float *part = reinterpret_cast<float *>(meta + 2 * wh + 2);
+    const int c = threadIdx.x;
+    if (c < ncw) {
+#pragma unroll
+        for (int k = 0; k < KS * KS * NV; k++) part[k * ncw + c] = acc[k];
     }
     __syncthreads();
     for (int p = threadIdx.x; p < g.tw * g.th; p += blockDim.x) {
         const int lx = p % g.tw, ly = p / g.tw;
         const int x = tx0 + lx, y = ty0 + ly;
         if (x >= g.W || y >= g.H) continue;
-        float acc[NV];
+        float sum[NV];
 #pragma unroll
-        for (int q = 0; q < NV; q++) acc[q] = 0.0f;
+        for (int q = 0; q < NV; q++) sum[q] = 0.0f;
 #pragma unroll
         for (int i = 0; i < KS; i++)
 #pragma unroll
             for (int j = 0; j < KS; j++) {
                 // pixel = x0 + j with x0 = cx - (KS-1)  =>  local cell column lx - j + KS - 1
-                const int c = (ly - i + KS - 1) * ww + (lx - j + KS - 1);
+                const int cc = (ly - i + KS - 1) * ww + (lx - j + KS - 1);
 #pragma unroll
-                for (int q = 0; q < NV; q++) acc[q] += part[((i * KS + j) * NV + q) * ncw + c];
+                for (int q = 0; q < NV; q++) sum[q] += part[((i * KS + j) * NV + q) * ncw + cc];
             }
-        epi((size_t)y * g.W + x, x, y, acc);
+        epi((size_t)y * g.W + x, x, y, sum);
     }
 }
 
-template <int KS, int NV>
-__host__ __device__ __forceinline__ size_t cell_pull_lds(int tw, int th)
-{
-    return sizeof(float) * (size_t)KS * KS * NV * cell_pull_cells<KS>(tw, th);
-}
-
-// The pull used by every splat kernel: cell-owner for ks <= 3, LDS slots for
-// larger footprints (whose cells hold few entries).  `lds` is the kernel's
-// pull region (cell_pull: part planes; tile_pull: meta ints then slots).
-template <int KS, int NV, bool SQUARED, class Vals, class Epi>
+// The pull used by every splat kernel: cell-owner with LDS staging for
+// ks <= 3, LDS slots for larger footprints (whose cells hold few entries).
+// `lds` is the kernel's pull region; `cap` is the staged-entry capacity
+// (ks <= 3) or the slot capacity (ks >= 4).
+template <int KS, int NV, int SQ, class Vals, class Epi>
 __device__ __forceinline__ void any_pull(const TileGeom &g, const TileEntries &te, int t, int *lds, int cap,
                                          Vals vals, Epi epi)
 {
     if constexpr (KS <= 3) {
-        cell_pull<KS, NV, SQUARED>(g, te, t, reinterpret_cast<float *>(lds), vals, epi);
+        cell_pull<KS, NV, SQ>(g, te, t, lds, cap, vals, epi);
     } else {
         float *slots = reinterpret_cast<float *>(lds + pull_meta_ints(g.tw, g.th, KS, blockDim.x));
-        tile_pull<KS, NV, SQUARED>(g, te, t, lds, slots, cap, vals, epi);
+        tile_pull<KS, NV, SQ>(g, te, t, lds, slots, cap, vals, epi);
     }
 }
 
-// LDS bytes of a pull with NV values per slot and `cap` slots.
+// LDS bytes of a slot pull with NV values per slot and `cap` slots.
 inline size_t pull_lds_bytes(int tw, int th, int ks, int bs, int nv, int cap)
 {
     return sizeof(int) * (size_t)pull_meta_ints(tw, th, ks, bs) + sizeof(float) * (size_t)(nv + 2) * cap;

@@ -42,6 +42,7 @@ class Params(ctypes.Structure):
         ("lambda_max_factor", ctypes.c_float), ("inner_iterations", ctypes.c_int),
         ("occlusion_tol_px", ctypes.c_float), ("sigma_z", ctypes.c_float),
         ("backproject_source_cam", ctypes.c_int), ("tile_w", ctypes.c_int), ("tile_h", ctypes.c_int),
+        ("pull_lds_kb", ctypes.c_int),
     ]
 
 

@@ -35,9 +35,16 @@ def test_warps_from_depth_matches_oracle(small_scene):
     assert np.mean(np.abs(zb[fin] - oz[fin]) > 1e-3) < 2e-3
 
 
-def test_pipeline_stages_match_oracle(small_scene):
+@pytest.fixture(scope="module", params=["dsf1", "dsf2"])
+def stage_scene(request, small_scene):
+    if request.param == "dsf1":
+        return small_scene      # 2x2 footprints: the CG operator runs as an assembled 9-point stencil
+    return lfrv.synth(2, seed=5, w=48, h=40, dsf=2)   # 3x3 footprints: the entry-walk operator
+
+
+def test_pipeline_stages_match_oracle(stage_scene):
     """Every stage fed with the GPU's own warps, compared with the oracle stage."""
-    s = small_scene
+    s = stage_scene
     sig = 1.0 / 255.0
     with _ctx(s) as ctx:
         ctx.warps_from_depth()
@@ -148,3 +155,32 @@ def test_sobel_gate_matches_oracle(small_scene):
     ogx, ogy = oracle.sobel(img, vis, prev[0], prev[1])
     assert np.max(np.abs(dgx.numpy() - ogx)) < 1e-6
     assert np.max(np.abs(dgy.numpy() - ogy)) < 1e-6
+
+
+@pytest.mark.parametrize("precond", [False, True])
+def test_poisson_cg_matches_oracle(small_scene, precond):
+    """The screened Poisson system of possion_conj_grad (solver.cu:361-505):
+    (lambda I + D^T D) u = lambda u0 + D^T g from the hole-filled initial image."""
+    s = small_scene
+    lam = 1.0
+    with _ctx(s) as ctx:
+        ctx.warps_from_depth()
+        ctx.prepare()
+        ctx.setup_visibility()
+        ctx.compute_weights()
+        ctx.compute_initial_image()
+        ctx.init_u_gradient()
+        H, W = s["H"], s["W"]
+        U0 = ctx.get_solution().astype(np.float64)
+        gx = ctx.get_buffer("U_GRAD_X").reshape(H, W)
+        gy = ctx.get_buffer("U_GRAD_Y").reshape(H, W)
+        tx = ctx.get_buffer("TAU_X").reshape(s["V"], s["h"], s["w"])
+        ty = ctx.get_buffer("TAU_Y").reshape(s["V"], s["h"], s["w"])
+        st = ctx.poisson_cg(lam, 1e-6, 3000, precond=precond)
+        U = ctx.get_solution()
+    wz = np.zeros(tx.shape)
+    ob = oracle.rhs(W, H, s["dsf"], tx, ty, s["views"], wz, 0.0, lam, U0, 1.0, gx, gy)
+    oX, ost = oracle.pcg(W, H, s["dsf"], tx, ty, wz, 0.0, lam, 1.0, ob, U0, 1e-6, 3000, precond)
+    assert all(st["converged"]) and all(ost["converged"])
+    assert max(st["true_rel_residual"]) < 5e-6
+    assert rel_err(U, oX) < 1e-5
