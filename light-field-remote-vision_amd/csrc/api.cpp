@@ -195,7 +195,10 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
                       &ctx->cache_sq})
         A(b, N);
     A(&ctx->hole_list, N);
-    if (ctx->ks == 2) A(&ctx->gram, 5 * N);
+    if (ctx->ks == 2) {
+        A(&ctx->gram, 5 * N);
+        A(&ctx->cg_extra, 9 * N);
+    }
     ctx->pp_cap = 3 * (N + N / 2) + 3 * N + N / 4 + 64;
     A(&ctx->pp_scratch, ctx->pp_cap);
     A(&ctx->cg_state, 1);
@@ -229,7 +232,7 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
              ctx->cg_d, ctx->cg_q, ctx->cg_b, ctx->reg_g, ctx->norm_mask, ctx->coverage, ctx->u_grad_x,
              ctx->u_grad_y, ctx->target_mask, ctx->visibility, ctx->target_depth, ctx->tmpN, ctx->cg_minv,
              ctx->hole_list, ctx->pp_scratch, ctx->cg_state, ctx->counters, ctx->scal, ctx->partials,
-             ctx->cache_num, ctx->cache_sq, ctx->gram})
+             ctx->cache_num, ctx->cache_sq, ctx->gram, ctx->cg_extra})
         if (p) hipFree(p);
     if (ctx->cg_state_host) hipHostFree(ctx->cg_state_host);
     if (ctx->scal_host) hipHostFree(ctx->scal_host);

@@ -21,40 +21,40 @@ __device__ __forceinline__ int cell_of(float t0, float t1, int Wc)
     return (y0 + KS - 1) * Wc + (x0 + KS - 1);
 }
 
-// Source index of thread t: the views are interleaved (t = m * V + v), so a
-// wave touches the same target neighbourhood from every view at once and the
-// atomics and scattered stores of the binning stay within few cache lines.
-__device__ __forceinline__ size_t interleaved_source(size_t t, int V, size_t n)
-{
-    const size_t m = t / (size_t)V, v = t - m * (size_t)V;
-    return v * n + m;
-}
-
+// Binning: one thread per source pixel position, looping over the views, so
+// the loads are coalesced and the V entries that land in the same target
+// neighbourhood are counted / stored by the same wave close in time.
 template <int KS>
-__global__ void k_cell_count(size_t tot, int V, size_t n, int W, int H, int Wc, const float *__restrict__ tx,
+__global__ void k_cell_count(int V, int w, int h, int W, int H, int Wc, const float *__restrict__ tx,
                              const float *__restrict__ ty, int *__restrict__ count)
 {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= tot) return;
-    const size_t k = interleaved_source(t, V, n);
-    const float t0 = tx[k], t1 = ty[k];
-    if (!cell_valid(t0, t1, W, H)) return;
-    atomicAdd(count + cell_of<KS>(t0, t1, Wc), 1);
+    const int mx = blockIdx.x * blockDim.x + threadIdx.x, my = blockIdx.y * blockDim.y + threadIdx.y;
+    if (mx >= w || my >= h) return;
+    const size_t n = (size_t)w * h;
+    for (int v = 0; v < V; v++) {
+        const size_t k = (size_t)v * n + (size_t)my * w + mx;
+        const float t0 = tx[k], t1 = ty[k];
+        if (!cell_valid(t0, t1, W, H)) continue;
+        atomicAdd(count + cell_of<KS>(t0, t1, Wc), 1);
+    }
 }
 
 template <int KS>
-__global__ void k_cell_fill(size_t tot, int V, size_t n, int W, int H, int Wc, const float *__restrict__ tx,
+__global__ void k_cell_fill(int V, int w, int h, int W, int H, int Wc, const float *__restrict__ tx,
                             const float *__restrict__ ty, const int *__restrict__ start,
                             int *__restrict__ cursor, int *__restrict__ e_src)
 {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= tot) return;
-    const size_t k = interleaved_source(t, V, n);
-    const float t0 = tx[k], t1 = ty[k];
-    if (!cell_valid(t0, t1, W, H)) return;
-    const int c = cell_of<KS>(t0, t1, Wc);
-    const int pos = start[c] + atomicAdd(cursor + c, 1);
-    e_src[pos] = (int)k;
+    const int mx = blockIdx.x * blockDim.x + threadIdx.x, my = blockIdx.y * blockDim.y + threadIdx.y;
+    if (mx >= w || my >= h) return;
+    const size_t n = (size_t)w * h;
+    for (int v = 0; v < V; v++) {
+        const size_t k = (size_t)v * n + (size_t)my * w + mx;
+        const float t0 = tx[k], t1 = ty[k];
+        if (!cell_valid(t0, t1, W, H)) continue;
+        const int c = cell_of<KS>(t0, t1, Wc);
+        const int pos = start[c] + atomicAdd(cursor + c, 1);
+        e_src[pos] = (int)k;
+    }
 }
 
 // Order every cell's entries by source index and gather their tau.  A block
@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(1024) k_pull_init_image(TileGeom g, TileEntrie
         g, te, t, smem, pc.cap,
         PULL_VALS(4) {
             const int k = te.src[e];
-            const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
+            const unsigned vi = (unsigned)k / (unsigned)n, m = (unsigned)k - vi * (unsigned)n;
             const float *vp = views + vi * 3 * n + m;
             v[0] = om * vp[0];
             v[1] = om * vp[n];
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(1024) k_pull_gibr_setup(TileGeom g, TileEntrie
         g, te, t, smem, pc.cap,
         PULL_VALS(7) {
             const int k = te.src[e];
-            const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
+            const unsigned vi = (unsigned)k / (unsigned)n, m = (unsigned)k - vi * (unsigned)n;
             const float *vp = views + vi * 3 * n + m;
             v[0] = om * vp[0];
             v[1] = om * vp[n];
@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(1024) k_pull_u_gradient(TileGeom g, TileEntrie
         g, te, t, smem, pc.cap,
         PULL_VALS(4) {
             const int k = te.src[e];
-            const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
+            const unsigned vi = (unsigned)k / (unsigned)n, m = (unsigned)k - vi * (unsigned)n;
             v[0] = vgrad[vi * 2 * n + m];
             v[1] = vgrad[vi * 2 * n + n + m];
             v[2] = om;
@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(1024) k_pull_rhs(TileGeom g, TileEntries te, P
         g, te, t, smem, pc.cap,
         PULL_VALS(3) {
             const int k = te.src[e];
-            const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
+            const unsigned vi = (unsigned)k / (unsigned)n, m = (unsigned)k - vi * (unsigned)n;
             const float ow = om * wscale;
             const float *vp = views + vi * 3 * n + m;
             v[0] = ow * vp[0];
@@ -521,19 +521,19 @@ size_t pull_lds(const lfrv_ctx *ctx, int nv, int cap)
 lfrv_status run_prepare(lfrv_ctx *ctx)
 {
     const TileGeom g = tile_geom(ctx);
-    const size_t tot = (size_t)ctx->V * ctx->n;
     const int nc = ctx->ncells;
     TimedRegion tr(ctx, 5);
     LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_count, 0, sizeof(int) * (nc + 1), ctx->stream));
     LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_cursor, 0, sizeof(int) * nc, ctx->stream));
-#define CC(K) k_cell_count<K><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(tot, ctx->V, ctx->n, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_count)
+    const dim3 bgrid((ctx->w + 63) / 64, (ctx->h + 3) / 4), bblock(64, 4);
+#define CC(K) k_cell_count<K><<<bgrid, bblock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_count)
     LFRV_KS_DISPATCH(ctx->ks, CC);
 #undef CC
     const int nb = (nc + 256 * kScanPer - 1) / (256 * kScanPer);
     k_scan_blocks<<<nb, 256, 0, ctx->stream>>>(nc, ctx->cell_count, ctx->cell_start, ctx->scan_tmp);
     k_scan_sums<<<1, 1024, 0, ctx->stream>>>(nb, ctx->scan_tmp, ctx->cell_start + nc);
     k_scan_add<<<blocks_for(nc), kBlock, 0, ctx->stream>>>(nc, ctx->cell_start, ctx->scan_tmp);
-#define CF(K) k_cell_fill<K><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(tot, ctx->V, ctx->n, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_start, ctx->cell_cursor, ctx->e_src)
+#define CF(K) k_cell_fill<K><<<bgrid, bblock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_start, ctx->cell_cursor, ctx->e_src)
     LFRV_KS_DISPATCH(ctx->ks, CF);
 #undef CF
     k_cell_sort<<<(nc + kSortCells - 1) / kSortCells, kSortCells, 0, ctx->stream>>>(

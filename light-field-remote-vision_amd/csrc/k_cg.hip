@@ -200,12 +200,12 @@ __global__ void __launch_bounds__(1024) k_gram2(TileGeom g, TileEntries te, int 
     }
 }
 
-constexpr int kSt2W = 64, kSt2H = 4;  // tile of the stencil operator (256 threads)
+constexpr int kSt2W = 64, kSt2H = 16;  // tile of the stencil kernels, one thread per pixel
 
 // q = A din (mode 0) or, fused with the direction update (mode 1),
 // d = Minv r + beta d_prev, q = A d, d.q partials;
 // A d = wscale * stencil(d) + c0 d + beta_reg D^T D d.
-__global__ void __launch_bounds__(256) k_cg_op_st2(int W, int H, int ntx, int ntiles, float wscale, float c0,
+__global__ void __launch_bounds__(kSt2W * kSt2H) k_cg_op_st2(int W, int H, int ntx, int ntiles, float wscale, float c0,
                                                   float beta_reg, int mode, const float *__restrict__ din,
                                                   const float *__restrict__ r, const float *__restrict__ minv,
                                                   float *__restrict__ dout, float *__restrict__ q,
@@ -285,6 +285,205 @@ __global__ void __launch_bounds__(256) k_cg_op_st2(int W, int H, int ntx, int nt
     }
     if (mode == 0) return;
     block_partials<3>(dq, partials, red);
+}
+
+// ---- dsf = 1: single-reduction PCG on the assembled stencil -----------------
+//
+// The Chronopoulos-Gear form of preconditioned CG needs one global reduction
+// per iteration instead of two: with u = Minv r, w = A u, gamma = (r, u),
+// delta = (w, u),
+//   beta_i  = gamma_i / gamma_{i-1},  alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1}),
+//   p_i = u_i + beta_i p_{i-1},  s_i = w_i + beta_i s_{i-1},
+//   x_{i+1} = x_i + alpha_i p_i,  r_{i+1} = r_i - alpha_i s_i.
+// (Same iterates as standard PCG in exact arithmetic.)  One kernel does a
+// whole iteration: it rebuilds r_{i+1} and u_{i+1} on its tile and a 1-pixel
+// halo (s_i and r_{i+1} of the halo are recomputed from w_i, s_{i-1}, r_i),
+// applies the stencil to u_{i+1}, writes p_i, x_{i+1}, s_i, r_{i+1}, w_{i+1}
+// and reduces gamma, delta and r.r of the new residual.  r, s and w are read
+// across tiles, so they are double-buffered.
+//
+// start = 1: no update; w_0 = A (Minv r_0) and gamma_0, delta_0, rho_0.
+// (A variant that lets every block of the next launch finish the reduction
+// itself measured slower: the redundant per-block sums cost more than the
+// one-block launch they replace.)
+struct CgcgArgs {
+    int W, H, ntx, ntiles, start;
+    float wscale, c0, beta_reg;
+    const float *r_in, *w_in, *s_in;
+    float *r_out, *w_out, *s_out, *p, *x;
+    const float *minv;
+    const float *gD, *gE, *gS, *gSE, *gSW;
+};
+
+__global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const CgState *st, double *partials)
+{
+    constexpr int LW = kSt2W + 2, LH = kSt2H + 2, LP = LW * LH;
+    __shared__ float ul[3 * LP];
+    __shared__ float rl[3 * LP];  // r_{i+1}
+    __shared__ double red[9 * 16];
+    if (!a.start && st->done) return;
+    const int t = xcd_tile(blockIdx.x, a.ntiles);
+    const int tx0 = (t % a.ntx) * kSt2W, ty0 = (t / a.ntx) * kSt2H;
+    const int W = a.W, H = a.H;
+    const size_t N = (size_t)W * H;
+    float al[3] = {0.0f, 0.0f, 0.0f}, be[3] = {0.0f, 0.0f, 0.0f};
+    int first = 1;
+    if (!a.start) {
+        first = st->first;
+        for (int c = 0; c < 3; c++) {
+            al[c] = (float)st->alpha[c];
+            be[c] = (float)st->beta[c];
+        }
+    }
+    for (int i = threadIdx.x; i < LP; i += blockDim.x) {
+        const int lx = i % LW, ly = i / LW;
+        const int x = tx0 - 1 + lx, y = ty0 - 1 + ly;
+        const bool in = (x >= 0 && x < W && y >= 0 && y < H);
+        float un[3] = {0.0f, 0.0f, 0.0f}, rnew[3] = {0.0f, 0.0f, 0.0f};
+        if (in) {
+            const size_t o = (size_t)y * W + x;
+            const float mi = a.minv[o];
+            const bool interior = lx >= 1 && lx <= kSt2W && ly >= 1 && ly <= kSt2H;
+            for (int c = 0; c < 3; c++) {
+                const size_t k = c * N + o;
+                const float rv = a.r_in[k];
+                if (a.start) {
+                    un[c] = mi * rv;
+                    rnew[c] = rv;
+                    continue;
+                }
+                const float sv = first ? a.w_in[k] : a.w_in[k] + be[c] * a.s_in[k];
+                const float rn = rv - al[c] * sv;
+                un[c] = mi * rn;
+                rnew[c] = rn;
+                if (interior) {
+                    const float uv = mi * rv;
+                    const float pv = first ? uv : uv + be[c] * a.p[k];
+                    a.p[k] = pv;
+                    a.x[k] += al[c] * pv;
+                    a.s_out[k] = sv;
+                    a.r_out[k] = rn;
+                }
+            }
+        }
+        for (int c = 0; c < 3; c++) {
+            ul[c * LP + i] = un[c];
+            rl[c * LP + i] = rnew[c];
+        }
+    }
+    __syncthreads();
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int lx = threadIdx.x % kSt2W, ly = threadIdx.x / kSt2W;
+    const int x = tx0 + lx, y = ty0 + ly;
+    if (x < W && y < H) {
+        const size_t o = (size_t)y * W + x;
+        const float cD = a.gD[o], cE = a.gE[o], cS = a.gS[o], cSE = a.gSE[o], cSW = a.gSW[o];
+        const float cEm = x > 0 ? a.gE[o - 1] : 0.0f;
+        const float cSm = y > 0 ? a.gS[o - W] : 0.0f;
+        const float cSEm = (x > 0 && y > 0) ? a.gSE[o - W - 1] : 0.0f;
+        const float cSWm = (x < W - 1 && y > 0) ? a.gSW[o - W + 1] : 0.0f;
+        const int li = (ly + 1) * LW + lx + 1;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float *up = ul + c * LP;
+            const float uv = up[li];
+            const float sten = cD * uv + cE * up[li + 1] + cEm * up[li - 1] + cS * up[li + LW] + cSm * up[li - LW] +
+                               cSE * up[li + LW + 1] + cSEm * up[li - LW - 1] + cSW * up[li + LW - 1] +
+                               cSWm * up[li - LW + 1];
+            float lap = 0.0f;
+            if (W > 1) {
+                if (x == 0) lap += uv - up[li + 1];
+                else if (x < W - 1) lap += 2.0f * uv - up[li - 1] - up[li + 1];
+                else lap += uv - up[li - 1];
+            }
+            if (H > 1) {
+                if (y == 0) lap += uv - up[li + LW];
+                else if (y < H - 1) lap += 2.0f * uv - up[li - LW] - up[li + LW];
+                else lap += uv - up[li - LW];
+            }
+            const float wv = a.wscale * sten + a.c0 * uv + a.beta_reg * lap;
+            a.w_out[c * N + o] = wv;
+            const float rn = rl[c * LP + li];
+            v[c] += (double)rn * (double)uv;
+            v[3 + c] += (double)wv * (double)uv;
+            v[6 + c] += (double)rn * (double)rn;
+        }
+    }
+    block_partials<9>(v, partials, red);
+}
+
+// The scalar step between two passes (one thread).  After the start pass:
+// alpha_0 = gamma_0 / delta_0.  After an iteration: convergence of the new
+// residual (rho), then beta and alpha of the next iteration.
+// `pv` = gamma[3], delta[3], rho[3].
+__device__ void cgcg_scalars(CgState &s, const double (&pv)[9], bool after_start)
+{
+    if (s.done) return;
+    int any = 0;
+    for (int c = 0; c < 3; c++) {
+        const double gam = pv[c], del = pv[3 + c];
+        if (after_start) {
+            s.rz[c] = gam;
+            s.dq[c] = del;
+            s.beta[c] = 0.0;
+            s.alpha[c] = 0.0;
+            if (s.active[c]) {
+                if (!(del > 0.0)) {
+                    s.breakdown[c] = 1;
+                    s.active[c] = 0;
+                } else {
+                    s.alpha[c] = gam / del;
+                }
+            }
+        } else if (s.active[c]) {
+            s.iter[c] += 1;
+            s.rr[c] = pv[6 + c];
+            if (pv[6 + c] <= s.tol2 * s.bb[c] || s.iter[c] >= s.maxit) {
+                s.active[c] = 0;
+                s.alpha[c] = 0.0;
+            } else {
+                const double bt = gam / s.rz[c];
+                const double den = del - bt * gam / s.alpha[c];
+                if (!(den > 0.0)) {
+                    s.breakdown[c] = 1;
+                    s.active[c] = 0;
+                    s.alpha[c] = 0.0;
+                } else {
+                    s.beta[c] = bt;
+                    s.alpha[c] = gam / den;
+                    s.rz[c] = gam;
+                    s.dq[c] = del;
+                }
+            }
+        }
+        any |= s.active[c];
+    }
+    s.done = !any;
+}
+
+// Scalars after the start pass: alpha_0 = gamma_0 / delta_0.
+__global__ void __launch_bounds__(kFinalizeThreads) k_cgcg_start(CgState *st, const double *__restrict__ partials,
+                                                                 int G)
+{
+    __shared__ double red[9 * 16];
+    double v[9];
+    reduce_partials<9>(partials, G, v, red);
+    if (threadIdx.x == 0) cgcg_scalars(*st, v, true);
+}
+
+// Scalars after iteration i: convergence on r_{i+1}.r_{i+1}, then beta and
+// alpha of the next iteration.
+__global__ void __launch_bounds__(kFinalizeThreads) k_cgcg_step(CgState *st, const double *__restrict__ partials,
+                                                                int G)
+{
+    __shared__ double red[9 * 16];
+    if (st->done) return;
+    double v[9];
+    reduce_partials<9>(partials, G, v, red);
+    if (threadIdx.x == 0) {
+        cgcg_scalars(*st, v, false);
+        st->first = 0;
+    }
 }
 
 // r = b - q (q = A x0), r.r, r.z, b.b; initialises the scalar state.
@@ -515,7 +714,7 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
 
 #define OPK(K, MODE, DIN, DOUT)                                                                               \
     if (st2)                                                                                                  \
-        k_cg_op_st2<<<st2_tiles, 256, 0, ctx->stream>>>(ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, \
+        k_cg_op_st2<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, \
                                                         (float)beta, MODE, DIN, r, minv, DOUT, q, gD, gE, gS,  \
                                                         gSE, gSW, stt, part);                                 \
     else                                                                                                      \
@@ -531,7 +730,7 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     }
 
     {
-        TimedRegion tr(ctx, 0);
+        TimedRegion tr(ctx, 5);
         LAUNCH_OP(0, x, nullptr);
     }
     k_cg_init<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, r, minv, part);
@@ -540,6 +739,65 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     int it = 0, cur = 0;
     const int chunk = 8;
     bool done = false;
+    if (st2) {
+        // single-reduction iterations on the stencil; r, s, w double-buffered
+        float *rbuf[2] = {ctx->cg_r, ctx->cg_extra};
+        float *sbuf[2] = {ctx->cg_extra + 3 * N, ctx->tmp3N};
+        float *wbuf[2] = {ctx->cg_q, ctx->cg_extra + 6 * N};
+        CgcgArgs a{};
+        a.W = ctx->W;
+        a.H = ctx->H;
+        a.ntx = st2_ntx;
+        a.ntiles = st2_tiles;
+        a.wscale = wscale;
+        a.c0 = (float)c0;
+        a.beta_reg = (float)beta;
+        a.minv = minv;
+        a.p = ctx->cg_d;
+        a.x = x;
+        a.gD = gD;
+        a.gE = gE;
+        a.gS = gS;
+        a.gSE = gSE;
+        a.gSW = gSW;
+        a.start = 1;
+        a.r_in = rbuf[0];
+        a.w_in = wbuf[0];
+        a.s_in = sbuf[0];
+        a.r_out = rbuf[1];
+        a.w_out = wbuf[0];
+        a.s_out = sbuf[1];
+        {
+            TimedRegion tr(ctx, 5);
+            k_cgcg_st2<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(a, stt, part);
+        }
+        k_cgcg_start<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, st2_tiles);
+        LFRV_LAUNCH(ctx, "cg start");
+        a.start = 0;
+        while (it < maxit && !done) {
+            const int nk = std::min(chunk, maxit - it);
+            for (int k = 0; k < nk; k++) {
+                a.r_in = rbuf[cur];
+                a.w_in = wbuf[cur];
+                a.s_in = sbuf[cur];
+                a.r_out = rbuf[cur ^ 1];
+                a.w_out = wbuf[cur ^ 1];
+                a.s_out = sbuf[cur ^ 1];
+                {
+                    TimedRegion tr(ctx, 0);
+                    k_cgcg_st2<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(a, stt, part);
+                }
+                k_cgcg_step<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, st2_tiles);
+                cur ^= 1;
+            }
+            LFRV_LAUNCH(ctx, "cg iteration");
+            it += nk;
+            LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+            LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            done = ctx->cg_state_host->done != 0;
+        }
+        done = true;
+    }
     while (it < maxit && !done) {
         const int nk = std::min(chunk, maxit - it);
         for (int k = 0; k < nk; k++) {
@@ -565,7 +823,7 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
 #undef LAUNCH_OP
     // final state and true residual
     {
-        TimedRegion tr(ctx, 0);
+        TimedRegion tr(ctx, 5);
         switch (ctx->ks) {
         case 2: OPK(2, 0, x, nullptr); break;
         case 3: OPK(3, 0, x, nullptr); break;

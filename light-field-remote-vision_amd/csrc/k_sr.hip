@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(1024) k_sr_dataterm(TileGeom g, TileEntries te
         g, te, t, plds, cap,
         [&](int e, int x0, int y0, const float(&wx)[KS], const float(&wy)[KS], float om, float(&r)[6]) {
             const int k = te.src[e];
-            const size_t vi = (size_t)k / n, m = (size_t)k - vi * n;
+            const unsigned vi = (unsigned)k / (unsigned)n, m = (unsigned)k - vi * (unsigned)n;
             const int bx = x0 - tx0 + HAL, by = y0 - ty0 + HAL;
             float su[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
 #pragma unroll

@@ -7,6 +7,18 @@ namespace lfrv {
 
 namespace {
 
+// Per-source-pixel kernels run on grid (x tiles, y tiles, views) with 2-D
+// blocks: pixel (mx, my) of view v, m = my * w + mx, k = v * n + m.
+#define SOURCE_INDEX()                                                      \
+    const int mx = blockIdx.x * blockDim.x + threadIdx.x;                   \
+    const int my = blockIdx.y * blockDim.y + threadIdx.y;                   \
+    const int v = blockIdx.z;                                               \
+    if (mx >= w || my >= h) return;                                         \
+    const size_t n = (size_t)w * h;                                         \
+    const int m = my * w + mx;                                              \
+    const size_t k = (size_t)v * n + (size_t)m;                             \
+    (void)k;
+
 constexpr float kInvalidDepth = 1000.0f;  // import.cpp:23
 
 // Order-preserving map float -> int for atomicMin on signed floats.
@@ -26,11 +38,7 @@ __global__ void k_tau_from_depth(int V, int w, int h, int dsf, const float *__re
                                  float *__restrict__ dpx, float *__restrict__ dpy,
                                  int *__restrict__ zkey)
 {
-    const size_t n = (size_t)w * h;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (size_t)V * n) return;
-    const int v = (int)(k / n);
-    const int m = (int)(k - (size_t)v * n);
+    SOURCE_INDEX()
     const int W = w * dsf, H = h * dsf;
     const WarpCam &cam = cams[v];
     const float d = depth[k];
@@ -41,7 +49,7 @@ __global__ void k_tau_from_depth(int V, int w, int h, int dsf, const float *__re
         dpy[k] = 0.0f;
         return;
     }
-    const float xx = (float)(m % w) + 0.5f, yy = (float)(m / w) + 0.5f;
+    const float xx = (float)mx + 0.5f, yy = (float)my + 0.5f;
     const float Mx0 = cam.M[0] * xx + cam.M[1] * yy + cam.M[2];
     const float Mx1 = cam.M[3] * xx + cam.M[4] * yy + cam.M[5];
     const float Mx2 = cam.M[6] * xx + cam.M[7] * yy + cam.M[8];
@@ -79,22 +87,26 @@ __global__ void k_tau_visibility(int V, int w, int h, int dsf, const float *__re
                                  const WarpCam *__restrict__ cams, float tol,
                                  float *__restrict__ tx, float *__restrict__ ty)
 {
-    const size_t n = (size_t)w * h;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (size_t)V * n) return;
-    const int v = (int)(k / n);
-    const int m = (int)(k - (size_t)v * n);
+    SOURCE_INDEX()
     const int W = w * dsf, H = h * dsf;
     const WarpCam &cam = cams[v];
     const float tux = tx[k], tuy = ty[k];
     float u_d = 0.0f, u_norm = 0.0f;
     const float hd = 0.5f * (float)dsf;
+    const int bx = (int)floorf(tux - hd + 0.5f), by = (int)floorf(tuy - hd + 0.5f);
+    // the footprint's depths are loaded unconditionally (clamped addresses,
+    // masked below) so that they are in flight together
+    float zt[kMaxKs * kMaxKs];
     for (int i = 0; i < dsf + 1; i++)
         for (int j = 0; j < dsf + 1; j++) {
-            const int px = (int)floorf(tux - hd + 0.5f) + j;
-            const int py = (int)floorf(tuy - hd + 0.5f) + i;
+            const int px = min(max(bx + j, 0), W - 1), py = min(max(by + i, 0), H - 1);
+            zt[i * (dsf + 1) + j] = ud[(size_t)py * W + px];
+        }
+    for (int i = 0; i < dsf + 1; i++)
+        for (int j = 0; j < dsf + 1; j++) {
+            const int px = bx + j, py = by + i;
             if (0 <= px && px <= W - 1 && 0 <= py && py <= H - 1) {
-                const float z = ud[(size_t)py * W + px];
+                const float z = zt[i * (dsf + 1) + j];
                 if (z == kInvalidDepth) continue;
                 float dx = 1.0f, dy = 1.0f;
                 if (j == 0) dx = ((float)px + 0.5f) - (tux - hd);
@@ -116,7 +128,6 @@ __global__ void k_tau_visibility(int V, int w, int h, int dsf, const float *__re
     const float X1 = u_d * (cam.Nb[3] * xx + cam.Nb[4] * yy + cam.Nb[5]) + cam.e[1];
     const float X2 = u_d * (cam.Nb[6] * xx + cam.Nb[7] * yy + cam.Nb[8]) + cam.e[2];
     const int ox = (int)(X0 / X2), oy = (int)(X1 / X2);
-    const int mx = m % w, my = m / w;
     const float ddx = (float)(ox - mx), ddy = (float)(oy - my);
     if (ddx < -tol || tol < ddx || ddy < -tol || tol < ddy) {
         tx[k] = -(float)w * 2.0f;
@@ -230,12 +241,8 @@ __global__ void k_sobel(int W, int H, const float *__restrict__ r, const float *
 __global__ void k_view_stencils(int V, int w, int h, const float *__restrict__ views,
                                 float *__restrict__ vgrad, float *__restrict__ vlap)
 {
-    const size_t n = (size_t)w * h;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (size_t)V * n) return;
-    const int v = (int)(k / n);
-    const size_t m = k - (size_t)v * n;
-    const int ox = (int)(m % w), oy = (int)(m / w);
+    SOURCE_INDEX()
+    const int ox = mx, oy = my;
     const float *r = views + (size_t)v * 3 * n, *g = r + n, *b = g + n;
     float GX = 0.0f, GY = 0.0f;
     if (0 < ox && ox < w - 1 && 0 < oy && oy < h - 1) {
@@ -311,11 +318,7 @@ __global__ void k_weights(int V, int W, int H, int w, int h, const float *__rest
                           const float *__restrict__ dpy, float sigma, float thr,
                           float *__restrict__ omega)
 {
-    const size_t n = (size_t)w * h;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (size_t)V * n) return;
-    const int v = (int)(k / n);
-    const size_t m = k - (size_t)v * n;
+    SOURCE_INDEX()
     const float *vtx = tx + (size_t)v * n, *vty = ty + (size_t)v * n;
     // warped u gradient (convolution_nonsep_param); identically zero, and not
     // gathered, when the solution is the cleared image
@@ -347,24 +350,35 @@ __global__ void k_weights(int V, int W, int H, int w, int h, const float *__rest
     float sg = ZERO_GRAD ? 0.0f : gxv * dpx[k] + gyv * dpy[k];
     if (sg > 0.5f) sg = 0.5f;
     const float ang = 1.0f / (sg * sg + sigma * sigma);
-    // deformation weight
-    const int mx = (int)(m % w), my = (int)(m / w);
-    auto inval = [&](size_t q) {
-        return vtx[q] < 0.0f || vtx[q] > (float)W || vty[q] < 0.0f || vty[q] > (float)H;
-    };
+    // deformation weight; the 4-neighbourhood is loaded up front (clamped at
+    // the border, where it is not used) so the loads are not serialised
+    // behind the validity branches
+    const int ml = my * w + (mx > 0 ? mx - 1 : mx), mr = my * w + (mx < w - 1 ? mx + 1 : mx);
+    const int mu = (my > 0 ? my - 1 : my) * w + mx, md = (my < h - 1 ? my + 1 : my) * w + mx;
+    const float xc = vtx[m], yc = vty[m];
+    const float xl = vtx[ml], yl = vty[ml], xr = vtx[mr], yr = vty[mr];
+    const float xu = vtx[mu], yu = vty[mu], xd = vtx[md], yd = vty[md];
+    auto inval = [&](float a, float b) { return a < 0.0f || a > (float)W || b < 0.0f || b > (float)H; };
     float dw;
-    if (inval(m)) dw = 0.0f;
+    if (inval(xc, yc)) dw = 0.0f;
     else if (mx == 0 || my == 0 || mx == w - 1 || my == h - 1) dw = 1.0f;
-    else if (inval(m - 1) || inval(m + 1) || inval(m - w) || inval(m + w)) dw = 1.0f;
+    else if (inval(xl, yl) || inval(xr, yr) || inval(xu, yu) || inval(xd, yd)) dw = 1.0f;
     else {
-        const float Dxx = 0.5f * (vtx[m + 1] - vtx[m - 1]);
-        const float Dyx = 0.5f * (vty[m + 1] - vty[m - 1]);
-        const float Dxy = 0.5f * (vtx[m + w] - vtx[m - w]);
-        const float Dyy = 0.5f * (vty[m + w] - vty[m - w]);
+        const float Dxx = 0.5f * (xr - xl);
+        const float Dyx = 0.5f * (yr - yl);
+        const float Dxy = 0.5f * (xd - xu);
+        const float Dyy = 0.5f * (yd - yu);
         const float Nd = fabsf(Dxx * Dyy - Dxy * Dyx);
         dw = (Nd > 1.0f) ? 1.0f / Nd : 1.0f;
     }
     omega[k] = ang * dw;
+}
+
+const dim3 kSourceBlock(64, 4);
+
+inline dim3 source_grid(const lfrv_ctx *ctx)
+{
+    return dim3((ctx->w + 63) / 64, (ctx->h + 3) / 4, ctx->V);
 }
 
 }  // namespace
@@ -446,12 +460,11 @@ lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
         std::memcpy(&inv_key, &f, sizeof inv_key);  // positive float: key == bit pattern
     }
     k_fill_key<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(zkey, ctx->N, inv_key);
-    const size_t tot = (size_t)ctx->V * ctx->n;
-    k_tau_from_depth<<<blocks_for(tot), kBlock, 0, ctx->stream>>>(
+    k_tau_from_depth<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
         ctx->V, ctx->w, ctx->h, ctx->dsf, ctx->depth, dcams, ctx->p.sigma_z, ctx->tau_x, ctx->tau_y,
         ctx->dpart_x, ctx->dpart_y, zkey);
     k_key_to_depth<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(zkey, ctx->target_depth, ctx->N);
-    k_tau_visibility<<<blocks_for(tot), kBlock, 0, ctx->stream>>>(
+    k_tau_visibility<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
         ctx->V, ctx->w, ctx->h, ctx->dsf, ctx->target_depth, dcams, ctx->p.occlusion_tol_px,
         ctx->tau_x, ctx->tau_y);
     LFRV_LAUNCH(ctx, "warps_from_depth kernels");
@@ -463,13 +476,12 @@ lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
 template <int KS>
 static void launch_weights_ks(lfrv_ctx *ctx, const float *gxs, const float *gys, bool zero_grad)
 {
-    const size_t tot = (size_t)ctx->V * ctx->n;
     if (zero_grad)
-        k_weights<KS, true><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(
+        k_weights<KS, true><<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
             ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, gxs, gys, ctx->dpart_x,
             ctx->dpart_y, ctx->p.sigma_sensor, ctx->p.ugrad_threshold, ctx->omega);
     else
-        k_weights<KS, false><<<blocks_for(tot), kBlock, 0, ctx->stream>>>(
+        k_weights<KS, false><<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
             ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, gxs, gys, ctx->dpart_x,
             ctx->dpart_y, ctx->p.sigma_sensor, ctx->p.ugrad_threshold, ctx->omega);
 }
@@ -520,8 +532,7 @@ lfrv_status run_view_stencils(lfrv_ctx *ctx)
         LFRV_LAUNCH(ctx, "view stencils");
         return LFRV_OK;
     }
-    const size_t tot = (size_t)ctx->V * ctx->n;
-    k_view_stencils<<<blocks_for(tot), kBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->views,
+    k_view_stencils<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->views,
                                                                  ctx->view_grad, ctx->view_lap);
     LFRV_LAUNCH(ctx, "view stencils");
     return LFRV_OK;

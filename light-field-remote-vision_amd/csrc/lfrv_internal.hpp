@@ -142,19 +142,25 @@ template <int NQ>
 __device__ __forceinline__ void reduce_partials(const double *partials, int G, double (&out)[NQ], double *lds)
 {
     const int bs = blockDim.x;
+    if (G <= bs) {
+        // one value per thread and sum: all NQ loads in flight at once
 #pragma unroll
-    for (int q = 0; q < NQ; q++) {
-        const double *pq = partials + (size_t)q * G;
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int b = threadIdx.x;
-        for (; b + 3 * bs < G; b += 4 * bs) {
-            a0 += pq[b];
-            a1 += pq[b + bs];
-            a2 += pq[b + 2 * bs];
-            a3 += pq[b + 3 * bs];
+        for (int q = 0; q < NQ; q++) out[q] = (int)threadIdx.x < G ? partials[(size_t)q * G + threadIdx.x] : 0.0;
+    } else {
+#pragma unroll 1
+        for (int q = 0; q < NQ; q++) {
+            const double *pq = partials + (size_t)q * G;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int b = threadIdx.x;
+            for (; b + 3 * bs < G; b += 4 * bs) {
+                a0 += pq[b];
+                a1 += pq[b + bs];
+                a2 += pq[b + 2 * bs];
+                a3 += pq[b + 3 * bs];
+            }
+            for (; b < G; b += bs) a0 += pq[b];
+            out[q] = (a0 + a1) + (a2 + a3);
         }
-        for (; b < G; b += bs) a0 += pq[b];
-        out[q] = (a0 + a1) + (a2 + a3);
     }
     block_sum<NQ>(out, lds);
 }
@@ -242,6 +248,7 @@ struct lfrv_ctx {
     uint64_t setup_epoch = 0;  // epoch of the last gradient-IBR set-up pull
     float *cache_num = nullptr, *cache_sq = nullptr;  // its raw sums: 3N and N
     float *gram = nullptr;     // dsf = 1: 5N raw stencil coefficients of sum_i S_i^T omega_i S_i
+    float *cg_extra = nullptr; // dsf = 1: 9N, the second r, s and w buffers of the single-reduction PCG
     uint64_t gram_epoch = 0;
     int pull_budget = 48 * 1024;  // LDS bytes of a pull's entry cache
 
