@@ -24,18 +24,26 @@ __device__ __forceinline__ int cell_of(float t0, float t1, int Wc)
 // Binning: one thread per source pixel position, looping over the views, so
 // the loads are coalesced and the V entries that land in the same target
 // neighbourhood are counted / stored by the same wave close in time.
+constexpr int kViewBatch = 4;  // views whose tau is loaded together by the binning kernels
+
 template <int KS>
 __global__ void k_cell_count(int V, int w, int h, int W, int H, int Wc, const float *__restrict__ tx,
                              const float *__restrict__ ty, int *__restrict__ count)
 {
     const int mx = blockIdx.x * blockDim.x + threadIdx.x, my = blockIdx.y * blockDim.y + threadIdx.y;
     if (mx >= w || my >= h) return;
-    const size_t n = (size_t)w * h;
-    for (int v = 0; v < V; v++) {
-        const size_t k = (size_t)v * n + (size_t)my * w + mx;
-        const float t0 = tx[k], t1 = ty[k];
-        if (!cell_valid(t0, t1, W, H)) continue;
-        atomicAdd(count + cell_of<KS>(t0, t1, Wc), 1);
+    const size_t n = (size_t)w * h, m = (size_t)my * w + mx;
+    for (int v0 = 0; v0 < V; v0 += kViewBatch) {
+        float t0[kViewBatch], t1[kViewBatch];
+#pragma unroll
+        for (int b = 0; b < kViewBatch; b++) {
+            const size_t k = (size_t)min(v0 + b, V - 1) * n + m;
+            t0[b] = tx[k];
+            t1[b] = ty[k];
+        }
+#pragma unroll
+        for (int b = 0; b < kViewBatch; b++)
+            if (v0 + b < V && cell_valid(t0[b], t1[b], W, H)) atomicAdd(count + cell_of<KS>(t0[b], t1[b], Wc), 1);
     }
 }
 
@@ -46,14 +54,25 @@ __global__ void k_cell_fill(int V, int w, int h, int W, int H, int Wc, const flo
 {
     const int mx = blockIdx.x * blockDim.x + threadIdx.x, my = blockIdx.y * blockDim.y + threadIdx.y;
     if (mx >= w || my >= h) return;
-    const size_t n = (size_t)w * h;
-    for (int v = 0; v < V; v++) {
-        const size_t k = (size_t)v * n + (size_t)my * w + mx;
-        const float t0 = tx[k], t1 = ty[k];
-        if (!cell_valid(t0, t1, W, H)) continue;
-        const int c = cell_of<KS>(t0, t1, Wc);
-        const int pos = start[c] + atomicAdd(cursor + c, 1);
-        e_src[pos] = (int)k;
+    const size_t n = (size_t)w * h, m = (size_t)my * w + mx;
+    for (int v0 = 0; v0 < V; v0 += kViewBatch) {
+        float t0[kViewBatch], t1[kViewBatch];
+#pragma unroll
+        for (int b = 0; b < kViewBatch; b++) {
+            const size_t k = (size_t)min(v0 + b, V - 1) * n + m;
+            t0[b] = tx[k];
+            t1[b] = ty[k];
+        }
+        int c[kViewBatch], pos[kViewBatch];
+#pragma unroll
+        for (int b = 0; b < kViewBatch; b++) {
+            const bool ok = v0 + b < V && cell_valid(t0[b], t1[b], W, H);
+            c[b] = ok ? cell_of<KS>(t0[b], t1[b], Wc) : -1;
+            pos[b] = ok ? atomicAdd(cursor + c[b], 1) : 0;
+        }
+#pragma unroll
+        for (int b = 0; b < kViewBatch; b++)
+            if (c[b] >= 0) e_src[start[c[b]] + pos[b]] = (int)((size_t)(v0 + b) * n + m);
     }
 }
 
@@ -412,14 +431,22 @@ __global__ void k_rhs_diag_cached(int W, int H, float wscale, float c0, const fl
                                   const float *__restrict__ num, const float *__restrict__ sq, int precond,
                                   float *__restrict__ b, float *__restrict__ minv)
 {
-    const size_t N = (size_t)W * H;
-    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= N) return;
-    const int x = (int)(o % W), y = (int)(o / W);
-    const float dtg = beta != 0.0f ? beta * dtg_at(gx, gy, o, x, y, W, H) : 0.0f;
-    for (int c = 0; c < 3; c++) b[c * N + o] = wscale * num[c * N + o] + (c0 != 0.0f ? c0 * u0[c * N + o] : 0.0f) + dtg;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const size_t N = (size_t)W * H, o = (size_t)y * W + x;
+    // neighbours loaded unconditionally (clamped), masked afterwards
+    const float gxl = gx[x > 0 ? o - 1 : o], gxc = gx[o], gyu = gy[y > 0 ? o - W : o], gyc = gy[o];
+    const float sqv = sq[o];
+    float nm[3], uv[3] = {0.0f, 0.0f, 0.0f};
+    for (int c = 0; c < 3; c++) {
+        nm[c] = num[c * N + o];
+        if (c0 != 0.0f) uv[c] = u0[c * N + o];
+    }
+    const float dtg = (x > 0 ? gxl : 0.0f) - (x < W - 1 ? gxc : 0.0f) + (y > 0 ? gyu : 0.0f) - (y < H - 1 ? gyc : 0.0f);
+    const float bdt = beta != 0.0f ? beta * dtg : 0.0f;
+    for (int c = 0; c < 3; c++) b[c * N + o] = wscale * nm[c] + (c0 != 0.0f ? c0 * uv[c] : 0.0f) + bdt;
     if (precond) {
-        const float d = sq[o] * wscale + c0 + beta * (float)dtd_degree(x, y, W, H);
+        const float d = sqv * wscale + c0 + beta * (float)dtd_degree(x, y, W, H);
         minv[o] = d > 0.0f ? 1.0f / d : 1.0f;
     } else {
         minv[o] = 1.0f;
@@ -668,7 +695,7 @@ lfrv_status run_rhs_and_diag(lfrv_ctx *ctx, double alpha, double c0, double beta
     const float wscale = use_views ? (float)(alpha * s2) : 0.0f;
     TimedRegion tr(ctx, 5);
     if (use_views && setup_cache_valid(ctx)) {
-        k_rhs_diag_cached<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(
+        k_rhs_diag_cached<<<dim3((ctx->W + 63) / 64, (ctx->H + 3) / 4), dim3(64, 4), 0, ctx->stream>>>(
             ctx->W, ctx->H, wscale, (float)c0, u0, (float)beta, ctx->u_grad_x, ctx->u_grad_y, ctx->cache_num,
             ctx->cache_sq, use_precond ? 1 : 0, ctx->cg_b, ctx->cg_minv);
         LFRV_LAUNCH(ctx, "rhs / diag");

@@ -83,11 +83,13 @@ __global__ void k_key_to_depth(const int *__restrict__ zkey, float *__restrict__
 // Pass 2 (import.cpp:351-442): footprint-averaged target depth at tau, round
 // trip of the target pixel int(tau)+.5 into view v; a miss by more than tol
 // pixels marks the source pixel occluded.
-__global__ void k_tau_visibility(int V, int w, int h, int dsf, const float *__restrict__ ud,
+template <int KS>
+__global__ void k_tau_visibility(int V, int w, int h, const float *__restrict__ ud,
                                  const WarpCam *__restrict__ cams, float tol,
                                  float *__restrict__ tx, float *__restrict__ ty)
 {
     SOURCE_INDEX()
+    constexpr int dsf = KS - 1;
     const int W = w * dsf, H = h * dsf;
     const WarpCam &cam = cams[v];
     const float tux = tx[k], tuy = ty[k];
@@ -96,17 +98,21 @@ __global__ void k_tau_visibility(int V, int w, int h, int dsf, const float *__re
     const int bx = (int)floorf(tux - hd + 0.5f), by = (int)floorf(tuy - hd + 0.5f);
     // the footprint's depths are loaded unconditionally (clamped addresses,
     // masked below) so that they are in flight together
-    float zt[kMaxKs * kMaxKs];
-    for (int i = 0; i < dsf + 1; i++)
-        for (int j = 0; j < dsf + 1; j++) {
+    float zt[KS * KS];
+#pragma unroll
+    for (int i = 0; i < KS; i++)
+#pragma unroll
+        for (int j = 0; j < KS; j++) {
             const int px = min(max(bx + j, 0), W - 1), py = min(max(by + i, 0), H - 1);
-            zt[i * (dsf + 1) + j] = ud[(size_t)py * W + px];
+            zt[i * KS + j] = ud[(size_t)py * W + px];
         }
-    for (int i = 0; i < dsf + 1; i++)
-        for (int j = 0; j < dsf + 1; j++) {
+#pragma unroll
+    for (int i = 0; i < KS; i++)
+#pragma unroll
+        for (int j = 0; j < KS; j++) {
             const int px = bx + j, py = by + i;
             if (0 <= px && px <= W - 1 && 0 <= py && py <= H - 1) {
-                const float z = zt[i * (dsf + 1) + j];
+                const float z = zt[i * KS + j];
                 if (z == kInvalidDepth) continue;
                 float dx = 1.0f, dy = 1.0f;
                 if (j == 0) dx = ((float)px + 0.5f) - (tux - hd);
@@ -358,19 +364,20 @@ __global__ void k_weights(int V, int W, int H, int w, int h, const float *__rest
     const float xc = vtx[m], yc = vty[m];
     const float xl = vtx[ml], yl = vty[ml], xr = vtx[mr], yr = vty[mr];
     const float xu = vtx[mu], yu = vty[mu], xd = vtx[md], yd = vty[md];
-    auto inval = [&](float a, float b) { return a < 0.0f || a > (float)W || b < 0.0f || b > (float)H; };
-    float dw;
-    if (inval(xc, yc)) dw = 0.0f;
-    else if (mx == 0 || my == 0 || mx == w - 1 || my == h - 1) dw = 1.0f;
-    else if (inval(xl, yl) || inval(xr, yr) || inval(xu, yu) || inval(xd, yd)) dw = 1.0f;
-    else {
-        const float Dxx = 0.5f * (xr - xl);
-        const float Dyx = 0.5f * (yr - yl);
-        const float Dxy = 0.5f * (xd - xu);
-        const float Dyy = 0.5f * (yd - yu);
-        const float Nd = fabsf(Dxx * Dyy - Dxy * Dyx);
-        dw = (Nd > 1.0f) ? 1.0f / Nd : 1.0f;
-    }
+    // non-short-circuit tests: every load above is consumed unconditionally,
+    // which keeps the compiler from sinking them into dependent branches
+    const float Wf = (float)W, Hf = (float)H;
+    auto inval = [&](float a, float b) { return (a < 0.0f) | (a > Wf) | (b < 0.0f) | (b > Hf); };
+    const bool bad_c = inval(xc, yc);
+    const bool border = (mx == 0) | (my == 0) | (mx == w - 1) | (my == h - 1);
+    const bool bad_n = inval(xl, yl) | inval(xr, yr) | inval(xu, yu) | inval(xd, yd);
+    const float Dxx = 0.5f * (xr - xl);
+    const float Dyx = 0.5f * (yr - yl);
+    const float Dxy = 0.5f * (xd - xu);
+    const float Dyy = 0.5f * (yd - yu);
+    const float Nd = fabsf(Dxx * Dyy - Dxy * Dyx);
+    const float dwi = (Nd > 1.0f) ? 1.0f / Nd : 1.0f;
+    const float dw = bad_c ? 0.0f : ((border | bad_n) ? 1.0f : dwi);
     omega[k] = ang * dw;
 }
 
@@ -464,9 +471,15 @@ lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
         ctx->V, ctx->w, ctx->h, ctx->dsf, ctx->depth, dcams, ctx->p.sigma_z, ctx->tau_x, ctx->tau_y,
         ctx->dpart_x, ctx->dpart_y, zkey);
     k_key_to_depth<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(zkey, ctx->target_depth, ctx->N);
-    k_tau_visibility<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
-        ctx->V, ctx->w, ctx->h, ctx->dsf, ctx->target_depth, dcams, ctx->p.occlusion_tol_px,
-        ctx->tau_x, ctx->tau_y);
+#define TV(K) k_tau_visibility<K><<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->target_depth, dcams, ctx->p.occlusion_tol_px, ctx->tau_x, ctx->tau_y)
+    switch (ctx->ks) {
+    case 2: TV(2); break;
+    case 3: TV(3); break;
+    case 4: TV(4); break;
+    case 5: TV(5); break;
+    default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
+    }
+#undef TV
     LFRV_LAUNCH(ctx, "warps_from_depth kernels");
     ctx->prepared = false;
     ctx->epoch++;
