@@ -147,15 +147,15 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
         return LFRV_E_ARG;
     }
     ctx->pull_budget = (p->pull_lds_kb > 0 ? p->pull_lds_kb : 48) * 1024;
-    if (ctx->tile_w < ctx->ks || ctx->tile_h < ctx->ks || ctx->tile_w * ctx->tile_h > 1024 ||
-        (ctx->tile_w + ctx->ks - 1) * (ctx->tile_h + ctx->ks - 1) > 1024) {
+    if (ctx->tile_w < ctx->ks || ctx->tile_h < ctx->ks || ctx->tile_w * ctx->tile_h > kPullMaxThreads ||
+        (ctx->tile_w + ctx->ks - 1) * (ctx->tile_h + ctx->ks - 1) > kPullMaxThreads) {
         delete ctx;
         return LFRV_E_ARG;
     }
     {
         const int cells = (ctx->tile_w + ctx->ks - 1) * (ctx->tile_h + ctx->ks - 1);
         const int need = std::max(cells, ctx->tile_w * ctx->tile_h);
-        ctx->pull_bs = std::min(1024, (need + 63) / 64 * 64);
+        ctx->pull_bs = std::min(kPullMaxThreads, (need + 63) / 64 * 64);
     }
     ctx->ntx = (ctx->W + ctx->tile_w - 1) / ctx->tile_w;
     ctx->nty = (ctx->H + ctx->tile_h - 1) / ctx->tile_h;

@@ -217,7 +217,7 @@ struct PullCfg {
 
 // coverage = sum_i B_i^T 1
 template <int KS>
-__global__ void __launch_bounds__(1024) k_pull_coverage(TileGeom g, TileEntries te, PullCfg pc, float *__restrict__ coverage)
+__global__ void __launch_bounds__(kPullMaxThreads) k_pull_coverage(TileGeom g, TileEntries te, PullCfg pc, float *__restrict__ coverage)
 {
     PULL_PROLOGUE()
     any_pull<KS, 1, 0>(
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(1024) k_pull_coverage(TileGeom g, TileEntries 
 // U = sum B^T (omega v) / sum B^T omega where the weight is non-zero, else 0;
 // norm_mask = sum B^T omega.
 template <int KS>
-__global__ void __launch_bounds__(1024) k_pull_init_image(TileGeom g, TileEntries te, PullCfg pc, size_t n,
+__global__ void __launch_bounds__(kPullMaxThreads) k_pull_init_image(TileGeom g, TileEntries te, PullCfg pc, size_t n,
                                                          const float *__restrict__ views, float *__restrict__ U,
                                                          float *__restrict__ norm_mask)
 {
@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(1024) k_pull_init_image(TileGeom g, TileEntrie
 // sum B^T 1 is the coverage plane of lfrv_prepare (the same sums in the same
 // order as the u-gradient pull's own normaliser).
 template <int KS>
-__global__ void __launch_bounds__(1024) k_pull_gibr_setup(TileGeom g, TileEntries te, PullCfg pc, size_t n,
+__global__ void __launch_bounds__(kPullMaxThreads) k_pull_gibr_setup(TileGeom g, TileEntries te, PullCfg pc, size_t n,
                                                          const float *__restrict__ views,
                                                          const float *__restrict__ vgrad,
                                                          const float *__restrict__ coverage, float sigma2,
@@ -310,10 +310,96 @@ __global__ void __launch_bounds__(1024) k_pull_gibr_setup(TileGeom g, TileEntrie
         });
 }
 
+// dsf = 1: the set-up pull above fused with the assembly of the CG stencil
+// (the six cross Gram sums of each 2 x 2 footprint; the diagonal is the sq
+// sum).  omega is read through the staged source index, so the sorted entry
+// weights need no refresh.  gram = 5 planes D, E, S, SE, SW; D is sq.
+__global__ void __launch_bounds__(kPullMaxThreads) k_gibr_setup_st2(TileGeom g, TileEntries te, PullCfg pc, size_t n,
+                                                        const float *__restrict__ views,
+                                                        const float *__restrict__ vgrad,
+                                                        const float *__restrict__ coverage, float sigma2,
+                                                        float *__restrict__ U, float *__restrict__ norm_mask,
+                                                        float *__restrict__ ugx, float *__restrict__ ugy,
+                                                        float *__restrict__ tmask, float *__restrict__ num,
+                                                        float *__restrict__ gram)
+{
+    constexpr int NV = 7, NT = 4 * NV, NC = NT + 6;
+    extern __shared__ int smem[];
+    const int t = xcd_tile(blockIdx.x, g.ntiles);
+    const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
+    const int ww = g.tw + 1, wh = g.th + 1, ncw = ww * wh;
+    const unsigned nu = (unsigned)n;
+    float acc[NC];
+    cell_accumulate<2, NC, true>(
+        g, te, t, smem, pc.cap, acc,
+        [&](int k, int, int, const float(&wx)[2], const float(&wy)[2], float om, float(&a)[NC]) {
+            const unsigned vi = (unsigned)k / nu, m = (unsigned)k - vi * nu;
+            const float *vp = views + (size_t)vi * 3 * n + m;
+            const float v[NV] = {om * vp[0], om * vp[n], om * vp[2 * n], om, vgrad[(size_t)vi * 2 * n + m],
+                                 vgrad[(size_t)vi * 2 * n + n + m], om};
+            const float tl = wy[0] * wx[0], tr = wy[0] * wx[1], bl = wy[1] * wx[0], br = wy[1] * wx[1];
+            const float tap[4] = {tl, tr, bl, br};
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++)
+#pragma unroll
+                for (int q = 0; q < NV; q++) a[k2 * NV + q] += (q == 6 ? tap[k2] * tap[k2] : tap[k2]) * v[q];
+            a[NT + 0] += om * tl * tr;  // E of TL
+            a[NT + 1] += om * bl * br;  // E of BL
+            a[NT + 2] += om * tl * bl;  // S of TL
+            a[NT + 3] += om * tr * br;  // S of TR
+            a[NT + 4] += om * tl * br;  // SE of TL
+            a[NT + 5] += om * tr * bl;  // SW of TR
+        });
+    float *part = reinterpret_cast<float *>(smem + 2 * wh + 2);
+    const int c = threadIdx.x;
+    if (c < ncw) {
+#pragma unroll
+        for (int k = 0; k < NC; k++) part[k * ncw + c] = acc[k];
+    }
+    __syncthreads();
+    const size_t N = (size_t)g.W * g.H;
+    float *gD = gram, *gE = gram + N, *gS = gE + N, *gSE = gS + N, *gSW = gSE + N;
+    for (int p = threadIdx.x; p < g.tw * g.th; p += blockDim.x) {
+        const int lx = p % g.tw, ly = p / g.tw;
+        const int x = tx0 + lx, y = ty0 + ly;
+        if (x >= g.W || y >= g.H) continue;
+        // the cell whose top-left tap is (x - dx, y - dy) sits at local (lx + 1 - dx, ly + 1 - dy);
+        // there p is tap (i = dy, j = dx)
+        const int c00 = (ly + 1) * ww + lx + 1, c10 = c00 - 1, c01 = c00 - ww, c11 = c00 - ww - 1;
+        float sum[NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+            sum[q] = part[(0 * NV + q) * ncw + c00] + part[(1 * NV + q) * ncw + c10] +
+                     part[(2 * NV + q) * ncw + c01] + part[(3 * NV + q) * ncw + c11];
+        const size_t o = (size_t)y * g.W + x;
+        const float wsum = sum[3];
+        for (int ch = 0; ch < 3; ch++) {
+            U[ch * N + o] = (wsum != 0.0f) ? sum[ch] / wsum : sum[ch];
+            num[ch * N + o] = sum[ch];
+        }
+        norm_mask[o] = wsum;
+        const float nm = coverage[o];
+        float gx = sum[4], gy = sum[5], tm = sum[3];
+        if (nm != 0.0f) {
+            gx /= nm;
+            gy /= nm;
+            tm /= nm;
+        }
+        ugx[o] = gx;
+        ugy[o] = gy;
+        tmask[o] = tm * sigma2;
+        gD[o] = sum[6];
+        gE[o] = part[(NT + 0) * ncw + c00] + part[(NT + 1) * ncw + c01];
+        gS[o] = part[(NT + 2) * ncw + c00] + part[(NT + 3) * ncw + c10];
+        gSE[o] = part[(NT + 4) * ncw + c00];
+        gSW[o] = part[(NT + 5) * ncw + c10];
+    }
+}
+
 // Target gradient (init_u_gradient, gradientIBR.cu:858-913): pull (gx, gy,
 // omega, 1) with the footprint weights, normalise by the last plane.
 template <int KS>
-__global__ void __launch_bounds__(1024) k_pull_u_gradient(TileGeom g, TileEntries te, PullCfg pc, size_t n,
+__global__ void __launch_bounds__(kPullMaxThreads) k_pull_u_gradient(TileGeom g, TileEntries te, PullCfg pc, size_t n,
                                                          const float *__restrict__ vgrad, float sigma2,
                                                          float *__restrict__ ugx, float *__restrict__ ugy,
                                                          float *__restrict__ tmask, float *__restrict__ norm_mask)
@@ -348,7 +434,7 @@ __global__ void __launch_bounds__(1024) k_pull_u_gradient(TileGeom g, TileEntrie
 // vtv_sr_kernels.cu:1172-1194): the reference accumulates sum B^T 1 on top of
 // whatever its normalisation buffer held, so norm_mask += coverage here.
 template <int KS>
-__global__ void __launch_bounds__(1024) k_pull_reg_weight(TileGeom g, TileEntries te, PullCfg pc,
+__global__ void __launch_bounds__(kPullMaxThreads) k_pull_reg_weight(TileGeom g, TileEntries te, PullCfg pc,
                                                          const float *__restrict__ coverage, float sigma2,
                                                          float lmax, float lmin, float nviews,
                                                          float *__restrict__ norm_mask, float *__restrict__ graw)
@@ -385,7 +471,7 @@ __device__ __forceinline__ int dtd_degree(int x, int y, int W, int H)
 
 // CG right-hand side: b = alpha sum B^T (omega sigma^2 v) + c0 u0 + beta D^T g.
 template <int KS>
-__global__ void __launch_bounds__(1024) k_pull_rhs(TileGeom g, TileEntries te, PullCfg pc, size_t n,
+__global__ void __launch_bounds__(kPullMaxThreads) k_pull_rhs(TileGeom g, TileEntries te, PullCfg pc, size_t n,
                                                   const float *__restrict__ views, float wscale, float c0,
                                                   const float *__restrict__ u0, float beta,
                                                   const float *__restrict__ gx, const float *__restrict__ gy,
@@ -412,7 +498,7 @@ __global__ void __launch_bounds__(1024) k_pull_rhs(TileGeom g, TileEntries te, P
 
 // Jacobi preconditioner 1 / diag(A), diag = alpha sum omega sigma^2 B^2 + c0 + beta deg.
 template <int KS>
-__global__ void __launch_bounds__(1024) k_pull_diag(TileGeom g, TileEntries te, PullCfg pc, float wscale, float c0,
+__global__ void __launch_bounds__(kPullMaxThreads) k_pull_diag(TileGeom g, TileEntries te, PullCfg pc, float wscale, float c0,
                                                    float beta, float *__restrict__ minv)
 {
     PULL_PROLOGUE()
@@ -504,6 +590,7 @@ TileEntries tile_entries(const lfrv_ctx *ctx)
     t.tx = ctx->e_tx;
     t.ty = ctx->e_ty;
     t.w = ctx->e_w;
+    t.omega = ctx->omega;
     t.active = 1;
     return t;
 }
@@ -610,11 +697,25 @@ lfrv_status run_visibility(lfrv_ctx *ctx)
 lfrv_status run_initial_image(lfrv_ctx *ctx)
 {
     if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "initial image before lfrv_prepare");
-    lfrv_status st = run_refresh_entry_weights(ctx, 1.0f);
-    if (st != LFRV_OK) return st;
     const TileGeom g = tile_geom(ctx);
     const TileEntries te = tile_entries(ctx);
     TimedRegion tr(ctx, 5);
+    if (ctx->p.mode == LFRV_MODE_GRADIENT_IBR && ctx->ks == 2 && ctx->gram) {
+        // one pass over the entries: initial image, target gradient, CG sums
+        // and the CG stencil (omega read through the source index)
+        TileEntries tes = te;
+        PullCfg pc{pull_cap(ctx, 9, 0, ctx->pull_budget)};  // 4 * 9 = 36 >= 34 sums per window cell
+        const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
+        k_gibr_setup_st2<<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 9, pc.cap), ctx->stream>>>(
+            g, tes, pc, ctx->n, ctx->views, ctx->view_grad, ctx->coverage, s2, ctx->U, ctx->norm_mask,
+            ctx->u_grad_x, ctx->u_grad_y, ctx->target_mask, ctx->cache_num, ctx->gram);
+        LFRV_LAUNCH(ctx, "set-up pull");
+        ctx->setup_epoch = ctx->epoch;
+        ctx->gram_epoch = ctx->epoch;
+        return LFRV_OK;
+    }
+    lfrv_status st = run_refresh_entry_weights(ctx, 1.0f);
+    if (st != LFRV_OK) return st;
     if (ctx->p.mode == LFRV_MODE_GRADIENT_IBR) {
         // one pull for the initial image, the target gradient and the CG sums
         PullCfg pc{pull_cap(ctx, 7, 0, ctx->pull_budget)};
@@ -637,14 +738,12 @@ lfrv_status run_initial_image(lfrv_ctx *ctx)
 // True when the set-up pull ran on the current entries, weights and views.
 static bool setup_cache_valid(const lfrv_ctx *ctx)
 {
-    return ctx->setup_epoch == ctx->epoch && ctx->entry_weights_valid && ctx->prepared;
+    return ctx->setup_epoch == ctx->epoch && ctx->prepared;
 }
 
 lfrv_status run_u_gradient(lfrv_ctx *ctx)
 {
     if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "lfrv_init_u_gradient before lfrv_prepare");
-    lfrv_status st = run_refresh_entry_weights(ctx, 1.0f);
-    if (st != LFRV_OK) return st;
     if (setup_cache_valid(ctx)) {
         // g and the target mask came out of the set-up pull; the reference
         // leaves sum B^T 1 in its normalisation buffer here
@@ -652,6 +751,8 @@ lfrv_status run_u_gradient(lfrv_ctx *ctx)
                                      ctx->stream));
         return LFRV_OK;
     }
+    lfrv_status st = run_refresh_entry_weights(ctx, 1.0f);
+    if (st != LFRV_OK) return st;
     const TileGeom g = tile_geom(ctx);
     const TileEntries te = tile_entries(ctx);
     PullCfg pc{pull_cap(ctx, 4, 0, ctx->pull_budget)};
@@ -697,9 +798,14 @@ lfrv_status run_rhs_and_diag(lfrv_ctx *ctx, double alpha, double c0, double beta
     if (use_views && setup_cache_valid(ctx)) {
         k_rhs_diag_cached<<<dim3((ctx->W + 63) / 64, (ctx->H + 3) / 4), dim3(64, 4), 0, ctx->stream>>>(
             ctx->W, ctx->H, wscale, (float)c0, u0, (float)beta, ctx->u_grad_x, ctx->u_grad_y, ctx->cache_num,
-            ctx->cache_sq, use_precond ? 1 : 0, ctx->cg_b, ctx->cg_minv);
+            ctx->ks == 2 && ctx->gram ? ctx->gram : ctx->cache_sq,  // dsf = 1: sq is the stencil diagonal
+            use_precond ? 1 : 0, ctx->cg_b, ctx->cg_minv);
         LFRV_LAUNCH(ctx, "rhs / diag");
         return LFRV_OK;
+    }
+    if (use_views) {
+        lfrv_status st = run_refresh_entry_weights(ctx, 1.0f);
+        if (st != LFRV_OK) return st;
     }
     {
         PullCfg pc{pull_cap(ctx, 3, 0, ctx->pull_budget)};

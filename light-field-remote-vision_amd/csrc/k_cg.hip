@@ -21,7 +21,7 @@ namespace {
 // d = Minv r + beta d_prev, q = A d, written to dout / q, d.q reduced.
 // LDS: d tile with halo ks-1 (3 planes), then the pull's meta and slots.
 template <int KS>
-__global__ void __launch_bounds__(1024) k_cg_op(TileGeom g, TileEntries te, int cap, float wscale, float c0,
+__global__ void __launch_bounds__(kPullMaxThreads) k_cg_op(TileGeom g, TileEntries te, int cap, float wscale, float c0,
                                                float beta_reg, int mode, const float *__restrict__ din,
                                                const float *__restrict__ r, const float *__restrict__ minv,
                                                float *__restrict__ dout, float *__restrict__ q,
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_cg_alpha(CgState *st, cons
 // Assembly: each window cell sums its 4 squared and 6 cross tap products,
 // each pixel then adds the cells that hold it (cell (x0, y0) has taps
 // TL = (x0, y0), TR, BL, BR).
-__global__ void __launch_bounds__(1024) k_gram2(TileGeom g, TileEntries te, int cap, float *__restrict__ gD,
+__global__ void __launch_bounds__(kPullMaxThreads) k_gram2(TileGeom g, TileEntries te, int cap, float *__restrict__ gD,
                                                float *__restrict__ gE, float *__restrict__ gS,
                                                float *__restrict__ gSE, float *__restrict__ gSW)
 {
@@ -672,7 +672,8 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
 {
     auto t_start = std::chrono::steady_clock::now();
     if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "CG before lfrv_prepare");
-    if (use_views) {
+    // the sorted entry weights are needed unless the stencil is already assembled
+    if (use_views && !(ctx->ks == 2 && ctx->gram && ctx->gram_epoch == ctx->epoch)) {
         lfrv_status s = run_refresh_entry_weights(ctx, 1.0f);
         if (s != LFRV_OK) return s;
     }

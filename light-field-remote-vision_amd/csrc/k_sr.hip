@@ -25,7 +25,7 @@ struct DataTermArgs {
 // Writes F (into G), Uq = U and norm_mask = cov (the reference's buffer ends
 // holding the coverage).
 template <int KS>
-__global__ void __launch_bounds__(1024) k_sr_dataterm(TileGeom g, TileEntries te, int cap, size_t n, DataTermArgs da,
+__global__ void __launch_bounds__(kPullMaxThreads) k_sr_dataterm(TileGeom g, TileEntries te, int cap, size_t n, DataTermArgs da,
                                                      const float *__restrict__ U, const float *__restrict__ views,
                                                      const float *__restrict__ vlap, const float *__restrict__ coverage,
                                                      float *__restrict__ Gout, float *__restrict__ Uq,

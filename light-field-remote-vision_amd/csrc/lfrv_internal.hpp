@@ -18,6 +18,7 @@ namespace lfrv {
 constexpr int kMaxKs = 5;       // dsf <= 4
 constexpr int kBlock = 256;     // threads per block of the elementwise kernels
 constexpr int kNumXcd = 8;      // MI355X: 8 XCDs, blocks are dealt round-robin
+constexpr int kPullMaxThreads = 512;  // threads of the pull kernels at most (window cells, tile pixels)
 
 // Map a linear block id to a tile id so that each XCD receives a contiguous
 // run of tiles (neighbouring tiles then share that XCD's L2 for their halos).

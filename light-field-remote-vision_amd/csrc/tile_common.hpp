@@ -33,7 +33,8 @@ struct TileEntries {
     const int *cell_start;
     const int *src;
     const float *tx, *ty, *w;
-    int active;  // 0: the tile kernels see no entries
+    const float *omega;  // per-source weights (for pulls that read omega[src] instead of w)
+    int active;          // 0: the tile kernels see no entries
 };
 
 TileGeom tile_geom(const lfrv_ctx *ctx);
@@ -232,7 +233,7 @@ __host__ __device__ __forceinline__ size_t cell_pull_region_floats(int tw, int t
 // return every thread has finished reading the staged entries, so the region
 // may be overwritten.  Requires blockDim.x >= ncw and the caller's LDS tiles
 // complete (the first barrier below orders them).
-template <int KS, int NC, class EntryFn>
+template <int KS, int NC, bool W_FROM_SRC = false, class EntryFn>
 __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEntries &te, int t, int *meta, int ecap,
                                                 float (&acc)[NC], EntryFn entry_fn)
 {
@@ -280,7 +281,8 @@ __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEnt
                 const int e = rbeg[r] + (i - roff[r]);
                 a[k] = te.tx[e];
                 b[k] = te.ty[e];
-                w[k] = te.w[e];
+                // W_FROM_SRC: the third staged word is the source index
+                w[k] = W_FROM_SRC ? __int_as_float(te.src[e]) : te.w[e];
             }
         }
 #pragma unroll
@@ -307,20 +309,24 @@ __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEnt
             const int lbase = roff[r] - rbeg[r];
             for (int e = e0; e < e1; e++) {
                 float t0, t1, om;
+                int id = e;  // entry index, or the source index with W_FROM_SRC
                 if (staged) {
                     const int li = lbase + e;
                     t0 = stx[li];
                     t1 = sty[li];
-                    om = sw[li];
+                    if (W_FROM_SRC) id = __float_as_int(sw[li]);
+                    else om = sw[li];
                 } else {
                     t0 = te.tx[e];
                     t1 = te.ty[e];
-                    om = te.w[e];
+                    if (W_FROM_SRC) id = te.src[e];
+                    else om = te.w[e];
                 }
+                if (W_FROM_SRC) om = te.omega[id];
                 int x0, y0;
                 float wx[KS], wy[KS];
                 if (!footprint<KS>(t0, t1, g.W, g.H, x0, y0, wx, wy)) continue;
-                entry_fn(e, x0, y0, wx, wy, om, acc);
+                entry_fn(id, x0, y0, wx, wy, om, acc);
             }
         }
     }
