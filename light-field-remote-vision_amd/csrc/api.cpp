@@ -276,6 +276,7 @@ extern "C" lfrv_status lfrv_set_tau(lfrv_ctx *ctx, int i, const float *tau_x, co
     lfrv_status st = copy_in(ctx, ctx->tau_x + (size_t)i * ctx->n, tau_x, ctx->n);
     if (st == LFRV_OK) st = copy_in(ctx, ctx->tau_y + (size_t)i * ctx->n, tau_y, ctx->n);
     ctx->prepared = false;
+    ctx->counts_valid = false;
     ctx->epoch++;
     return st;
 }
@@ -502,7 +503,10 @@ extern "C" lfrv_status lfrv_set_buffer(lfrv_ctx *ctx, int which, const float *in
     if (which == LFRV_BUF_U) ctx->u_zero = false;
     lfrv_status st = copy_in(ctx, p, in, count);
     if (st != LFRV_OK) return st;
-    if (which == LFRV_BUF_TAU_X || which == LFRV_BUF_TAU_Y) ctx->prepared = false;
+    if (which == LFRV_BUF_TAU_X || which == LFRV_BUF_TAU_Y) {
+        ctx->prepared = false;
+        ctx->counts_valid = false;
+    }
     if (which == LFRV_BUF_OMEGA) ctx->entry_weights_valid = false;
     ctx->epoch++;  // any overwritten input invalidates the set-up sums
     if (which == LFRV_BUF_REG_G) ctx->has_reg_g = true;
@@ -531,7 +535,15 @@ extern "C" lfrv_status lfrv_get_timing(lfrv_ctx *ctx, int kclass, double *total_
 extern "C" lfrv_status lfrv_get_bin_stats(lfrv_ctx *ctx, int64_t *entries, int *tiles)
 {
     if (!ctx) return LFRV_E_ARG;
-    if (entries) *entries = ctx->e_total;
+    if (entries) {
+        int total = 0;
+        if (ctx->prepared) {
+            LFRV_HIP(ctx, hipMemcpyAsync(&total, ctx->cell_start + ctx->ncells, sizeof(int), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+            LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
+        *entries = total;
+    }
     if (tiles) *tiles = ctx->ntiles;
     return LFRV_OK;
 }

@@ -12,15 +12,6 @@ namespace lfrv {
 
 namespace {
 
-template <int KS>
-__device__ __forceinline__ int cell_of(float t0, float t1, int Wc)
-{
-    constexpr float hd = 0.5f * (float)(KS - 1);
-    const int x0 = (int)floorf(t0 - 0.5f - hd + 0.5f);
-    const int y0 = (int)floorf(t1 - 0.5f - hd + 0.5f);
-    return (y0 + KS - 1) * Wc + (x0 + KS - 1);
-}
-
 // Binning: one thread per source pixel position, looping over the views, so
 // the loads are coalesced and the V entries that land in the same target
 // neighbourhood are counted / stored by the same wave close in time.
@@ -85,10 +76,35 @@ __global__ void k_cell_fill(int V, int w, int h, int W, int H, int Wc, const flo
 constexpr int kSortCells = 256;
 constexpr int kSortCap = 4096;
 
-__global__ void __launch_bounds__(kSortCells) k_cell_sort(int ncells, const int *__restrict__ start,
+// Per-cell footprint tap sums sum_e a_e(i, j) of the cell's sorted entries
+// (the coverage pull's per-cell phase), written as tap planes [KS*KS][ncells].
+template <int KS>
+__device__ __forceinline__ void cell_tap_sums(int c, int a, int b, const int *keys, const float *__restrict__ tx,
+                                              const float *__restrict__ ty, int W, int H, int ncells,
+                                              float *__restrict__ cell_tap)
+{
+    float acc[KS * KS];
+#pragma unroll
+    for (int q = 0; q < KS * KS; q++) acc[q] = 0.0f;
+    for (int i = a; i < b; i++) {
+        const int k = keys[i];
+        int x0, y0;
+        float wx[KS], wy[KS];
+        if (!footprint<KS>(tx[k], ty[k], W, H, x0, y0, wx, wy)) continue;
+#pragma unroll
+        for (int ii = 0; ii < KS; ii++)
+#pragma unroll
+            for (int jj = 0; jj < KS; jj++) acc[ii * KS + jj] += wy[ii] * wx[jj];
+    }
+#pragma unroll
+    for (int q = 0; q < KS * KS; q++) cell_tap[(size_t)q * ncells + c] = acc[q];
+}
+
+template <int KS>
+__global__ void __launch_bounds__(kSortCells) k_cell_sort(int ncells, int W, int H, const int *__restrict__ start,
                                                          int *__restrict__ e_src, const float *__restrict__ tx,
                                                          const float *__restrict__ ty, float *__restrict__ e_tx,
-                                                         float *__restrict__ e_ty)
+                                                         float *__restrict__ e_ty, float *__restrict__ cell_tap)
 {
     __shared__ int keys[kSortCap];
     const int c0 = blockIdx.x * kSortCells;
@@ -117,6 +133,8 @@ __global__ void __launch_bounds__(kSortCells) k_cell_sort(int ncells, const int 
             e_tx[base + i] = tx[k];
             e_ty[base + i] = ty[k];
         }
+        if (cell_tap && c < c1) cell_tap_sums<KS>(c, start[c] - base, start[c + 1] - base, keys, tx, ty, W, H, ncells,
+                                                  cell_tap);
         return;
     }
     if (c >= c1) return;
@@ -135,6 +153,28 @@ __global__ void __launch_bounds__(kSortCells) k_cell_sort(int ncells, const int 
         e_tx[i] = tx[k];
         e_ty[i] = ty[k];
     }
+    if (cell_tap) cell_tap_sums<KS>(c, a, b, e_src, tx, ty, W, H, ncells, cell_tap);
+}
+
+// coverage(p) = sum of the tap sums of the cells whose footprint holds p, in
+// the order of the pull's pixel phase (taps (i, j) row-major, cell
+// (x + KS-1-j, y + KS-1-i)).
+template <int KS>
+__global__ void k_coverage_from_taps(int W, int H, int Wc, int ncells, const float *__restrict__ cell_tap,
+                                     float *__restrict__ coverage)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    float v[KS * KS];
+#pragma unroll
+    for (int i = 0; i < KS; i++)
+#pragma unroll
+        for (int j = 0; j < KS; j++)
+            v[i * KS + j] = cell_tap[(size_t)(i * KS + j) * ncells + (size_t)(y + KS - 1 - i) * Wc + (x + KS - 1 - j)];
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < KS * KS; q++) s += v[q];
+    coverage[(size_t)y * W + x] = s;
 }
 
 // Device-wide exclusive scan of `in` (n ints) into `out` (n + 1 ints):
@@ -196,11 +236,12 @@ __global__ void k_scan_add(int n, int *__restrict__ out, const int *__restrict__
     if (i < n) out[i] += bsum[i / (256 * kScanPer)];
 }
 
-__global__ void k_refresh_weights(int64_t ne, const int *__restrict__ e_src,
+// e_w = omega[e_src] for the first *total entries (total read on the device).
+__global__ void k_refresh_weights(int64_t cap, const int *__restrict__ total, const int *__restrict__ e_src,
                                   const float *__restrict__ omega, float *__restrict__ e_w)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < ne) e_w[e] = omega[e_src[e]];
+    if (e < cap && e < *total) e_w[e] = omega[e_src[e]];
 }
 
 // ---- pull kernels -----------------------------------------------------------
@@ -637,12 +678,16 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
     const TileGeom g = tile_geom(ctx);
     const int nc = ctx->ncells;
     TimedRegion tr(ctx, 5);
-    LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_count, 0, sizeof(int) * (nc + 1), ctx->stream));
-    LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_cursor, 0, sizeof(int) * nc, ctx->stream));
     const dim3 bgrid((ctx->w + 63) / 64, (ctx->h + 3) / 4), bblock(64, 4);
+    if (!ctx->counts_valid) {
+        // (lfrv_warps_from_depth counts the cells of the warps it produces)
+        LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_count, 0, sizeof(int) * (nc + 1), ctx->stream));
 #define CC(K) k_cell_count<K><<<bgrid, bblock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_count)
-    LFRV_KS_DISPATCH(ctx->ks, CC);
+        LFRV_KS_DISPATCH(ctx->ks, CC);
 #undef CC
+        ctx->counts_valid = true;
+    }
+    LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_cursor, 0, sizeof(int) * nc, ctx->stream));
     const int nb = (nc + 256 * kScanPer - 1) / (256 * kScanPer);
     k_scan_blocks<<<nb, 256, 0, ctx->stream>>>(nc, ctx->cell_count, ctx->cell_start, ctx->scan_tmp);
     k_scan_sums<<<1, 1024, 0, ctx->stream>>>(nb, ctx->scan_tmp, ctx->cell_start + nc);
@@ -650,19 +695,24 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
 #define CF(K) k_cell_fill<K><<<bgrid, bblock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_start, ctx->cell_cursor, ctx->e_src)
     LFRV_KS_DISPATCH(ctx->ks, CF);
 #undef CF
-    k_cell_sort<<<(nc + kSortCells - 1) / kSortCells, kSortCells, 0, ctx->stream>>>(
-        nc, ctx->cell_start, ctx->e_src, ctx->tau_x, ctx->tau_y, ctx->e_tx, ctx->e_ty);
+    // dsf = 1: the sort also forms the per-cell tap sums of the coverage
+    // (scratch: the single-reduction CG buffers, idle here)
+    float *cell_tap = (ctx->ks == 2 && ctx->cg_extra) ? ctx->cg_extra : nullptr;
+    const int sort_blocks = (nc + kSortCells - 1) / kSortCells;
+#define CS(K) k_cell_sort<K><<<sort_blocks, kSortCells, 0, ctx->stream>>>(nc, ctx->W, ctx->H, ctx->cell_start, ctx->e_src, ctx->tau_x, ctx->tau_y, ctx->e_tx, ctx->e_ty, cell_tap)
+    LFRV_KS_DISPATCH(ctx->ks, CS);
+#undef CS
     LFRV_LAUNCH(ctx, "splat structure");
-    int total = 0;
-    LFRV_HIP(ctx, hipMemcpyAsync(&total, ctx->cell_start + nc, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if ((size_t)total > ctx->e_cap) return fail(ctx, LFRV_E_ARG, "splat structure overflow");
-    ctx->e_total = total;
-    const TileEntries te = tile_entries(ctx);
-    PullCfg pc{pull_cap(ctx, 1, 0, ctx->pull_budget)};
+    if (cell_tap) {
+        k_coverage_from_taps<2><<<dim3((ctx->W + 63) / 64, (ctx->H + 3) / 4), dim3(64, 4), 0, ctx->stream>>>(
+            ctx->W, ctx->H, g.Wc, nc, cell_tap, ctx->coverage);
+    } else {
+        const TileEntries te = tile_entries(ctx);
+        PullCfg pc{pull_cap(ctx, 1, 0, ctx->pull_budget)};
 #define COV(K) k_pull_coverage<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap), ctx->stream>>>(g, te, pc, ctx->coverage)
-    LFRV_KS_DISPATCH(ctx->ks, COV);
+        LFRV_KS_DISPATCH(ctx->ks, COV);
 #undef COV
+    }
     LFRV_LAUNCH(ctx, "coverage");
     lfrv_status st = run_view_stencils(ctx);
     if (st != LFRV_OK) return st;
@@ -675,9 +725,8 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
 lfrv_status run_refresh_entry_weights(lfrv_ctx *ctx, float)
 {
     if (ctx->entry_weights_valid) return LFRV_OK;
-    if (ctx->e_total > 0)
-        k_refresh_weights<<<blocks_for((size_t)ctx->e_total), kBlock, 0, ctx->stream>>>(ctx->e_total, ctx->e_src,
-                                                                                       ctx->omega, ctx->e_w);
+    k_refresh_weights<<<blocks_for(ctx->e_cap), kBlock, 0, ctx->stream>>>(
+        (int64_t)ctx->e_cap, ctx->cell_start + ctx->ncells, ctx->e_src, ctx->omega, ctx->e_w);
     LFRV_LAUNCH(ctx, "refresh weights");
     ctx->entry_weights_valid = true;
     return LFRV_OK;

@@ -86,7 +86,7 @@ __global__ void k_key_to_depth(const int *__restrict__ zkey, float *__restrict__
 template <int KS>
 __global__ void k_tau_visibility(int V, int w, int h, const float *__restrict__ ud,
                                  const WarpCam *__restrict__ cams, float tol,
-                                 float *__restrict__ tx, float *__restrict__ ty)
+                                 float *__restrict__ tx, float *__restrict__ ty, int *__restrict__ cell_count)
 {
     SOURCE_INDEX()
     constexpr int dsf = KS - 1;
@@ -138,7 +138,10 @@ __global__ void k_tau_visibility(int V, int w, int h, const float *__restrict__ 
     if (ddx < -tol || tol < ddx || ddy < -tol || tol < ddy) {
         tx[k] = -(float)w * 2.0f;
         ty[k] = -(float)h * 2.0f;
+        return;
     }
+    // the final warp is known here: count it into the splat structure's cells
+    if (cell_valid(tux, tuy, W, H)) atomicAdd(cell_count + cell_of<KS>(tux, tuy, W + KS - 1), 1);
 }
 
 // v = S u (gradientIBR_kernels.cuh:196-290), C <= 4 planes.
@@ -471,7 +474,8 @@ lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
         ctx->V, ctx->w, ctx->h, ctx->dsf, ctx->depth, dcams, ctx->p.sigma_z, ctx->tau_x, ctx->tau_y,
         ctx->dpart_x, ctx->dpart_y, zkey);
     k_key_to_depth<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(zkey, ctx->target_depth, ctx->N);
-#define TV(K) k_tau_visibility<K><<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->target_depth, dcams, ctx->p.occlusion_tol_px, ctx->tau_x, ctx->tau_y)
+    LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_count, 0, sizeof(int) * ((size_t)ctx->ncells + 1), ctx->stream));
+#define TV(K) k_tau_visibility<K><<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->target_depth, dcams, ctx->p.occlusion_tol_px, ctx->tau_x, ctx->tau_y, ctx->cell_count)
     switch (ctx->ks) {
     case 2: TV(2); break;
     case 3: TV(3); break;
@@ -483,6 +487,7 @@ lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
     LFRV_LAUNCH(ctx, "warps_from_depth kernels");
     ctx->prepared = false;
     ctx->epoch++;
+    ctx->counts_valid = true;
     return LFRV_OK;
 }
 

@@ -44,6 +44,17 @@ __host__ __device__ __forceinline__ bool cell_valid(float tx, float ty, int W, i
     return !(px < 0 || py < 0 || px > W - 1 || py > H - 1);
 }
 
+// Cell of a cell-valid warped pixel in the extended grid of (W+KS-1) x
+// (H+KS-1) cells: the cell of footprint origin (x0, y0) is (x0+KS-1, y0+KS-1).
+template <int KS>
+__device__ __forceinline__ int cell_of(float t0, float t1, int Wc)
+{
+    constexpr float hd = 0.5f * (float)(KS - 1);
+    const int x0 = (int)floorf(t0 - 0.5f - hd + 0.5f);
+    const int y0 = (int)floorf(t1 - 0.5f - hd + 0.5f);
+    return (y0 + KS - 1) * Wc + (x0 + KS - 1);
+}
+
 // Footprint weights of one warped pixel (set_B_bilinear,
 // gradientIBR_kernels.cuh:447-507).  The 2-D weights are wy[i]*wx[j]; both
 // factors come out normalised over the in-image taps (the 2-D normaliser of
@@ -242,9 +253,9 @@ struct lfrv_ctx {
     int *e_src = nullptr;       // view * n + m
     float *e_tx = nullptr, *e_ty = nullptr, *e_w = nullptr;
     size_t e_cap = 0;
-    int64_t e_total = 0;
     bool prepared = false;
     bool entry_weights_valid = false;
+    bool counts_valid = false;  // cell_count holds the cell counts of the current warps
     uint64_t epoch = 1;        // bumped whenever entries, weights or views change
     uint64_t setup_epoch = 0;  // epoch of the last gradient-IBR set-up pull
     float *cache_num = nullptr, *cache_sq = nullptr;  // its raw sums: 3N and N
