@@ -15,6 +15,7 @@ import argparse
 import json
 import os
 import platform
+import re
 import sys
 import time
 
@@ -29,6 +30,17 @@ from lfrv import pipelines  # noqa: E402
 METRIC = "ms per novel view to rel. residual 1e-5 (9 views, 1024²); achieved HBM GB/s"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 TOL = 1e-5
+# rocprofv3 summary of the roofline kernel on this workload; it records the
+# HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE (copy-calibrated)
+PROFILE = os.path.join(ROOT, "profiles", "k_cgcg_st2_gradient_ibr_9views_1024.md")
+
+
+def measured_traffic():
+    try:
+        m = re.search(r"^traffic_bytes_per_launch: (\d+)", open(PROFILE).read(), re.M)
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def cell_valid_count(tx, ty, W, H):
@@ -81,6 +93,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=str, default="", help="tile_w x tile_h of the binned splat kernels")
+    ap.add_argument("--lds-kb", type=int, default=0, help="LDS budget of the pull kernels' entry cache (KiB)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the timed region, run the library's plain copy kernel once over MIB MiB "
                          "(reference dispatch for the HBM byte counters)")
@@ -96,6 +109,8 @@ def main():
     if args.tile:
         tw, th = (int(v) for v in args.tile.lower().split("x"))
         kw = dict(tile_w=tw, tile_h=th)
+    if args.lds_kb:
+        kw["pull_lds_kb"] = args.lds_kb
     ctx = lfrv.Context(pipelines.gradient_ibr_params(scene, **kw))
     pipelines.load_scene(ctx, scene)
     ctx.sync()
@@ -141,6 +156,7 @@ def main():
         op_kernel = "k_cg_op (fused PCG operator, entry walk)"
         op_bytes = 52 * N + 12 * e_unique
     op_avg_s = op_ms / max(op_n, 1) / 1e3
+    traffic = measured_traffic() if (scene["dsf"] == 1 and scene["V"] == 9 and N == 1024 * 1024) else None
     achieved = op_bytes / op_avg_s / 1e9
     up_bytes = 76 * N
     up_avg_s = up_ms / max(up_n, 1) / 1e3 if up_n else None
@@ -175,7 +191,7 @@ def main():
                    "seed": args.seed, "tile": tile},
         "roofline": {"bound": "hbm", "kernel": op_kernel,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": op_bytes, "avg_launch_us": round(op_avg_s * 1e6, 2),
                      "launches": op_n},
         "ms_per_step_median": round(float(np.median(per_step)), 3),

@@ -53,6 +53,23 @@ TimedRegion::~TimedRegion()
     if (pending(ctx).size() > 4096) flush_timing(ctx);
 }
 
+KernelEvents kernel_events(lfrv_ctx *ctx)
+{
+    KernelEvents ev;
+    if (!ctx->timing) return ev;
+    ev.a = get_event(ctx);
+    ev.b = ev.a ? get_event(ctx) : nullptr;
+    if (!ev.b) ev.a = nullptr;
+    return ev;
+}
+
+void record_kernel_events(lfrv_ctx *ctx, int k, const KernelEvents &ev)
+{
+    if (!ev.a || !ev.b) return;
+    pending(ctx).emplace_back(k, ev.a, ev.b);
+    if (pending(ctx).size() > 4096) flush_timing(ctx);
+}
+
 }  // namespace lfrv
 
 // pending event pairs live beside the context (kept out of the header)

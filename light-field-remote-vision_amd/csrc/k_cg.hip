@@ -11,6 +11,8 @@
 // d is double-buffered because neighbouring tiles read it in their halos.
 #include "tile_common.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <chrono>
 
 namespace lfrv {
@@ -302,12 +304,12 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cg_op_st2(int W, int H, int n
 // and reduces gamma, delta and r.r of the new residual.  r, s and w are read
 // across tiles, so they are double-buffered.
 //
-// start = 1: no update; w_0 = A (Minv r_0) and gamma_0, delta_0, rho_0.
+// START: no update; w_0 = A (Minv r_0) and gamma_0, delta_0, rho_0.
 // (A variant that lets every block of the next launch finish the reduction
 // itself measured slower: the redundant per-block sums cost more than the
 // one-block launch they replace.)
 struct CgcgArgs {
-    int W, H, ntx, ntiles, start;
+    int W, H, ntx, ntiles;
     float wscale, c0, beta_reg;
     const float *r_in, *w_in, *s_in;
     float *r_out, *w_out, *s_out, *p, *x;
@@ -315,20 +317,21 @@ struct CgcgArgs {
     const float *gD, *gE, *gS, *gSE, *gSW;
 };
 
+template <bool START>
 __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const CgState *st, double *partials)
 {
     constexpr int LW = kSt2W + 2, LH = kSt2H + 2, LP = LW * LH;
     __shared__ float ul[3 * LP];
     __shared__ float rl[3 * LP];  // r_{i+1}
     __shared__ double red[9 * 16];
-    if (!a.start && st->done) return;
+    if (!START && st->done) return;
     const int t = xcd_tile(blockIdx.x, a.ntiles);
     const int tx0 = (t % a.ntx) * kSt2W, ty0 = (t / a.ntx) * kSt2H;
     const int W = a.W, H = a.H;
     const size_t N = (size_t)W * H;
     float al[3] = {0.0f, 0.0f, 0.0f}, be[3] = {0.0f, 0.0f, 0.0f};
     int first = 1;
-    if (!a.start) {
+    if (!START) {
         first = st->first;
         for (int c = 0; c < 3; c++) {
             al[c] = (float)st->alpha[c];
@@ -347,7 +350,7 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const Cg
             for (int c = 0; c < 3; c++) {
                 const size_t k = c * N + o;
                 const float rv = a.r_in[k];
-                if (a.start) {
+                if (START) {
                     un[c] = mi * rv;
                     rnew[c] = rv;
                     continue;
@@ -761,7 +764,6 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         a.gS = gS;
         a.gSE = gSE;
         a.gSW = gSW;
-        a.start = 1;
         a.r_in = rbuf[0];
         a.w_in = wbuf[0];
         a.s_in = sbuf[0];
@@ -770,13 +772,16 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         a.s_out = sbuf[1];
         {
             TimedRegion tr(ctx, 5);
-            k_cgcg_st2<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(a, stt, part);
+            k_cgcg_st2<true><<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(a, stt, part);
         }
         k_cgcg_start<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, st2_tiles);
         LFRV_LAUNCH(ctx, "cg start");
-        a.start = 0;
+        // launch as many iterations as the last solve needed before the first
+        // look at the convergence flag, then small chunks
+        int next_chunk = std::max(1, ctx->cg_iter_hint);
         while (it < maxit && !done) {
-            const int nk = std::min(chunk, maxit - it);
+            const int nk = std::min(next_chunk, maxit - it);
+            next_chunk = 4;
             for (int k = 0; k < nk; k++) {
                 a.r_in = rbuf[cur];
                 a.w_in = wbuf[cur];
@@ -785,8 +790,10 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
                 a.w_out = wbuf[cur ^ 1];
                 a.s_out = sbuf[cur ^ 1];
                 {
-                    TimedRegion tr(ctx, 0);
-                    k_cgcg_st2<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(a, stt, part);
+                    const KernelEvents ev = kernel_events(ctx);
+                    hipExtLaunchKernelGGL(k_cgcg_st2<false>, dim3(st2_tiles), dim3(kSt2W * kSt2H), 0, ctx->stream,
+                                          ev.a, ev.b, 0, a, (const CgState *)stt, part);
+                    record_kernel_events(ctx, 0, ev);
                 }
                 k_cgcg_step<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, st2_tiles);
                 cur ^= 1;
@@ -797,6 +804,8 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
             LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
             done = ctx->cg_state_host->done != 0;
         }
+        const CgState &hs = *ctx->cg_state_host;
+        ctx->cg_iter_hint = std::max(hs.iter[0], std::max(hs.iter[1], hs.iter[2]));
         done = true;
     }
     while (it < maxit && !done) {

@@ -262,6 +262,7 @@ struct lfrv_ctx {
     float *gram = nullptr;     // dsf = 1: 5N raw stencil coefficients of sum_i S_i^T omega_i S_i
     float *cg_extra = nullptr; // dsf = 1: 9N, the second r, s and w buffers of the single-reduction PCG
     uint64_t gram_epoch = 0;
+    int cg_iter_hint = 8;      // iterations of the last single-reduction solve (first look at the flag)
     int pull_budget = 48 * 1024;  // LDS bytes of a pull's entry cache
 
     // target side (3 planes each unless noted)
@@ -336,6 +337,14 @@ struct TimedRegion {
     ~TimedRegion();
 };
 void flush_timing(lfrv_ctx *ctx);
+
+// Kernel-exact timing: an event pair handed to hipExtLaunchKernelGGL, which
+// stamps the kernel's own start and stop (no dispatch overhead in between).
+struct KernelEvents {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+KernelEvents kernel_events(lfrv_ctx *ctx);          // empty pair unless timing is on
+void record_kernel_events(lfrv_ctx *ctx, int k, const KernelEvents &ev);
 
 inline unsigned int blocks_for(size_t n, int bs = kBlock) { return (unsigned int)((n + bs - 1) / bs); }
 
