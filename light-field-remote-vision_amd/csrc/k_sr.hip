@@ -239,9 +239,10 @@ __global__ void __launch_bounds__(256) k_count_moving(size_t N, const float *__r
     __shared__ double red[16];
     double v[1] = {0.0};
     for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < N; o += (size_t)gridDim.x * blockDim.x) {
-        float mx = 0.0f;
-        for (int c = 0; c < 3; c++) mx = fmaxf(mx, fabsf(U[c * N + o] - prev[c * N + o]));
-        if (!(mx < thr)) v[0] += 1.0;
+        // the difference in double, as sr_compare_images takes it of its double planes
+        double mx = 0.0;
+        for (int c = 0; c < 3; c++) mx = fmax(mx, fabs((double)U[c * N + o] - (double)prev[c * N + o]));
+        if (!(mx < (double)thr)) v[0] += 1.0;
     }
     block_partials<1>(v, partials, red);
 }

@@ -1,0 +1,39 @@
+// Command-line driver with the reference's switches (main.cpp:56-79, :347-356):
+//   lfrv_cli -config scene.cfg -algorithm IBR_gradient [-name value ...] [+flag]
+// The configuration is parsed as coco::config does, written to <outdir>/config,
+// and the algorithm switch selects IBR_gradient or IBR_color.  The OpenGL
+// algorithms of the reference (IBR_direct, IBR_laplacian, refocus, IBR_optical,
+// fromMVE2coco, IBR_video and the default scene viewer) are not part of this
+// library and are reported as such.
+#include <cstdio>
+#include <exception>
+#include <string>
+
+#include "../../include/lfrv_ref.hpp"
+
+int main(int argc, char **argv)
+{
+    using namespace lfrv_ref;
+    Config cfg;
+    cfg.parse_command_line(argc, argv);
+    ConfigData cd;
+    cd.parse_config(cfg);
+    cfg.dump(cd._outdir + "/config");
+
+    std::string algo;
+    cfg.get_switch("algorithm", algo);
+    try {
+        if (algo == "IBR_gradient") {
+            IBR_gradient(&cd);
+        } else if (algo == "IBR_color") {
+            sr_synthesize_view(&cd);
+        } else {
+            std::fprintf(stderr, "lfrv_cli: algorithm '%s' is not provided (IBR_gradient, IBR_color)\n", algo.c_str());
+            return 2;
+        }
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "lfrv_cli: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

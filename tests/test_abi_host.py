@@ -16,9 +16,9 @@ import oracle
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+def declared_functions(headers=None):
     names = set()
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+    for h in headers or glob.glob(os.path.join(ROOT, "include", "*.h")):
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"//[^\n]*", "", text)
@@ -36,9 +36,30 @@ def test_headers_declare_the_boundary():
 
 
 def test_library_exports_every_declared_function():
+    io_h = os.path.join(ROOT, "include", "lfrv_io.h")
+    core = [h for h in glob.glob(os.path.join(ROOT, "include", "*.h")) if h != io_h]
     L = lfrv.lib()
-    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    missing = [n for n in declared_functions(core) if not hasattr(L, n)]
     assert not missing, missing
+    from lfrv import io as lio
+    H = lio.lib()
+    missing = [n for n in declared_functions([io_h]) if not hasattr(H, n)]
+    assert not missing, missing
+
+
+def test_host_library_exports_the_reference_named_interface():
+    """Every function include/lfrv_ref.hpp declares is defined in liblfrv_host.so."""
+    import subprocess
+    from lfrv import io as lio
+    text = open(os.path.join(ROOT, "include", "lfrv_ref.hpp")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    declared = set(re.findall(r"^[A-Za-z_][\w:<>,* ]*?[ *&]([A-Za-z_][A-Za-z0-9_]*)\(", text, flags=re.M))
+    assert {"init_data", "possion_conj_grad", "IBR_gradient", "coco_vtv_sr_iteration_fista_unstructured",
+            "sr_synthesize_view", "import_cam"} <= declared
+    out = subprocess.run(["nm", "-DC", "--defined-only", lio.HOST_LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    defined = set(re.findall(r"lfrv_ref::(\w+)\(", out))
+    assert not (declared - defined), sorted(declared - defined)
 
 
 def test_library_is_built_for_gfx950_only():

@@ -1,0 +1,193 @@
+"""The command-line driver end to end on the GPU: a scene written as the
+reference's files (PFM views and depth maps, CONTOUR camera files, a config
+file), run through lfrv_cli, compared with the same sequence over the C-ABI
+in this process and with the FP64 oracle.
+
+IBR_gradient follows gradientIBR.cpp:12-150, IBR_color super-resolution.cpp:54-338.
+"""
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import lfrv
+import oracle
+from lfrv import io as lio
+from lfrv import pipelines
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _write_cam(path, P):
+    P = np.asarray(P, np.float64).reshape(3, 4)
+    with open(path, "w") as f:
+        f.write("CONTOUR\n")
+        for r in range(3):
+            f.write(" ".join(f"{v:.17g}" for v in P[r]) + "\n")
+
+
+def _write_scene(d, s, s_rmv=-1):
+    """Views and depths at indices 0..V (skipping s_rmv), cameras likewise,
+    and the target camera at nviews (+1 when a view is removed), as
+    import_warps_from_depth reads it (import.cpp:254-258)."""
+    V = s["V"]
+    idx = [i for i in range(V + (1 if s_rmv != -1 else 0)) if i != s_rmv]
+    for v, i in enumerate(idx):
+        lio.save_image(str(d / f"v_{i:02d}.pfm"), s["views"][v])
+        lio.save_image(str(d / f"d_{i:02d}.pfm"), np.repeat(s["depths"][v][None], 3, 0))
+        _write_cam(d / f"cam_{i:02d}.txt", s["P_views"][v])
+    _write_cam(d / f"cam_{V + (1 if s_rmv != -1 else 0):02d}.txt", s["P_target"])
+    return 0, idx[-1]
+
+
+def _config(d, s, s_min, s_max, s_rmv, algorithm, extra):
+    cfg = d / "scene.cfg"
+    lines = [f"algorithm {algorithm}", f"lf_name {d}/v_%02d.pfm", f"depth_name {d}/d_%02d.pfm",
+             f"cam_name {d}/cam_%02d.txt", f"tau_name {d}/tau_%02d.pfm", f"outdir {d}/out", "outfile u.png",
+             f"s_min {s_min}", f"s_max {s_max}", f"s_rmv {s_rmv}", f"dsf {s['dsf']}", "nchannels 3",
+             "sigma_sensor 1.0"] + extra
+    cfg.write_text("\n".join(lines) + "\n")
+    return cfg
+
+
+def _run_cli(cfg):
+    print(f"lfrv_cli -config {cfg}", flush=True)
+    r = subprocess.run([lio.CLI_PATH, "-config", str(cfg)], capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+@pytest.mark.parametrize("s_rmv", [-1, 2])
+def test_cli_ibr_gradient_matches_abi_and_oracle(tmp_path, small_scene, s_rmv):
+    s = small_scene
+    s_min, s_max = _write_scene(tmp_path, s, s_rmv)
+    cfg = _config(tmp_path, s, s_min, s_max, s_rmv, "IBR_gradient", ["dt_alpha 1.0", "dt_beta 0.5"])
+    _run_cli(cfg)
+    out = tmp_path / "out"
+    U_cli = lio.load_image(str(out / "output.pfm"))
+    # the same sequence over the C-ABI in this process
+    prm = pipelines.gradient_ibr_params(s, dt_alpha=1.0, dt_beta=0.5, sigma_sensor=np.float32(1.0) / np.float32(255))
+    with lfrv.Context(prm) as ctx:
+        pipelines.load_scene(ctx, s)
+        ctx.clear_solution()
+        ctx.warps_from_depth()
+        ctx.prepare()
+        ctx.setup_visibility()
+        ctx.compute_weights()
+        ctx.compute_initial_image()
+        U0 = ctx.get_solution()
+        ctx.init_u_gradient()
+        st = ctx.cg_solve(1e-5, 5000)
+        U = ctx.get_solution()
+        tx = ctx.get_buffer("TAU_X").reshape(s["V"], s["h"], s["w"])
+        ty = ctx.get_buffer("TAU_Y").reshape(s["V"], s["h"], s["w"])
+    assert all(st["converged"])
+    assert np.max(np.abs(U_cli - U)) <= 1e-6 * max(1.0, np.abs(U).max())
+    # files of the driver: the PNG of the result and of the initial image, the tau PFMs
+    want_png = np.clip(np.trunc(U.astype(np.float64) * 255.0), 0, 255) / 255.0
+    assert np.max(np.abs(lio.load_image(str(out / "u.png")) - want_png)) <= 1.0 / 255.0 + 1e-7
+    want0 = np.clip(np.trunc(U0.astype(np.float64) * 255.0), 0, 255) / 255.0
+    assert np.max(np.abs(lio.load_image(str(out / "u_init_filled.png")) - want0)) <= 1.0 / 255.0 + 1e-7
+    idx = [i for i in range(s_max + 1) if i != s_rmv]
+    for v, i in enumerate(idx):
+        t = lio.load_image(str(tmp_path / f"tau_{i:02d}.pfm"))
+        assert np.array_equal(t[0], tx[v]) and np.array_equal(t[1], ty[v]) and np.array_equal(t[2], ty[v])
+    # and the oracle's pipeline on the same inputs
+    r = oracle.gibr_pipeline(s["views"], s["depths"], s["P_views"], s["P_target"], s["dsf"], 1 / 255.0, 1.0,
+                             1.0, 0.5, 1e-5, 2000)
+    assert rel_err(U_cli, r["U"]) < 1e-3
+
+
+def test_cli_ibr_gradient_from_tau_files(tmp_path, small_scene):
+    """Warps read from tau / dpart PFMs (import_warps) instead of depth maps."""
+    s = small_scene
+    with lfrv.Context(pipelines.gradient_ibr_params(s, dt_beta=0.5)) as ctx:
+        pipelines.load_scene(ctx, s)
+        ctx.warps_from_depth()
+        V, h, w = s["V"], s["h"], s["w"]
+        tx = ctx.get_buffer("TAU_X").reshape(V, h, w)
+        ty = ctx.get_buffer("TAU_Y").reshape(V, h, w)
+        dpx = ctx.get_buffer("DPART_X").reshape(V, h, w)
+        dpy = ctx.get_buffer("DPART_Y").reshape(V, h, w)
+        ctx.prepare()
+        ctx.setup_visibility()
+        ctx.compute_weights()
+        ctx.compute_initial_image()
+        ctx.init_u_gradient()
+        ctx.cg_solve(1e-5, 5000)
+        U = ctx.get_solution()
+    for v in range(V):
+        lio.save_image(str(tmp_path / f"v_{v:02d}.pfm"), s["views"][v])
+        lio.save_image(str(tmp_path / f"t_{v:02d}.pfm"), np.stack([tx[v], ty[v], ty[v]]))
+        # sigma_z in R is not read by the solver; dtau/dz in G and B
+        lio.save_image(str(tmp_path / f"p_{v:02d}.pfm"), np.stack([np.zeros_like(dpx[v]), dpx[v], dpy[v]]))
+    cfg = tmp_path / "scene.cfg"
+    cfg.write_text("\n".join([
+        "algorithm IBR_gradient", f"lf_name {tmp_path}/v_%02d.pfm", f"tau_name {tmp_path}/t_%02d.pfm",
+        f"dpart_name {tmp_path}/p_%02d.pfm", f"outdir {tmp_path}/out", "outfile u.pfm", "s_min 0",
+        f"s_max {V - 1}", f"dsf {s['dsf']}", "nchannels 3", "sigma_sensor 1", "dt_beta 0.5"]) + "\n")
+    _run_cli(cfg)
+    U_cli = lio.load_image(str(tmp_path / "out" / "u.pfm"))
+    assert np.max(np.abs(U_cli - U)) <= 1e-6 * max(1.0, np.abs(U).max())
+
+
+def _sr_driver(ctx, W, H, niter, nmeta):
+    """super-resolution.cpp:177-300 restated over the C-ABI in the reference's
+    float (cuflt) arithmetic, with host copies of the previous solutions."""
+    f = np.float32
+    EPS, MAXD, PCT = f(1e-4), f(1e-3), f(0.01)
+    prev_meta = ctx.get_solution()
+    prev = prev_meta.copy()
+    energy = prev_e = prev_e_rw = f(0)
+    for r in range(nmeta):
+        ctx.compute_weights()
+        ctx.sr_init_regularizer_weight()
+        energy = f(ctx.sr_primal_energy()[0])
+        prev_e = energy
+        for k in range(niter):
+            ctx.sr_iteration_fista()
+            energy = f(ctx.sr_primal_energy()[0])
+            if energy < EPS:
+                break
+            if abs((energy - prev_e) / energy) < EPS:
+                break
+            prev_e = energy
+            if f(ctx.sr_count_moving(prev, MAXD)) / f(W * H) < PCT:
+                break
+            prev = ctx.get_solution()
+        if energy < EPS:
+            if prev_e_rw < EPS:
+                break
+            prev_e_rw = energy
+            continue
+        if f(ctx.sr_count_moving(prev_meta, MAXD)) / f(W * H) < PCT:
+            break
+        prev_meta = ctx.get_solution()
+    return ctx.get_solution()
+
+
+def test_cli_ibr_color_matches_abi(tmp_path):
+    d = lfrv.synth_desc(1, 5)
+    d.w = d.h = 40
+    d.dsf = d.render_scale = 2
+    d.focal = 40.0
+    d.baseline = 0.7 * 4.0 / 80
+    s = lfrv.synth(d)
+    s_min, s_max = _write_scene(tmp_path, s)
+    niter, nmeta = 15, 3
+    cfg = _config(tmp_path, s, s_min, s_max, -1, "IBR_color",
+                  ["lambda 0.05", "lambda_max_factor 100", f"niter {niter}", f"nmeta_iter {nmeta}",
+                   "dt_alpha 1", "dt_beta 0.3", "gradient_step 1.0"])
+    _run_cli(cfg)
+    U_cli = lio.load_image(str(tmp_path / "out" / "output.pfm"))
+    prm = pipelines.color_sr_params(s, lambda_=0.05, lambda_max_factor=100.0, inner_iterations=niter, dt_beta=0.3,
+                                    gradient_step=1.0, sigma_sensor=np.float32(1.0) / np.float32(255))
+    with lfrv.Context(prm) as ctx:
+        pipelines.load_scene(ctx, s)
+        ctx.clear_solution()
+        pipelines.color_sr_setup(ctx)
+        U = _sr_driver(ctx, s["W"], s["H"], niter, nmeta)
+    assert np.max(np.abs(U_cli - U)) <= 1e-5
