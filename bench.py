@@ -116,7 +116,7 @@ def main():
     ctx.sync()
 
     for _ in range(args.warmup):
-        pipelines.gradient_ibr_solve(ctx, TOL, 5000)
+        ctx.ibr_gradient(TOL, 5000)
     ctx.sync()
 
     ctx.set_timing(True)
@@ -126,7 +126,7 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
-        st = pipelines.gradient_ibr_solve(ctx, TOL, 5000)
+        st = ctx.ibr_gradient(TOL, 5000)
         ctx.sync()
         per_step.append(1e3 * (time.perf_counter() - ts))
         iters.append(st["iterations"])

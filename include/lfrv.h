@@ -194,6 +194,15 @@ typedef struct {
  * warped intensity term of vtv_sr_dataterm_intensities (optimization.cu:1835-1966). */
 lfrv_status lfrv_cg_solve(lfrv_ctx *ctx, double tol, int maxit, lfrv_cg_stats *st);
 
+/* The device-resident part of the IBR_gradient driver (gradientIBR.cpp:12-150
+ * without its file I/O), in one call: U = 0 (init_data), warps from depth when
+ * every view's depth and the target camera are set (else the tau / dpart set
+ * with lfrv_set_tau / lfrv_set_partial_tau), lfrv_prepare,
+ * lfrv_setup_visibility, lfrv_compute_weights, lfrv_compute_initial_image
+ * (NULL), lfrv_init_u_gradient and lfrv_cg_solve (tol, maxit, st).  Same
+ * results as those calls made one by one. */
+lfrv_status lfrv_ibr_gradient(lfrv_ctx *ctx, double tol, int maxit, lfrv_cg_stats *st);
+
 /* The screened Poisson system possion_conj_grad sets up (solver.cu:361-505):
  * (lambda I + D^T D) u = lambda u0 + D^T g, u0 = the current solution,
  * plain CG (use_precond = 0) or Jacobi PCG, to ||r|| <= tol ||b||. */

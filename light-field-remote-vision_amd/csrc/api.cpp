@@ -227,6 +227,8 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     A(&ctx->partials, ctx->partials_cap);
     if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->cg_state_host, sizeof(CgState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->scal_host, 64 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->hole_count_host, sizeof(unsigned int), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->hole_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         lfrv_destroy(ctx);
@@ -253,6 +255,8 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
         if (p) hipFree(p);
     if (ctx->cg_state_host) hipHostFree(ctx->cg_state_host);
     if (ctx->scal_host) hipHostFree(ctx->scal_host);
+    if (ctx->hole_count_host) hipHostFree(ctx->hole_count_host);
+    if (ctx->hole_ev) hipEventDestroy(ctx->hole_ev);
     for (hipEvent_t ev : ctx->ev_pool) hipEventDestroy(ev);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -427,6 +431,24 @@ extern "C" lfrv_status lfrv_cg_solve(lfrv_ctx *ctx, double tol, int maxit, lfrv_
     return run_cg(ctx, ctx->p.dt_alpha, 0.0, ctx->p.dt_beta, true, true, false, tol, maxit, st);
 }
 
+extern "C" lfrv_status lfrv_ibr_gradient(lfrv_ctx *ctx, double tol, int maxit, lfrv_cg_stats *st)
+{
+    if (!ctx) return LFRV_E_ARG;
+    if (ctx->p.mode != LFRV_MODE_GRADIENT_IBR) return fail(ctx, LFRV_E_STATE, "lfrv_ibr_gradient on an IBR_color context");
+    if (!(tol > 0.0) || maxit < 0) return fail(ctx, LFRV_E_ARG, "bad CG arguments");
+    bool depth = ctx->have_target_cam;
+    for (int v = 0; v < ctx->V; v++) depth = depth && ctx->have_depth[v];
+    lfrv_status s = lfrv_clear_solution(ctx);                  // init_data: U = 0
+    if (s == LFRV_OK && depth) s = run_warps_from_depth(ctx);  // else the tau / dpart already set
+    if (s == LFRV_OK) s = run_prepare(ctx);
+    if (s == LFRV_OK) s = lfrv_setup_visibility(ctx);
+    if (s == LFRV_OK) s = run_compute_weights(ctx);
+    if (s == LFRV_OK) s = lfrv_compute_initial_image(ctx, nullptr);
+    if (s == LFRV_OK) s = run_u_gradient(ctx);
+    if (s == LFRV_OK) s = lfrv_cg_solve(ctx, tol, maxit, st);
+    return s;
+}
+
 extern "C" lfrv_status lfrv_poisson_cg(lfrv_ctx *ctx, double lambda, double tol, int maxit, int use_precond,
                                        lfrv_cg_stats *st)
 {
@@ -527,7 +549,10 @@ extern "C" lfrv_status lfrv_set_buffer(lfrv_ctx *ctx, int which, const float *in
     if (which == LFRV_BUF_OMEGA) ctx->entry_weights_valid = false;
     ctx->epoch++;  // any overwritten input invalidates the set-up sums
     if (which == LFRV_BUF_REG_G) ctx->has_reg_g = true;
-    if (which == LFRV_BUF_VISIBILITY) ctx->vis_ready = true;
+    if (which == LFRV_BUF_VISIBILITY) {
+        ctx->vis_ready = true;
+        ctx->holes_listed = false;  // the hole list follows the visibility
+    }
     return LFRV_OK;
 }
 

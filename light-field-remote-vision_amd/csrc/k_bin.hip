@@ -230,10 +230,14 @@ __global__ void k_scan_sums(int nb, int *__restrict__ bsum, int *__restrict__ to
     if (threadIdx.x == 1023) *total_out = tmp[1023];
 }
 
-__global__ void k_scan_add(int n, int *__restrict__ out, const int *__restrict__ bsum)
+// Adds the block offsets; also clears the fill cursors of the n cells.
+__global__ void k_scan_add(int n, int *__restrict__ out, const int *__restrict__ bsum, int *__restrict__ cursor)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] += bsum[i / (256 * kScanPer)];
+    if (i < n) {
+        out[i] += bsum[i / (256 * kScanPer)];
+        cursor[i] = 0;
+    }
 }
 
 // e_w = omega[e_src] for the first *total entries (total read on the device).
@@ -593,12 +597,20 @@ __global__ void k_fill(float *__restrict__ p, size_t N, float v)
 // (coverage > 0, setup_visibility_mask gradientIBR_kernels.cuh:4-54 sets the
 // flag for dx*dy != 0 and the normalised weights are positive exactly there);
 // COLOR_SR requires coverage >= 1 (vtv_sr_kernels_unstructured.cu:76-93).
-__global__ void k_visibility(size_t N, const float *__restrict__ coverage, int color, float *__restrict__ vis)
+// Visibility from the coverage (IBR_color: coverage >= 1; IBR_gradient: any
+// splat).  IBR_gradient also lists the invisible pixels (the holes) into
+// `list` / `count`, and clears `count_next` for the following pass.
+__global__ void k_visibility(size_t N, const float *__restrict__ coverage, int color, float *__restrict__ vis,
+                             int *__restrict__ list, unsigned int *__restrict__ count,
+                             unsigned int *__restrict__ count_next)
 {
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (list && o == 0) *count_next = 0u;
     if (o >= N) return;
     const float c = coverage[o];
-    vis[o] = color ? (c < 1.0f ? 0.0f : 1.0f) : (c > 0.0f ? 1.0f : 0.0f);
+    const float v = color ? (c < 1.0f ? 0.0f : 1.0f) : (c > 0.0f ? 1.0f : 0.0f);
+    vis[o] = v;
+    if (list && v == 0.0f) list[atomicAdd(count, 1u)] = (int)o;
 }
 
 __global__ void k_add_to(size_t N, const float *__restrict__ src, float *__restrict__ dst)
@@ -687,11 +699,10 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
 #undef CC
         ctx->counts_valid = true;
     }
-    LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_cursor, 0, sizeof(int) * nc, ctx->stream));
     const int nb = (nc + 256 * kScanPer - 1) / (256 * kScanPer);
     k_scan_blocks<<<nb, 256, 0, ctx->stream>>>(nc, ctx->cell_count, ctx->cell_start, ctx->scan_tmp);
     k_scan_sums<<<1, 1024, 0, ctx->stream>>>(nb, ctx->scan_tmp, ctx->cell_start + nc);
-    k_scan_add<<<blocks_for(nc), kBlock, 0, ctx->stream>>>(nc, ctx->cell_start, ctx->scan_tmp);
+    k_scan_add<<<blocks_for(nc), kBlock, 0, ctx->stream>>>(nc, ctx->cell_start, ctx->scan_tmp, ctx->cell_cursor);
 #define CF(K) k_cell_fill<K><<<bgrid, bblock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_start, ctx->cell_cursor, ctx->e_src)
     LFRV_KS_DISPATCH(ctx->ks, CF);
 #undef CF
@@ -736,10 +747,20 @@ lfrv_status run_visibility(lfrv_ctx *ctx)
 {
     if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "lfrv_setup_visibility before lfrv_prepare");
     const int color = ctx->p.mode == LFRV_MODE_COLOR_SR;
-    k_visibility<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(ctx->N, ctx->coverage, color, ctx->visibility);
+    unsigned int *cnt = ctx->counters + 8 + ctx->hole_parity, *cnt_next = ctx->counters + 8 + (ctx->hole_parity ^ 1);
+    k_visibility<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(ctx->N, ctx->coverage, color, ctx->visibility,
+                                                                 color ? nullptr : ctx->hole_list, cnt, cnt_next);
     if (color)  // the reference splats into its normalisation buffer without clearing it (optimization.cu:108-117)
         k_add_to<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(ctx->N, ctx->coverage, ctx->norm_mask);
     LFRV_LAUNCH(ctx, "visibility");
+    ctx->holes_listed = false;
+    if (!color) {
+        LFRV_HIP(ctx, hipMemcpyAsync(ctx->hole_count_host, cnt, sizeof(unsigned int), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        LFRV_HIP(ctx, hipEventRecord(ctx->hole_ev, ctx->stream));
+        ctx->hole_parity ^= 1;
+        ctx->holes_listed = true;
+    }
     return LFRV_OK;
 }
 

@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_gram2(TileGeom g, TileEntri
     }
 }
 
-constexpr int kSt2W = 64, kSt2H = 16;  // tile of the stencil kernels, one thread per pixel
+constexpr int kSt2W = 64, kSt2H = 8;  // tile of the stencil kernels, one thread per pixel (512 threads)
 
 // q = A din (mode 0) or, fused with the direction update (mode 1),
 // d = Minv r + beta d_prev, q = A d, d.q partials;
@@ -317,12 +317,64 @@ struct CgcgArgs {
     const float *gD, *gE, *gS, *gSE, *gSW;
 };
 
+// w = A u at pixel (x, y) from the LDS tile `up` (row stride LW, centre li):
+// the assembled 9-point stencil cf = {D, E, S, SE, SW, E(x-1), S(y-1),
+// SE(x-1,y-1), SW(x+1,y-1)} (zero where the neighbour is outside), the
+// diagonal shift c0 and the Neumann Laplacian D^T D.
+__device__ __forceinline__ float st2_apply(const float *up, int li, int LW, const float (&cf)[9], int x, int y, int W,
+                                           int H, float wscale, float c0, float beta_reg)
+{
+    const float uv = up[li];
+    const float sten = cf[0] * uv + cf[1] * up[li + 1] + cf[5] * up[li - 1] + cf[2] * up[li + LW] +
+                       cf[6] * up[li - LW] + cf[3] * up[li + LW + 1] + cf[7] * up[li - LW - 1] +
+                       cf[4] * up[li + LW - 1] + cf[8] * up[li - LW + 1];
+    float lap = 0.0f;
+    if (W > 1) {
+        if (x == 0) lap += uv - up[li + 1];
+        else if (x < W - 1) lap += 2.0f * uv - up[li - 1] - up[li + 1];
+        else lap += uv - up[li - 1];
+    }
+    if (H > 1) {
+        if (y == 0) lap += uv - up[li + LW];
+        else if (y < H - 1) lap += 2.0f * uv - up[li - LW] - up[li + LW];
+        else lap += uv - up[li - LW];
+    }
+    return wscale * sten + c0 * uv + beta_reg * lap;
+}
+
+// Coefficients of the stencil at pixel o = (xc, yc) (clamped into the image);
+// every load is issued unconditionally at an in-bounds address and masked
+// afterwards, so they all go out before the caller's barrier.
+__device__ __forceinline__ void st2_coeffs(const CgcgArgs &a, size_t o, int xc, int yc, float (&cf)[9])
+{
+    const int W = a.W;
+    const bool xm = xc > 0, ym = yc > 0, xp = xc < W - 1;
+    cf[0] = a.gD[o];
+    cf[1] = a.gE[o];
+    cf[2] = a.gS[o];
+    cf[3] = a.gSE[o];
+    cf[4] = a.gSW[o];
+    const float e = a.gE[xm ? o - 1 : o];
+    const float sm = a.gS[ym ? o - W : o];
+    const float se = a.gSE[(xm && ym) ? o - W - 1 : o];
+    const float sw = a.gSW[(xp && ym) ? o - W + 1 : o];
+    cf[5] = xm ? e : 0.0f;
+    cf[6] = ym ? sm : 0.0f;
+    cf[7] = (xm && ym) ? se : 0.0f;
+    cf[8] = (xp && ym) ? sw : 0.0f;
+}
+
+// One thread per tile pixel; the first 2 (LW + TH) threads also own one cell
+// of the halo ring, whose r_{i+1} and u_{i+1} are recomputed from r_i, w_i,
+// s_{i-1} (never written).  All global loads of the pass (the pixel's
+// vectors, its halo cell's, the stencil coefficients) are issued before the
+// one barrier.
 template <bool START>
 __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const CgState *st, double *partials)
 {
-    constexpr int LW = kSt2W + 2, LH = kSt2H + 2, LP = LW * LH;
+    constexpr int LW = kSt2W + 2, LH = kSt2H + 2, LP = LW * LH, NH = 2 * LW + 2 * kSt2H;
+    static_assert(NH <= kSt2W * kSt2H, "one thread per halo cell");
     __shared__ float ul[3 * LP];
-    __shared__ float rl[3 * LP];  // r_{i+1}
     __shared__ double red[9 * 16];
     if (!START && st->done) return;
     const int t = xcd_tile(blockIdx.x, a.ntiles);
@@ -338,78 +390,104 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const Cg
             be[c] = (float)st->beta[c];
         }
     }
-    for (int i = threadIdx.x; i < LP; i += blockDim.x) {
-        const int lx = i % LW, ly = i / LW;
-        const int x = tx0 - 1 + lx, y = ty0 - 1 + ly;
-        const bool in = (x >= 0 && x < W && y >= 0 && y < H);
-        float un[3] = {0.0f, 0.0f, 0.0f}, rnew[3] = {0.0f, 0.0f, 0.0f};
-        if (in) {
-            const size_t o = (size_t)y * W + x;
-            const float mi = a.minv[o];
-            const bool interior = lx >= 1 && lx <= kSt2W && ly >= 1 && ly <= kSt2H;
-            for (int c = 0; c < 3; c++) {
-                const size_t k = c * N + o;
-                const float rv = a.r_in[k];
-                if (START) {
-                    un[c] = mi * rv;
-                    rnew[c] = rv;
-                    continue;
-                }
-                const float sv = first ? a.w_in[k] : a.w_in[k] + be[c] * a.s_in[k];
-                const float rn = rv - al[c] * sv;
-                un[c] = mi * rn;
-                rnew[c] = rn;
-                if (interior) {
-                    const float uv = mi * rv;
-                    const float pv = first ? uv : uv + be[c] * a.p[k];
-                    a.p[k] = pv;
-                    a.x[k] += al[c] * pv;
-                    a.s_out[k] = sv;
-                    a.r_out[k] = rn;
-                }
+    // this thread's pixel
+    const int lx = threadIdx.x % kSt2W, ly = threadIdx.x / kSt2W;
+    const int x = tx0 + lx, y = ty0 + ly;
+    const bool pin = x < W && y < H;
+    const int xc = min(x, W - 1), yc = min(y, H - 1);
+    const size_t o = (size_t)yc * W + xc;
+    float cf[9];
+    st2_coeffs(a, o, xc, yc, cf);
+    const float mi = a.minv[o];
+    float rv[3], wv[3], sv[3], pv[3], xv[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const size_t k = c * N + o;
+        rv[c] = a.r_in[k];
+        wv[c] = sv[c] = pv[c] = xv[c] = 0.0f;
+        if (!START) {
+            wv[c] = a.w_in[k];
+            xv[c] = a.x[k];
+            if (!first) {
+                sv[c] = a.s_in[k];
+                pv[c] = a.p[k];
             }
         }
+    }
+    // this thread's halo cell
+    const int h = threadIdx.x;
+    const bool hact = h < NH;
+    int hlx, hly;
+    if (h < LW) {
+        hlx = h;
+        hly = 0;
+    } else if (h < 2 * LW) {
+        hlx = h - LW;
+        hly = LH - 1;
+    } else if (h < 2 * LW + kSt2H) {
+        hlx = 0;
+        hly = 1 + (h - 2 * LW);
+    } else {
+        hlx = LW - 1;
+        hly = 1 + (h - 2 * LW - kSt2H);
+    }
+    const int hx = tx0 - 1 + hlx, hy = ty0 - 1 + hly;
+    const bool hin = hact && hx >= 0 && hx < W && hy >= 0 && hy < H;
+    float hm = 0.0f, hr[3] = {0.0f, 0.0f, 0.0f}, hw[3] = {0.0f, 0.0f, 0.0f}, hs[3] = {0.0f, 0.0f, 0.0f};
+    if (hact) {
+        const size_t ho = (size_t)min(max(hy, 0), H - 1) * W + min(max(hx, 0), W - 1);
+        hm = a.minv[ho];
+#pragma unroll
         for (int c = 0; c < 3; c++) {
-            ul[c * LP + i] = un[c];
-            rl[c * LP + i] = rnew[c];
+            hr[c] = a.r_in[c * N + ho];
+            if (!START) {
+                hw[c] = a.w_in[c * N + ho];
+                if (!first) hs[c] = a.s_in[c * N + ho];
+            }
+        }
+    }
+    // vector updates: s_i, p_i, x_{i+1}, r_{i+1} on the pixel; u_{i+1} to LDS
+    float rn[3], un[3];
+    const int li = (ly + 1) * LW + lx + 1;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (START) {
+            rn[c] = rv[c];
+        } else {
+            const float s = first ? wv[c] : wv[c] + be[c] * sv[c];
+            rn[c] = rv[c] - al[c] * s;
+            const float uo = mi * rv[c];
+            const float pn = first ? uo : uo + be[c] * pv[c];
+            if (pin) {
+                const size_t k = c * N + o;
+                a.p[k] = pn;
+                a.x[k] = xv[c] + al[c] * pn;
+                a.s_out[k] = s;
+                a.r_out[k] = rn[c];
+            }
+        }
+        un[c] = pin ? mi * rn[c] : 0.0f;
+        ul[c * LP + li] = un[c];
+    }
+    if (hact) {
+        const int hl = hly * LW + hlx;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float r1 = hr[c];
+            if (!START) r1 -= al[c] * (first ? hw[c] : hw[c] + be[c] * hs[c]);
+            ul[c * LP + hl] = hin ? hm * r1 : 0.0f;
         }
     }
     __syncthreads();
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int lx = threadIdx.x % kSt2W, ly = threadIdx.x / kSt2W;
-    const int x = tx0 + lx, y = ty0 + ly;
-    if (x < W && y < H) {
-        const size_t o = (size_t)y * W + x;
-        const float cD = a.gD[o], cE = a.gE[o], cS = a.gS[o], cSE = a.gSE[o], cSW = a.gSW[o];
-        const float cEm = x > 0 ? a.gE[o - 1] : 0.0f;
-        const float cSm = y > 0 ? a.gS[o - W] : 0.0f;
-        const float cSEm = (x > 0 && y > 0) ? a.gSE[o - W - 1] : 0.0f;
-        const float cSWm = (x < W - 1 && y > 0) ? a.gSW[o - W + 1] : 0.0f;
-        const int li = (ly + 1) * LW + lx + 1;
+    if (pin) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float *up = ul + c * LP;
-            const float uv = up[li];
-            const float sten = cD * uv + cE * up[li + 1] + cEm * up[li - 1] + cS * up[li + LW] + cSm * up[li - LW] +
-                               cSE * up[li + LW + 1] + cSEm * up[li - LW - 1] + cSW * up[li + LW - 1] +
-                               cSWm * up[li - LW + 1];
-            float lap = 0.0f;
-            if (W > 1) {
-                if (x == 0) lap += uv - up[li + 1];
-                else if (x < W - 1) lap += 2.0f * uv - up[li - 1] - up[li + 1];
-                else lap += uv - up[li - 1];
-            }
-            if (H > 1) {
-                if (y == 0) lap += uv - up[li + LW];
-                else if (y < H - 1) lap += 2.0f * uv - up[li - LW] - up[li + LW];
-                else lap += uv - up[li - LW];
-            }
-            const float wv = a.wscale * sten + a.c0 * uv + a.beta_reg * lap;
-            a.w_out[c * N + o] = wv;
-            const float rn = rl[c * LP + li];
-            v[c] += (double)rn * (double)uv;
-            v[3 + c] += (double)wv * (double)uv;
-            v[6 + c] += (double)rn * (double)rn;
+            const float wn = st2_apply(ul + c * LP, li, LW, cf, x, y, W, H, a.wscale, a.c0, a.beta_reg);
+            a.w_out[c * N + o] = wn;
+            v[c] += (double)rn[c] * (double)un[c];
+            v[3 + c] += (double)wn * (double)un[c];
+            v[6 + c] += (double)rn[c] * (double)rn[c];
         }
     }
     block_partials<9>(v, partials, red);
@@ -742,7 +820,7 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     LFRV_LAUNCH(ctx, "cg init");
     int it = 0, cur = 0;
     const int chunk = 8;
-    bool done = false;
+    bool done = false, final_ready = false;
     if (st2) {
         // single-reduction iterations on the stencil; r, s, w double-buffered
         float *rbuf[2] = {ctx->cg_r, ctx->cg_extra};
@@ -800,9 +878,24 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
             }
             LFRV_LAUNCH(ctx, "cg iteration");
             it += nk;
+            // the true residual of the current x goes out with the poll: if the
+            // chunk converged it is the answer, and the host waits once.
+            // (q of the final operator is a buffer the iterations do not use.)
+            {
+                TimedRegion tr(ctx, 5);
+                k_cg_op_st2<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(
+                    ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, (float)beta, 0, x, r, minv, nullptr,
+                    ctx->G, gD, gE, gS, gSE, gSW, stt, part);
+            }
+            k_resid<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, ctx->G, part);
+            k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, (int)rb, ctx->scal);
+            LFRV_LAUNCH(ctx, "cg residual");
             LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+            LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                         ctx->stream));
             LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
             done = ctx->cg_state_host->done != 0;
+            final_ready = true;
         }
         const CgState &hs = *ctx->cg_state_host;
         ctx->cg_iter_hint = std::max(hs.iter[0], std::max(hs.iter[1], hs.iter[2]));
@@ -831,23 +924,26 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         done = ctx->cg_state_host->done != 0;
     }
 #undef LAUNCH_OP
-    // final state and true residual
-    {
-        TimedRegion tr(ctx, 5);
-        switch (ctx->ks) {
-        case 2: OPK(2, 0, x, nullptr); break;
-        case 3: OPK(3, 0, x, nullptr); break;
-        case 4: OPK(4, 0, x, nullptr); break;
-        case 5: OPK(5, 0, x, nullptr); break;
+    // final state and true residual (the stencil path has it with its last poll)
+    if (!final_ready) {
+        {
+            TimedRegion tr(ctx, 5);
+            switch (ctx->ks) {
+            case 2: OPK(2, 0, x, nullptr); break;
+            case 3: OPK(3, 0, x, nullptr); break;
+            case 4: OPK(4, 0, x, nullptr); break;
+            case 5: OPK(5, 0, x, nullptr); break;
+            }
         }
+        k_resid<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, part);
+        k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, (int)rb, ctx->scal);
+        LFRV_LAUNCH(ctx, "cg residual");
+        LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+        LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
 #undef OPK
-    k_resid<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, part);
-    k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, (int)rb, ctx->scal);
-    LFRV_LAUNCH(ctx, "cg residual");
-    LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
-    LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
     flush_timing(ctx);
     const CgState &hs = *ctx->cg_state_host;
     lfrv_status status = LFRV_OK;

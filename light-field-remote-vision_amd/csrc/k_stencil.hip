@@ -306,12 +306,18 @@ __global__ void __launch_bounds__(256) k_pd_step(int W, int H, float tau, float 
 lfrv_status run_hole_filling(lfrv_ctx *ctx, int iters, double eps, int *ran)
 {
     const size_t N = ctx->N;
-    unsigned int *cnt = ctx->counters + 8;
-    LFRV_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(unsigned int), ctx->stream));
-    k_hole_list<<<blocks_for(N), kBlock, 0, ctx->stream>>>(N, ctx->visibility, ctx->hole_list, cnt);
     unsigned int nh = 0;
-    LFRV_HIP(ctx, hipMemcpyAsync(&nh, cnt, sizeof nh, hipMemcpyDeviceToHost, ctx->stream));
-    LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->holes_listed) {
+        // listed by the visibility pass; its length arrived behind hole_ev
+        LFRV_HIP(ctx, hipEventSynchronize(ctx->hole_ev));
+        nh = *ctx->hole_count_host;
+    } else {
+        unsigned int *cnt = ctx->counters + 10;
+        LFRV_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(unsigned int), ctx->stream));
+        k_hole_list<<<blocks_for(N), kBlock, 0, ctx->stream>>>(N, ctx->visibility, ctx->hole_list, cnt);
+        LFRV_HIP(ctx, hipMemcpyAsync(&nh, cnt, sizeof nh, hipMemcpyDeviceToHost, ctx->stream));
+        LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     ctx->n_holes = (int)nh;
     if (ran) *ran = 0;
     if (nh == 0 || iters <= 0) return LFRV_OK;

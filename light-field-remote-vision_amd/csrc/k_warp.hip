@@ -68,10 +68,12 @@ __global__ void k_tau_from_depth(int V, int w, int h, int dsf, const float *__re
         }
 }
 
-__global__ void k_fill_key(int *__restrict__ zkey, size_t N, int key)
+// Fills the z-buffer keys; also clears the ncnt cell counters.
+__global__ void k_fill_key(int *__restrict__ zkey, size_t N, int key, int *__restrict__ cnt, size_t ncnt)
 {
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o < N) zkey[o] = key;
+    if (o < ncnt) cnt[o] = 0;
 }
 
 __global__ void k_key_to_depth(const int *__restrict__ zkey, float *__restrict__ depth, size_t N)
@@ -469,12 +471,13 @@ lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
         const float f = kInvalidDepth;
         std::memcpy(&inv_key, &f, sizeof inv_key);  // positive float: key == bit pattern
     }
-    k_fill_key<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(zkey, ctx->N, inv_key);
+    const size_t ncnt = (size_t)ctx->ncells + 1;
+    k_fill_key<<<blocks_for(std::max(ctx->N, ncnt)), kBlock, 0, ctx->stream>>>(zkey, ctx->N, inv_key, ctx->cell_count,
+                                                                              ncnt);
     k_tau_from_depth<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
         ctx->V, ctx->w, ctx->h, ctx->dsf, ctx->depth, dcams, ctx->p.sigma_z, ctx->tau_x, ctx->tau_y,
         ctx->dpart_x, ctx->dpart_y, zkey);
     k_key_to_depth<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(zkey, ctx->target_depth, ctx->N);
-    LFRV_HIP(ctx, hipMemsetAsync(ctx->cell_count, 0, sizeof(int) * ((size_t)ctx->ncells + 1), ctx->stream));
 #define TV(K) k_tau_visibility<K><<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->target_depth, dcams, ctx->p.occlusion_tol_px, ctx->tau_x, ctx->tau_y, ctx->cell_count)
     switch (ctx->ks) {
     case 2: TV(2); break;

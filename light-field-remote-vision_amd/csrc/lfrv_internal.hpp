@@ -291,6 +291,14 @@ struct lfrv_ctx {
     double *scal = nullptr;        // device scalars (reduction outputs)
     double *scal_host = nullptr;   // pinned mirror
     int *hole_list = nullptr;      // N
+    // hole list built by the visibility pass (IBR_gradient): its length is
+    // copied to pinned memory behind an event, so the hole filling reads it
+    // without draining the stream; the two device counters alternate, each
+    // pass clearing the one the next pass uses
+    unsigned int *hole_count_host = nullptr;  // pinned
+    hipEvent_t hole_ev = nullptr;
+    int hole_parity = 0;
+    bool holes_listed = false;
     int n_holes = 0;
     int hole_iters = 0;            // sweeps run by the last hole filling
     float *pp_scratch = nullptr;   // push-pull pyramid scratch

@@ -106,6 +106,7 @@ def lib():
         "lfrv_clear_solution": (st, [_vp]),
         "lfrv_init_u_gradient": (st, [_vp]),
         "lfrv_cg_solve": (st, [_vp, ctypes.c_double, st, ctypes.POINTER(CgStats)]),
+        "lfrv_ibr_gradient": (st, [_vp, ctypes.c_double, st, ctypes.POINTER(CgStats)]),
         "lfrv_poisson_cg": (st, [_vp, ctypes.c_double, ctypes.c_double, st, st, ctypes.POINTER(CgStats)]),
         "lfrv_gibr_energy": (st, [_vp, _dp]),
         "lfrv_sr_init_regularizer_weight": (st, [_vp]),
@@ -227,6 +228,14 @@ class Context:
     def cg_solve(self, tol=1e-5, maxit=1000, check=True):
         s = CgStats()
         st = lib().lfrv_cg_solve(self._h, tol, maxit, ctypes.byref(s))
+        if check:
+            self._check(st)
+        return s.as_dict()
+
+    def ibr_gradient(self, tol=1e-5, maxit=5000, check=True):
+        """The whole device-resident IBR_gradient sequence in one call (lfrv_ibr_gradient)."""
+        s = CgStats()
+        st = lib().lfrv_ibr_gradient(self._h, tol, maxit, ctypes.byref(s))
         if check:
             self._check(st)
         return s.as_dict()
