@@ -40,7 +40,7 @@ TimedRegion::TimedRegion(lfrv_ctx *c, int kk) : ctx(c), k(kk)
 {
     if (!ctx->timing) return;
     a = get_event(ctx);
-    if (a) hipEventRecord(a, ctx->stream);
+    if (a) (void)hipEventRecord(a, ctx->stream);
 }
 
 TimedRegion::~TimedRegion()
@@ -48,7 +48,7 @@ TimedRegion::~TimedRegion()
     if (!ctx->timing || !a) return;
     b = get_event(ctx);
     if (!b) return;
-    hipEventRecord(b, ctx->stream);
+    (void)hipEventRecord(b, ctx->stream);
     pending(ctx).emplace_back(k, a, b);
     if (pending(ctx).size() > 4096) flush_timing(ctx);
 }
@@ -82,7 +82,7 @@ void flush_timing(lfrv_ctx *ctx)
 {
     auto it = g_pending.find(ctx);
     if (it == g_pending.end() || it->second.empty()) return;
-    hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
     for (auto &t : it->second) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, std::get<1>(t), std::get<2>(t)) == hipSuccess) {
@@ -220,6 +220,7 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     A(&ctx->pp_scratch, ctx->pp_cap);
     A(&ctx->cg_state, 1);
     A(&ctx->counters, 32);
+    A(&ctx->dcams, V);
     A(&ctx->scal, 64);
     const size_t cam_doubles = (V * sizeof(WarpCam) + 7) / 8;
     const size_t st2_tiles = (size_t)((ctx->W + 63) / 64) * ((ctx->H + 3) / 4);  // stencil operator tiles
@@ -241,7 +242,7 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
 extern "C" void lfrv_destroy(lfrv_ctx *ctx)
 {
     if (!ctx) return;
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     flush_timing(ctx);
     g_pending.erase(ctx);
     for (void *p : std::initializer_list<void *>{
@@ -250,15 +251,15 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
              ctx->e_w, ctx->U, ctx->Uq, ctx->G, ctx->PX, ctx->PY, ctx->U2, ctx->P2X, ctx->P2Y, ctx->tmp3N, ctx->cg_r,
              ctx->cg_d, ctx->cg_q, ctx->cg_b, ctx->reg_g, ctx->norm_mask, ctx->coverage, ctx->u_grad_x,
              ctx->u_grad_y, ctx->target_mask, ctx->visibility, ctx->target_depth, ctx->tmpN, ctx->cg_minv,
-             ctx->hole_list, ctx->pp_scratch, ctx->cg_state, ctx->counters, ctx->scal, ctx->partials,
+             ctx->hole_list, ctx->pp_scratch, ctx->cg_state, ctx->counters, ctx->dcams, ctx->scal, ctx->partials,
              ctx->cache_num, ctx->cache_sq, ctx->gram, ctx->cg_extra})
-        if (p) hipFree(p);
-    if (ctx->cg_state_host) hipHostFree(ctx->cg_state_host);
-    if (ctx->scal_host) hipHostFree(ctx->scal_host);
-    if (ctx->hole_count_host) hipHostFree(ctx->hole_count_host);
-    if (ctx->hole_ev) hipEventDestroy(ctx->hole_ev);
-    for (hipEvent_t ev : ctx->ev_pool) hipEventDestroy(ev);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+        if (p) (void)hipFree(p);
+    if (ctx->cg_state_host) (void)hipHostFree(ctx->cg_state_host);
+    if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
+    if (ctx->hole_count_host) (void)hipHostFree(ctx->hole_count_host);
+    if (ctx->hole_ev) (void)hipEventDestroy(ctx->hole_ev);
+    for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -332,6 +333,7 @@ extern "C" lfrv_status lfrv_set_depth(lfrv_ctx *ctx, int i, const float *depth, 
     if (st != LFRV_OK) return st;
     std::memcpy(ctx->P_views.data() + 12 * i, P, 12 * sizeof(double));
     ctx->have_depth[i] = 1;
+    ctx->cams_valid = false;
     return LFRV_OK;
 }
 
@@ -340,6 +342,7 @@ extern "C" lfrv_status lfrv_set_target_camera(lfrv_ctx *ctx, const double *P)
     if (!ctx || !P) return LFRV_E_ARG;
     std::memcpy(ctx->P_target, P, 12 * sizeof(double));
     ctx->have_target_cam = true;
+    ctx->cams_valid = false;
     return LFRV_OK;
 }
 
@@ -438,8 +441,11 @@ extern "C" lfrv_status lfrv_ibr_gradient(lfrv_ctx *ctx, double tol, int maxit, l
     if (!(tol > 0.0) || maxit < 0) return fail(ctx, LFRV_E_ARG, "bad CG arguments");
     bool depth = ctx->have_target_cam;
     for (int v = 0; v < ctx->V; v++) depth = depth && ctx->have_depth[v];
-    lfrv_status s = lfrv_clear_solution(ctx);                  // init_data: U = 0
-    if (s == LFRV_OK && depth) s = run_warps_from_depth(ctx);  // else the tau / dpart already set
+    // init_data's U = 0 only enters through the weights, which take the
+    // zero-gradient form for it; compute_initial_image then writes all of U
+    ctx->u_zero = true;
+    lfrv_status s = LFRV_OK;
+    if (depth) s = run_warps_from_depth(ctx);  // else the tau / dpart already set
     if (s == LFRV_OK) s = run_prepare(ctx);
     if (s == LFRV_OK) s = lfrv_setup_visibility(ctx);
     if (s == LFRV_OK) s = run_compute_weights(ctx);

@@ -204,91 +204,6 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_gram2(TileGeom g, TileEntri
 
 constexpr int kSt2W = 64, kSt2H = 8;  // tile of the stencil kernels, one thread per pixel (512 threads)
 
-// q = A din (mode 0) or, fused with the direction update (mode 1),
-// d = Minv r + beta d_prev, q = A d, d.q partials;
-// A d = wscale * stencil(d) + c0 d + beta_reg D^T D d.
-__global__ void __launch_bounds__(kSt2W * kSt2H) k_cg_op_st2(int W, int H, int ntx, int ntiles, float wscale, float c0,
-                                                  float beta_reg, int mode, const float *__restrict__ din,
-                                                  const float *__restrict__ r, const float *__restrict__ minv,
-                                                  float *__restrict__ dout, float *__restrict__ q,
-                                                  const float *__restrict__ gD, const float *__restrict__ gE,
-                                                  const float *__restrict__ gS, const float *__restrict__ gSE,
-                                                  const float *__restrict__ gSW, const CgState *st,
-                                                  double *partials)
-{
-    constexpr int LW = kSt2W + 2, LH = kSt2H + 2, LP = LW * LH;
-    __shared__ float dl[3 * LP];
-    __shared__ double red[3 * 16];
-    if (mode == 1 && st->done) return;
-    const int t = xcd_tile(blockIdx.x, ntiles);
-    const int tx0 = (t % ntx) * kSt2W, ty0 = (t / ntx) * kSt2H;
-    const size_t N = (size_t)W * H;
-    float bc[3] = {0.0f, 0.0f, 0.0f};
-    int first = 1;
-    if (mode == 1) {
-        first = st->first;
-        for (int c = 0; c < 3; c++) bc[c] = (float)st->beta[c];
-    }
-    for (int i = threadIdx.x; i < LP; i += blockDim.x) {
-        const int lx = i % LW, ly = i / LW;
-        const int x = tx0 - 1 + lx, y = ty0 - 1 + ly;
-        const bool in = (x >= 0 && x < W && y >= 0 && y < H);
-        const size_t o = in ? (size_t)y * W + x : 0;
-        float v[3] = {0.0f, 0.0f, 0.0f};
-        if (in) {
-            if (mode == 0) {
-                for (int c = 0; c < 3; c++) v[c] = din[c * N + o];
-            } else {
-                const float mi = minv[o];
-                for (int c = 0; c < 3; c++) {
-                    const float z = mi * r[c * N + o];
-                    v[c] = first ? z : z + bc[c] * din[c * N + o];
-                }
-                if (lx >= 1 && lx <= kSt2W && ly >= 1 && ly <= kSt2H)
-                    for (int c = 0; c < 3; c++) dout[c * N + o] = v[c];
-            }
-        }
-        for (int c = 0; c < 3; c++) dl[c * LP + i] = v[c];
-    }
-    __syncthreads();
-    double dq[3] = {0.0, 0.0, 0.0};
-    const int lx = threadIdx.x % kSt2W, ly = threadIdx.x / kSt2W;
-    const int x = tx0 + lx, y = ty0 + ly;
-    if (x < W && y < H) {
-        const size_t o = (size_t)y * W + x;
-        const float cD = gD[o], cE = gE[o], cS = gS[o], cSE = gSE[o], cSW = gSW[o];
-        const float cEm = x > 0 ? gE[o - 1] : 0.0f;
-        const float cSm = y > 0 ? gS[o - W] : 0.0f;
-        const float cSEm = (x > 0 && y > 0) ? gSE[o - W - 1] : 0.0f;
-        const float cSWm = (x < W - 1 && y > 0) ? gSW[o - W + 1] : 0.0f;
-        const int li = (ly + 1) * LW + lx + 1;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float *dp = dl + c * LP;
-            const float dv = dp[li];
-            const float s = cD * dv + cE * dp[li + 1] + cEm * dp[li - 1] + cS * dp[li + LW] + cSm * dp[li - LW] +
-                            cSE * dp[li + LW + 1] + cSEm * dp[li - LW - 1] + cSW * dp[li + LW - 1] +
-                            cSWm * dp[li - LW + 1];
-            float lap = 0.0f;
-            if (W > 1) {
-                if (x == 0) lap += dv - dp[li + 1];
-                else if (x < W - 1) lap += 2.0f * dv - dp[li - 1] - dp[li + 1];
-                else lap += dv - dp[li - 1];
-            }
-            if (H > 1) {
-                if (y == 0) lap += dv - dp[li + LW];
-                else if (y < H - 1) lap += 2.0f * dv - dp[li - LW] - dp[li + LW];
-                else lap += dv - dp[li - LW];
-            }
-            const float qv = wscale * s + c0 * dv + beta_reg * lap;
-            q[c * N + o] = qv;
-            dq[c] += (double)dv * (double)qv;
-        }
-    }
-    if (mode == 0) return;
-    block_partials<3>(dq, partials, red);
-}
-
 // ---- dsf = 1: single-reduction PCG on the assembled stencil -----------------
 //
 // The Chronopoulos-Gear form of preconditioned CG needs one global reduction
@@ -308,13 +223,19 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cg_op_st2(int W, int H, int n
 // (A variant that lets every block of the next launch finish the reduction
 // itself measured slower: the redundant per-block sums cost more than the
 // one-block launch they replace.)
+// The five coefficient planes of the assembled stencil: centre and the E, S,
+// SE, SW couplings of each pixel (the W, N, NW, NE ones are its neighbours').
+struct St2Coeffs {
+    const float *gD, *gE, *gS, *gSE, *gSW;
+};
+
 struct CgcgArgs {
     int W, H, ntx, ntiles;
     float wscale, c0, beta_reg;
     const float *r_in, *w_in, *s_in;
     float *r_out, *w_out, *s_out, *p, *x;
     const float *minv;
-    const float *gD, *gE, *gS, *gSE, *gSW;
+    St2Coeffs g;
 };
 
 // w = A u at pixel (x, y) from the LDS tile `up` (row stride LW, centre li):
@@ -345,19 +266,18 @@ __device__ __forceinline__ float st2_apply(const float *up, int li, int LW, cons
 // Coefficients of the stencil at pixel o = (xc, yc) (clamped into the image);
 // every load is issued unconditionally at an in-bounds address and masked
 // afterwards, so they all go out before the caller's barrier.
-__device__ __forceinline__ void st2_coeffs(const CgcgArgs &a, size_t o, int xc, int yc, float (&cf)[9])
+__device__ __forceinline__ void st2_coeffs(const St2Coeffs &g, int W, size_t o, int xc, int yc, float (&cf)[9])
 {
-    const int W = a.W;
     const bool xm = xc > 0, ym = yc > 0, xp = xc < W - 1;
-    cf[0] = a.gD[o];
-    cf[1] = a.gE[o];
-    cf[2] = a.gS[o];
-    cf[3] = a.gSE[o];
-    cf[4] = a.gSW[o];
-    const float e = a.gE[xm ? o - 1 : o];
-    const float sm = a.gS[ym ? o - W : o];
-    const float se = a.gSE[(xm && ym) ? o - W - 1 : o];
-    const float sw = a.gSW[(xp && ym) ? o - W + 1 : o];
+    cf[0] = g.gD[o];
+    cf[1] = g.gE[o];
+    cf[2] = g.gS[o];
+    cf[3] = g.gSE[o];
+    cf[4] = g.gSW[o];
+    const float e = g.gE[xm ? o - 1 : o];
+    const float sm = g.gS[ym ? o - W : o];
+    const float se = g.gSE[(xm && ym) ? o - W - 1 : o];
+    const float sw = g.gSW[(xp && ym) ? o - W + 1 : o];
     cf[5] = xm ? e : 0.0f;
     cf[6] = ym ? sm : 0.0f;
     cf[7] = (xm && ym) ? se : 0.0f;
@@ -397,7 +317,7 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const Cg
     const int xc = min(x, W - 1), yc = min(y, H - 1);
     const size_t o = (size_t)yc * W + xc;
     float cf[9];
-    st2_coeffs(a, o, xc, yc, cf);
+    st2_coeffs(a.g, W, o, xc, yc, cf);
     const float mi = a.minv[o];
     float rv[3], wv[3], sv[3], pv[3], xv[3];
 #pragma unroll
@@ -491,6 +411,95 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const Cg
         }
     }
     block_partials<9>(v, partials, red);
+}
+
+// A x on the stencil tiles, fused with what the solve needs of it (same
+// halo-ring layout as k_cgcg_st2):
+//   INIT:  r = b - A x written, partials r.r, r.(Minv r), b.b per channel;
+//   else:  the true residual, partials ||b - A x||^2 per channel in FP64
+//          (nothing written).
+template <bool INIT>
+__global__ void __launch_bounds__(kSt2W * kSt2H) k_st2_residual(int W, int H, int ntx, int ntiles, float wscale,
+                                                                float c0, float beta_reg, St2Coeffs g,
+                                                                const float *__restrict__ x,
+                                                                const float *__restrict__ b,
+                                                                const float *__restrict__ minv,
+                                                                float *__restrict__ r, double *partials)
+{
+    constexpr int LW = kSt2W + 2, LH = kSt2H + 2, LP = LW * LH, NH = 2 * LW + 2 * kSt2H;
+    constexpr int NQ = INIT ? 9 : 3;
+    __shared__ float xl[3 * LP];
+    __shared__ double red[NQ * 16];
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const int tx0 = (t % ntx) * kSt2W, ty0 = (t / ntx) * kSt2H;
+    const size_t N = (size_t)W * H;
+    const int lx = threadIdx.x % kSt2W, ly = threadIdx.x / kSt2W;
+    const int px = tx0 + lx, py = ty0 + ly;
+    const bool pin = px < W && py < H;
+    const int xc = min(px, W - 1), yc = min(py, H - 1);
+    const size_t o = (size_t)yc * W + xc;
+    float cf[9];
+    st2_coeffs(g, W, o, xc, yc, cf);
+    const float mi = INIT ? minv[o] : 0.0f;
+    float xv[3], bv[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        xv[c] = x[c * N + o];
+        bv[c] = b[c * N + o];
+    }
+    const int h = threadIdx.x;
+    const bool hact = h < NH;
+    int hlx, hly;
+    if (h < LW) {
+        hlx = h;
+        hly = 0;
+    } else if (h < 2 * LW) {
+        hlx = h - LW;
+        hly = LH - 1;
+    } else if (h < 2 * LW + kSt2H) {
+        hlx = 0;
+        hly = 1 + (h - 2 * LW);
+    } else {
+        hlx = LW - 1;
+        hly = 1 + (h - 2 * LW - kSt2H);
+    }
+    const int hx = tx0 - 1 + hlx, hy = ty0 - 1 + hly;
+    const bool hin = hact && hx >= 0 && hx < W && hy >= 0 && hy < H;
+    float hxv[3] = {0.0f, 0.0f, 0.0f};
+    if (hact) {
+        const size_t ho = (size_t)min(max(hy, 0), H - 1) * W + min(max(hx, 0), W - 1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) hxv[c] = x[c * N + ho];
+    }
+    const int li = (ly + 1) * LW + lx + 1;
+#pragma unroll
+    for (int c = 0; c < 3; c++) xl[c * LP + li] = pin ? xv[c] : 0.0f;
+    if (hact) {
+        const int hl = hly * LW + hlx;
+#pragma unroll
+        for (int c = 0; c < 3; c++) xl[c * LP + hl] = hin ? hxv[c] : 0.0f;
+    }
+    __syncthreads();
+    double v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) v[q] = 0.0;
+    if (pin) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float q = st2_apply(xl + c * LP, li, LW, cf, px, py, W, H, wscale, c0, beta_reg);
+            if constexpr (INIT) {
+                const float rv = bv[c] - q;
+                r[c * N + o] = rv;
+                v[c] += (double)rv * rv;
+                v[3 + c] += (double)rv * (double)(mi * rv);
+                v[6 + c] += (double)bv[c] * bv[c];
+            } else {
+                const double d = (double)bv[c] - (double)q;
+                v[c] += d * d;
+            }
+        }
+    }
+    block_partials<NQ>(v, partials, red);
 }
 
 // The scalar step between two passes (one thread).  After the start pass:
@@ -794,14 +803,9 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     }
     const int op_blocks = st2 ? st2_tiles : ctx->ntiles;
 
-#define OPK(K, MODE, DIN, DOUT)                                                                               \
-    if (st2)                                                                                                  \
-        k_cg_op_st2<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, \
-                                                        (float)beta, MODE, DIN, r, minv, DOUT, q, gD, gE, gS,  \
-                                                        gSE, gSW, stt, part);                                 \
-    else                                                                                                      \
-        k_cg_op<K><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, wscale, (float)c0, (float)beta, \
-                                                                    MODE, DIN, r, minv, DOUT, q, stt, part)
+#define OPK(K, MODE, DIN, DOUT)                                                                          \
+    k_cg_op<K><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, wscale, (float)c0, (float)beta, \
+                                                                MODE, DIN, r, minv, DOUT, q, stt, part)
 #define LAUNCH_OP(MODE, DIN, DOUT)                     \
     switch (ctx->ks) {                                 \
     case 2: OPK(2, MODE, DIN, DOUT); break;            \
@@ -811,12 +815,21 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     default: return fail(ctx, LFRV_E_ARG, "dsf out of range"); \
     }
 
-    {
+    const St2Coeffs gco{gD, gE, gS, gSE, gSW};
+    if (st2) {
+        // r = b - A x with its sums in one stencil pass
         TimedRegion tr(ctx, 5);
-        LAUNCH_OP(0, x, nullptr);
+        k_st2_residual<true><<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(
+            ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, (float)beta, gco, x, ctx->cg_b, minv, r, part);
+        k_cg_init_state<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, st2_tiles, tol * tol, maxit);
+    } else {
+        {
+            TimedRegion tr(ctx, 5);
+            LAUNCH_OP(0, x, nullptr);
+        }
+        k_cg_init<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, r, minv, part);
+        k_cg_init_state<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, (int)rb, tol * tol, maxit);
     }
-    k_cg_init<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, r, minv, part);
-    k_cg_init_state<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, (int)rb, tol * tol, maxit);
     LFRV_LAUNCH(ctx, "cg init");
     int it = 0, cur = 0;
     const int chunk = 8;
@@ -837,11 +850,7 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         a.minv = minv;
         a.p = ctx->cg_d;
         a.x = x;
-        a.gD = gD;
-        a.gE = gE;
-        a.gS = gS;
-        a.gSE = gSE;
-        a.gSW = gSW;
+        a.g = gco;
         a.r_in = rbuf[0];
         a.w_in = wbuf[0];
         a.s_in = sbuf[0];
@@ -879,16 +888,14 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
             LFRV_LAUNCH(ctx, "cg iteration");
             it += nk;
             // the true residual of the current x goes out with the poll: if the
-            // chunk converged it is the answer, and the host waits once.
-            // (q of the final operator is a buffer the iterations do not use.)
+            // chunk converged it is the answer, and the host waits once
             {
                 TimedRegion tr(ctx, 5);
-                k_cg_op_st2<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(
-                    ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, (float)beta, 0, x, r, minv, nullptr,
-                    ctx->G, gD, gE, gS, gSE, gSW, stt, part);
+                k_st2_residual<false><<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(
+                    ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, (float)beta, gco, x, ctx->cg_b, minv,
+                    nullptr, part);
             }
-            k_resid<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, ctx->G, part);
-            k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, (int)rb, ctx->scal);
+            k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, st2_tiles, ctx->scal);
             LFRV_LAUNCH(ctx, "cg residual");
             LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
             LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, 3 * sizeof(double), hipMemcpyDeviceToHost,

@@ -426,44 +426,47 @@ lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
     if (!ctx->have_target_cam) return fail(ctx, LFRV_E_STATE, "lfrv_warps_from_depth: target camera not set");
     for (int v = 0; v < ctx->V; v++)
         if (!ctx->have_depth[v]) return fail(ctx, LFRV_E_STATE, "lfrv_warps_from_depth: depth of a view not set");
-    // camera algebra in FP64 on the host (import.cpp:301-320, 412-430)
-    std::vector<WarpCam> hc(ctx->V);
-    double Ku[9], Ru[9], tu[3], Cu[3], Kui[9];
-    decompose_P(ctx->P_target, Ku, Ru, tu, Cu);
-    kinv_ref(Ku, Kui);
-    const double *Pu = ctx->P_target;
-    for (int v = 0; v < ctx->V; v++) {
-        const double *Pv = ctx->P_views.data() + 12 * v;
-        double Kv[9], Rv[9], tv[3], Cv[3], Kvi[9], RvT[9], Pu3[9], M[9], Pv3[9], RbT[9], Nb[9];
-        decompose_P(Pv, Kv, Rv, tv, Cv);
-        kinv_ref(Kv, Kvi);
-        const double *Rb = ctx->p.backproject_source_cam ? Rv : Ru;
-        const double *Cb = ctx->p.backproject_source_cam ? Cv : Cu;
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                RvT[3 * i + j] = Rv[3 * j + i];
-                RbT[3 * i + j] = Rb[3 * j + i];
-                Pu3[3 * i + j] = Pu[4 * i + j];
-                Pv3[3 * i + j] = Pv[4 * i + j];
+    // camera algebra in FP64 on the host (import.cpp:301-320, 412-430), redone
+    // only when a camera changed
+    if (!ctx->cams_valid) {
+        std::vector<WarpCam> hc(ctx->V);
+        double Ku[9], Ru[9], tu[3], Cu[3], Kui[9];
+        decompose_P(ctx->P_target, Ku, Ru, tu, Cu);
+        kinv_ref(Ku, Kui);
+        const double *Pu = ctx->P_target;
+        for (int v = 0; v < ctx->V; v++) {
+            const double *Pv = ctx->P_views.data() + 12 * v;
+            double Kv[9], Rv[9], tv[3], Cv[3], Kvi[9], RvT[9], Pu3[9], M[9], Pv3[9], RbT[9], Nb[9];
+            decompose_P(Pv, Kv, Rv, tv, Cv);
+            kinv_ref(Kv, Kvi);
+            const double *Rb = ctx->p.backproject_source_cam ? Rv : Ru;
+            const double *Cb = ctx->p.backproject_source_cam ? Cv : Cu;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    RvT[3 * i + j] = Rv[3 * j + i];
+                    RbT[3 * i + j] = Rb[3 * j + i];
+                    Pu3[3 * i + j] = Pu[4 * i + j];
+                    Pv3[3 * i + j] = Pv[4 * i + j];
+                }
+            mat3_mul(RvT, Kvi, M);
+            mat3_mul(Pu3, M, M);
+            mat3_mul(RbT, Kui, Nb);
+            mat3_mul(Pv3, Nb, Nb);
+            WarpCam &c = hc[v];
+            for (int i = 0; i < 9; i++) {
+                c.M[i] = (float)M[i];
+                c.Nb[i] = (float)Nb[i];
             }
-        mat3_mul(RvT, Kvi, M);
-        mat3_mul(Pu3, M, M);
-        mat3_mul(RbT, Kui, Nb);
-        mat3_mul(Pv3, Nb, Nb);
-        WarpCam &c = hc[v];
-        for (int i = 0; i < 9; i++) {
-            c.M[i] = (float)M[i];
-            c.Nb[i] = (float)Nb[i];
+            for (int i = 0; i < 3; i++) {
+                c.c[i] = (float)(Pu[4 * i] * Cv[0] + Pu[4 * i + 1] * Cv[1] + Pu[4 * i + 2] * Cv[2] + Pu[4 * i + 3]);
+                c.e[i] = (float)(Pv[4 * i] * Cb[0] + Pv[4 * i + 1] * Cb[1] + Pv[4 * i + 2] * Cb[2] + Pv[4 * i + 3]);
+            }
         }
-        for (int i = 0; i < 3; i++) {
-            c.c[i] = (float)(Pu[4 * i] * Cv[0] + Pu[4 * i + 1] * Cv[1] + Pu[4 * i + 2] * Cv[2] + Pu[4 * i + 3]);
-            c.e[i] = (float)(Pv[4 * i] * Cb[0] + Pv[4 * i + 1] * Cb[1] + Pv[4 * i + 2] * Cb[2] + Pv[4 * i + 3]);
-        }
+        LFRV_HIP(ctx, hipMemcpyAsync(ctx->dcams, hc.data(), hc.size() * sizeof(WarpCam), hipMemcpyHostToDevice,
+                                     ctx->stream));
+        ctx->cams_valid = true;
     }
-    WarpCam *dcams = reinterpret_cast<WarpCam *>(ctx->partials);  // scratch, >= V*sizeof(WarpCam)
-    if (ctx->partials_cap * sizeof(double) < hc.size() * sizeof(WarpCam))
-        return fail(ctx, LFRV_E_ARG, "too many views for the camera scratch");
-    LFRV_HIP(ctx, hipMemcpyAsync(dcams, hc.data(), hc.size() * sizeof(WarpCam), hipMemcpyHostToDevice, ctx->stream));
+    const WarpCam *dcams = ctx->dcams;
     int *zkey = reinterpret_cast<int *>(ctx->tmpN);
     TimedRegion tr(ctx, 4);
     int inv_key;

@@ -119,13 +119,10 @@ __device__ __forceinline__ void block_sum(double (&v)[NQ], double *lds /* >= NQ 
         if (lane == 0) lds[q * 16 + wid] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wid == 0) {
+        // the per-wave sums, summed by wave 0 (fixed order: deterministic)
 #pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            double s = 0.0;
-            for (int k = 0; k < nw; k++) s += lds[q * 16 + k];
-            v[q] = s;
-        }
+        for (int q = 0; q < NQ; q++) v[q] = wave_sum(lane < nw ? lds[q * 16 + lane] : 0.0);
     }
     __syncthreads();
 }
@@ -136,28 +133,49 @@ __device__ __forceinline__ void block_sum(double (&v)[NQ], double *lds /* >= NQ 
 // Instead each block stores its NQ partial sums (layout [q][block], plain
 // stores) and a one-block finalize kernel launched after it on the same
 // stream (the kernel boundary orders the stores) reduces them in block order.
+// The per-wave sums of value q are added by thread q (in wave order) and
+// stored by it, so no thread walks all NQ columns.
 template <int NQ>
-__device__ __forceinline__ void block_partials(double (&v)[NQ], double *partials, double *lds)
+__device__ __forceinline__ void block_partials(double (&v)[NQ], double *partials, double *lds /* >= NQ * 16 */)
 {
-    block_sum<NQ>(v, lds);
-    if (threadIdx.x == 0) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
 #pragma unroll
-        for (int q = 0; q < NQ; q++) partials[(size_t)q * gridDim.x + blockIdx.x] = v[q];
+    for (int q = 0; q < NQ; q++) {
+        const double s = wave_sum(v[q]);
+        if (lane == 0) lds[q * 16 + wid] = s;
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if (q < NQ) {
+        double s = 0.0;
+        for (int k = 0; k < nw; k++) s += lds[q * 16 + k];
+        partials[(size_t)q * gridDim.x + blockIdx.x] = s;
     }
 }
 
 // Deterministic sum of G block partials (layout [q][block]) by one block;
-// the result is valid in thread 0.  Four independent loads per thread and
-// value are in flight at a time (the partials come from other XCDs' L2s, so
-// a dependent load chain would pay the full memory latency per element).
+// the result is valid in thread 0.  Up to 2 x blocks-size partials, all
+// loads of a thread are independent and in flight at once (the partials come
+// from other XCDs' L2s, so a dependent load chain would pay the full memory
+// latency per element); beyond that, four independent loads per thread and
+// value at a time.
 template <int NQ>
 __device__ __forceinline__ void reduce_partials(const double *partials, int G, double (&out)[NQ], double *lds)
 {
-    const int bs = blockDim.x;
+    const int bs = blockDim.x, t = threadIdx.x;
     if (G <= bs) {
-        // one value per thread and sum: all NQ loads in flight at once
 #pragma unroll
-        for (int q = 0; q < NQ; q++) out[q] = (int)threadIdx.x < G ? partials[(size_t)q * G + threadIdx.x] : 0.0;
+        for (int q = 0; q < NQ; q++) out[q] = t < G ? partials[(size_t)q * G + t] : 0.0;
+    } else if (G <= 2 * bs) {
+        double hi[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            out[q] = partials[(size_t)q * G + t];
+            hi[q] = t + bs < G ? partials[(size_t)q * G + t + bs] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; q++) out[q] += hi[q];
     } else {
 #pragma unroll 1
         for (int q = 0; q < NQ; q++) {
@@ -242,6 +260,8 @@ struct lfrv_ctx {
     std::vector<double> P_views;  // V*12
     double P_target[12] = {0};
     bool have_target_cam = false;
+    lfrv::WarpCam *dcams = nullptr;  // V per-view camera records on the device
+    bool cams_valid = false;      // dcams match P_views / P_target
     std::vector<int> have_depth;
 
     // splat structure: entries sorted by footprint cell, target tiling of the pulls
