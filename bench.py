@@ -85,6 +85,144 @@ def cpu_baseline(scene, gpu_iters, threads, cg_sample_iters=8):
                 cpu=platform.processor() or platform.machine())
 
 
+# IBR_color workloads: (synthetic scene preset, FISTA steps per view, dt_beta, description)
+COLOR_WORKLOADS = {
+    "color_sr_c3": (3, 500, 1.0, "5x5 views 512² -> 2048²"),
+    "color_sr_c1": (1, 300, 0.0, "3x3 views 128² -> 256²"),
+}
+
+
+def cpu_baseline_sr(scene, prm, steps, threads, sample_steps=2):
+    """The FP64 oracle on the host for an IBR_color view: warps, weights,
+    visibility, initial image (push-pull), TV weight, then `sample_steps`
+    FISTA steps, scaled to the `steps` of the GPU run."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.set_threads(threads)
+    s = scene
+    dsf, W, H = s["dsf"], s["W"], s["H"]
+    sig = float(prm.sigma_sensor)
+    t0 = time.perf_counter()
+    tx, ty, dpx, dpy, _ = oracle.warps_from_depth(s["depths"], s["P_views"], s["P_target"], dsf)
+    st = oracle.SrState(W, H, np.zeros((3, H, W)))
+    om, gx, gy = oracle.sr_weights(W, H, dsf, tx, ty, dpx, dpy, st.U, np.zeros((H, W), np.uint8), st.PX[0],
+                                   st.PY[0], sig, 1.0)
+    st.PX[0], st.PY[0] = gx, gy
+    cov = oracle.coverage(W, H, dsf, tx, ty)
+    vis = (cov >= 1.0).astype(np.uint8)
+    st.norm_mask += cov
+    U0, nm = oracle.initial_image(W, H, dsf, tx, ty, s["views"], om)
+    st.norm_mask = nm
+    st.U = oracle.push_pull(U0, vis)
+    st.Uq = st.U.copy()
+    g, st.norm_mask = oracle.regularizer_weight(W, H, dsf, tx, ty, om, sig, prm.lambda_ * prm.lambda_max_factor,
+                                                prm.lambda_, st.norm_mask)
+    t_setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    for _ in range(sample_steps):
+        oracle.fista_step(W, H, dsf, tx, ty, s["views"], om, sig, prm.dt_alpha, prm.dt_beta, prm.gradient_step,
+                          prm.gw_type, prm.lambda_, prm.inner_iterations, g, st)
+    per_step = (time.perf_counter() - t1) / sample_steps
+    est_ms = 1e3 * (t_setup + per_step * steps)
+    sample = (f"FP64 oracle on the host: set-up of one view ({t_setup:.1f} s) plus {sample_steps} FISTA steps "
+              f"({per_step * 1e3:.0f} ms each), scaled to the {steps} steps of the GPU run")
+    return dict(value=round(est_ms, 1), unit="ms per novel view", cores=threads, kind="port", sample=sample,
+                cpu=platform.processor() or platform.machine())
+
+
+def bench_color_sr(args, workload):
+    """One step = one IBR_color novel view: from device-resident views, depths
+    and cameras, warps from depth, splat structure, weights, visibility,
+    push-pull initial image and TV weight, then the FISTA / primal-dual steps
+    (one data-term pull, one fused PD step and the over-relaxation each)."""
+    preset, steps, beta, desc = COLOR_WORKLOADS[workload]
+    scene = lfrv.synth(preset, seed=args.seed)
+    prm = pipelines.color_sr_params(scene, lambda_=0.05, lambda_max_factor=100.0, inner_iterations=1,
+                                    dt_beta=beta, gradient_step=1.0)
+    ctx = lfrv.Context(prm)
+    pipelines.load_scene(ctx, scene)
+    ctx.sync()
+
+    def one_view():
+        ctx.clear_solution()
+        pipelines.color_sr_setup(ctx)
+        ctx.sr_init_regularizer_weight()
+        for _ in range(steps):
+            ctx.sr_iteration_fista()
+        ctx.sync()
+
+    for _ in range(args.warmup):
+        one_view()
+    ctx.set_timing(True)
+    per_view = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        one_view()
+        per_view.append(1e3 * (time.perf_counter() - ts))
+    t1 = time.perf_counter()
+    ms_per_view = 1e3 * (t1 - t0) / args.steps
+    dt_ms, dt_n = ctx.get_timing("dataterm")
+    pd_ms, pd_n = ctx.get_timing("pd_step")
+    ctx.set_timing(False)
+    U = ctx.get_solution()
+    e_binned, ntiles = ctx.bin_stats()
+    psnr = lfrv.psnr(U, scene["target"])
+    ctx.close()
+    N = scene["W"] * scene["H"]
+    e = e_binned
+    ks = scene["dsf"] + 1
+    # data-term pull, per cell-valid entry: e_src, e_tx, e_ty, e_w (16 B) and the
+    # source view (12 B; + its Laplacian, 12 B, with the gradient term); per
+    # target pixel: U, coverage read, G, Uq, norm_mask written (44 B)
+    dt_bytes = (28 + (12 if beta > 0 else 0)) * e + 44 * N
+    # fused PD step: Uq, px, py, g, F read; px, py, U written (88 B per pixel)
+    pd_bytes = 88 * N
+    dt_s = dt_ms / max(dt_n, 1) / 1e3
+    pd_s = pd_ms / max(pd_n, 1) / 1e3
+    kern = {
+        "k_sr_dataterm": dict(bytes_per_launch=dt_bytes, avg_launch_us=round(dt_s * 1e6, 2),
+                              achieved_gbs=round(dt_bytes / dt_s / 1e9, 1), launches=dt_n),
+        "k_pd_step": dict(bytes_per_launch=pd_bytes, avg_launch_us=round(pd_s * 1e6, 2),
+                          achieved_gbs=round(pd_bytes / pd_s / 1e9, 1), launches=pd_n),
+    }
+    dom = "k_sr_dataterm" if dt_ms >= pd_ms else "k_pd_step"
+    k = kern[dom]
+    line = {
+        "metric": f"ms per novel view ({desc}, {steps} TV primal-dual steps); achieved HBM GB/s",
+        "value": round(ms_per_view, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_view, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"IBR_color TV-L2 super-resolution with warped data terms: {desc} RGB, "
+                               f"{steps} FISTA steps of one primal-dual iteration, lambda 0.05, "
+                               f"lambda_max_factor 100, dt_beta {beta}",
+                   "views": scene["V"], "width": scene["W"], "height": scene["H"], "dsf": scene["dsf"],
+                   "seed": args.seed},
+        "roofline": {"bound": "hbm", "kernel": f"{dom}<{ks}>" if dom == "k_sr_dataterm" else dom,
+                     "achieved": k["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(k["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": k["bytes_per_launch"], "avg_launch_us": k["avg_launch_us"],
+                     "launches": k["launches"]},
+        "ms_per_step_median": round(float(np.median(per_view)), 3),
+        "ms_per_step_p10": round(float(np.percentile(per_view, 10)), 3),
+        "ms_per_step_p90": round(float(np.percentile(per_view, 90)), 3),
+        "psnr_vs_ground_truth_db": round(psnr, 3),
+        "kernels": dict(kern, binned_entries=e_binned, tiles=ntiles),
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline_sr(scene, prm, steps, threads)
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,12 +235,17 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the timed region, run the library's plain copy kernel once over MIB MiB "
                          "(reference dispatch for the HBM byte counters)")
+    ap.add_argument("--workload", default="gradient_ibr_c2", choices=["gradient_ibr_c2"] + sorted(COLOR_WORKLOADS),
+                    help="gradient_ibr_c2 (the headline metric, default) or an IBR_color configuration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs one novel-view solve per GPU "
               "and every box has one GPU (multi-GPU sharding is a design note, not a measured path)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload in COLOR_WORKLOADS:
+        bench_color_sr(args, args.workload)
+        return
 
     scene = lfrv.synth(2, seed=args.seed)
     kw = {}

@@ -240,6 +240,27 @@ __global__ void k_scan_add(int n, int *__restrict__ out, const int *__restrict__
     }
 }
 
+// The payload of the IBR_color data term for the first *total entries: the
+// entry's source pixel in its view (3 planes) and in its view's Laplacian.
+__global__ void k_entry_payload(int64_t cap, const int *__restrict__ total, const int *__restrict__ e_src, size_t n,
+                                const float *__restrict__ views, const float *__restrict__ vlap,
+                                float *__restrict__ pay)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= cap || e >= *total) return;
+    const int k = e_src[e];
+    const unsigned vi = (unsigned)k / (unsigned)n, m = (unsigned)k - vi * (unsigned)n;
+    const float *vp = views + (size_t)vi * 3 * n + m, *lp = vlap + (size_t)vi * 3 * n + m;
+    float v[6];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        v[c] = vp[c * n];
+        v[3 + c] = lp[c * n];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) pay[q * cap + e] = v[q];
+}
+
 // e_w = omega[e_src] for the first *total entries (total read on the device).
 __global__ void k_refresh_weights(int64_t cap, const int *__restrict__ total, const int *__restrict__ e_src,
                                   const float *__restrict__ omega, float *__restrict__ e_w)
@@ -258,14 +279,15 @@ struct PullCfg {
     extern __shared__ int smem[];     \
     const int t = xcd_tile(blockIdx.x, g.ntiles);
 
-#define PULL_VALS(NV) [&](int e, int x0, int y0, const float(&wx)[KS], const float(&wy)[KS], float om, float(&v)[NV])
+// (the footprint weights are not used by a splat-only pull's values)
+#define PULL_VALS(NV) [&](int e, int, int, const auto &, const auto &, float om, float(&v)[NV])
 
 // coverage = sum_i B_i^T 1
 template <int KS>
 __global__ void __launch_bounds__(kPullMaxThreads) k_pull_coverage(TileGeom g, TileEntries te, PullCfg pc, float *__restrict__ coverage)
 {
     PULL_PROLOGUE()
-    any_pull<KS, 1, 0>(
+    splat_pull<KS, 1, 0>(
         g, te, t, smem, pc.cap, PULL_VALS(1) { v[0] = 1.0f; },
         [&](size_t o, int, int, const float(&a)[1]) { coverage[o] = a[0]; });
 }
@@ -280,7 +302,7 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_pull_init_image(TileGeom g,
 {
     PULL_PROLOGUE()
     const size_t N = (size_t)g.W * g.H;
-    any_pull<KS, 4, 0>(
+    splat_pull<KS, 4, 0>(
         g, te, t, smem, pc.cap,
         PULL_VALS(4) {
             const int k = te.src[e];
@@ -320,7 +342,7 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_pull_gibr_setup(TileGeom g,
 {
     PULL_PROLOGUE()
     const size_t N = (size_t)g.W * g.H;
-    any_pull<KS, 7, (1 << 6)>(
+    splat_pull<KS, 7, (1 << 6)>(
         g, te, t, smem, pc.cap,
         PULL_VALS(7) {
             const int k = te.src[e];
@@ -450,7 +472,7 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_pull_u_gradient(TileGeom g,
                                                          float *__restrict__ tmask, float *__restrict__ norm_mask)
 {
     PULL_PROLOGUE()
-    any_pull<KS, 4, 0>(
+    splat_pull<KS, 4, 0>(
         g, te, t, smem, pc.cap,
         PULL_VALS(4) {
             const int k = te.src[e];
@@ -485,7 +507,7 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_pull_reg_weight(TileGeom g,
                                                          float *__restrict__ norm_mask, float *__restrict__ graw)
 {
     PULL_PROLOGUE()
-    any_pull<KS, 1, 0>(
+    splat_pull<KS, 1, 0>(
         g, te, t, smem, pc.cap, PULL_VALS(1) { v[0] = om; },
         [&](size_t o, int, int, const float(&a)[1]) {
             const float nm = norm_mask[o] + coverage[o];
@@ -524,7 +546,7 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_pull_rhs(TileGeom g, TileEn
 {
     PULL_PROLOGUE()
     const size_t N = (size_t)g.W * g.H;
-    any_pull<KS, 3, 0>(
+    splat_pull<KS, 3, 0>(
         g, te, t, smem, pc.cap,
         PULL_VALS(3) {
             const int k = te.src[e];
@@ -547,7 +569,7 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_pull_diag(TileGeom g, TileE
                                                    float beta, float *__restrict__ minv)
 {
     PULL_PROLOGUE()
-    any_pull<KS, 1, 1>(
+    splat_pull<KS, 1, 1>(
         g, te, t, smem, pc.cap, PULL_VALS(1) { v[0] = om * wscale; },
         [&](size_t o, int x, int y, const float(&a)[1]) {
             const float d = a[0] + c0 + beta * (float)dtd_degree(x, y, g.W, g.H);
@@ -644,18 +666,20 @@ TileEntries tile_entries(const lfrv_ctx *ctx)
     t.ty = ctx->e_ty;
     t.w = ctx->e_w;
     t.omega = ctx->omega;
+    t.pay = ctx->e_pay;
+    t.pay_stride = ctx->e_cap;
     t.active = 1;
     return t;
 }
 
 // Capacity of a pull's LDS cache within `budget` bytes besides `extra` bytes:
 // staged entries for the cell pull (ks <= 3), slots for the slot pull.
-int pull_cap(const lfrv_ctx *ctx, int nv, size_t extra, size_t budget)
+int pull_cap(const lfrv_ctx *ctx, int nv, size_t extra, size_t budget, PullKind kind, int npay)
 {
-    if (ctx->ks <= 3) {
+    if (kind != PullKind::Slot) {
         const int wh = ctx->tile_h + ctx->ks - 1;
         const long avail = (long)budget - (long)extra - (long)sizeof(int) * (2 * wh + 2);
-        const long ecap = avail / (long)(3 * sizeof(float));
+        const long ecap = avail / (long)((3 + npay) * sizeof(float));
         return (int)std::max(0L, std::min(ecap, (long)kStagePer * ctx->pull_bs));
     }
     const size_t meta = sizeof(int) * (size_t)pull_meta_ints(ctx->tile_w, ctx->tile_h, ctx->ks, ctx->pull_bs);
@@ -664,13 +688,13 @@ int pull_cap(const lfrv_ctx *ctx, int nv, size_t extra, size_t budget)
     return (int)std::max(64L, std::min(cap, 16384L));
 }
 
-size_t pull_lds(const lfrv_ctx *ctx, int nv, int cap)
+size_t pull_lds(const lfrv_ctx *ctx, int nv, int cap, PullKind kind, int npay)
 {
-    if (ctx->ks <= 3) {
+    if (kind != PullKind::Slot) {
         const int wh = ctx->tile_h + ctx->ks - 1;
         const size_t cells = (size_t)(ctx->tile_w + ctx->ks - 1) * wh;
-        const size_t part = (size_t)ctx->ks * ctx->ks * nv * cells;
-        return sizeof(int) * (size_t)(2 * wh + 2) + sizeof(float) * std::max(part, (size_t)3 * cap);
+        const size_t part = (kind == PullKind::Box ? 3 : (size_t)ctx->ks * ctx->ks) * nv * cells;
+        return sizeof(int) * (size_t)(2 * wh + 2) + sizeof(float) * std::max(part, (size_t)(3 + npay) * cap);
     }
     return pull_lds_bytes(ctx->tile_w, ctx->tile_h, ctx->ks, ctx->pull_bs, nv, cap);
 }
@@ -719,14 +743,21 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
             ctx->W, ctx->H, g.Wc, nc, cell_tap, ctx->coverage);
     } else {
         const TileEntries te = tile_entries(ctx);
-        PullCfg pc{pull_cap(ctx, 1, 0, ctx->pull_budget)};
-#define COV(K) k_pull_coverage<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap), ctx->stream>>>(g, te, pc, ctx->coverage)
+        const PullKind pk = splat_kind(ctx, false);
+        PullCfg pc{pull_cap(ctx, 1, 0, ctx->pull_budget, pk)};
+#define COV(K) k_pull_coverage<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap, pk), ctx->stream>>>(g, te, pc, ctx->coverage)
         LFRV_KS_DISPATCH(ctx->ks, COV);
 #undef COV
     }
     LFRV_LAUNCH(ctx, "coverage");
     lfrv_status st = run_view_stencils(ctx);
     if (st != LFRV_OK) return st;
+    if (ctx->e_pay) {
+        k_entry_payload<<<blocks_for(ctx->e_cap), kBlock, 0, ctx->stream>>>(
+            (int64_t)ctx->e_cap, ctx->cell_start + ctx->ncells, ctx->e_src, ctx->n, ctx->views, ctx->view_lap,
+            ctx->e_pay);
+        LFRV_LAUNCH(ctx, "entry payload");
+    }
     ctx->prepared = true;
     ctx->entry_weights_valid = false;
     ctx->epoch++;
@@ -774,9 +805,9 @@ lfrv_status run_initial_image(lfrv_ctx *ctx)
         // one pass over the entries: initial image, target gradient, CG sums
         // and the CG stencil (omega read through the source index)
         TileEntries tes = te;
-        PullCfg pc{pull_cap(ctx, 9, 0, ctx->pull_budget)};  // 4 * 9 = 36 >= 34 sums per window cell
+        PullCfg pc{pull_cap(ctx, 9, 0, ctx->pull_budget, PullKind::Cell)};  // 4 * 9 = 36 >= 34 sums per window cell
         const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
-        k_gibr_setup_st2<<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 9, pc.cap), ctx->stream>>>(
+        k_gibr_setup_st2<<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 9, pc.cap, PullKind::Cell), ctx->stream>>>(
             g, tes, pc, ctx->n, ctx->views, ctx->view_grad, ctx->coverage, s2, ctx->U, ctx->norm_mask,
             ctx->u_grad_x, ctx->u_grad_y, ctx->target_mask, ctx->cache_num, ctx->gram);
         LFRV_LAUNCH(ctx, "set-up pull");
@@ -788,17 +819,19 @@ lfrv_status run_initial_image(lfrv_ctx *ctx)
     if (st != LFRV_OK) return st;
     if (ctx->p.mode == LFRV_MODE_GRADIENT_IBR) {
         // one pull for the initial image, the target gradient and the CG sums
-        PullCfg pc{pull_cap(ctx, 7, 0, ctx->pull_budget)};
+        const PullKind pk = splat_kind(ctx, true);
+        PullCfg pc{pull_cap(ctx, 7, 0, ctx->pull_budget, pk)};
         const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
-#define SETUP(K) k_pull_gibr_setup<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 7, pc.cap), ctx->stream>>>(g, te, pc, ctx->n, ctx->views, ctx->view_grad, ctx->coverage, s2, ctx->U, ctx->norm_mask, ctx->u_grad_x, ctx->u_grad_y, ctx->target_mask, ctx->cache_num, ctx->cache_sq)
+#define SETUP(K) k_pull_gibr_setup<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 7, pc.cap, pk), ctx->stream>>>(g, te, pc, ctx->n, ctx->views, ctx->view_grad, ctx->coverage, s2, ctx->U, ctx->norm_mask, ctx->u_grad_x, ctx->u_grad_y, ctx->target_mask, ctx->cache_num, ctx->cache_sq)
         LFRV_KS_DISPATCH(ctx->ks, SETUP);
 #undef SETUP
         LFRV_LAUNCH(ctx, "set-up pull");
         ctx->setup_epoch = ctx->epoch;
         return LFRV_OK;
     }
-    PullCfg pc{pull_cap(ctx, 4, 0, ctx->pull_budget)};
-#define INIT(K) k_pull_init_image<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 4, pc.cap), ctx->stream>>>(g, te, pc, ctx->n, ctx->views, ctx->U, ctx->norm_mask)
+    const PullKind pk = splat_kind(ctx, false);
+    PullCfg pc{pull_cap(ctx, 4, 0, ctx->pull_budget, pk)};
+#define INIT(K) k_pull_init_image<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 4, pc.cap, pk), ctx->stream>>>(g, te, pc, ctx->n, ctx->views, ctx->U, ctx->norm_mask)
     LFRV_KS_DISPATCH(ctx->ks, INIT);
 #undef INIT
     LFRV_LAUNCH(ctx, "initial image");
@@ -825,10 +858,11 @@ lfrv_status run_u_gradient(lfrv_ctx *ctx)
     if (st != LFRV_OK) return st;
     const TileGeom g = tile_geom(ctx);
     const TileEntries te = tile_entries(ctx);
-    PullCfg pc{pull_cap(ctx, 4, 0, ctx->pull_budget)};
+    const PullKind pk = splat_kind(ctx, false);
+    PullCfg pc{pull_cap(ctx, 4, 0, ctx->pull_budget, pk)};
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
     TimedRegion tr(ctx, 5);
-#define UG(K) k_pull_u_gradient<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 4, pc.cap), ctx->stream>>>(g, te, pc, ctx->n, ctx->view_grad, s2, ctx->u_grad_x, ctx->u_grad_y, ctx->target_mask, ctx->norm_mask)
+#define UG(K) k_pull_u_gradient<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 4, pc.cap, pk), ctx->stream>>>(g, te, pc, ctx->n, ctx->view_grad, s2, ctx->u_grad_x, ctx->u_grad_y, ctx->target_mask, ctx->norm_mask)
     LFRV_KS_DISPATCH(ctx->ks, UG);
 #undef UG
     LFRV_LAUNCH(ctx, "u gradient");
@@ -842,11 +876,12 @@ lfrv_status run_regularizer_weight(lfrv_ctx *ctx)
     if (st != LFRV_OK) return st;
     const TileGeom g = tile_geom(ctx);
     const TileEntries te = tile_entries(ctx);
-    PullCfg pc{pull_cap(ctx, 1, 0, ctx->pull_budget)};
+    const PullKind pk = splat_kind(ctx, false);
+    PullCfg pc{pull_cap(ctx, 1, 0, ctx->pull_budget, pk)};
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
     const float lmax = ctx->p.lambda_max_factor * ctx->p.lambda, lmin = ctx->p.lambda;
     float *graw = ctx->tmpN;
-#define RW(K) k_pull_reg_weight<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap), ctx->stream>>>(g, te, pc, ctx->coverage, s2, lmax, lmin, (float)ctx->V, ctx->norm_mask, graw)
+#define RW(K) k_pull_reg_weight<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap, pk), ctx->stream>>>(g, te, pc, ctx->coverage, s2, lmax, lmin, (float)ctx->V, ctx->norm_mask, graw)
     LFRV_KS_DISPATCH(ctx->ks, RW);
 #undef RW
     LFRV_LAUNCH(ctx, "regularizer weight");
@@ -878,14 +913,16 @@ lfrv_status run_rhs_and_diag(lfrv_ctx *ctx, double alpha, double c0, double beta
         if (st != LFRV_OK) return st;
     }
     {
-        PullCfg pc{pull_cap(ctx, 3, 0, ctx->pull_budget)};
-#define RHS(K) k_pull_rhs<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 3, pc.cap), ctx->stream>>>(g, te, pc, ctx->n, ctx->views, wscale, (float)c0, u0, (float)beta, ctx->u_grad_x, ctx->u_grad_y, ctx->cg_b)
+        const PullKind pk = splat_kind(ctx, false);
+        PullCfg pc{pull_cap(ctx, 3, 0, ctx->pull_budget, pk)};
+#define RHS(K) k_pull_rhs<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 3, pc.cap, pk), ctx->stream>>>(g, te, pc, ctx->n, ctx->views, wscale, (float)c0, u0, (float)beta, ctx->u_grad_x, ctx->u_grad_y, ctx->cg_b)
         LFRV_KS_DISPATCH(ctx->ks, RHS);
 #undef RHS
     }
     if (use_precond) {
-        PullCfg pc{pull_cap(ctx, 1, 0, ctx->pull_budget)};
-#define DG(K) k_pull_diag<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap), ctx->stream>>>(g, te, pc, wscale, (float)c0, (float)beta, ctx->cg_minv)
+        const PullKind pk = splat_kind(ctx, true);
+        PullCfg pc{pull_cap(ctx, 1, 0, ctx->pull_budget, pk)};
+#define DG(K) k_pull_diag<K><<<ctx->ntiles, ctx->pull_bs, pull_lds(ctx, 1, pc.cap, pk), ctx->stream>>>(g, te, pc, wscale, (float)c0, (float)beta, ctx->cg_minv)
         LFRV_KS_DISPATCH(ctx->ks, DG);
 #undef DG
     } else {

@@ -781,8 +781,8 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     te.active = use_views ? 1 : 0;
     const int HAL = ctx->ks - 1;
     const size_t dtile = sizeof(float) * 3 * (size_t)(ctx->tile_w + 2 * HAL) * (ctx->tile_h + 2 * HAL);
-    const int cap = pull_cap(ctx, 3, dtile, ctx->pull_budget);
-    const size_t lds = dtile + pull_lds(ctx, 3, cap);
+    const int cap = pull_cap(ctx, 3, dtile, ctx->pull_budget, walk_kind(ctx));
+    const size_t lds = dtile + pull_lds(ctx, 3, cap, walk_kind(ctx));
     const unsigned int rb = reduce_blocks(N);
     float *x = ctx->U, *r = ctx->cg_r, *q = ctx->cg_q, *minv = ctx->cg_minv;
     float *dbuf[2] = {ctx->cg_d, ctx->tmp3N};
@@ -795,8 +795,8 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
     float *gD = ctx->gram, *gE = gD + N, *gS = gE + N, *gSE = gS + N, *gSW = gSE + N;
     if (st2 && ctx->gram_epoch != ctx->epoch) {
         TimedRegion tr(ctx, 5);
-        const int gcap = pull_cap(ctx, 3, 0, ctx->pull_budget);
-        const size_t glds = pull_lds(ctx, 3, gcap);  // 12 floats per window cell >= the 10 Gram sums
+        const int gcap = pull_cap(ctx, 3, 0, ctx->pull_budget, PullKind::Cell);
+        const size_t glds = pull_lds(ctx, 3, gcap, PullKind::Cell);  // 12 floats per window cell >= the 10 Gram sums
         k_gram2<<<ctx->ntiles, ctx->pull_bs, glds, ctx->stream>>>(g, te, gcap, gD, gE, gS, gSE, gSW);
         LFRV_LAUNCH(ctx, "stencil assembly");
         ctx->gram_epoch = ctx->epoch;

@@ -2,6 +2,7 @@
 // gradient, the outer proximal step, the primal energy.
 #include "tile_common.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace lfrv {
@@ -125,6 +126,144 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_sr_dataterm(TileGeom g, Til
                 }
                 G = fminf(1.0f, fmaxf(-1.0f, G));
                 const float uv = ul[c * up + ui];
+                G *= da.prox_factor;
+                G += uv;
+                G = fminf(1.0f, fmaxf(0.0f, G));
+                Gout[c * N + o] = G;
+                Uq[c * N + o] = uv;
+            }
+            if (da.alpha > 0.0f || need_lap) norm_mask[o] = cov;
+        });
+}
+
+// The same data term for dsf >= 2 in box form (see box_pull): the gathers
+// S_i U and S_i lap U of an entry are 2 x 2 gathers, at its anchors, of the
+// dsf x dsf box sums of U and lap U (zero outside the image), and the splat
+// of its residuals is a box pull.  U (the box region of the tile's anchors
+// plus a 1-pixel ring) and lap U are staged in the pull's region first; the
+// box sums stay in LDS beside it.
+template <int KS>
+__global__ void __launch_bounds__(kPullMaxThreads)
+    k_sr_dataterm_box(TileGeom g, TileEntries te, int cap, size_t n, DataTermArgs da, const float *__restrict__ U,
+                      const float *__restrict__ views, const float *__restrict__ vlap,
+                      const float *__restrict__ coverage, float *__restrict__ Gout, float *__restrict__ Uq,
+                      float *__restrict__ norm_mask)
+{
+    constexpr int D = KS - 1;
+    extern __shared__ int smem[];
+    const int t = xcd_tile(blockIdx.x, g.ntiles);
+    const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
+    const int ww = g.tw + KS - 1, wh = g.th + KS - 1;
+    const int aw = ww + 1, ah = wh + 1, ap = aw * ah;  // anchors (ac, ar): box start (tx0 + ac - D, ty0 + ar - D)
+    const int uw = g.tw + 2 * D + 2, uh = g.th + 2 * D + 2, up = uw * uh;  // U from (tx0 - D - 1, ty0 - D - 1)
+    const int lw = g.tw + 2 * D, lh = g.th + 2 * D, lpp = lw * lh;        // lap U from (tx0 - D, ty0 - D)
+    float *bu = reinterpret_cast<float *>(smem);  // 3 * ap box sums of U
+    float *bl = bu + 3 * ap;                      // 3 * ap box sums of lap U
+    int *pull = reinterpret_cast<int *>(bl + 3 * ap);
+    float *us = reinterpret_cast<float *>(pull + 2 * wh + 2);  // staging, inside the pull's region
+    float *ls = us + 3 * up;
+    const size_t N = (size_t)g.W * g.H;
+    const bool need_lap = da.beta > 0.0f;
+    for (int i = threadIdx.x; i < up; i += blockDim.x) {
+        const int x = tx0 - D - 1 + i % uw, y = ty0 - D - 1 + i / uw;
+        const bool in = x >= 0 && x < g.W && y >= 0 && y < g.H;
+        const size_t o = in ? (size_t)y * g.W + x : 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) us[c * up + i] = in ? U[c * N + o] : 0.0f;
+    }
+    __syncthreads();
+    if (need_lap) {
+        // cuda_laplacian_device (vtv_sr_kernels_unstructured.cu:720-761), no hole gating
+        for (int i = threadIdx.x; i < lpp; i += blockDim.x) {
+            const int lx = i % lw, ly = i / lw;
+            const int x = tx0 - D + lx, y = ty0 - D + ly;
+            const int ui = (ly + 1) * uw + lx + 1;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float v = 0.0f;
+                if (x >= 0 && x < g.W && y >= 0 && y < g.H) {
+                    const float *u = us + c * up;
+                    float sm = 0.0f, wsum = 0.0f;
+                    if (x > 0) { sm += u[ui - 1]; wsum += 1.0f; }
+                    if (y > 0) { sm += u[ui - uw]; wsum += 1.0f; }
+                    if (x < g.W - 1) { sm += u[ui + 1]; wsum += 1.0f; }
+                    if (y < g.H - 1) { sm += u[ui + uw]; wsum += 1.0f; }
+                    v = sm / wsum - u[ui];
+                }
+                ls[c * lpp + i] = v;
+            }
+        }
+        __syncthreads();
+    }
+    // box sums at the anchors: anchor (ac, ar) covers U-staging pixels
+    // (ac + 1 .. ac + D, ar + 1 .. ar + D) and lap-staging (ac .. ac + D - 1, ...)
+    for (int a = threadIdx.x; a < ap; a += blockDim.x) {
+        const int ar = a / aw, ac = a - ar * aw;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float su = 0.0f, sl = 0.0f;
+#pragma unroll
+            for (int l = 0; l < D; l++)
+#pragma unroll
+                for (int k = 0; k < D; k++) {
+                    su += us[c * up + (ar + l + 1) * uw + ac + k + 1];
+                    if (need_lap) sl += ls[c * lpp + (ar + l) * lw + ac + k];
+                }
+            bu[c * ap + a] = su;
+            bl[c * ap + a] = sl;
+        }
+    }
+    // (the pull's first barrier orders these sums before the walk and the
+    // staging before its own use of the region)
+    box_pull<KS, 6, 6>(
+        g, te, t, pull, cap,
+        [&](int, int x0, int y0, const float(&ax)[2], const float(&ay)[2], float om, const float(&pay)[6],
+            float(&r)[6]) {
+            const int a0 = (y0 - ty0 + D) * aw + (x0 - tx0 + D);
+            float su[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const float w = ay[i] * ax[j];
+                    const int a = a0 + i * aw + j;
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        su[c] += w * bu[c * ap + a];
+                        if (need_lap) sl[c] += w * bl[c * ap + a];
+                    }
+                }
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                r[c] = (da.alpha > 0.0f) ? (su[c] - pay[c]) * om : 0.0f;
+                float rg = 0.0f;
+                if (need_lap) {
+                    rg = sl[c] * da.dsf2 - pay[3 + c];
+                    if (da.gw_type != 0) rg *= om;
+                }
+                r[3 + c] = rg;
+            }
+        },
+        [&](size_t o, int, int, const float(&acc)[6]) {
+            const float cov = coverage[o];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float G = 0.0f;
+                if (da.alpha > 0.0f) {
+                    float gi = acc[c];
+                    if (cov != 0.0f) gi /= cov;
+                    gi *= da.sigma2;
+                    G += da.alpha * gi;
+                }
+                if (need_lap) {
+                    float gg = acc[3 + c];
+                    if (cov != 0.0f) gg /= cov;
+                    if (da.gw_type != 0) gg *= da.sigma2;
+                    gg *= -da.grad_step;
+                    G += da.beta * gg;
+                }
+                G = fminf(1.0f, fmaxf(-1.0f, G));
+                const float uv = U[c * N + o];
                 G *= da.prox_factor;
                 G += uv;
                 G = fminf(1.0f, fmaxf(0.0f, G));
@@ -272,16 +411,33 @@ lfrv_status run_fista(lfrv_ctx *ctx)
     da.prox_factor = (float)(-1.0 / ((double)ctx->p.lambda * (double)L));
     da.gw_type = ctx->p.gw_type;
     const int HAL = ctx->ks - 1;
-    const size_t up = (size_t)(ctx->tile_w + 2 * HAL + 2) * (ctx->tile_h + 2 * HAL + 2);
-    const size_t lpp = (size_t)(ctx->tile_w + 2 * HAL) * (ctx->tile_h + 2 * HAL);
-    const size_t tiles = sizeof(float) * (3 * up + 3 * lpp);
-    const int cap = pull_cap(ctx, 6, tiles, ctx->pull_budget);
-    const size_t lds = tiles + pull_lds(ctx, 6, cap);
-    {
+    if (ctx->ks == 2) {
+        const size_t up = (size_t)(ctx->tile_w + 2 * HAL + 2) * (ctx->tile_h + 2 * HAL + 2);
+        const size_t lpp = (size_t)(ctx->tile_w + 2 * HAL) * (ctx->tile_h + 2 * HAL);
+        const size_t tiles = sizeof(float) * (3 * up + 3 * lpp);
+        const int cap = pull_cap(ctx, 6, tiles, ctx->pull_budget, PullKind::Cell);
+        const size_t lds = tiles + pull_lds(ctx, 6, cap, PullKind::Cell);
         TimedRegion tr(ctx, 3);
-#define DT(K) k_sr_dataterm<K><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, ctx->n, da, ctx->U, ctx->views, ctx->view_lap, ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask)
+        k_sr_dataterm<2><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, ctx->n, da, ctx->U, ctx->views,
+                                                                          ctx->view_lap, ctx->coverage, ctx->G, ctx->Uq,
+                                                                          ctx->norm_mask);
+        LFRV_LAUNCH(ctx, "data term");
+    } else {
+        // box form: box sums of U and lap U at the anchors beside the pull,
+        // whose region first holds the staged U and lap U
+        const int D = HAL, ww = ctx->tile_w + ctx->ks - 1, wh = ctx->tile_h + ctx->ks - 1;
+        const size_t ap = (size_t)(ww + 1) * (wh + 1);
+        const size_t sums = sizeof(float) * 6 * ap;
+        const size_t up = (size_t)(ctx->tile_w + 2 * D + 2) * (ctx->tile_h + 2 * D + 2);
+        const size_t lpp = (size_t)(ctx->tile_w + 2 * D) * (ctx->tile_h + 2 * D);
+        const size_t stage = sizeof(int) * (2 * wh + 2) + sizeof(float) * (3 * up + 3 * lpp);
+        // entries staged with their 6 payload floats; up to 64 KiB of LDS
+        const size_t budget = std::max<size_t>(ctx->pull_budget, 65536 - 512);
+        const int cap = pull_cap(ctx, 6, sums, budget, PullKind::Box, 6);
+        const size_t lds = sums + std::max(stage, pull_lds(ctx, 6, cap, PullKind::Box, 6));
+        TimedRegion tr(ctx, 3);
+#define DT(K) k_sr_dataterm_box<K><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, ctx->n, da, ctx->U, ctx->views, ctx->view_lap, ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask)
         switch (ctx->ks) {
-        case 2: DT(2); break;
         case 3: DT(3); break;
         case 4: DT(4); break;
         case 5: DT(5); break;

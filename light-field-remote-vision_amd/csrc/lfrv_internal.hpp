@@ -272,6 +272,7 @@ struct lfrv_ctx {
     int *scan_tmp = nullptr;              // block sums of the cell scan
     int *e_src = nullptr;       // view * n + m
     float *e_tx = nullptr, *e_ty = nullptr, *e_w = nullptr;
+    float *e_pay = nullptr;  // IBR_color, dsf >= 2: per entry its view's RGB and Laplacian (6 planes)
     size_t e_cap = 0;
     bool prepared = false;
     bool entry_weights_valid = false;

@@ -34,14 +34,31 @@ struct TileEntries {
     const int *src;
     const float *tx, *ty, *w;
     const float *omega;  // per-source weights (for pulls that read omega[src] instead of w)
+    const float *pay;    // per-entry payload planes (pay[q * pay_stride + e]), entry order
+    size_t pay_stride;
     int active;          // 0: the tile kernels see no entries
 };
 
 TileGeom tile_geom(const lfrv_ctx *ctx);
 TileEntries tile_entries(const lfrv_ctx *ctx);
 
-int pull_cap(const lfrv_ctx *ctx, int nv, size_t extra, size_t budget);
-size_t pull_lds(const lfrv_ctx *ctx, int nv, int cap);
+// The three pulls (device side below): the cell-owner pull, the box pull
+// (ks >= 3, plain weights) and the slot pull.  Kernels that only splat use
+// splat_pull (cell-owner for ks = 2, box for larger footprints; when a value
+// sums squared weights, cell-owner up to ks = 3 and slot beyond); kernels
+// whose values gather through the
+// footprint use any_pull (cell-owner for ks <= 3, slot beyond).
+enum class PullKind { Cell, Box, Slot };
+inline PullKind splat_kind(const lfrv_ctx *ctx, bool sq)
+{
+    return ctx->ks <= 2 ? PullKind::Cell : (!sq ? PullKind::Box : (ctx->ks <= 3 ? PullKind::Cell : PullKind::Slot));
+}
+inline PullKind walk_kind(const lfrv_ctx *ctx) { return ctx->ks <= 3 ? PullKind::Cell : PullKind::Slot; }
+// Staged-entry (Cell, Box) or slot (Slot) capacity within `budget` bytes of
+// LDS, `extra` of them taken by the kernel's own tiles; the pull's LDS bytes.
+// `npay`: payload floats staged per entry besides tau and omega (box pull).
+int pull_cap(const lfrv_ctx *ctx, int nv, size_t extra, size_t budget, PullKind kind, int npay = 0);
+size_t pull_lds(const lfrv_ctx *ctx, int nv, int cap, PullKind kind, int npay = 0);
 
 lfrv_status run_rhs_and_diag(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use_views,
                              bool use_precond, const float *u0);
@@ -93,6 +110,47 @@ __device__ __forceinline__ void tap_accumulate(float (&acc)[NV], float a, const 
 {
 #pragma unroll
     for (int q = 0; q < NV; q++) acc[q] += (((SQ >> q) & 1) ? a * a : a) * v[q];
+}
+
+// The two anchor weights per axis: ax = ((1-f)/sx, f/sx), as d_0 / sx and
+// d_dsf / sx of footprint<KS> (same arithmetic).  False when every in-image
+// tap has zero weight.
+template <int KS>
+__device__ __forceinline__ bool box_weights(float tx, float ty, int W, int H, int &x0, int &y0, float (&ax)[2],
+                                            float (&ay)[2])
+{
+    constexpr float hd = 0.5f * (float)(KS - 1);
+    const float px = tx - 0.5f, py = ty - 0.5f;
+    x0 = (int)floorf(px - hd + 0.5f);
+    y0 = (int)floorf(py - hd + 0.5f);
+    float sx = 0.0f, sy = 0.0f, fx0 = 0.0f, fx1 = 0.0f, fy0 = 0.0f, fy1 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < KS; j++) {
+        const int x = x0 + j;
+        float d = 1.0f;
+        if (j == 0) d = ((float)x + 0.5f) - (px - hd);
+        else if (j == KS - 1) d = (px + hd) - ((float)x - 0.5f);
+        if (j == 0) fx0 = d;
+        if (j == KS - 1) fx1 = d;
+        sx += (x >= 0 && x <= W - 1) ? d : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < KS; i++) {
+        const int y = y0 + i;
+        float d = 1.0f;
+        if (i == 0) d = ((float)y + 0.5f) - (py - hd);
+        else if (i == KS - 1) d = (py + hd) - ((float)y - 0.5f);
+        if (i == 0) fy0 = d;
+        if (i == KS - 1) fy1 = d;
+        sy += (y >= 0 && y <= H - 1) ? d : 0.0f;
+    }
+    if (sx * sy == 0.0f) return false;
+    const float isx = 1.0f / sx, isy = 1.0f / sy;
+    ax[0] = fx0 * isx;
+    ax[1] = fx1 * isx;
+    ay[0] = fy0 * isy;
+    ay[1] = fy1 * isy;
+    return true;
 }
 
 // Slot pull (ks >= 4: few entries per cell).  `vals(e, x0, y0, wx, wy, om, v)`
@@ -233,7 +291,7 @@ __host__ __device__ __forceinline__ size_t cell_pull_region_floats(int tw, int t
 // return every thread has finished reading the staged entries, so the region
 // may be overwritten.  Requires blockDim.x >= ncw and the caller's LDS tiles
 // complete (the first barrier below orders them).
-template <int KS, int NC, bool W_FROM_SRC = false, class EntryFn>
+template <int KS, int NC, bool W_FROM_SRC = false, bool ANCHOR = false, int NPAY = 0, class EntryFn>
 __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEntries &te, int t, int *meta, int ecap,
                                                 float (&acc)[NC], EntryFn entry_fn)
 {
@@ -243,6 +301,7 @@ __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEnt
     int *roff = meta + wh;  // wh + 1: staged offset of the row
     float *region = reinterpret_cast<float *>(meta + 2 * wh + 2);
     float *stx = region, *sty = region + ecap, *sw = region + 2 * ecap;
+    float *spay = region + 3 * ecap;  // NPAY planes of ecap
 
     __syncthreads();
     if ((int)threadIdx.x < wh) {
@@ -270,7 +329,7 @@ __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEnt
     __syncthreads();
     const int total = roff[wh];
     const bool staged = total <= ecap;
-    if (staged) {
+    if (staged && NPAY == 0) {
         float a[kStagePer], b[kStagePer], w[kStagePer];
         int r = 0;
 #pragma unroll
@@ -294,6 +353,37 @@ __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEnt
                 sw[i] = w[k];
             }
         }
+    } else if (staged) {
+        // with a payload: fewer entries per round, every load of a round in flight
+        constexpr int SP = kStagePer / 4;
+        int r = 0;
+        for (int k0 = 0; k0 < kStagePer; k0 += SP) {
+            float a[SP], b[SP], w[SP], pv[SP][NPAY > 0 ? NPAY : 1];
+#pragma unroll
+            for (int k = 0; k < SP; k++) {
+                const int i = threadIdx.x + (k0 + k) * blockDim.x;
+                if (i < total) {
+                    while (roff[r + 1] <= i) r++;
+                    const int e = rbeg[r] + (i - roff[r]);
+                    a[k] = te.tx[e];
+                    b[k] = te.ty[e];
+                    w[k] = te.w[e];
+#pragma unroll
+                    for (int q = 0; q < NPAY; q++) pv[k][q] = te.pay[q * te.pay_stride + e];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < SP; k++) {
+                const int i = threadIdx.x + (k0 + k) * blockDim.x;
+                if (i < total) {
+                    stx[i] = a[k];
+                    sty[i] = b[k];
+                    sw[i] = w[k];
+#pragma unroll
+                    for (int q = 0; q < NPAY; q++) spay[q * ecap + i] = pv[k][q];
+                }
+            }
+        }
     }
     __syncthreads();
 
@@ -309,6 +399,7 @@ __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEnt
             const int lbase = roff[r] - rbeg[r];
             for (int e = e0; e < e1; e++) {
                 float t0, t1, om;
+                float pv[NPAY > 0 ? NPAY : 1];
                 int id = e;  // entry index, or the source index with W_FROM_SRC
                 if (staged) {
                     const int li = lbase + e;
@@ -316,17 +407,28 @@ __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEnt
                     t1 = sty[li];
                     if (W_FROM_SRC) id = __float_as_int(sw[li]);
                     else om = sw[li];
+#pragma unroll
+                    for (int q = 0; q < NPAY; q++) pv[q] = spay[q * ecap + li];
                 } else {
                     t0 = te.tx[e];
                     t1 = te.ty[e];
                     if (W_FROM_SRC) id = te.src[e];
                     else om = te.w[e];
+#pragma unroll
+                    for (int q = 0; q < NPAY; q++) pv[q] = te.pay[q * te.pay_stride + e];
                 }
                 if (W_FROM_SRC) om = te.omega[id];
                 int x0, y0;
-                float wx[KS], wy[KS];
-                if (!footprint<KS>(t0, t1, g.W, g.H, x0, y0, wx, wy)) continue;
-                entry_fn(id, x0, y0, wx, wy, om, acc);
+                if constexpr (ANCHOR) {
+                    float ax[2], ay[2];
+                    if (!box_weights<KS>(t0, t1, g.W, g.H, x0, y0, ax, ay)) continue;
+                    if constexpr (NPAY > 0) entry_fn(id, x0, y0, ax, ay, om, pv, acc);
+                    else entry_fn(id, x0, y0, ax, ay, om, acc);
+                } else {
+                    float wx[KS], wy[KS];
+                    if (!footprint<KS>(t0, t1, g.W, g.H, x0, y0, wx, wy)) continue;
+                    entry_fn(id, x0, y0, wx, wy, om, acc);
+                }
             }
         }
     }
@@ -379,6 +481,141 @@ __device__ __forceinline__ void cell_pull(const TileGeom &g, const TileEntries &
                 for (int q = 0; q < NV; q++) sum[q] += part[((i * KS + j) * NV + q) * ncw + cc];
             }
         epi((size_t)y * g.W + x, x, y, sum);
+    }
+}
+
+// ---- box decomposition of large footprints ---------------------------------
+//
+// Along x the footprint weights of an entry are d = (1-f, 1, ..., 1, f) on the
+// ks = dsf + 1 taps x0 .. x0+dsf (f = the fraction of the box start), divided
+// by their in-image sum sx; out-of-image taps are 0.  That is
+//   (1-f)/sx * box(x0) + f/sx * box(x0+1),  box(b) = 1 on b .. b+dsf-1,
+// restricted to the image, and the 2-D weights are the product of the x and y
+// forms.  So a splat is a 2 x 2 (bilinear-like) splat of the entry values
+// onto box starts (anchors) followed by a dsf x dsf box sum, and a gather is a
+// 2 x 2 gather of the box-summed image (zero outside) at the anchors.  Per
+// entry 4 taps instead of ks^2.  The anchor of cell (cx, cy) is the entry's
+// box start x0 = cx - dsf; tap (i, j) goes to anchor (x0 + j, y0 + i).
+
+// LDS floats of the box pull's region (beside its 2 wh + 2 meta ints): the
+// staged entries (tau, omega and NPAY payload floats each), then the taps
+// shared between neighbouring cells (2 NV per cell) and the anchor sums (NV).
+template <int KS, int NV, int NPAY = 0>
+__host__ __device__ __forceinline__ size_t box_pull_region_floats(int tw, int th, int ecap)
+{
+    const size_t cells = (size_t)(tw + KS - 1) * (th + KS - 1);
+    const size_t part = 3 * (size_t)NV * cells;
+    const size_t stage = (3 + NPAY) * (size_t)ecap;
+    return part > stage ? part : stage;
+}
+
+// Box pull (ks >= 3, plain weights).  Walk: thread c owns window cell c and
+// accumulates, over its entries, the 2 x 2 anchor taps of vals(e, x0, y0, ax,
+// ay, om, [pay,] v).  Cell (col, row) puts tap (i, j) on anchor (col + j,
+// row + i): each anchor adds its own cell's (0, 0) tap, the left cell's
+// (0, 1), the upper cell's (1, 0) and the upper-left cell's (1, 1) -- first
+// along x (the (., 1) taps through LDS), then along y.  Each pixel finally
+// adds the dsf x dsf anchors whose box covers it.  Requires blockDim.x >= ncw.
+template <int KS, int NV, int NPAY = 0, class Vals, class Epi>
+__device__ __forceinline__ void box_pull(const TileGeom &g, const TileEntries &te, int t, int *meta, int ecap,
+                                         Vals vals, Epi epi)
+{
+    constexpr int D = KS - 1;
+    const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
+    const int ww = g.tw + KS - 1, wh = g.th + KS - 1, ncw = ww * wh;
+    float acc[4 * NV];
+    auto add_taps = [&](const float(&ax)[2], const float(&ay)[2], const float(&v)[NV], float(&a)[4 * NV]) {
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const float w = ay[i] * ax[j];
+#pragma unroll
+                for (int q = 0; q < NV; q++) a[(i * 2 + j) * NV + q] += w * v[q];
+            }
+    };
+    if constexpr (NPAY > 0) {
+        cell_accumulate<KS, 4 * NV, false, true, NPAY>(
+            g, te, t, meta, ecap, acc,
+            [&](int e, int x0, int y0, const float(&ax)[2], const float(&ay)[2], float om,
+                const float(&pay)[NPAY], float(&a)[4 * NV]) {
+                float v[NV];
+                vals(e, x0, y0, ax, ay, om, pay, v);
+                add_taps(ax, ay, v, a);
+            });
+    } else {
+        cell_accumulate<KS, 4 * NV, false, true>(
+            g, te, t, meta, ecap, acc,
+            [&](int e, int x0, int y0, const float(&ax)[2], const float(&ay)[2], float om, float(&a)[4 * NV]) {
+                float v[NV];
+                vals(e, x0, y0, ax, ay, om, v);
+                add_taps(ax, ay, v, a);
+            });
+    }
+    float *sh = reinterpret_cast<float *>(meta + 2 * wh + 2);  // 2 NV * ncw
+    float *anc = sh + 2 * NV * ncw;                             // NV * ncw
+    const int c = threadIdx.x;
+    const int row = c / ww, col = c - row * ww;
+    const bool own = c < ncw;
+    // along x: h_i = own (i, 0) tap + the left cell's (i, 1) tap
+    if (own) {
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int q = 0; q < NV; q++) sh[(i * NV + q) * ncw + c] = acc[(i * 2 + 1) * NV + q];
+    }
+    __syncthreads();
+    float h[2 * NV];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+            h[i * NV + q] = acc[(i * 2) * NV + q] + ((own && col > 0) ? sh[(i * NV + q) * ncw + c - 1] : 0.0f);
+    __syncthreads();
+    // along y: anchor = own h_0 + the upper cell's h_1
+    if (own) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) sh[q * ncw + c] = h[NV + q];
+    }
+    __syncthreads();
+    if (own) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) anc[q * ncw + c] = h[q] + (row > 0 ? sh[q * ncw + c - ww] : 0.0f);
+    }
+    __syncthreads();
+    // pixel (x, y) = the anchors ac in [lx + 1, lx + D], ar in [ly + 1, ly + D]
+    for (int p = threadIdx.x; p < g.tw * g.th; p += blockDim.x) {
+        const int lx = p % g.tw, ly = p / g.tw;
+        const int x = tx0 + lx, y = ty0 + ly;
+        if (x >= g.W || y >= g.H) continue;
+        float sum[NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++) sum[q] = 0.0f;
+#pragma unroll
+        for (int i = 1; i <= D; i++)
+#pragma unroll
+            for (int j = 1; j <= D; j++) {
+                const int a = (ly + i) * ww + lx + j;
+#pragma unroll
+                for (int q = 0; q < NV; q++) sum[q] += anc[q * ncw + a];
+            }
+        epi((size_t)y * g.W + x, x, y, sum);
+    }
+}
+
+// The pull of the splat-only kernels: `vals` must not use the weights it is
+// given (their number depends on the pull); the choice is splat_kind's.
+template <int KS, int NV, int SQ, class Vals, class Epi>
+__device__ __forceinline__ void splat_pull(const TileGeom &g, const TileEntries &te, int t, int *lds, int cap,
+                                           Vals vals, Epi epi)
+{
+    if constexpr (KS <= 2 || (SQ != 0 && KS <= 3)) {
+        cell_pull<KS, NV, SQ>(g, te, t, lds, cap, vals, epi);
+    } else if constexpr (SQ == 0) {
+        box_pull<KS, NV>(g, te, t, lds, cap, vals, epi);
+    } else {
+        float *slots = reinterpret_cast<float *>(lds + pull_meta_ints(g.tw, g.th, KS, blockDim.x));
+        tile_pull<KS, NV, SQ>(g, te, t, lds, slots, cap, vals, epi);
     }
 }
 

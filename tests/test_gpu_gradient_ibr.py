@@ -79,7 +79,8 @@ def test_pipeline_stages_match_oracle(stage_scene):
         ogx, ogy, otm, _ = oracle.u_gradient(W, H, s["dsf"], tx, ty, s["views"], om, sig)
         assert np.max(np.abs(gx - ogx)) < 1e-6
         assert np.max(np.abs(gy - ogy)) < 1e-6
-        assert np.max(np.abs(tm - otm)) < 1e-6
+        # (dsf >= 2 splats in box form: FP32 sums of the same terms in another order)
+        assert np.max(np.abs(tm - otm)) < 4e-6 * max(1.0, np.abs(otm).max())
         st = ctx.cg_solve(1e-5, 2000)
         U = ctx.get_solution()
         b = ctx.get_buffer("CG_B").reshape(3, H, W)
