@@ -165,7 +165,8 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     }
     ctx->pull_budget = (p->pull_lds_kb > 0 ? p->pull_lds_kb : 48) * 1024;
     if (ctx->tile_w < ctx->ks || ctx->tile_h < ctx->ks || ctx->tile_w * ctx->tile_h > kPullMaxThreads ||
-        (ctx->tile_w + ctx->ks - 1) * (ctx->tile_h + ctx->ks - 1) > kPullMaxThreads) {
+        (ctx->tile_w + ctx->ks - 1) * (ctx->tile_h + ctx->ks - 1) > kPullMaxThreads ||
+        ctx->tile_h + ctx->ks - 1 > 64) {  // the window rows are scanned by one wave
         delete ctx;
         return LFRV_E_ARG;
     }
@@ -204,7 +205,10 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     A(&ctx->e_tx, ctx->e_cap);
     A(&ctx->e_ty, ctx->e_cap);
     A(&ctx->e_w, ctx->e_cap);
-    if (p->mode == LFRV_MODE_COLOR_SR && ctx->ks >= 3) A(&ctx->e_pay, 6 * ctx->e_cap);
+    if (p->mode == LFRV_MODE_COLOR_SR && ctx->ks >= 3) {
+        A(&ctx->e_pay, 6 * ctx->e_cap);
+        A(&ctx->sr_ops, kSrOpsPlanes * (size_t)(ctx->W + ctx->ks - 1) * (ctx->H + ctx->ks - 1));
+    }
     for (float **b : {&ctx->U, &ctx->Uq, &ctx->G, &ctx->PX, &ctx->PY, &ctx->U2, &ctx->P2X, &ctx->P2Y, &ctx->tmp3N,
                       &ctx->cg_r, &ctx->cg_d, &ctx->cg_q, &ctx->cg_b, &ctx->cache_num})
         A(b, 3 * N);
@@ -249,7 +253,7 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
     for (void *p : std::initializer_list<void *>{
              ctx->views, ctx->tau_x, ctx->tau_y, ctx->dpart_x, ctx->dpart_y, ctx->omega, ctx->depth, ctx->view_lap,
              ctx->view_grad, ctx->cell_count, ctx->cell_start, ctx->cell_cursor, ctx->scan_tmp, ctx->e_src, ctx->e_tx, ctx->e_ty,
-             ctx->e_w, ctx->e_pay, ctx->U, ctx->Uq, ctx->G, ctx->PX, ctx->PY, ctx->U2, ctx->P2X, ctx->P2Y, ctx->tmp3N, ctx->cg_r,
+             ctx->e_w, ctx->e_pay, ctx->sr_ops, ctx->U, ctx->Uq, ctx->G, ctx->PX, ctx->PY, ctx->U2, ctx->P2X, ctx->P2Y, ctx->tmp3N, ctx->cg_r,
              ctx->cg_d, ctx->cg_q, ctx->cg_b, ctx->reg_g, ctx->norm_mask, ctx->coverage, ctx->u_grad_x,
              ctx->u_grad_y, ctx->target_mask, ctx->visibility, ctx->target_depth, ctx->tmpN, ctx->cg_minv,
              ctx->hole_list, ctx->pp_scratch, ctx->cg_state, ctx->counters, ctx->dcams, ctx->scal, ctx->partials,

@@ -680,7 +680,11 @@ int pull_cap(const lfrv_ctx *ctx, int nv, size_t extra, size_t budget, PullKind 
         const int wh = ctx->tile_h + ctx->ks - 1;
         const long avail = (long)budget - (long)extra - (long)sizeof(int) * (2 * wh + 2);
         const long ecap = avail / (long)((3 + npay) * sizeof(float));
-        return (int)std::max(0L, std::min(ecap, (long)kStagePer * ctx->pull_bs));
+        // the box pull reads what does not fit in place, so it stages at most
+        // a few times a window's typical entry count (more blocks per CU)
+        const long lim = kind == PullKind::Box ? std::min(2048L, (long)kStagePer * ctx->pull_bs)
+                                               : (long)kStagePer * ctx->pull_bs;
+        return (int)std::max(0L, std::min(ecap, lim));
     }
     const size_t meta = sizeof(int) * (size_t)pull_meta_ints(ctx->tile_w, ctx->tile_h, ctx->ks, ctx->pull_bs);
     const long avail = (long)budget - (long)meta - (long)extra;

@@ -136,142 +136,304 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_sr_dataterm(TileGeom g, Til
         });
 }
 
-// The same data term for dsf >= 2 in box form (see box_pull): the gathers
-// S_i U and S_i lap U of an entry are 2 x 2 gathers, at its anchors, of the
-// dsf x dsf box sums of U and lap U (zero outside the image), and the splat
-// of its residuals is a box pull.  U (the box region of the tile's anchors
-// plus a 1-pixel ring) and lap U are staged in the pull's region first; the
-// box sums stay in LDS beside it.
+// ---- dsf >= 2: the data term as an assembled operator -------------------------
+//
+// With t_e the 2 x 2 anchor taps of entry e (box_weights) and BU, BL the box
+// sums of U and lap U at the anchors, the data-term sums of one outer step are
+// linear in U while omega is fixed:
+//   sum_e omega t_e (t_e^T BU - v_e)                 = G9 BU - bI
+//   sum_e t_e (dsf^2 t_e^T BL - lap v_e) [* omega]     = dsf^2 G9' BL - bG
+// G9 = sum omega t t^T couples each anchor with its 8 neighbours (centre, E,
+// S, SE, SW planes; the others are the neighbours' couplings), G9' is the same
+// without omega (gw_type 0), bI = sum omega t v, bG = sum t lap v [* omega].
+// They are assembled once per set of weights (k_sr_assemble); every outer
+// step is then a tiled stencil (k_sr_dataterm_st): box sums, the 9-point
+// couplings, and the box sums back onto the pixels.
+//
+// Anchor grid: point x' in [0, W + dsf) has box start x' + 1 - dsf; it is the
+// top-left tap of the cell x' + 1 (cells hold box start cx - dsf).  Pixel x
+// lies in the boxes of the points x .. x + dsf - 1.
+// (kSrOpsPlanes planes: G9 (5), bI (3), G9' (5), bG (3))
+
 template <int KS>
-__global__ void __launch_bounds__(kPullMaxThreads)
-    k_sr_dataterm_box(TileGeom g, TileEntries te, int cap, size_t n, DataTermArgs da, const float *__restrict__ U,
-                      const float *__restrict__ views, const float *__restrict__ vlap,
-                      const float *__restrict__ coverage, float *__restrict__ Gout, float *__restrict__ Uq,
-                      float *__restrict__ norm_mask)
+__global__ void __launch_bounds__(kPullMaxThreads) k_sr_assemble(TileGeom g, TileEntries te, int cap, int grad,
+                                                                 int gw, size_t NA, float *__restrict__ ops)
 {
-    constexpr int D = KS - 1;
     extern __shared__ int smem[];
+    constexpr int NC = 44;  // 10 weighted couplings, 12 bI taps, 12 bG taps, 10 unweighted couplings
     const int t = xcd_tile(blockIdx.x, g.ntiles);
     const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
-    const int ww = g.tw + KS - 1, wh = g.th + KS - 1;
-    const int aw = ww + 1, ah = wh + 1, ap = aw * ah;  // anchors (ac, ar): box start (tx0 + ac - D, ty0 + ar - D)
-    const int uw = g.tw + 2 * D + 2, uh = g.th + 2 * D + 2, up = uw * uh;  // U from (tx0 - D - 1, ty0 - D - 1)
-    const int lw = g.tw + 2 * D, lh = g.th + 2 * D, lpp = lw * lh;        // lap U from (tx0 - D, ty0 - D)
-    float *bu = reinterpret_cast<float *>(smem);  // 3 * ap box sums of U
-    float *bl = bu + 3 * ap;                      // 3 * ap box sums of lap U
-    int *pull = reinterpret_cast<int *>(bl + 3 * ap);
-    float *us = reinterpret_cast<float *>(pull + 2 * wh + 2);  // staging, inside the pull's region
-    float *ls = us + 3 * up;
-    const size_t N = (size_t)g.W * g.H;
-    const bool need_lap = da.beta > 0.0f;
-    for (int i = threadIdx.x; i < up; i += blockDim.x) {
-        const int x = tx0 - D - 1 + i % uw, y = ty0 - D - 1 + i / uw;
-        const bool in = x >= 0 && x < g.W && y >= 0 && y < g.H;
-        const size_t o = in ? (size_t)y * g.W + x : 0;
+    const int ww = g.tw + 1, wh = g.th + 1, ncw = ww * wh;
+    float acc[NC];
+    cell_accumulate<KS, NC, false, true, 6, 1>(
+        g, te, t, smem, cap, acc,
+        [&](int, int, int, const float(&ax)[2], const float(&ay)[2], float om, const float(&pay)[6],
+            float(&a)[NC]) {
+            const float w[4] = {ay[0] * ax[0], ay[0] * ax[1], ay[1] * ax[0], ay[1] * ax[1]};  // tl tr bl br
+            const float pr[10] = {w[0] * w[0], w[1] * w[1], w[2] * w[2], w[3] * w[3], w[0] * w[1],
+                                  w[2] * w[3], w[0] * w[2], w[1] * w[3], w[0] * w[3], w[1] * w[2]};
 #pragma unroll
-        for (int c = 0; c < 3; c++) us[c * up + i] = in ? U[c * N + o] : 0.0f;
+            for (int k = 0; k < 10; k++) a[k] += om * pr[k];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) a[10 + k * 3 + c] += om * w[k] * pay[c];
+            if (grad) {
+                const float og = gw ? om : 1.0f;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) a[22 + k * 3 + c] += og * w[k] * pay[3 + c];
+#pragma unroll
+                for (int k = 0; k < 10; k++) a[34 + k] += pr[k];
+            }
+        });
+    float *part = reinterpret_cast<float *>(smem + 2 * wh + 2);
+    const int c = threadIdx.x;
+    if (c < ncw) {
+#pragma unroll
+        for (int k = 0; k < NC; k++) part[k * ncw + c] = acc[k];
     }
     __syncthreads();
-    if (need_lap) {
+    for (int p = threadIdx.x; p < g.tw * g.th; p += blockDim.x) {
+        const int lx = p % g.tw, ly = p / g.tw;
+        const int x = tx0 + lx, y = ty0 + ly;
+        if (x >= g.Wc || y >= g.Hc) continue;
+        // own cell (its tl tap), left cell (tr), upper cell (bl), upper-left cell (br)
+        const int c00 = (ly + 1) * ww + lx + 1, c10 = c00 - 1, c01 = c00 - ww, c11 = c00 - ww - 1;
+        const size_t o = (size_t)y * g.Wc + x;
+        auto couple = [&](int base, int plane) {
+            ops[(size_t)(plane + 0) * NA + o] = part[(base + 0) * ncw + c00] + part[(base + 1) * ncw + c10] +
+                                                part[(base + 2) * ncw + c01] + part[(base + 3) * ncw + c11];
+            ops[(size_t)(plane + 1) * NA + o] = part[(base + 4) * ncw + c00] + part[(base + 5) * ncw + c01];
+            ops[(size_t)(plane + 2) * NA + o] = part[(base + 6) * ncw + c00] + part[(base + 7) * ncw + c10];
+            ops[(size_t)(plane + 3) * NA + o] = part[(base + 8) * ncw + c00];
+            ops[(size_t)(plane + 4) * NA + o] = part[(base + 9) * ncw + c10];
+        };
+        auto taps = [&](int base, int plane) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++)
+                ops[(size_t)(plane + ch) * NA + o] =
+                    part[(base + 0 * 3 + ch) * ncw + c00] + part[(base + 1 * 3 + ch) * ncw + c10] +
+                    part[(base + 2 * 3 + ch) * ncw + c01] + part[(base + 3 * 3 + ch) * ncw + c11];
+        };
+        couple(0, 0);
+        taps(10, 5);
+        if (grad) {
+            couple(34, 8);
+            taps(22, 13);
+        }
+    }
+}
+
+// One outer step's data term on a 32 x 8 pixel tile from the assembled
+// operator.  Every global load of the block (U with its halo, the couplings
+// and right-hand sides of the tile's anchors, coverage) is issued first; the
+// rest runs in LDS: lap U, the box sums BU / BL at the anchors (+1 ring), the
+// anchor residuals QI = G9 BU - bI and QG = dsf^2 G9' BL - bG, then the box
+// sums back onto the pixels, split into rows and columns.
+constexpr int kStTW = 32, kStTH = 8;
+
+template <int KS>
+__global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, int Wc, int Hc, int ntx, int ntiles,
+                                                                   DataTermArgs da, const float *__restrict__ U,
+                                                                   const float *__restrict__ ops, size_t NA,
+                                                                   const float *__restrict__ coverage,
+                                                                   float *__restrict__ Gout, float *__restrict__ Uq,
+                                                                   float *__restrict__ norm_mask)
+{
+    constexpr int D = KS - 1, TW = kStTW, TH = kStTH, NT = TW * TH;
+    constexpr int QW = TW + D - 1, QH = TH + D - 1, NQ = QW * QH;  // anchor points x' = tx0 + qx
+    constexpr int BW = QW + 2, BH = QH + 2;                         // box sums at x' = tx0 - 1 + bx
+    constexpr int LW = BW + D - 1, LH = BH + D - 1;                 // pixels tx0 - D + lx (box sums' support)
+    constexpr int SW = LW + 2, SH = LH + 2;                         // U from tx0 - D - 1 (the Laplacian's ring)
+    constexpr int AREA = (3 * SH * SW + 3 * LH * LW) > (6 * QH * QW + 6 * QH * TW) ? (3 * SH * SW + 3 * LH * LW)
+                                                                                    : (6 * QH * QW + 6 * QH * TW);
+    constexpr int NQT = (NQ + NT - 1) / NT, NST = (SH * SW + NT - 1) / NT;
+    __shared__ float sa[AREA];           // U and lap U, later the anchor residuals and their row sums
+    __shared__ float sb[6 * BH * BW];    // BU (3) and BL (3)
+    float *us = sa, *ls = sa + 3 * SH * SW;
+    float *qs = sa, *rs = sa + 6 * QH * QW;
+    float *bus = sb, *bls = sb + 3 * BH * BW;
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const int tx0 = (t % ntx) * TW, ty0 = (t / ntx) * TH;
+    const size_t N = (size_t)W * H;
+    const bool grad = da.beta > 0.0f;
+    const float *g9 = da.gw_type ? ops : ops + 8 * NA;  // couplings of the gradient term
+    const int tid = threadIdx.x;
+
+    // ---- all global loads ----
+    float uv[NST][3];
+#pragma unroll
+    for (int k = 0; k < NST; k++) {
+        const int i = tid + k * NT;
+        const int x = tx0 - D - 1 + i % SW, y = ty0 - D - 1 + i / SW;
+        const bool in = i < SH * SW && x >= 0 && x < W && y >= 0 && y < H;
+        const size_t o = in ? (size_t)y * W + x : 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) uv[k][c] = in ? U[c * N + o] : 0.0f;
+    }
+    float cf[NQT][9], cg[NQT][9], bi[NQT][3], bg[NQT][3];
+#pragma unroll
+    for (int k = 0; k < NQT; k++) {
+        const int i = tid + k * NT;
+        const int xq = tx0 + i % QW, yq = ty0 + i / QW;
+        const bool in = i < NQ && xq < Wc && yq < Hc;
+        const int xc = min(xq, Wc - 1), yc = min(yq, Hc - 1);
+        const size_t o = (size_t)yc * Wc + xc;
+        const bool xm = xc > 0, ym = yc > 0, xp = xc < Wc - 1;
+        const size_t oE = xm ? o - 1 : o, oS = ym ? o - Wc : o, oSE = (xm && ym) ? o - Wc - 1 : o,
+                     oSW = (xp && ym) ? o - Wc + 1 : o;
+        auto load9 = [&](const float *P, float(&f)[9]) {
+            f[0] = P[o];
+            f[1] = P[NA + o];
+            f[2] = P[2 * NA + o];
+            f[3] = P[3 * NA + o];
+            f[4] = P[4 * NA + o];
+            const float e = P[NA + oE], sn = P[2 * NA + oS], se = P[3 * NA + oSE], sw = P[4 * NA + oSW];
+            f[5] = xm ? e : 0.0f;
+            f[6] = ym ? sn : 0.0f;
+            f[7] = (xm && ym) ? se : 0.0f;
+            f[8] = (xp && ym) ? sw : 0.0f;
+            if (!in)
+#pragma unroll
+                for (int q = 0; q < 9; q++) f[q] = 0.0f;
+        };
+        load9(ops, cf[k]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) bi[k][c] = in ? ops[(size_t)(5 + c) * NA + o] : 0.0f;
+        if (grad) {
+            load9(g9, cg[k]);
+#pragma unroll
+            for (int c = 0; c < 3; c++) bg[k][c] = in ? ops[(size_t)(13 + c) * NA + o] : 0.0f;
+        }
+    }
+    const int px = tx0 + tid % TW, py = ty0 + tid / TW;
+    const bool pin = px < W && py < H;
+    const size_t po = pin ? (size_t)py * W + px : 0;
+    const float cov = pin ? coverage[po] : 0.0f;
+    float up[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) up[c] = pin ? U[c * N + po] : 0.0f;
+
+    // ---- U, lap U, box sums ----
+#pragma unroll
+    for (int k = 0; k < NST; k++) {
+        const int i = tid + k * NT;
+        if (i < SH * SW)
+#pragma unroll
+            for (int c = 0; c < 3; c++) us[c * SH * SW + i] = uv[k][c];
+    }
+    __syncthreads();
+    if (grad) {
         // cuda_laplacian_device (vtv_sr_kernels_unstructured.cu:720-761), no hole gating
-        for (int i = threadIdx.x; i < lpp; i += blockDim.x) {
-            const int lx = i % lw, ly = i / lw;
+        for (int i = tid; i < LH * LW; i += NT) {
+            const int lx = i % LW, ly = i / LW;
             const int x = tx0 - D + lx, y = ty0 - D + ly;
-            const int ui = (ly + 1) * uw + lx + 1;
+            const int ui = (ly + 1) * SW + lx + 1;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 float v = 0.0f;
-                if (x >= 0 && x < g.W && y >= 0 && y < g.H) {
-                    const float *u = us + c * up;
+                if (x >= 0 && x < W && y >= 0 && y < H) {
+                    const float *u = us + c * SH * SW;
                     float sm = 0.0f, wsum = 0.0f;
                     if (x > 0) { sm += u[ui - 1]; wsum += 1.0f; }
-                    if (y > 0) { sm += u[ui - uw]; wsum += 1.0f; }
-                    if (x < g.W - 1) { sm += u[ui + 1]; wsum += 1.0f; }
-                    if (y < g.H - 1) { sm += u[ui + uw]; wsum += 1.0f; }
+                    if (y > 0) { sm += u[ui - SW]; wsum += 1.0f; }
+                    if (x < W - 1) { sm += u[ui + 1]; wsum += 1.0f; }
+                    if (y < H - 1) { sm += u[ui + SW]; wsum += 1.0f; }
                     v = sm / wsum - u[ui];
                 }
-                ls[c * lpp + i] = v;
+                ls[c * LH * LW + i] = v;
             }
         }
         __syncthreads();
     }
-    // box sums at the anchors: anchor (ac, ar) covers U-staging pixels
-    // (ac + 1 .. ac + D, ar + 1 .. ar + D) and lap-staging (ac .. ac + D - 1, ...)
-    for (int a = threadIdx.x; a < ap; a += blockDim.x) {
-        const int ar = a / aw, ac = a - ar * aw;
+    for (int i = tid; i < BH * BW; i += NT) {
+        const int bx = i % BW, by = i / BW;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             float su = 0.0f, sl = 0.0f;
-#pragma unroll
+#pragma unroll 1
             for (int l = 0; l < D; l++)
 #pragma unroll
                 for (int k = 0; k < D; k++) {
-                    su += us[c * up + (ar + l + 1) * uw + ac + k + 1];
-                    if (need_lap) sl += ls[c * lpp + (ar + l) * lw + ac + k];
+                    su += us[c * SH * SW + (by + l + 1) * SW + bx + k + 1];
+                    if (grad) sl += ls[c * LH * LW + (by + l) * LW + bx + k];
                 }
-            bu[c * ap + a] = su;
-            bl[c * ap + a] = sl;
+            bus[c * BH * BW + i] = su;
+            bls[c * BH * BW + i] = sl;
         }
     }
-    // (the pull's first barrier orders these sums before the walk and the
-    // staging before its own use of the region)
-    box_pull<KS, 6, 6>(
-        g, te, t, pull, cap,
-        [&](int, int x0, int y0, const float(&ax)[2], const float(&ay)[2], float om, const float(&pay)[6],
-            float(&r)[6]) {
-            const int a0 = (y0 - ty0 + D) * aw + (x0 - tx0 + D);
-            float su[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
+    __syncthreads();
+
+    // ---- anchor residuals ----
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+    for (int k = 0; k < NQT; k++) {
+        const int i = tid + k * NT;
+        if (i >= NQ) continue;
+        const int qx = i % QW, qy = i / QW;
+        const int b = (qy + 1) * BW + qx + 1;  // the point in the box-sum tile
+        auto st9 = [&](const float *B, const float(&f)[9]) {
+            return f[0] * B[b] + f[1] * B[b + 1] + f[5] * B[b - 1] + f[2] * B[b + BW] + f[6] * B[b - BW] +
+                   f[3] * B[b + BW + 1] + f[7] * B[b - BW - 1] + f[4] * B[b + BW - 1] + f[8] * B[b - BW + 1];
+        };
 #pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    const float w = ay[i] * ax[j];
-                    const int a = a0 + i * aw + j;
+        for (int c = 0; c < 3; c++) {
+            qs[c * NQ + i] = st9(bus + c * BH * BW, cf[k]) - bi[k][c];
+            if (grad) qs[(3 + c) * NQ + i] = da.dsf2 * st9(bls + c * BH * BW, cg[k]) - bg[k][c];
+        }
+    }
+    __syncthreads();
+    // rows: r(x, qy) = sum_k q(x + k, qy)
+    for (int i = tid; i < QH * TW; i += NT) {
+        const int lx = i % TW, qy = i / TW;
 #pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        su[c] += w * bu[c * ap + a];
-                        if (need_lap) sl[c] += w * bl[c * ap + a];
-                    }
-                }
+        for (int c = 0; c < 6; c++) {
+            if (c >= 3 && !grad) break;
+            float sm = 0.0f;
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                r[c] = (da.alpha > 0.0f) ? (su[c] - pay[c]) * om : 0.0f;
-                float rg = 0.0f;
-                if (need_lap) {
-                    rg = sl[c] * da.dsf2 - pay[3 + c];
-                    if (da.gw_type != 0) rg *= om;
-                }
-                r[3 + c] = rg;
-            }
-        },
-        [&](size_t o, int, int, const float(&acc)[6]) {
-            const float cov = coverage[o];
+            for (int k = 0; k < D; k++) sm += qs[c * NQ + qy * QW + lx + k];
+            rs[(c * QH + qy) * TW + lx] = sm;
+        }
+    }
+    __syncthreads();
+
+    // ---- pixel: columns of the row sums, then the step of vtv_sr_ista_step_unstructured ----
+    if (!pin) return;
+    const int lx = tid % TW, ly = tid / TW;
+    float acc[6];
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                float G = 0.0f;
-                if (da.alpha > 0.0f) {
-                    float gi = acc[c];
-                    if (cov != 0.0f) gi /= cov;
-                    gi *= da.sigma2;
-                    G += da.alpha * gi;
-                }
-                if (need_lap) {
-                    float gg = acc[3 + c];
-                    if (cov != 0.0f) gg /= cov;
-                    if (da.gw_type != 0) gg *= da.sigma2;
-                    gg *= -da.grad_step;
-                    G += da.beta * gg;
-                }
-                G = fminf(1.0f, fmaxf(-1.0f, G));
-                const float uv = U[c * N + o];
-                G *= da.prox_factor;
-                G += uv;
-                G = fminf(1.0f, fmaxf(0.0f, G));
-                Gout[c * N + o] = G;
-                Uq[c * N + o] = uv;
-            }
-            if (da.alpha > 0.0f || need_lap) norm_mask[o] = cov;
-        });
+    for (int c = 0; c < 6; c++) {
+        float sm = 0.0f;
+        if (c < 3 || grad)
+#pragma unroll
+            for (int l = 0; l < D; l++) sm += rs[(c * QH + ly + l) * TW + lx];
+        acc[c] = sm;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float G = 0.0f;
+        if (da.alpha > 0.0f) {
+            float gi = acc[c];
+            if (cov != 0.0f) gi /= cov;
+            gi *= da.sigma2;
+            G += da.alpha * gi;
+        }
+        if (grad) {
+            float gg = acc[3 + c];
+            if (cov != 0.0f) gg /= cov;
+            if (da.gw_type != 0) gg *= da.sigma2;
+            gg *= -da.grad_step;
+            G += da.beta * gg;
+        }
+        G = fminf(1.0f, fmaxf(-1.0f, G));
+        G *= da.prox_factor;
+        G += up[c];
+        G = fminf(1.0f, fmaxf(0.0f, G));
+        Gout[c * N + po] = G;
+        Uq[c * N + po] = up[c];
+    }
+    if (da.alpha > 0.0f || grad) norm_mask[po] = cov;
 }
 
 // theta over-relaxation (vtv_rof.cu:28-45): uq <- u + theta (uq - u); u <- uq_old
@@ -423,20 +585,33 @@ lfrv_status run_fista(lfrv_ctx *ctx)
                                                                           ctx->norm_mask);
         LFRV_LAUNCH(ctx, "data term");
     } else {
-        // box form: box sums of U and lap U at the anchors beside the pull,
-        // whose region first holds the staged U and lap U
-        const int D = HAL, ww = ctx->tile_w + ctx->ks - 1, wh = ctx->tile_h + ctx->ks - 1;
-        const size_t ap = (size_t)(ww + 1) * (wh + 1);
-        const size_t sums = sizeof(float) * 6 * ap;
-        const size_t up = (size_t)(ctx->tile_w + 2 * D + 2) * (ctx->tile_h + 2 * D + 2);
-        const size_t lpp = (size_t)(ctx->tile_w + 2 * D) * (ctx->tile_h + 2 * D);
-        const size_t stage = sizeof(int) * (2 * wh + 2) + sizeof(float) * (3 * up + 3 * lpp);
-        // entries staged with their 6 payload floats; up to 64 KiB of LDS
-        const size_t budget = std::max<size_t>(ctx->pull_budget, 65536 - 512);
-        const int cap = pull_cap(ctx, 6, sums, budget, PullKind::Box, 6);
-        const size_t lds = sums + std::max(stage, pull_lds(ctx, 6, cap, PullKind::Box, 6));
+        // assembled operator on the anchor grid, rebuilt when the weights or
+        // the entries changed; the step itself is a stencil
+        const int Wc = ctx->W + ctx->ks - 1, Hc = ctx->H + ctx->ks - 1;
+        const size_t NA = (size_t)Wc * Hc;
+        const int grad = ctx->p.dt_beta > 0.0f ? 1 : 0;
+        if (ctx->sr_ops_epoch != ctx->epoch) {
+            TileGeom ga = g;
+            ga.ntx = (Wc + g.tw - 1) / g.tw;
+            ga.ntiles = ga.ntx * ((Hc + g.th - 1) / g.th);
+            const int wh = g.th + 1, ncw = (g.tw + 1) * wh;
+            const int cap = std::min(1024, kStagePer * ctx->pull_bs);
+            const size_t lds = sizeof(int) * (2 * wh + 2) + sizeof(float) * std::max<size_t>(44 * (size_t)ncw, 9 * (size_t)cap);
+            TimedRegion tr(ctx, 5);
+#define AS(K) k_sr_assemble<K><<<ga.ntiles, ctx->pull_bs, lds, ctx->stream>>>(ga, te, cap, grad, ctx->p.gw_type != 0, NA, ctx->sr_ops)
+            switch (ctx->ks) {
+            case 3: AS(3); break;
+            case 4: AS(4); break;
+            case 5: AS(5); break;
+            default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
+            }
+#undef AS
+            LFRV_LAUNCH(ctx, "data-term assembly");
+            ctx->sr_ops_epoch = ctx->epoch;
+        }
+        const int ntx = (ctx->W + kStTW - 1) / kStTW, ntiles = ntx * ((ctx->H + kStTH - 1) / kStTH);
         TimedRegion tr(ctx, 3);
-#define DT(K) k_sr_dataterm_box<K><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, ctx->n, da, ctx->U, ctx->views, ctx->view_lap, ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask)
+#define DT(K) k_sr_dataterm_st<K><<<ntiles, kStTW * kStTH, 0, ctx->stream>>>(ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, ctx->U, ctx->sr_ops, NA, ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask)
         switch (ctx->ks) {
         case 3: DT(3); break;
         case 4: DT(4); break;

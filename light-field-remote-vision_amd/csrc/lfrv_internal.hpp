@@ -196,6 +196,7 @@ __device__ __forceinline__ void reduce_partials(const double *partials, int G, d
 }
 
 constexpr int kFinalizeThreads = 1024;  // threads of the one-block finalize kernels
+constexpr int kSrOpsPlanes = 16;        // planes of the assembled IBR_color data term
 
 // Finalize kernel of the plain sums.
 template <int NQ>
@@ -273,6 +274,8 @@ struct lfrv_ctx {
     int *e_src = nullptr;       // view * n + m
     float *e_tx = nullptr, *e_ty = nullptr, *e_w = nullptr;
     float *e_pay = nullptr;  // IBR_color, dsf >= 2: per entry its view's RGB and Laplacian (6 planes)
+    float *sr_ops = nullptr;   // IBR_color, dsf >= 2: the assembled data-term operator (kSrOpsPlanes anchor planes)
+    uint64_t sr_ops_epoch = 0;
     size_t e_cap = 0;
     bool prepared = false;
     bool entry_weights_valid = false;

@@ -119,31 +119,18 @@ template <int KS>
 __device__ __forceinline__ bool box_weights(float tx, float ty, int W, int H, int &x0, int &y0, float (&ax)[2],
                                             float (&ay)[2])
 {
-    constexpr float hd = 0.5f * (float)(KS - 1);
+    constexpr int D = KS - 1;
+    constexpr float hd = 0.5f * (float)D;
     const float px = tx - 0.5f, py = ty - 0.5f;
     x0 = (int)floorf(px - hd + 0.5f);
     y0 = (int)floorf(py - hd + 0.5f);
-    float sx = 0.0f, sy = 0.0f, fx0 = 0.0f, fx1 = 0.0f, fy0 = 0.0f, fy1 = 0.0f;
-#pragma unroll
-    for (int j = 0; j < KS; j++) {
-        const int x = x0 + j;
-        float d = 1.0f;
-        if (j == 0) d = ((float)x + 0.5f) - (px - hd);
-        else if (j == KS - 1) d = (px + hd) - ((float)x - 0.5f);
-        if (j == 0) fx0 = d;
-        if (j == KS - 1) fx1 = d;
-        sx += (x >= 0 && x <= W - 1) ? d : 0.0f;
-    }
-#pragma unroll
-    for (int i = 0; i < KS; i++) {
-        const int y = y0 + i;
-        float d = 1.0f;
-        if (i == 0) d = ((float)y + 0.5f) - (py - hd);
-        else if (i == KS - 1) d = (py + hd) - ((float)y - 0.5f);
-        if (i == 0) fy0 = d;
-        if (i == KS - 1) fy1 = d;
-        sy += (y >= 0 && y <= H - 1) ? d : 0.0f;
-    }
+    // end taps as footprint<KS>; the D - 1 interior taps weigh 1 each
+    const float fx0 = ((float)x0 + 0.5f) - (px - hd), fx1 = (px + hd) - ((float)(x0 + D) - 0.5f);
+    const float fy0 = ((float)y0 + 0.5f) - (py - hd), fy1 = (py + hd) - ((float)(y0 + D) - 0.5f);
+    const int nix = max(0, min(x0 + D - 1, W - 1) - max(x0 + 1, 0) + 1);  // interior taps inside the image
+    const int niy = max(0, min(y0 + D - 1, H - 1) - max(y0 + 1, 0) + 1);
+    const float sx = (x0 >= 0 && x0 <= W - 1 ? fx0 : 0.0f) + (float)nix + (x0 + D >= 0 && x0 + D <= W - 1 ? fx1 : 0.0f);
+    const float sy = (y0 >= 0 && y0 <= H - 1 ? fy0 : 0.0f) + (float)niy + (y0 + D >= 0 && y0 + D <= H - 1 ? fy1 : 0.0f);
     if (sx * sy == 0.0f) return false;
     const float isx = 1.0f / sx, isy = 1.0f / sy;
     ax[0] = fx0 * isx;
@@ -291,51 +278,66 @@ __host__ __device__ __forceinline__ size_t cell_pull_region_floats(int tw, int t
 // return every thread has finished reading the staged entries, so the region
 // may be overwritten.  Requires blockDim.x >= ncw and the caller's LDS tiles
 // complete (the first barrier below orders them).
-template <int KS, int NC, bool W_FROM_SRC = false, bool ANCHOR = false, int NPAY = 0, class EntryFn>
+// WEXT: the window is (tw + WEXT) x (th + WEXT) cells from the tile origin
+// (KS - 1 for target-pixel tiles; 1 for tiles of box anchors, where the cell
+// of box start b holds the anchor taps of b and b + 1).
+template <int KS, int NC, bool W_FROM_SRC = false, bool ANCHOR = false, int NPAY = 0, int WEXT = KS - 1,
+          class EntryFn>
 __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEntries &te, int t, int *meta, int ecap,
                                                 float (&acc)[NC], EntryFn entry_fn)
 {
     const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
-    const int ww = g.tw + KS - 1, wh = g.th + KS - 1, ncw = ww * wh;
+    const int ww = g.tw + WEXT, wh = g.th + WEXT, ncw = ww * wh;
     int *rbeg = meta;       // wh: CSR index of the row's first entry
     int *roff = meta + wh;  // wh + 1: staged offset of the row
     float *region = reinterpret_cast<float *>(meta + 2 * wh + 2);
     float *stx = region, *sty = region + ecap, *sw = region + 2 * ecap;
     float *spay = region + 3 * ecap;  // NPAY planes of ecap
 
-    __syncthreads();
+    // global loads first (they do not touch LDS): the window rows' CSR ranges
+    // and the owned cell's range
+    const int c = threadIdx.x;
+    int rb = 0, rcnt = 0, e0 = 0, e1 = 0;
     if ((int)threadIdx.x < wh) {
-        const int r = threadIdx.x, cy = ty0 + r;
-        int b = 0, cnt = 0;
+        const int cy = ty0 + (int)threadIdx.x;
         if (te.active && cy < g.Hc) {
-            const int c0 = cy * g.Wc + tx0;
-            const int c1 = cy * g.Wc + min(tx0 + ww, g.Wc);
-            b = te.cell_start[c0];
-            cnt = te.cell_start[c1] - b;
+            rb = te.cell_start[cy * g.Wc + tx0];
+            rcnt = te.cell_start[cy * g.Wc + min(tx0 + ww, g.Wc)] - rb;
         }
-        rbeg[r] = b;
-        roff[r] = cnt;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int s = 0;
-        for (int r = 0; r < wh; r++) {
-            const int c = roff[r];
-            roff[r] = s;
-            s += c;
+    const int crow = c / ww, ccol = c - crow * ww;
+    if (c < ncw && te.active && tx0 + ccol < g.Wc && ty0 + crow < g.Hc) {
+        const int gc = (ty0 + crow) * g.Wc + tx0 + ccol;
+        e0 = te.cell_start[gc];
+        e1 = te.cell_start[gc + 1];
+    }
+    __syncthreads();  // the caller's LDS tiles are complete
+    if (threadIdx.x < 64) {
+        // staged offsets of the rows: exclusive scan of the counts by wave 0
+        int v = (int)threadIdx.x < wh ? rcnt : 0;
+        int incl = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(incl, off, 64);
+            if ((int)threadIdx.x >= off) incl += u;
         }
-        roff[wh] = s;
+        if ((int)threadIdx.x < wh) {
+            rbeg[threadIdx.x] = rb;
+            roff[threadIdx.x] = incl - v;
+        }
+        if ((int)threadIdx.x == wh - 1) roff[wh] = incl;
     }
     __syncthreads();
     const int total = roff[wh];
-    const bool staged = total <= ecap;
-    if (staged && NPAY == 0) {
+    // the first ecap entries of the window are staged, the rest read in place
+    const int nst = min(total, ecap);
+    if (NPAY == 0) {
         float a[kStagePer], b[kStagePer], w[kStagePer];
         int r = 0;
 #pragma unroll
         for (int k = 0; k < kStagePer; k++) {
             const int i = threadIdx.x + k * blockDim.x;
-            if (i < total) {
+            if (i < nst) {
                 while (roff[r + 1] <= i) r++;
                 const int e = rbeg[r] + (i - roff[r]);
                 a[k] = te.tx[e];
@@ -347,13 +349,13 @@ __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEnt
 #pragma unroll
         for (int k = 0; k < kStagePer; k++) {
             const int i = threadIdx.x + k * blockDim.x;
-            if (i < total) {
+            if (i < nst) {
                 stx[i] = a[k];
                 sty[i] = b[k];
                 sw[i] = w[k];
             }
         }
-    } else if (staged) {
+    } else {
         // with a payload: fewer entries per round, every load of a round in flight
         constexpr int SP = kStagePer / 4;
         int r = 0;
@@ -362,7 +364,7 @@ __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEnt
 #pragma unroll
             for (int k = 0; k < SP; k++) {
                 const int i = threadIdx.x + (k0 + k) * blockDim.x;
-                if (i < total) {
+                if (i < nst) {
                     while (roff[r + 1] <= i) r++;
                     const int e = rbeg[r] + (i - roff[r]);
                     a[k] = te.tx[e];
@@ -375,7 +377,7 @@ __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEnt
 #pragma unroll
             for (int k = 0; k < SP; k++) {
                 const int i = threadIdx.x + (k0 + k) * blockDim.x;
-                if (i < total) {
+                if (i < nst) {
                     stx[i] = a[k];
                     sty[i] = b[k];
                     sw[i] = w[k];
@@ -389,20 +391,15 @@ __device__ __forceinline__ void cell_accumulate(const TileGeom &g, const TileEnt
 
 #pragma unroll
     for (int k = 0; k < NC; k++) acc[k] = 0.0f;
-    const int c = threadIdx.x;
     if (c < ncw) {
-        const int r = c / ww, col = c - r * ww;
-        const int cx = tx0 + col, cy = ty0 + r;
-        if (te.active && cx < g.Wc && cy < g.Hc) {
-            const int gc = cy * g.Wc + cx;
-            const int e0 = te.cell_start[gc], e1 = te.cell_start[gc + 1];
-            const int lbase = roff[r] - rbeg[r];
+        {
+            const int lbase = roff[crow] - rbeg[crow];
             for (int e = e0; e < e1; e++) {
                 float t0, t1, om;
                 float pv[NPAY > 0 ? NPAY : 1];
                 int id = e;  // entry index, or the source index with W_FROM_SRC
-                if (staged) {
-                    const int li = lbase + e;
+                const int li = lbase + e;
+                if (li < nst) {
                     t0 = stx[li];
                     t1 = sty[li];
                     if (W_FROM_SRC) id = __float_as_int(sw[li]);
@@ -591,7 +588,7 @@ __device__ __forceinline__ void box_pull(const TileGeom &g, const TileEntries &t
         float sum[NV];
 #pragma unroll
         for (int q = 0; q < NV; q++) sum[q] = 0.0f;
-#pragma unroll
+#pragma unroll 1
         for (int i = 1; i <= D; i++)
 #pragma unroll
             for (int j = 1; j <= D; j++) {
