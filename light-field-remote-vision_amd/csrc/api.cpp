@@ -229,7 +229,7 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     A(&ctx->scal, 64);
     const size_t cam_doubles = (V * sizeof(WarpCam) + 7) / 8;
     const size_t st2_tiles = (size_t)((ctx->W + 63) / 64) * ((ctx->H + 3) / 4);  // stencil operator tiles
-    ctx->partials_cap = std::max<size_t>({(size_t)ctx->ntiles * 9, st2_tiles * 9, 2048 * 9, cam_doubles});
+    ctx->partials_cap = std::max<size_t>({(size_t)ctx->ntiles * 9, st2_tiles * 12, 2048 * 9, cam_doubles});
     A(&ctx->partials, ctx->partials_cap);
     if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->cg_state_host, sizeof(CgState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->scal_host, 64 * sizeof(double), hipHostMallocDefault);

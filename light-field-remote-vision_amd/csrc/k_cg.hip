@@ -502,6 +502,144 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_st2_residual(int W, int H, in
     block_partials<NQ>(v, partials, red);
 }
 
+__device__ void cgcg_scalars(CgState &s, const double (&pv)[9], bool after_start);
+
+// The start of a stencil-path solve in one pass: r_0 = b - A x_0 and
+// u_0 = Minv r_0 on the tile and a 1-pixel ring (x_0 staged with a 2-pixel
+// ring), w_0 = A u_0 on the tile; writes r_0 and w_0 and the partial sums
+// r.r, r.(Minv r), b.b and w.u per channel.  Every global load is issued
+// before the first barrier.
+__global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_begin_st2(int W, int H, int ntx, int ntiles, float wscale,
+                                                                  float c0, float beta_reg, St2Coeffs g,
+                                                                  const float *__restrict__ x,
+                                                                  const float *__restrict__ b,
+                                                                  const float *__restrict__ minv,
+                                                                  float *__restrict__ r_out,
+                                                                  float *__restrict__ w_out, double *partials)
+{
+    constexpr int NT = kSt2W * kSt2H;
+    constexpr int XW = kSt2W + 4, XH = kSt2H + 4, XP = XW * XH;  // x from (tx0 - 2, ty0 - 2)
+    constexpr int UW = kSt2W + 2, UH = kSt2H + 2, UP = UW * UH;  // r, u from (tx0 - 1, ty0 - 1)
+    constexpr int NXT = (XP + NT - 1) / NT, NUT = (UP + NT - 1) / NT;
+    __shared__ float xl[3 * XP];
+    __shared__ float ul[3 * UP];
+    __shared__ double red[12 * 16];
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const int tx0 = (t % ntx) * kSt2W, ty0 = (t / ntx) * kSt2H;
+    const size_t N = (size_t)W * H;
+    const int tid = threadIdx.x;
+    // ---- global loads ----
+    float xv[NXT][3];
+#pragma unroll
+    for (int k = 0; k < NXT; k++) {
+        const int i = tid + k * NT;
+        const int px = tx0 - 2 + i % XW, py = ty0 - 2 + i / XW;
+        const bool in = i < XP && px >= 0 && px < W && py >= 0 && py < H;
+        const size_t o = in ? (size_t)py * W + px : 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) xv[k][c] = in ? x[c * N + o] : 0.0f;
+    }
+    float rcf[NUT][9], rb[NUT][3], rmi[NUT];
+    bool rin[NUT];
+#pragma unroll
+    for (int k = 0; k < NUT; k++) {
+        const int i = tid + k * NT;
+        const int px = tx0 - 1 + i % UW, py = ty0 - 1 + i / UW;
+        rin[k] = i < UP && px >= 0 && px < W && py >= 0 && py < H;
+        const int xc = min(max(px, 0), W - 1), yc = min(max(py, 0), H - 1);
+        const size_t o = (size_t)yc * W + xc;
+        st2_coeffs(g, W, o, xc, yc, rcf[k]);
+        rmi[k] = minv[o];
+#pragma unroll
+        for (int c = 0; c < 3; c++) rb[k][c] = b[c * N + o];
+    }
+    const int lx = tid % kSt2W, ly = tid / kSt2W;
+    const int px = tx0 + lx, py = ty0 + ly;
+    const bool pin = px < W && py < H;
+    const int pxc = min(px, W - 1), pyc = min(py, H - 1);
+    float cf[9];
+    st2_coeffs(g, W, (size_t)pyc * W + pxc, pxc, pyc, cf);
+    // ---- x, then r and u on the ring ----
+#pragma unroll
+    for (int k = 0; k < NXT; k++) {
+        const int i = tid + k * NT;
+        if (i < XP)
+#pragma unroll
+            for (int c = 0; c < 3; c++) xl[c * XP + i] = xv[k][c];
+    }
+    __syncthreads();
+    double v[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) v[q] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NUT; k++) {
+        const int i = tid + k * NT;
+        if (i >= UP) continue;
+        const int ux = i % UW, uy = i / UW;
+        const int gx = tx0 - 1 + ux, gy = ty0 - 1 + uy;
+        const int xi = (uy + 1) * XW + ux + 1;  // the point in the x tile
+        const bool interior = ux >= 1 && ux <= kSt2W && uy >= 1 && uy <= kSt2H;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float u = 0.0f;
+            if (rin[k]) {
+                const float q = st2_apply(xl + c * XP, xi, XW, rcf[k], gx, gy, W, H, wscale, c0, beta_reg);
+                const float rv = rb[k][c] - q;
+                u = rmi[k] * rv;
+                if (interior) {
+                    r_out[c * N + (size_t)gy * W + gx] = rv;
+                    v[c] += (double)rv * rv;
+                    v[3 + c] += (double)rv * (double)u;
+                    v[6 + c] += (double)rb[k][c] * rb[k][c];
+                }
+            }
+            ul[c * UP + i] = u;
+        }
+    }
+    __syncthreads();
+    // ---- w = A u on the tile ----
+    if (pin) {
+        const int li = (ly + 1) * UW + lx + 1;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float wv = st2_apply(ul + c * UP, li, UW, cf, px, py, W, H, wscale, c0, beta_reg);
+            w_out[c * N + (size_t)py * W + px] = wv;
+            v[9 + c] += (double)wv * (double)ul[c * UP + li];
+        }
+    }
+    block_partials<12>(v, partials, red);
+}
+
+// Scalars of a new solve from the begin pass: the state of k_cg_init_state,
+// then alpha_0 = gamma_0 / delta_0 as after the start pass.
+__global__ void __launch_bounds__(kFinalizeThreads) k_cgcg_begin(CgState *st, const double *__restrict__ partials,
+                                                                 int G, double tol2, int maxit)
+{
+    __shared__ double red[12 * 16];
+    double v[12];
+    reduce_partials<12>(partials, G, v, red);
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int c = 0; c < 3; c++) {
+            st->rr[c] = v[c];
+            st->rz[c] = v[3 + c];
+            st->bb[c] = v[6 + c];
+            st->alpha[c] = 0.0;
+            st->beta[c] = 0.0;
+            st->iter[c] = 0;
+            st->breakdown[c] = 0;
+            st->active[c] = (v[c] > tol2 * v[6 + c]) && maxit > 0;
+            any |= st->active[c];
+        }
+        st->tol2 = tol2;
+        st->maxit = maxit;
+        st->first = 1;
+        st->done = !any;
+        const double pv[9] = {v[3], v[4], v[5], v[9], v[10], v[11], v[0], v[1], v[2]};
+        cgcg_scalars(*st, pv, true);
+    }
+}
+
 // The scalar step between two passes (one thread).  After the start pass:
 // alpha_0 = gamma_0 / delta_0.  After an iteration: convergence of the new
 // residual (rho), then beta and alpha of the next iteration.
@@ -817,11 +955,14 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
 
     const St2Coeffs gco{gD, gE, gS, gSE, gSW};
     if (st2) {
-        // r = b - A x with its sums in one stencil pass
-        TimedRegion tr(ctx, 5);
-        k_st2_residual<true><<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(
-            ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, (float)beta, gco, x, ctx->cg_b, minv, r, part);
-        k_cg_init_state<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, st2_tiles, tol * tol, maxit);
+        // r_0 = b - A x_0, u_0 = Minv r_0, w_0 = A u_0 and the start sums in one pass
+        {
+            TimedRegion tr(ctx, 5);
+            k_cgcg_begin_st2<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(
+                ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, (float)beta, gco, x, ctx->cg_b, minv, r,
+                ctx->cg_q, part);
+        }
+        k_cgcg_begin<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, st2_tiles, tol * tol, maxit);
     } else {
         {
             TimedRegion tr(ctx, 5);
@@ -851,17 +992,6 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         a.p = ctx->cg_d;
         a.x = x;
         a.g = gco;
-        a.r_in = rbuf[0];
-        a.w_in = wbuf[0];
-        a.s_in = sbuf[0];
-        a.r_out = rbuf[1];
-        a.w_out = wbuf[0];
-        a.s_out = sbuf[1];
-        {
-            TimedRegion tr(ctx, 5);
-            k_cgcg_st2<true><<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(a, stt, part);
-        }
-        k_cgcg_start<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, st2_tiles);
         LFRV_LAUNCH(ctx, "cg start");
         // launch as many iterations as the last solve needed before the first
         // look at the convergence flag, then small chunks

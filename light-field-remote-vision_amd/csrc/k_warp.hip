@@ -526,13 +526,15 @@ lfrv_status run_compute_weights(lfrv_ctx *ctx)
         gys = py0;
     } else {
         // compute_gradient over the whole target (gradientIBR.cu:578-580 launched
-        // it on the low-res grid only; the build covers W x H).
-        k_sobel<<<gs, bs, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->U + N, ctx->U + 2 * N,
-                                            ctx->u_grad_x, ctx->u_grad_y, nullptr, nullptr);
+        // it on the low-res grid only; the build covers W x H).  The ungated
+        // Sobel of the cleared image is zero everywhere, so is its warp: the
+        // weights then take the zero-gradient form and need no Sobel.
+        if (!ctx->u_zero)
+            k_sobel<<<gs, bs, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->U + N, ctx->U + 2 * N,
+                                                ctx->u_grad_x, ctx->u_grad_y, nullptr, nullptr);
         gxs = ctx->u_grad_x;
         gys = ctx->u_grad_y;
     }
-    // the ungated Sobel of the cleared image is zero everywhere, so is its warp
     const bool zero_grad = ctx->p.mode == LFRV_MODE_GRADIENT_IBR && ctx->u_zero;
     switch (ctx->ks) {
     case 2: launch_weights_ks<2>(ctx, gxs, gys, zero_grad); break;
