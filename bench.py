@@ -130,6 +130,19 @@ def cpu_baseline_sr(scene, prm, steps, threads, sample_steps=2):
                 cpu=platform.processor() or platform.machine())
 
 
+def calib_copy(args):
+    """A plain copy of --calib-copy MiB with k_copy16 (lfrv_dev_copy_kernel):
+    the reference kernel of the PMC byte calibration."""
+    if args.calib_copy > 0:
+        nf = args.calib_copy * (1 << 20) // 4
+        src = lfrv.DeviceArray((nf,))
+        dst = lfrv.DeviceArray((nf,))
+        lfrv.copy_kernel(dst, src)
+        lfrv.device_sync()
+        src.free()
+        dst.free()
+
+
 def bench_color_sr(args, workload):
     """One step = one IBR_color novel view: from device-resident views, depths
     and cameras, warps from depth, splat structure, weights, visibility,
@@ -171,6 +184,7 @@ def bench_color_sr(args, workload):
     e_binned, ntiles = ctx.bin_stats()
     psnr = lfrv.psnr(U, scene["target"])
     ctx.close()
+    calib_copy(args)
     N = scene["W"] * scene["H"]
     e = e_binned
     ks = scene["dsf"] + 1
@@ -188,8 +202,10 @@ def bench_color_sr(args, workload):
         NA = (scene["W"] + ks - 1) * (scene["H"] + ks - 1)
         dt_kernel = f"k_sr_dataterm_st<{ks}>"
         dt_bytes = 4 * NA * (8 + (8 if beta > 0 else 0)) + 44 * N
-    # fused PD step: Uq, px, py, g, F read; px, py, U written (88 B per pixel)
-    pd_bytes = 88 * N
+    # fused PD step: Uq, px, py, g, F read; px, py, new primal written (88 B
+    # per pixel), with the outer step's over-relaxation fused in: U read, and
+    # U and the over-relaxed Uq written (+24 B)
+    pd_bytes = 112 * N
     dt_s = dt_ms / max(dt_n, 1) / 1e3
     pd_s = pd_ms / max(pd_n, 1) / 1e3
     kern = {
@@ -319,14 +335,7 @@ def main():
     psnr = lfrv.psnr(U, scene["target"])
     tile = [ctx.params.tile_w or 32, ctx.params.tile_h or 8]  # 0 = library default 32x8
     ctx.close()
-    if args.calib_copy > 0:
-        nf = args.calib_copy * (1 << 20) // 4
-        src = lfrv.DeviceArray((nf,))
-        dst = lfrv.DeviceArray((nf,))
-        lfrv.copy_kernel(dst, src)
-        lfrv.device_sync()
-        src.free()
-        dst.free()
+    calib_copy(args)
 
     line = {
         "metric": METRIC,

@@ -219,10 +219,11 @@ constexpr int kSt2W = 64, kSt2H = 8;  // tile of the stencil kernels, one thread
 // and reduces gamma, delta and r.r of the new residual.  r, s and w are read
 // across tiles, so they are double-buffered.
 //
-// START: no update; w_0 = A (Minv r_0) and gamma_0, delta_0, rho_0.
-// (A variant that lets every block of the next launch finish the reduction
-// itself measured slower: the redundant per-block sums cost more than the
-// one-block launch they replace.)
+// The solve starts with one pass that forms r_0, u_0, w_0 = A u_0 and their
+// sums (k_cgcg_begin_st2).  (A variant that lets every block of the next
+// launch finish the reduction itself measured slower: the redundant per-block
+// sums cost more than the one-block launch they replace.)
+//
 // The five coefficient planes of the assembled stencil: centre and the E, S,
 // SE, SW couplings of each pixel (the W, N, NW, NE ones are its neighbours').
 struct St2Coeffs {
@@ -289,26 +290,22 @@ __device__ __forceinline__ void st2_coeffs(const St2Coeffs &g, int W, size_t o, 
 // s_{i-1} (never written).  All global loads of the pass (the pixel's
 // vectors, its halo cell's, the stencil coefficients) are issued before the
 // one barrier.
-template <bool START>
 __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const CgState *st, double *partials)
 {
     constexpr int LW = kSt2W + 2, LH = kSt2H + 2, LP = LW * LH, NH = 2 * LW + 2 * kSt2H;
     static_assert(NH <= kSt2W * kSt2H, "one thread per halo cell");
     __shared__ float ul[3 * LP];
     __shared__ double red[9 * 16];
-    if (!START && st->done) return;
+    if (st->done) return;
     const int t = xcd_tile(blockIdx.x, a.ntiles);
     const int tx0 = (t % a.ntx) * kSt2W, ty0 = (t / a.ntx) * kSt2H;
     const int W = a.W, H = a.H;
     const size_t N = (size_t)W * H;
-    float al[3] = {0.0f, 0.0f, 0.0f}, be[3] = {0.0f, 0.0f, 0.0f};
-    int first = 1;
-    if (!START) {
-        first = st->first;
-        for (int c = 0; c < 3; c++) {
-            al[c] = (float)st->alpha[c];
-            be[c] = (float)st->beta[c];
-        }
+    float al[3], be[3];
+    const int first = st->first;
+    for (int c = 0; c < 3; c++) {
+        al[c] = (float)st->alpha[c];
+        be[c] = (float)st->beta[c];
     }
     // this thread's pixel
     const int lx = threadIdx.x % kSt2W, ly = threadIdx.x / kSt2W;
@@ -324,14 +321,12 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const Cg
     for (int c = 0; c < 3; c++) {
         const size_t k = c * N + o;
         rv[c] = a.r_in[k];
-        wv[c] = sv[c] = pv[c] = xv[c] = 0.0f;
-        if (!START) {
-            wv[c] = a.w_in[k];
-            xv[c] = a.x[k];
-            if (!first) {
-                sv[c] = a.s_in[k];
-                pv[c] = a.p[k];
-            }
+        wv[c] = a.w_in[k];
+        xv[c] = a.x[k];
+        sv[c] = pv[c] = 0.0f;
+        if (!first) {
+            sv[c] = a.s_in[k];
+            pv[c] = a.p[k];
         }
     }
     // this thread's halo cell
@@ -360,10 +355,8 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const Cg
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             hr[c] = a.r_in[c * N + ho];
-            if (!START) {
-                hw[c] = a.w_in[c * N + ho];
-                if (!first) hs[c] = a.s_in[c * N + ho];
-            }
+            hw[c] = a.w_in[c * N + ho];
+            if (!first) hs[c] = a.s_in[c * N + ho];
         }
     }
     // vector updates: s_i, p_i, x_{i+1}, r_{i+1} on the pixel; u_{i+1} to LDS
@@ -371,9 +364,7 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const Cg
     const int li = (ly + 1) * LW + lx + 1;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        if (START) {
-            rn[c] = rv[c];
-        } else {
+        {
             const float s = first ? wv[c] : wv[c] + be[c] * sv[c];
             rn[c] = rv[c] - al[c] * s;
             const float uo = mi * rv[c];
@@ -393,8 +384,7 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const Cg
         const int hl = hly * LW + hlx;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            float r1 = hr[c];
-            if (!START) r1 -= al[c] * (first ? hw[c] : hw[c] + be[c] * hs[c]);
+            const float r1 = hr[c] - al[c] * (first ? hw[c] : hw[c] + be[c] * hs[c]);
             ul[c * LP + hl] = hin ? hm * r1 : 0.0f;
         }
     }
@@ -413,21 +403,15 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const Cg
     block_partials<9>(v, partials, red);
 }
 
-// A x on the stencil tiles, fused with what the solve needs of it (same
-// halo-ring layout as k_cgcg_st2):
-//   INIT:  r = b - A x written, partials r.r, r.(Minv r), b.b per channel;
-//   else:  the true residual, partials ||b - A x||^2 per channel in FP64
-//          (nothing written).
-template <bool INIT>
+// The true residual of a stencil-path solve: partials ||b - A x||^2 per
+// channel in FP64 (same halo-ring layout as k_cgcg_st2; nothing written).
 __global__ void __launch_bounds__(kSt2W * kSt2H) k_st2_residual(int W, int H, int ntx, int ntiles, float wscale,
                                                                 float c0, float beta_reg, St2Coeffs g,
                                                                 const float *__restrict__ x,
-                                                                const float *__restrict__ b,
-                                                                const float *__restrict__ minv,
-                                                                float *__restrict__ r, double *partials)
+                                                                const float *__restrict__ b, double *partials)
 {
     constexpr int LW = kSt2W + 2, LH = kSt2H + 2, LP = LW * LH, NH = 2 * LW + 2 * kSt2H;
-    constexpr int NQ = INIT ? 9 : 3;
+    constexpr int NQ = 3;
     __shared__ float xl[3 * LP];
     __shared__ double red[NQ * 16];
     const int t = xcd_tile(blockIdx.x, ntiles);
@@ -440,7 +424,6 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_st2_residual(int W, int H, in
     const size_t o = (size_t)yc * W + xc;
     float cf[9];
     st2_coeffs(g, W, o, xc, yc, cf);
-    const float mi = INIT ? minv[o] : 0.0f;
     float xv[3], bv[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -487,16 +470,8 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_st2_residual(int W, int H, in
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const float q = st2_apply(xl + c * LP, li, LW, cf, px, py, W, H, wscale, c0, beta_reg);
-            if constexpr (INIT) {
-                const float rv = bv[c] - q;
-                r[c * N + o] = rv;
-                v[c] += (double)rv * rv;
-                v[3 + c] += (double)rv * (double)(mi * rv);
-                v[6 + c] += (double)bv[c] * bv[c];
-            } else {
-                const double d = (double)bv[c] - (double)q;
-                v[c] += d * d;
-            }
+            const double d = (double)bv[c] - (double)q;
+            v[c] += d * d;
         }
     }
     block_partials<NQ>(v, partials, red);
@@ -687,16 +662,6 @@ __device__ void cgcg_scalars(CgState &s, const double (&pv)[9], bool after_start
         any |= s.active[c];
     }
     s.done = !any;
-}
-
-// Scalars after the start pass: alpha_0 = gamma_0 / delta_0.
-__global__ void __launch_bounds__(kFinalizeThreads) k_cgcg_start(CgState *st, const double *__restrict__ partials,
-                                                                 int G)
-{
-    __shared__ double red[9 * 16];
-    double v[9];
-    reduce_partials<9>(partials, G, v, red);
-    if (threadIdx.x == 0) cgcg_scalars(*st, v, true);
 }
 
 // Scalars after iteration i: convergence on r_{i+1}.r_{i+1}, then beta and
@@ -1008,7 +973,7 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
                 a.s_out = sbuf[cur ^ 1];
                 {
                     const KernelEvents ev = kernel_events(ctx);
-                    hipExtLaunchKernelGGL(k_cgcg_st2<false>, dim3(st2_tiles), dim3(kSt2W * kSt2H), 0, ctx->stream,
+                    hipExtLaunchKernelGGL(k_cgcg_st2, dim3(st2_tiles), dim3(kSt2W * kSt2H), 0, ctx->stream,
                                           ev.a, ev.b, 0, a, (const CgState *)stt, part);
                     record_kernel_events(ctx, 0, ev);
                 }
@@ -1021,9 +986,8 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
             // chunk converged it is the answer, and the host waits once
             {
                 TimedRegion tr(ctx, 5);
-                k_st2_residual<false><<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(
-                    ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, (float)beta, gco, x, ctx->cg_b, minv,
-                    nullptr, part);
+                k_st2_residual<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(
+                    ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, (float)beta, gco, x, ctx->cg_b, part);
             }
             k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, st2_tiles, ctx->scal);
             LFRV_LAUNCH(ctx, "cg residual");

@@ -7,7 +7,8 @@
 
 namespace lfrv {
 
-lfrv_status run_pd_steps(lfrv_ctx *ctx, int iters, const float *F);  // k_stencil.hip
+// k_stencil.hip; with `overrelax` the last step also does the over-relaxation
+lfrv_status run_pd_steps(lfrv_ctx *ctx, int iters, const float *F, bool overrelax = false, float theta = 0.0f);
 
 namespace {
 
@@ -621,13 +622,17 @@ lfrv_status run_fista(lfrv_ctx *ctx)
 #undef DT
         LFRV_LAUNCH(ctx, "data term");
     }
-    st = run_pd_steps(ctx, ctx->p.inner_iterations, ctx->G);
-    if (st != LFRV_OK) return st;
-    // FISTA weights (optimization.cu:1744-1746)
+    // FISTA weights (optimization.cu:1744-1746); the over-relaxation is
+    // fused into the last primal-dual step
     const double an = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * std::pow(ctx->fista_alpha, 2.0)));
     const float theta = (float)((ctx->fista_alpha - 1.0) / an);
-    k_overrelax<<<blocks_for(3 * ctx->N), kBlock, 0, ctx->stream>>>(3 * ctx->N, theta, ctx->Uq, ctx->U);
-    LFRV_LAUNCH(ctx, "overrelaxation");
+    if (ctx->p.inner_iterations > 0) {
+        st = run_pd_steps(ctx, ctx->p.inner_iterations, ctx->G, true, theta);
+        if (st != LFRV_OK) return st;
+    } else {
+        k_overrelax<<<blocks_for(3 * ctx->N), kBlock, 0, ctx->stream>>>(3 * ctx->N, theta, ctx->Uq, ctx->U);
+        LFRV_LAUNCH(ctx, "overrelaxation");
+    }
     ctx->fista_alpha = an;
     return LFRV_OK;
 }
