@@ -227,13 +227,13 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_sr_assemble(TileGeom g, Til
     }
 }
 
-// One outer step's data term on a 32 x 8 pixel tile from the assembled
+// One outer step's data term on a 32 x 16 pixel tile from the assembled
 // operator.  Every global load of the block (U with its halo, the couplings
 // and right-hand sides of the tile's anchors, coverage) is issued first; the
 // rest runs in LDS: lap U, the box sums BU / BL at the anchors (+1 ring), the
 // anchor residuals QI = G9 BU - bI and QG = dsf^2 G9' BL - bG, then the box
 // sums back onto the pixels, split into rows and columns.
-constexpr int kStTW = 32, kStTH = 8;
+constexpr int kStTW = 32, kStTH = 16;
 
 template <int KS>
 __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, int Wc, int Hc, int ntx, int ntiles,
