@@ -89,7 +89,47 @@ def cpu_baseline(scene, gpu_iters, threads, cg_sample_iters=8):
 COLOR_WORKLOADS = {
     "color_sr_c3": (3, 500, 1.0, "5x5 views 512² -> 2048²"),
     "color_sr_c1": (1, 300, 0.0, "3x3 views 128² -> 256²"),
+    "color_sr_c1_irls": (1, 300, 0.0, "3x3 views 128² -> 256²"),
 }
+# re-weighting passes of the workloads that run the driver's loop with its
+# stopping rules (lfrv_sr_solve) instead of a fixed step count
+IRLS_PASSES = {"color_sr_c1_irls": 3}
+
+
+def host_driven_sr_loop(ctx, W, H, niter, nmeta):
+    """The loop of super-resolution.cpp:177-300 driven from the host over the
+    C-ABI the way the reference drives it: after each FISTA step the energy
+    and the moving-pixel count come back to the host (the previous solution
+    a host copy, as the reference's R_prev).  Returns the FISTA steps run."""
+    f = np.float32
+    EPS, MAXD, PCT = f(1e-4), f(1e-3), f(0.01)
+    prev_meta = ctx.get_solution()
+    prev = prev_meta.copy()
+    prev_e_rw = f(0)
+    steps = 0
+    for _ in range(nmeta):
+        ctx.compute_weights()
+        ctx.sr_init_regularizer_weight()
+        energy = prev_e = f(ctx.sr_primal_energy()[0])
+        for _ in range(niter):
+            ctx.sr_iteration_fista()
+            steps += 1
+            energy = f(ctx.sr_primal_energy()[0])
+            if energy < EPS or abs((energy - prev_e) / energy) < EPS:
+                break
+            prev_e = energy
+            if f(ctx.sr_count_moving(prev, MAXD)) / f(W * H) < PCT:
+                break
+            prev = ctx.get_solution()
+        if energy < EPS:
+            if prev_e_rw < EPS:
+                break
+            prev_e_rw = energy
+            continue
+        if f(ctx.sr_count_moving(prev_meta, MAXD)) / f(W * H) < PCT:
+            break
+        prev_meta = ctx.get_solution()
+    return steps
 
 
 def cpu_baseline_sr(scene, prm, steps, threads, sample_steps=2):
@@ -151,24 +191,32 @@ def bench_color_sr(args, workload):
     dsf >= 2 the data term is assembled once into anchor-grid couplings and
     each step applies it as a stencil)."""
     preset, steps, beta, desc = COLOR_WORKLOADS[workload]
+    nmeta = IRLS_PASSES.get(workload, 0)
     scene = lfrv.synth(preset, seed=args.seed)
     prm = pipelines.color_sr_params(scene, lambda_=0.05, lambda_max_factor=100.0, inner_iterations=1,
                                     dt_beta=beta, gradient_step=1.0)
     ctx = lfrv.Context(prm)
     pipelines.load_scene(ctx, scene)
     ctx.sync()
+    irls = []
 
     def one_view():
         ctx.clear_solution()
         pipelines.color_sr_setup(ctx)
-        ctx.sr_init_regularizer_weight()
-        for _ in range(steps):
-            ctx.sr_iteration_fista()
+        if nmeta:
+            irls.append(ctx.sr_solve(steps, nmeta))
+        else:
+            ctx.sr_init_regularizer_weight()
+            for _ in range(steps):
+                ctx.sr_iteration_fista()
         ctx.sync()
 
     for _ in range(args.warmup):
         one_view()
-    ctx.set_timing(True)
+    # the driver loop enqueues steps ahead of its stopping rules and the
+    # device skips those behind a rule that fired: its kernels are timed in a
+    # fixed-step pass below, where every launch does the full work
+    ctx.set_timing(not nmeta)
     per_view = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -177,12 +225,29 @@ def bench_color_sr(args, workload):
         per_view.append(1e3 * (time.perf_counter() - ts))
     t1 = time.perf_counter()
     ms_per_view = 1e3 * (t1 - t0) / args.steps
-    dt_ms, dt_n = ctx.get_timing("dataterm")
-    pd_ms, pd_n = ctx.get_timing("pd_step")
-    ctx.set_timing(False)
     U = ctx.get_solution()
     e_binned, ntiles = ctx.bin_stats()
     psnr = lfrv.psnr(U, scene["target"])
+    if nmeta:
+        ctx.set_timing(True)
+        ctx.sr_init_regularizer_weight()
+        for _ in range(50):
+            ctx.sr_iteration_fista()
+        ctx.sync()
+    dt_ms, dt_n = ctx.get_timing("dataterm")
+    pd_ms, pd_n = ctx.get_timing("pd_step")
+    ctx.set_timing(False)
+    host_loop = None
+    if nmeta:
+        # the same view with the loop driven from the host, for comparison
+        ctx.clear_solution()
+        pipelines.color_sr_setup(ctx)
+        ctx.sync()
+        th = time.perf_counter()
+        host_steps = host_driven_sr_loop(ctx, scene["W"], scene["H"], steps, nmeta)
+        ctx.sync()
+        host_loop = dict(ms_per_view=round(1e3 * (time.perf_counter() - th), 3), fista_steps=host_steps,
+                         same_result=bool(np.array_equal(ctx.get_solution(), U)))
     ctx.close()
     calib_copy(args)
     N = scene["W"] * scene["H"]
@@ -216,8 +281,20 @@ def bench_color_sr(args, workload):
     }
     dom = "k_sr_dataterm" if dt_ms >= pd_ms else "k_pd_step"
     k = kern[dom]
+    if nmeta:
+        last = irls[-1]
+        metric = (f"ms per novel view ({desc}, up to {nmeta} re-weighting passes of up to {steps} FISTA steps, "
+                  f"stopping rules on the device); achieved HBM GB/s")
+        workload_desc = (f"IBR_color driver loop (super-resolution.cpp:177-300) on the device: {desc} RGB, up to "
+                         f"{nmeta} passes x {steps} FISTA steps of one primal-dual iteration, energy and "
+                         f"moving-pixel rules after every step, lambda 0.05, lambda_max_factor 100, dt_beta {beta}")
+    else:
+        metric = f"ms per novel view ({desc}, {steps} TV primal-dual steps); achieved HBM GB/s"
+        workload_desc = (f"IBR_color TV-L2 super-resolution with warped data terms: {desc} RGB, "
+                         f"{steps} FISTA steps of one primal-dual iteration, lambda 0.05, "
+                         f"lambda_max_factor 100, dt_beta {beta}")
     line = {
-        "metric": f"ms per novel view ({desc}, {steps} TV primal-dual steps); achieved HBM GB/s",
+        "metric": metric,
         "value": round(ms_per_view, 3),
         "unit": "ms",
         "n_gpus": 1,
@@ -229,9 +306,7 @@ def bench_color_sr(args, workload):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"IBR_color TV-L2 super-resolution with warped data terms: {desc} RGB, "
-                               f"{steps} FISTA steps of one primal-dual iteration, lambda 0.05, "
-                               f"lambda_max_factor 100, dt_beta {beta}",
+        "config": {"workload": workload_desc,
                    "views": scene["V"], "width": scene["W"], "height": scene["H"], "dsf": scene["dsf"],
                    "seed": args.seed},
         "roofline": {"bound": "hbm", "kernel": dt_kernel if dom == "k_sr_dataterm" else dom,
@@ -245,9 +320,13 @@ def bench_color_sr(args, workload):
         "psnr_vs_ground_truth_db": round(psnr, 3),
         "kernels": dict(kern, binned_entries=e_binned, tiles=ntiles),
     }
+    if nmeta:
+        line["irls"] = dict(meta_iterations=last["meta_iterations"], fista_steps=last["fista_steps"],
+                            last_stop=last["last_stop"], converged=last["converged"],
+                            energy=round(float(last["energy"]), 6), host_driven_loop=host_loop)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline_sr(scene, prm, steps, threads)
+        line["cpu_baseline"] = cpu_baseline_sr(scene, prm, irls[-1]["fista_steps"] if nmeta else steps, threads)
     print(json.dumps(line))
 
 

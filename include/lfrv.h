@@ -167,7 +167,11 @@ lfrv_status lfrv_get_solution(lfrv_ctx *ctx, float *rgb);
 lfrv_status lfrv_set_solution(lfrv_ctx *ctx, const float *rgb);
 
 /* Zero the current solution on the device (the cleared U of init_data,
- * gradientIBR.cu:281-287, and of compute_initial_image, :732-735). */
+ * gradientIBR.cu:281-287, and of compute_initial_image, :732-735).  For
+ * IBR_color it also starts a new view from the state of a fresh workspace,
+ * as coco_vtv_alloc and coco_vtv_initialize(ZERO) leave it (vtv.cu:60-89,
+ * 241-263): Uq, G, the dual variables and the normalisation buffer zeroed,
+ * the FISTA weight alpha = 1, no TV weight. */
 lfrv_status lfrv_clear_solution(lfrv_ctx *ctx);
 
 /* ---- gradient-domain IBR --------------------------------------------------- */
@@ -233,6 +237,34 @@ lfrv_status lfrv_sr_primal_energy(lfrv_ctx *ctx, double *energy, double *parts);
  * device) is >= max_diff, i.e. the "moving" count of sr_compare_images
  * (super-resolution.cpp:16-51). */
 lfrv_status lfrv_sr_count_moving(lfrv_ctx *ctx, const float *prev, float max_diff, int64_t *moving);
+
+typedef struct lfrv_sr_stats {
+    int meta_iterations;  /* re-weighting passes started */
+    int fista_steps;      /* FISTA steps run, all passes */
+    int last_stop;        /* how the last pass's FISTA loop ended: LFRV_SR_STOP_* */
+    int converged;        /* 1 when a re-weighting rule ended the solve before nmeta */
+    float energy;         /* energy after the last step, in float as the driver keeps it */
+    double ms;            /* wall time (host clock) */
+} lfrv_sr_stats;
+
+enum {
+    LFRV_SR_STOP_MAXIT = 0,     /* niter steps ran */
+    LFRV_SR_STOP_ENERGY = 1,    /* energy < 1e-4 */
+    LFRV_SR_STOP_RELATIVE = 2,  /* |E - E_prev| / E < 1e-4 */
+    LFRV_SR_STOP_MOVING = 3     /* < 1% of the pixels changed by >= 1e-3 */
+};
+
+/* The re-weighted least-squares loop of the IBR_color driver
+ * (super-resolution.cpp:177-300) from the current solution: up to nmeta
+ * passes of lfrv_compute_weights, lfrv_sr_init_regularizer_weight and up to
+ * niter FISTA steps, with the driver's stopping rules in float (EPSILON_TOL
+ * 1e-4, MAX_DIFF_THRESHOLD 1e-3, PERCENT_MOVING 0.01) and its previous
+ * solutions kept on the device.  Each step's energy and moving-pixel test
+ * run on the device, which skips the steps enqueued after a rule fired; the
+ * host reads the loop state every few steps.  Same results as the loop made
+ * of lfrv_sr_iteration_fista, lfrv_sr_primal_energy and lfrv_sr_count_moving
+ * calls. */
+lfrv_status lfrv_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *st);
 
 /* ---- inspection ------------------------------------------------------------ */
 

@@ -224,6 +224,7 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     ctx->pp_cap = 3 * (N + N / 2) + 3 * N + N / 4 + 64;
     A(&ctx->pp_scratch, ctx->pp_cap);
     A(&ctx->cg_state, 1);
+    A(&ctx->sr_loop, 1);
     A(&ctx->counters, 32);
     A(&ctx->dcams, V);
     A(&ctx->scal, 64);
@@ -235,6 +236,9 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->scal_host, 64 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->hole_count_host, sizeof(unsigned int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->hole_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->sr_loop_host, 3 * sizeof(SrLoop), hipHostMallocDefault);
+    for (hipEvent_t &ev : ctx->sr_ev)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         lfrv_destroy(ctx);
@@ -257,12 +261,15 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
              ctx->cg_d, ctx->cg_q, ctx->cg_b, ctx->reg_g, ctx->norm_mask, ctx->coverage, ctx->u_grad_x,
              ctx->u_grad_y, ctx->target_mask, ctx->visibility, ctx->target_depth, ctx->tmpN, ctx->cg_minv,
              ctx->hole_list, ctx->pp_scratch, ctx->cg_state, ctx->counters, ctx->dcams, ctx->scal, ctx->partials,
-             ctx->cache_num, ctx->cache_sq, ctx->gram, ctx->cg_extra})
+             ctx->cache_num, ctx->cache_sq, ctx->gram, ctx->cg_extra, ctx->sr_loop, ctx->sr_prev})
         if (p) (void)hipFree(p);
     if (ctx->cg_state_host) (void)hipHostFree(ctx->cg_state_host);
     if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
     if (ctx->hole_count_host) (void)hipHostFree(ctx->hole_count_host);
     if (ctx->hole_ev) (void)hipEventDestroy(ctx->hole_ev);
+    if (ctx->sr_loop_host) (void)hipHostFree(ctx->sr_loop_host);
+    for (hipEvent_t ev : ctx->sr_ev)
+        if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -421,6 +428,16 @@ extern "C" lfrv_status lfrv_clear_solution(lfrv_ctx *ctx)
 {
     if (!ctx) return LFRV_E_ARG;
     k_zero<<<blocks_for(3 * ctx->N), kBlock, 0, ctx->stream>>>(3 * ctx->N, ctx->U);
+    if (ctx->p.mode == LFRV_MODE_COLOR_SR) {
+        // a new view starts from the state of a fresh workspace: coco_vtv_alloc
+        // zeroes Uq, G, the duals and the normalisation buffer (vtv.cu:60-89),
+        // coco_vtv_initialize sets alpha = 1 (vtv.cu:241-263)
+        for (float *b : {ctx->Uq, ctx->G, ctx->PX, ctx->PY})
+            k_zero<<<blocks_for(3 * ctx->N), kBlock, 0, ctx->stream>>>(3 * ctx->N, b);
+        k_zero<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(ctx->N, ctx->norm_mask);
+        ctx->fista_alpha = 1.0;
+        ctx->has_reg_g = false;
+    }
     LFRV_LAUNCH(ctx, "clear solution");
     ctx->u_zero = true;
     return LFRV_OK;
@@ -501,6 +518,17 @@ extern "C" lfrv_status lfrv_sr_count_moving(lfrv_ctx *ctx, const float *prev, fl
     lfrv_status st = copy_in(ctx, ctx->tmp3N, prev, 3 * ctx->N);
     if (st != LFRV_OK) return st;
     return run_count_moving(ctx, ctx->tmp3N, max_diff, moving);
+}
+
+extern "C" lfrv_status lfrv_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *st)
+{
+    if (!ctx) return LFRV_E_ARG;
+    if (ctx->p.mode != LFRV_MODE_COLOR_SR) return fail(ctx, LFRV_E_STATE, "lfrv_sr_solve needs LFRV_MODE_COLOR_SR");
+    if (niter < 0 || nmeta < 0) return fail(ctx, LFRV_E_ARG, "negative iteration counts");
+    if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "lfrv_sr_solve before lfrv_prepare");
+    if (!ctx->vis_ready) return fail(ctx, LFRV_E_STATE, "lfrv_sr_solve before lfrv_setup_visibility");
+    ctx->u_zero = false;
+    return run_sr_solve(ctx, niter, nmeta, st);
 }
 
 // ---- inspection -----------------------------------------------------------------

@@ -3,12 +3,15 @@
 #include "tile_common.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <vector>
 
 namespace lfrv {
 
 // k_stencil.hip; with `overrelax` the last step also does the over-relaxation
-lfrv_status run_pd_steps(lfrv_ctx *ctx, int iters, const float *F, bool overrelax = false, float theta = 0.0f);
+lfrv_status run_pd_steps(lfrv_ctx *ctx, int iters, const float *F, bool overrelax = false, float theta = 0.0f,
+                         const int *halt = nullptr);
 
 namespace {
 
@@ -31,8 +34,9 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_sr_dataterm(TileGeom g, Til
                                                      const float *__restrict__ U, const float *__restrict__ views,
                                                      const float *__restrict__ vlap, const float *__restrict__ coverage,
                                                      float *__restrict__ Gout, float *__restrict__ Uq,
-                                                     float *__restrict__ norm_mask)
+                                                     float *__restrict__ norm_mask, const int *halt)
 {
+    if (halt && *halt) return;  // a step enqueued behind a stopping rule that fired
     constexpr int HAL = KS - 1;
     extern __shared__ int smem[];
     const int t = xcd_tile(blockIdx.x, g.ntiles);
@@ -241,8 +245,9 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
                                                                    const float *__restrict__ ops, size_t NA,
                                                                    const float *__restrict__ coverage,
                                                                    float *__restrict__ Gout, float *__restrict__ Uq,
-                                                                   float *__restrict__ norm_mask)
+                                                                   float *__restrict__ norm_mask, const int *halt)
 {
+    if (halt && *halt) return;  // a step enqueued behind a stopping rule that fired
     constexpr int D = KS - 1, TW = kStTW, TH = kStTH, NT = TW * TH;
     constexpr int QW = TW + D - 1, QH = TH + D - 1, NQ = QW * QH;  // anchor points x' = tx0 + qx
     constexpr int BW = QW + 2, BH = QH + 2;                         // box sums at x' = tx0 - 1 + bx
@@ -438,8 +443,9 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
 }
 
 // theta over-relaxation (vtv_rof.cu:28-45): uq <- u + theta (uq - u); u <- uq_old
-__global__ void k_overrelax(size_t n3, float theta, float *__restrict__ uq, float *__restrict__ u)
+__global__ void k_overrelax(size_t n3, float theta, float *__restrict__ uq, float *__restrict__ u, const int *halt)
 {
+    if (halt && *halt) return;
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n3) return;
     const float uv = u[k], uqv = uq[k];
@@ -451,8 +457,9 @@ __global__ void k_overrelax(size_t n3, float theta, float *__restrict__ uq, floa
 // of Y(U) for the gradient term (optimization.cu:1505-1534, 1622-1625).
 __global__ void __launch_bounds__(256) k_sr_energy_tv(int W, int H, const float *__restrict__ U,
                                                       const float *__restrict__ vis, float *__restrict__ ugx,
-                                                      float *__restrict__ ugy, double *partials)
+                                                      float *__restrict__ ugy, double *partials, const int *halt)
 {
+    if (halt && *halt) return;
     __shared__ double red[16];
     const size_t N = (size_t)W * H;
     double v[1] = {0.0};
@@ -488,8 +495,10 @@ __global__ void __launch_bounds__(256) k_sr_energy_views(int V, int W, int H, in
                                                          const float *__restrict__ omega, const float *__restrict__ views,
                                                          const float *__restrict__ vgrad, const float *__restrict__ U,
                                                          const float *__restrict__ ugx, const float *__restrict__ ugy,
-                                                         float sigma2, int gw_type, double *partials)
+                                                         float sigma2, int gw_type, double *partials,
+                                                         const int *halt)
 {
+    if (halt && *halt) return;
     __shared__ double red[2 * 16];
     const size_t n = (size_t)w * h, N = (size_t)W * H;
     double v[2] = {0.0, 0.0};
@@ -536,8 +545,9 @@ __global__ void __launch_bounds__(256) k_sr_energy_views(int V, int W, int H, in
 
 __global__ void __launch_bounds__(256) k_count_moving(size_t N, const float *__restrict__ U,
                                                       const float *__restrict__ prev, float thr,
-                                                      double *partials)
+                                                      double *partials, const int *halt)
 {
+    if (halt && *halt) return;
     __shared__ double red[16];
     double v[1] = {0.0};
     for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < N; o += (size_t)gridDim.x * blockDim.x) {
@@ -555,9 +565,87 @@ unsigned int rblocks(size_t N)
     return (unsigned int)(b < 2048 ? (b == 0 ? 1 : b) : 2048);
 }
 
+// The energy's block partials: E_TV at partials[0, 2048), E_int and E_grad
+// at partials + 2048 (2 x up to 2048 blocks).
+constexpr size_t kViewPartials = 2048;
+
+struct EnergyCoef {
+    double lambda, alpha, beta;
+};
+
+// The energy of the driver, E = lambda E_TV + dt_alpha E_int + dt_beta E_grad
+// (optimization.cu:1710), from the block partials; the driver keeps it in
+// float (super-resolution.cpp:181, 230).  BEGIN: the pass's starting energy
+// (:207-209), which also opens the pass.  Otherwise, after a FISTA step
+// (:230-244): E < 1e-4 stops; |(E - E_prev) / E| < 1e-4 stops; else E_prev = E.
+template <bool BEGIN>
+__global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_energy(SrLoop *L, const double *__restrict__ ptv, int Gtv,
+                                                                   const double *__restrict__ pviews, int Gv,
+                                                                   EnergyCoef c, float eps)
+{
+    if (!BEGIN && L->halt) return;
+    __shared__ double red[2 * 16];
+    double tv[1], dv[2];
+    reduce_partials<1>(ptv, Gtv, tv, red);
+    reduce_partials<2>(pviews, Gv, dv, red);
+    if (threadIdx.x != 0) return;
+    // the host's sum order, without contraction into FMAs
+    const float E = (float)__dadd_rn(__dadd_rn(__dmul_rn(c.lambda, tv[0]), __dmul_rn(c.alpha, dv[0])),
+                                     __dmul_rn(c.beta, dv[1]));
+    L->parts[0] = tv[0];
+    L->parts[1] = dv[0];
+    L->parts[2] = dv[1];
+    L->energy = E;
+    if (BEGIN) {
+        L->halt = 0;
+        L->steps = 0;
+        L->stop = LFRV_SR_STOP_MAXIT;
+        L->prev_energy = E;
+        return;
+    }
+    L->steps += 1;
+    if (E < eps) {
+        L->halt = 1;
+        L->stop = LFRV_SR_STOP_ENERGY;
+    } else if (fabsf((E - L->prev_energy) / E) < eps) {
+        L->halt = 1;
+        L->stop = LFRV_SR_STOP_RELATIVE;
+    } else {
+        L->prev_energy = E;
+    }
+}
+
+// The moving-pixel rule after a step (super-resolution.cpp:246-257): fewer
+// than pct of the pixels moved against the previous step's solution stops.
+__global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_moving(SrLoop *L, const double *__restrict__ partials,
+                                                                   int G, float npix, float pct)
+{
+    if (L->halt) return;
+    __shared__ double red[16];
+    double m[1];
+    reduce_partials<1>(partials, G, m, red);
+    if (threadIdx.x == 0 && (float)m[0] / npix < pct) {
+        L->halt = 1;
+        L->stop = LFRV_SR_STOP_MOVING;
+    }
+}
+
+// prev <- U unless a rule fired in this step (super-resolution.cpp:255).
+__global__ void k_copy_unless_halted(size_t n, const float *__restrict__ src, float *__restrict__ dst,
+                                     const int *halt)
+{
+    if (*halt) return;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+
 }  // namespace
 
-lfrv_status run_fista(lfrv_ctx *ctx)
+namespace {
+
+// One FISTA step on the stream; with `halt` every launch of the step is a
+// no-op once a stopping rule has fired on the device.
+lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
 {
     if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "FISTA step before lfrv_prepare");
     lfrv_status st = run_refresh_entry_weights(ctx, 1.0f);
@@ -583,7 +671,7 @@ lfrv_status run_fista(lfrv_ctx *ctx)
         TimedRegion tr(ctx, 3);
         k_sr_dataterm<2><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, ctx->n, da, ctx->U, ctx->views,
                                                                           ctx->view_lap, ctx->coverage, ctx->G, ctx->Uq,
-                                                                          ctx->norm_mask);
+                                                                          ctx->norm_mask, halt);
         LFRV_LAUNCH(ctx, "data term");
     } else {
         // assembled operator on the anchor grid, rebuilt when the weights or
@@ -612,7 +700,7 @@ lfrv_status run_fista(lfrv_ctx *ctx)
         }
         const int ntx = (ctx->W + kStTW - 1) / kStTW, ntiles = ntx * ((ctx->H + kStTH - 1) / kStTH);
         TimedRegion tr(ctx, 3);
-#define DT(K) k_sr_dataterm_st<K><<<ntiles, kStTW * kStTH, 0, ctx->stream>>>(ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, ctx->U, ctx->sr_ops, NA, ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask)
+#define DT(K) k_sr_dataterm_st<K><<<ntiles, kStTW * kStTH, 0, ctx->stream>>>(ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, ctx->U, ctx->sr_ops, NA, ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask, halt)
         switch (ctx->ks) {
         case 3: DT(3); break;
         case 4: DT(4); break;
@@ -622,31 +710,34 @@ lfrv_status run_fista(lfrv_ctx *ctx)
 #undef DT
         LFRV_LAUNCH(ctx, "data term");
     }
-    // FISTA weights (optimization.cu:1744-1746); the over-relaxation is
+    // FISTA weights (optimization.cu:1744-1746): alpha_new is a cuflt (float)
+    // there, theta the float of the double quotient; the over-relaxation is
     // fused into the last primal-dual step
-    const double an = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * std::pow(ctx->fista_alpha, 2.0)));
-    const float theta = (float)((ctx->fista_alpha - 1.0) / an);
+    const float an = (float)(0.5 * (1.0 + std::sqrt(1.0 + 4.0 * std::pow(ctx->fista_alpha, 2.0))));
+    const float theta = (float)((ctx->fista_alpha - 1.0) / (double)an);
     if (ctx->p.inner_iterations > 0) {
-        st = run_pd_steps(ctx, ctx->p.inner_iterations, ctx->G, true, theta);
+        st = run_pd_steps(ctx, ctx->p.inner_iterations, ctx->G, true, theta, halt);
         if (st != LFRV_OK) return st;
     } else {
-        k_overrelax<<<blocks_for(3 * ctx->N), kBlock, 0, ctx->stream>>>(3 * ctx->N, theta, ctx->Uq, ctx->U);
+        k_overrelax<<<blocks_for(3 * ctx->N), kBlock, 0, ctx->stream>>>(3 * ctx->N, theta, ctx->Uq, ctx->U, halt);
         LFRV_LAUNCH(ctx, "overrelaxation");
     }
     ctx->fista_alpha = an;
     return LFRV_OK;
 }
 
-lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts)
+// The energy's block partials (tv at partials[0], views at
+// partials + kViewPartials); ugx / ugy in tmp3N.
+lfrv_status enqueue_sr_energy(lfrv_ctx *ctx, const int *halt)
 {
     const size_t N = ctx->N;
     float *ugx = ctx->tmp3N, *ugy = ctx->tmp3N + N;
     k_sr_energy_tv<<<rblocks(N), 256, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->visibility, ugx, ugy,
-                                                        ctx->partials);
-    k_reduce_out<1><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->partials, (int)rblocks(N), ctx->scal + 8);
+                                                        ctx->partials, halt);
     const size_t tot = (size_t)ctx->V * ctx->n;
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
-#define EV(K) k_sr_energy_views<K><<<rblocks(tot), 256, 0, ctx->stream>>>(ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, ctx->omega, ctx->views, ctx->view_grad, ctx->U, ugx, ugy, s2, ctx->p.gw_type, ctx->partials)
+    double *pv = ctx->partials + kViewPartials;
+#define EV(K) k_sr_energy_views<K><<<rblocks(tot), 256, 0, ctx->stream>>>(ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, ctx->omega, ctx->views, ctx->view_grad, ctx->U, ugx, ugy, s2, ctx->p.gw_type, pv, halt)
     switch (ctx->ks) {
     case 2: EV(2); break;
     case 3: EV(3); break;
@@ -655,7 +746,30 @@ lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts)
     default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
     }
 #undef EV
-    k_reduce_out<2><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->partials, (int)rblocks(tot), ctx->scal + 9);
+    LFRV_LAUNCH(ctx, "sr energy");
+    return LFRV_OK;
+}
+
+EnergyCoef energy_coef(const lfrv_ctx *ctx)
+{
+    return EnergyCoef{(double)ctx->p.lambda, (double)ctx->p.dt_alpha, (double)ctx->p.dt_beta};
+}
+
+}  // namespace
+
+lfrv_status run_fista(lfrv_ctx *ctx)
+{
+    return enqueue_fista(ctx, nullptr);
+}
+
+lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts)
+{
+    lfrv_status st = enqueue_sr_energy(ctx, nullptr);
+    if (st != LFRV_OK) return st;
+    const size_t tot = (size_t)ctx->V * ctx->n;
+    k_reduce_out<1><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->partials, (int)rblocks(ctx->N), ctx->scal + 8);
+    k_reduce_out<2><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->partials + kViewPartials, (int)rblocks(tot),
+                                                             ctx->scal + 9);
     LFRV_LAUNCH(ctx, "sr energy");
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host + 8, ctx->scal + 8, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -671,12 +785,117 @@ lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts)
 
 lfrv_status run_count_moving(lfrv_ctx *ctx, const float *prev_dev, float max_diff, int64_t *moving)
 {
-    k_count_moving<<<rblocks(ctx->N), 256, 0, ctx->stream>>>(ctx->N, ctx->U, prev_dev, max_diff, ctx->partials);
+    k_count_moving<<<rblocks(ctx->N), 256, 0, ctx->stream>>>(ctx->N, ctx->U, prev_dev, max_diff, ctx->partials,
+                                                             nullptr);
     k_reduce_out<1><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->partials, (int)rblocks(ctx->N), ctx->scal + 12);
     LFRV_LAUNCH(ctx, "count moving");
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host + 12, ctx->scal + 12, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *moving = (int64_t)std::llround(ctx->scal_host[12]);
+    return LFRV_OK;
+}
+
+// The re-weighted least-squares loop of super-resolution.cpp:177-300.  Each
+// FISTA step is followed on the stream by its energy, the energy rules, the
+// moving-pixel count against the previous step's solution, the moving rule
+// and the copy to the previous solution; once a rule fires, the launches
+// enqueued after it return at once.  The host enqueues kSrPollSteps steps at
+// a time and reads the loop state of the chunk before the one it has just
+// enqueued, so the device runs ahead of the poll.
+lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *out)
+{
+    constexpr int kSrPollSteps = 8;
+    const float EPSILON_TOL = 1e-4f, MAX_DIFF_THRESHOLD = 1e-3f, PERCENT_MOVING = 0.01f;
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t N = ctx->N, n3 = 3 * N;
+    if (!ctx->sr_prev) LFRV_HIP(ctx, hipMalloc((void **)&ctx->sr_prev, 2 * n3 * sizeof(float)));
+    float *prev = ctx->sr_prev, *prev_meta = ctx->sr_prev + n3;
+    LFRV_HIP(ctx, hipMemcpyAsync(prev, ctx->U, n3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    LFRV_HIP(ctx, hipMemcpyAsync(prev_meta, ctx->U, n3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    const int *halt = &ctx->sr_loop->halt;
+    const EnergyCoef ec = energy_coef(ctx);
+    const int Gtv = (int)rblocks(N), Gv = (int)rblocks((size_t)ctx->V * ctx->n), Gm = (int)rblocks(N);
+    const float npix = (float)N;
+    lfrv_sr_stats s{};
+    float energy = 0.0f, previous_energy_reweight = 0.0f;
+    int r;
+    for (r = 0; r < nmeta; r++) {
+        lfrv_status st = run_compute_weights(ctx);
+        if (st == LFRV_OK) st = run_regularizer_weight(ctx);
+        if (st == LFRV_OK) st = enqueue_sr_energy(ctx, nullptr);
+        if (st != LFRV_OK) return st;
+        k_sr_loop_energy<true><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->sr_loop, ctx->partials, Gtv,
+                                                                        ctx->partials + kViewPartials, Gv, ec,
+                                                                        EPSILON_TOL);
+        LFRV_LAUNCH(ctx, "sr loop begin");
+        // alpha before each enqueued step, to undo the steps the device skipped
+        std::vector<double> alphas{ctx->fista_alpha};
+        int enq = 0, polls = 0;
+        bool halted = false;
+        while (enq < niter && !halted) {
+            const int chunk = std::min(kSrPollSteps, niter - enq);
+            for (int j = 0; j < chunk; j++) {
+                st = enqueue_fista(ctx, halt);
+                if (st == LFRV_OK) st = enqueue_sr_energy(ctx, halt);
+                if (st != LFRV_OK) return st;
+                alphas.push_back(ctx->fista_alpha);
+                k_sr_loop_energy<false><<<1, kFinalizeThreads, 0, ctx->stream>>>(
+                    ctx->sr_loop, ctx->partials, Gtv, ctx->partials + kViewPartials, Gv, ec, EPSILON_TOL);
+                k_count_moving<<<Gm, 256, 0, ctx->stream>>>(N, ctx->U, prev, MAX_DIFF_THRESHOLD, ctx->partials, halt);
+                k_sr_loop_moving<<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->sr_loop, ctx->partials, Gm, npix,
+                                                                         PERCENT_MOVING);
+                k_copy_unless_halted<<<blocks_for(n3), kBlock, 0, ctx->stream>>>(n3, ctx->U, prev, halt);
+                LFRV_LAUNCH(ctx, "sr loop step");
+            }
+            enq += chunk;
+            const int slot = polls & 1;
+            LFRV_HIP(ctx, hipMemcpyAsync(ctx->sr_loop_host + slot, ctx->sr_loop, sizeof(SrLoop), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+            LFRV_HIP(ctx, hipEventRecord(ctx->sr_ev[slot], ctx->stream));
+            if (++polls >= 2) {
+                LFRV_HIP(ctx, hipEventSynchronize(ctx->sr_ev[slot ^ 1]));
+                halted = ctx->sr_loop_host[slot ^ 1].halt != 0;
+            }
+        }
+        LFRV_HIP(ctx, hipMemcpyAsync(ctx->sr_loop_host + 2, ctx->sr_loop, sizeof(SrLoop), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const SrLoop L = ctx->sr_loop_host[2];
+        if (L.steps < 0 || L.steps > enq) return fail(ctx, LFRV_E_STATE, "FISTA loop state out of range");
+        // the skipped steps swapped the primal-dual buffers on the host only
+        const int skipped = enq - L.steps;
+        if (ctx->p.inner_iterations > 0 && ((int64_t)skipped * ctx->p.inner_iterations) % 2 != 0) {
+            std::swap(ctx->Uq, ctx->U2);
+            std::swap(ctx->PX, ctx->P2X);
+            std::swap(ctx->PY, ctx->P2Y);
+        }
+        ctx->fista_alpha = alphas[L.steps];
+        energy = L.energy;
+        s.fista_steps += L.steps;
+        s.last_stop = L.stop;
+        // the re-weighting rules (super-resolution.cpp:272-297)
+        if (energy < EPSILON_TOL) {
+            if (previous_energy_reweight < EPSILON_TOL) {
+                s.converged = 1;
+                break;
+            }
+            previous_energy_reweight = energy;
+            continue;
+        }
+        int64_t moving = 0;
+        st = run_count_moving(ctx, prev_meta, MAX_DIFF_THRESHOLD, &moving);
+        if (st != LFRV_OK) return st;
+        if ((float)moving / npix < PERCENT_MOVING) {
+            s.converged = 1;
+            break;
+        }
+        LFRV_HIP(ctx, hipMemcpyAsync(prev_meta, ctx->U, n3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s.meta_iterations = r < nmeta ? r + 1 : nmeta;
+    s.energy = energy;
+    s.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (out) *out = s;
     return LFRV_OK;
 }
 

@@ -230,13 +230,16 @@ constexpr int PD_TW = 64, PD_TH = 4;
 // With u_final (the last inner step of an outer step) the over-relaxation of
 // the outer step is fused in (vtv_rof.cu:28-45): u_final <- the new primal,
 // uout <- u_final_old + theta (new primal - u_final_old).
+// A non-null `halt` that holds nonzero makes the launch a no-op (a step
+// enqueued ahead of a stopping rule that has fired).
 __global__ void __launch_bounds__(256) k_pd_step(int W, int H, float tau, float lam, float sigma,
                                                  const float *__restrict__ uin, float *__restrict__ uout,
                                                  const float *__restrict__ pxin, const float *__restrict__ pyin,
                                                  float *__restrict__ pxout, float *__restrict__ pyout,
                                                  const float *__restrict__ F, const float *__restrict__ g,
-                                                 float *__restrict__ u_final, float theta)
+                                                 float *__restrict__ u_final, float theta, const int *halt)
 {
+    if (halt && *halt) return;
     constexpr int UW = PD_TW + 2, UH = PD_TH + 2;  // Uq region [x0-1, x0+TW] x [y0-1, y0+TH]
     constexpr int PW = PD_TW + 1, PH = PD_TH + 1;  // p region  [x0-1, x0+TW-1] x [y0-1, y0+TH-1]
     __shared__ float us[3][UH][UW];
@@ -433,7 +436,7 @@ lfrv_status run_push_pull(lfrv_ctx *ctx, int levels)
     return LFRV_OK;
 }
 
-lfrv_status run_pd_steps(lfrv_ctx *ctx, int iters, const float *F, bool overrelax, float theta)
+lfrv_status run_pd_steps(lfrv_ctx *ctx, int iters, const float *F, bool overrelax, float theta, const int *halt)
 {
     const int W = ctx->W, H = ctx->H;
     const float tau = (float)(0.3 / std::sqrt(8.0)), sigma = (float)(0.3 / std::sqrt(8.0));
@@ -447,7 +450,7 @@ lfrv_status run_pd_steps(lfrv_ctx *ctx, int iters, const float *F, bool overrela
         float *pxin = ctx->PX, *pyin = ctx->PY, *pxout = ctx->P2X, *pyout = ctx->P2Y;
         float *ufin = (overrelax && k == iters - 1) ? ctx->U : nullptr;
         k_pd_step<<<gs, 256, 0, ctx->stream>>>(W, H, tau, lam, sigma, uin, uout, pxin, pyin, pxout, pyout, F, g,
-                                               ufin, theta);
+                                               ufin, theta, halt);
         std::swap(ctx->Uq, ctx->U2);
         std::swap(ctx->PX, ctx->P2X);
         std::swap(ctx->PY, ctx->P2Y);

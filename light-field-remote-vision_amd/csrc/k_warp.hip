@@ -518,8 +518,7 @@ lfrv_status run_compute_weights(lfrv_ctx *ctx)
     if (ctx->p.mode == LFRV_MODE_COLOR_SR) {
         // vtv_sr_compute_gradient_device writes the gated Sobel of U into the
         // dual variable of channel 0 (optimization.cu:1090-1091); kept as is.
-        float *px0 = ctx->pd_swapped ? ctx->P2X : ctx->PX;
-        float *py0 = ctx->pd_swapped ? ctx->P2Y : ctx->PY;
+        float *px0 = ctx->PX, *py0 = ctx->PY;  // the current dual buffers
         k_sobel<<<gs, bs, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->U + N, ctx->U + 2 * N, px0, py0,
                                             ctx->visibility, nullptr);
         gxs = px0;

@@ -236,6 +236,17 @@ struct CgState {
     int breakdown[3];
 };
 
+// Device state of the IBR_color FISTA loop (the stopping rules of
+// super-resolution.cpp:215-252 evaluated on the device, one step at a time).
+struct SrLoop {
+    int halt;    // a rule fired: the steps enqueued after it return at once
+    int steps;   // FISTA steps run in this pass
+    int stop;    // LFRV_SR_STOP_* of the pass
+    int pad;
+    float energy, prev_energy;  // float, as the driver keeps them
+    double parts[3];            // E_TV, E_int, E_grad of the last energy
+};
+
 struct TimingSlot {
     double ms = 0.0;
     int64_t launches = 0;
@@ -303,6 +314,10 @@ struct lfrv_ctx {
     float *tmpN = nullptr;         // N scratch
     float *tmp3N = nullptr;        // 3N scratch
     double fista_alpha = 1.0;
+    lfrv::SrLoop *sr_loop = nullptr;       // device loop state of lfrv_sr_solve
+    lfrv::SrLoop *sr_loop_host = nullptr;  // pinned, 3 slots: two alternating polls, the final read
+    hipEvent_t sr_ev[2] = {nullptr, nullptr};
+    float *sr_prev = nullptr;              // 6N: the previous step's and pass's solutions (allocated on first use)
 
     // CG
     float *cg_r = nullptr, *cg_d = nullptr, *cg_q = nullptr, *cg_b = nullptr;  // 3N
@@ -332,9 +347,6 @@ struct lfrv_ctx {
     // U holds the cleared (all-zero) image: set by lfrv_clear_solution, reset by
     // every entry point that writes U
     bool u_zero = false;
-
-    // PD ping-pong orientation: which buffers currently hold Uq / p
-    bool pd_swapped = false;
 
     // timing
     bool timing = false;
@@ -397,6 +409,7 @@ lfrv_status run_regularizer_weight(lfrv_ctx *ctx);
 lfrv_status run_fista(lfrv_ctx *ctx);
 lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts);
 lfrv_status run_count_moving(lfrv_ctx *ctx, const float *prev_dev, float max_diff, int64_t *moving);
+lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *st);
 
 // stateless kernels
 hipError_t launch_gather(int W, int H, int w, int h, int dsf, const float *tx, const float *ty,

@@ -409,21 +409,11 @@ bool coco_vtv_sr_write_pfm_solution(coco_vtv_data *d)
     return save_pfm(d->_basedir + "/output.pfm", solution_image(d->ctx, d->_W, d->_H));
 }
 
-// Device buffer of the previous solution for the moving-pixel tests.
-struct DevBuf {
-    void *p = nullptr;
-    explicit DevBuf(size_t bytes)
-    {
-        if (lfrv_dev_alloc(bytes, &p) != LFRV_OK) throw Error("lfrv_dev_alloc failed");
-    }
-    ~DevBuf() { lfrv_dev_free(p); }
-};
-
 void sr_synthesize_view(ConfigData *cd)
 {
     check_channels(cd);
     std::vector<Image> lf = import_lightfield(cd);
-    const size_t W = cd->_W, H = cd->_H, N = W * H;
+    const size_t W = cd->_W, H = cd->_H;
     std::unique_ptr<coco_vtv_data, void (*)(coco_vtv_data *)> mtv(coco_vtv_alloc(W, H), [](coco_vtv_data *d) {
         if (d->ctx) coco_vtv_sr_free_unstructured(d);
         coco_vtv_free(d);
@@ -447,47 +437,14 @@ void sr_synthesize_view(ConfigData *cd)
     coco_vtv_setup_visibility_mask(mtv.get());
     coco_vtv_sr_compute_initial_image(mtv.get(), cd);
 
-    // stopping rules of super-resolution.cpp:177-300 in the reference's float
-    // (cuflt) arithmetic; the previous solutions stay on the device and the
-    // moving-pixel count is a device reduction
-    const float EPSILON_TOL = 1e-4f, MAX_DIFF_THRESHOLD = 1e-3f, PERCENT_MOVING = 0.01f;
-    DevBuf prev(3 * N * sizeof(float)), prev_meta(3 * N * sizeof(float));
-    check(mtv->ctx, lfrv_get_solution(mtv->ctx, (float *)prev_meta.p), "solution");
-    check(mtv->ctx, lfrv_get_solution(mtv->ctx, (float *)prev.p), "solution");
-    float energy = 0.0f, previous_energy = 0.0f, previous_energy_reweight = 0.0f;
-    auto moving_fraction_small = [&](const DevBuf &ref) {
-        int64_t nM = 0;
-        check(mtv->ctx, lfrv_sr_count_moving(mtv->ctx, (const float *)ref.p, MAX_DIFF_THRESHOLD, &nM), "moving");
-        return (float)nM / (float)N < PERCENT_MOVING;
-    };
-    int r, total_steps = 0;
-    for (r = 0; r < cd->_nmeta_iter; r++) {
-        coco_vtv_sr_compute_weights_unstructured(mtv.get());
-        coco_vtv_sr_init_regularizer_weight_unstructured(mtv.get());
-        energy = (float)coco_vtv_sr_primal_energy_unstructured(mtv.get());
-        previous_energy = energy;
-        for (int k = 0; k < cd->_niter; k++) {
-            coco_vtv_sr_iteration_fista_unstructured(mtv.get());
-            total_steps++;
-            energy = (float)coco_vtv_sr_primal_energy_unstructured(mtv.get(), previous_energy);
-            if (energy < EPSILON_TOL) break;
-            if (std::fabs((energy - previous_energy) / energy) < EPSILON_TOL) break;
-            previous_energy = energy;
-            if (moving_fraction_small(prev)) break;
-            check(mtv->ctx, lfrv_get_solution(mtv->ctx, (float *)prev.p), "solution");
-        }
-        if (energy < EPSILON_TOL) {
-            if (previous_energy_reweight < EPSILON_TOL) break;
-            previous_energy_reweight = energy;
-            continue;
-        }
-        if (moving_fraction_small(prev_meta)) break;
-        check(mtv->ctx, lfrv_get_solution(mtv->ctx, (float *)prev_meta.p), "solution");
-    }
+    // the re-weighted FISTA loop of super-resolution.cpp:177-300, its stopping
+    // rules (in the reference's float arithmetic) evaluated on the device
+    lfrv_sr_stats st{};
+    check(mtv->ctx, lfrv_sr_solve(mtv->ctx, cd->_niter, cd->_nmeta_iter, &st), "IBR_color solve");
     save_image(cd->_outdir + "/" + cd->_outfile, solution_image(mtv->ctx, W, H));
     coco_vtv_sr_write_pfm_solution(mtv.get());
     std::printf("IBR_color: %zu views %ux%u -> %zux%zu, %d meta-iterations, %d FISTA steps, energy %.6g\n",
-                mtv->_nviews, cd->_w, cd->_h, W, H, std::min(r + 1, cd->_nmeta_iter), total_steps, energy);
+                mtv->_nviews, cd->_w, cd->_h, W, H, st.meta_iterations, st.fista_steps, (double)st.energy);
 }
 
 }  // namespace lfrv_ref

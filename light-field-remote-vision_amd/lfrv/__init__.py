@@ -59,6 +59,17 @@ class CgStats(ctypes.Structure):
                     status=self.status, ms=self.ms)
 
 
+class SrStats(ctypes.Structure):
+    _fields_ = [
+        ("meta_iterations", ctypes.c_int), ("fista_steps", ctypes.c_int), ("last_stop", ctypes.c_int),
+        ("converged", ctypes.c_int), ("energy", ctypes.c_float), ("ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return dict(meta_iterations=self.meta_iterations, fista_steps=self.fista_steps, last_stop=self.last_stop,
+                    converged=self.converged, energy=self.energy, ms=self.ms)
+
+
 class SynthDesc(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int), ("seed", ctypes.c_uint64), ("grid", ctypes.c_int), ("w", ctypes.c_int),
@@ -113,6 +124,7 @@ def lib():
         "lfrv_sr_iteration_fista": (st, [_vp]),
         "lfrv_sr_primal_energy": (st, [_vp, _dp, _dp]),
         "lfrv_sr_count_moving": (st, [_vp, _vp, ctypes.c_float, ctypes.POINTER(ctypes.c_int64)]),
+        "lfrv_sr_solve": (st, [_vp, st, st, ctypes.POINTER(SrStats)]),
         "lfrv_get_buffer": (st, [_vp, st, _vp, ctypes.POINTER(ctypes.c_size_t)]),
         "lfrv_set_buffer": (st, [_vp, st, _vp]),
         "lfrv_set_timing": (st, [_vp, st]),
@@ -268,6 +280,12 @@ class Context:
         m = ctypes.c_int64()
         self._check(lib().lfrv_sr_count_moving(self._h, _ptr(_f32(prev)), max_diff, ctypes.byref(m)))
         return m.value
+
+    def sr_solve(self, niter, nmeta):
+        """The re-weighted FISTA loop with the driver's stopping rules on the device (lfrv_sr_solve)."""
+        s = SrStats()
+        self._check(lib().lfrv_sr_solve(self._h, niter, nmeta, ctypes.byref(s)))
+        return s.as_dict()
 
     def sync(self):
         self._check(lib().lfrv_sync(self._h))

@@ -47,6 +47,26 @@ def test_library_exports_every_declared_function():
     assert not missing, missing
 
 
+@pytest.mark.parametrize("cname,pyname", [("lfrv_cg_stats", "CgStats"), ("lfrv_sr_stats", "SrStats"),
+                                           ("lfrv_params", "Params")])
+def test_struct_layouts_match_the_header(tmp_path, cname, pyname):
+    """The ctypes mirrors of the C-ABI structs have the header's size and
+    field offsets (compiled with gcc against include/lfrv.h)."""
+    import subprocess
+    S = getattr(lfrv, pyname)
+    fields = [f[0] for f in S._fields_]
+    src = tmp_path / "layout.c"
+    cfield = {"lambda_": "lambda"}  # a Python keyword on the ctypes side
+    body = "".join(f'    printf("%zu\\n", offsetof({cname}, {cfield.get(f, f)}));\n' for f in fields)
+    src.write_text(f"#include <stddef.h>\n#include <stdio.h>\n#include \"lfrv.h\"\nint main(void) {{\n"
+                   f'    printf("%zu\\n", sizeof({cname}));\n{body}    return 0;\n}}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(S)
+    assert got[1:] == [getattr(S, f).offset for f in fields]
+
+
 def test_host_library_exports_the_reference_named_interface():
     """Every function include/lfrv_ref.hpp declares is defined in liblfrv_host.so."""
     import subprocess

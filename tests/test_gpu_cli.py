@@ -136,37 +136,52 @@ def test_cli_ibr_gradient_from_tau_files(tmp_path, small_scene):
 
 def _sr_driver(ctx, W, H, niter, nmeta):
     """super-resolution.cpp:177-300 restated over the C-ABI in the reference's
-    float (cuflt) arithmetic, with host copies of the previous solutions."""
+    float (cuflt) arithmetic, with host copies of the previous solutions.
+    Returns the solution and what the loop did: FISTA steps, re-weighting
+    passes, how each pass's FISTA loop ended (0 niter steps, 1 small energy,
+    2 small relative change, 3 few moving pixels), the last energy and
+    whether a re-weighting rule ended the solve."""
     f = np.float32
     EPS, MAXD, PCT = f(1e-4), f(1e-3), f(0.01)
     prev_meta = ctx.get_solution()
     prev = prev_meta.copy()
     energy = prev_e = prev_e_rw = f(0)
+    info = dict(steps=0, meta=0, stops=[], converged=0)
     for r in range(nmeta):
+        info["meta"] = r + 1
         ctx.compute_weights()
         ctx.sr_init_regularizer_weight()
         energy = f(ctx.sr_primal_energy()[0])
         prev_e = energy
+        stop = 0
         for k in range(niter):
             ctx.sr_iteration_fista()
+            info["steps"] += 1
             energy = f(ctx.sr_primal_energy()[0])
             if energy < EPS:
+                stop = 1
                 break
             if abs((energy - prev_e) / energy) < EPS:
+                stop = 2
                 break
             prev_e = energy
             if f(ctx.sr_count_moving(prev, MAXD)) / f(W * H) < PCT:
+                stop = 3
                 break
             prev = ctx.get_solution()
+        info["stops"].append(stop)
         if energy < EPS:
             if prev_e_rw < EPS:
+                info["converged"] = 1
                 break
             prev_e_rw = energy
             continue
         if f(ctx.sr_count_moving(prev_meta, MAXD)) / f(W * H) < PCT:
+            info["converged"] = 1
             break
         prev_meta = ctx.get_solution()
-    return ctx.get_solution()
+    info["energy"] = energy
+    return ctx.get_solution(), info
 
 
 def test_cli_ibr_color_matches_abi(tmp_path):
@@ -189,5 +204,82 @@ def test_cli_ibr_color_matches_abi(tmp_path):
         pipelines.load_scene(ctx, s)
         ctx.clear_solution()
         pipelines.color_sr_setup(ctx)
-        U = _sr_driver(ctx, s["W"], s["H"], niter, nmeta)
+        U, _ = _sr_driver(ctx, s["W"], s["H"], niter, nmeta)
     assert np.max(np.abs(U_cli - U)) <= 1e-5
+
+
+def _color_scene(dsf, w=40, flat=False):
+    d = lfrv.synth_desc(1, 5)
+    d.w = d.h = w
+    d.dsf = d.render_scale = dsf
+    d.focal = float(w)
+    d.baseline = 0.7 * 4.0 / (w * dsf)
+    s = lfrv.synth(d)
+    if flat:  # one colour everywhere: TV and gradient terms vanish
+        s["views"][:] = np.array([0.2, 0.5, 0.7], np.float32)[None, :, None, None]
+    return s
+
+
+@pytest.mark.parametrize("dsf,inner,beta,niter,nmeta,flat,scale,stops", [
+    (2, 3, 0.3, 15, 3, False, 1.0, {0}),      # niter reached in every pass; odd inner count
+    (2, 2, 0.0, 120, 4, False, 1.0, {0, 2, 3}),  # long passes: the relative-change and moving rules
+    (1, 1, 0.3, 120, 3, False, 1.0, {2, 3}),  # dsf = 1 data term
+    (3, 5, 0.3, 60, 2, False, 1.0, {0}),
+    (2, 1, 0.3, 0, 2, False, 1.0, {0}),       # no FISTA step: the passes only re-weight
+    (2, 1, 0.3, 20, 3, True, 1e-9, {1}),      # flat views, tiny weights: the small-energy rules
+])
+def test_sr_solve_matches_the_driver_loop(dsf, inner, beta, niter, nmeta, flat, scale, stops):
+    """lfrv_sr_solve (stopping rules on the device, steps enqueued ahead and
+    skipped once a rule fired) against the same loop driven from the host,
+    step by step, over the C-ABI: the same solution bit for bit, the same
+    step count, stop reasons and energy.  `stops` are the ways the passes'
+    FISTA loops end in each case (0 niter steps, 1 small energy, 2 small
+    relative change, 3 few moving pixels), so every rule is exercised."""
+    s = _color_scene(dsf, flat=flat)
+    prm = pipelines.color_sr_params(s, lambda_=0.05 * scale, lambda_max_factor=100.0, inner_iterations=inner,
+                                    dt_alpha=scale, dt_beta=beta * scale, gradient_step=1.0,
+                                    sigma_sensor=np.float32(1.0) / np.float32(255))
+    res = []
+    for device_loop in (False, True):
+        print(f"sr loop dsf={dsf} inner={inner} niter={niter} nmeta={nmeta} device={device_loop}", flush=True)
+        with lfrv.Context(prm) as ctx:
+            pipelines.load_scene(ctx, s)
+            ctx.clear_solution()
+            pipelines.color_sr_setup(ctx)
+            if device_loop:
+                st = ctx.sr_solve(niter, nmeta)
+                res.append((ctx.get_solution(), st))
+            else:
+                res.append(_sr_driver(ctx, s["W"], s["H"], niter, nmeta))
+    (U_h, info), (U_d, st) = res
+    print(info, st)
+    assert set(info["stops"]) == stops
+    assert st["fista_steps"] == info["steps"]
+    assert st["meta_iterations"] == info["meta"]
+    assert st["last_stop"] == (info["stops"][-1] if info["stops"] else 0)
+    assert st["converged"] == info["converged"]
+    assert np.float32(st["energy"]) == info["energy"]
+    assert np.array_equal(U_d, U_h)
+
+
+def test_sr_views_in_one_context_are_independent():
+    """lfrv_clear_solution starts an IBR_color view from a fresh workspace
+    (duals, G, normalisation buffer zero, FISTA alpha 1): a view solved after
+    another one in the same context equals the first bit for bit."""
+    s = _color_scene(2)
+    prm = pipelines.color_sr_params(s, lambda_=0.05, lambda_max_factor=100.0, inner_iterations=2, dt_beta=0.3,
+                                    gradient_step=1.0)
+    out = []
+    with lfrv.Context(prm) as ctx:
+        pipelines.load_scene(ctx, s)
+        for _ in range(2):
+            ctx.clear_solution()
+            pipelines.color_sr_setup(ctx)
+            st = ctx.sr_solve(40, 2)
+            out.append((ctx.get_solution(), st["fista_steps"]))
+            # a few more steps outside the loop change alpha and the duals
+            ctx.sr_init_regularizer_weight()
+            for _ in range(3):
+                ctx.sr_iteration_fista()
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
