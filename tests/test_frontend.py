@@ -174,3 +174,18 @@ def test_cli_requires_three_channels(tmp_path):
     r = subprocess.run([lio.CLI_PATH, "-outdir", str(tmp_path), "-algorithm", "IBR_color"],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "nchannels" in r.stderr
+
+
+@pytest.mark.parametrize("algo", ["no_such_algorithm", "IBR_colour"])
+def test_cli_runs_ibr_color_for_other_algorithm_names(tmp_path, algo):
+    """main.cpp:430-434: a name none of the branches knows runs
+    sr_synthesize_view (seen here by its channel check, before any GPU work)."""
+    r = subprocess.run([lio.CLI_PATH, "-outdir", str(tmp_path), "-algorithm", algo],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "nchannels" in r.stderr
+
+
+def test_cli_without_algorithm_is_the_opengl_viewer(tmp_path):
+    """No algorithm switch starts the reference's OpenGL scene viewer (main.cpp:405-428), not provided here."""
+    r = subprocess.run([lio.CLI_PATH, "-outdir", str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "OpenGL" in r.stderr
