@@ -239,30 +239,42 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_sr_assemble(TileGeom g, Til
 // sums back onto the pixels, split into rows and columns.
 constexpr int kStTW = 32, kStTH = 16;
 
+// Every LDS plane of the tile has the row pitch P of the widest one (U with
+// its ring) and every pass maps consecutive threads to consecutive points of
+// that pitch (idle where x is past the pass's width), so a wave's accesses are
+// contiguous and free of bank conflicts; a pitch per plane made them jump at
+// each row wrap.
 template <int KS>
-__global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, int Wc, int Hc, int ntx, int ntiles,
-                                                                   DataTermArgs da, const float *__restrict__ U,
-                                                                   const float *__restrict__ ops, size_t NA,
-                                                                   const float *__restrict__ coverage,
-                                                                   float *__restrict__ Gout, float *__restrict__ Uq,
-                                                                   float *__restrict__ norm_mask, const int *halt)
+struct StDims {
+    static constexpr int D = KS - 1, TW = kStTW, TH = kStTH, NT = TW * TH;
+    static constexpr int QW = TW + D - 1, QH = TH + D - 1;  // anchor points x' = tx0 + qx
+    static constexpr int BW = QW + 2, BH = QH + 2;          // box sums at x' = tx0 - 1 + bx
+    static constexpr int LW = BW + D - 1, LH = BH + D - 1;  // pixels tx0 - D + lx (box sums' support)
+    static constexpr int SW = LW + 2, SH = LH + 2;          // U from tx0 - D - 1 (the Laplacian's ring)
+    static constexpr int P = SW;                            // the common row pitch
+    static constexpr int A1 = 3 * SH * P + 3 * LH * P, A2 = 6 * QH * P + 6 * QH * TW;
+    static constexpr int AREA = A1 > A2 ? A1 : A2;  // sa: U and lap U, later the anchor residuals and their row sums
+    static constexpr int BAREA = 6 * BH * P;        // sb: BU (3) and BL (3)
+    static constexpr int NQP = QH * P;              // anchor slots (qx < QW used)
+    static constexpr int NQT = (NQP + NT - 1) / NT, NST = (SH * P + NT - 1) / NT;
+};
+
+// The data term of one 32 x 16 tile.  Every thread of the block calls it; on
+// return (the LDS free again) each thread holds F (the clamped gradient step)
+// and U at its pixel, and the pixel's coverage.  With ur, the U values of
+// [tx0-1, tx0+TW] x [ty0-1, ty0+TH] (3 planes, pitch TW+2) are also left there.
+template <int KS>
+__device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, int tx0, int ty0, const DataTermArgs &da,
+                                                 const float *__restrict__ U, const float *__restrict__ ops, size_t NA,
+                                                 const float *__restrict__ coverage, float *sa, float *sb,
+                                                 float (&F)[3], float (&up)[3], float &cov, float *ur = nullptr)
 {
-    if (halt && *halt) return;  // a step enqueued behind a stopping rule that fired
-    constexpr int D = KS - 1, TW = kStTW, TH = kStTH, NT = TW * TH;
-    constexpr int QW = TW + D - 1, QH = TH + D - 1, NQ = QW * QH;  // anchor points x' = tx0 + qx
-    constexpr int BW = QW + 2, BH = QH + 2;                         // box sums at x' = tx0 - 1 + bx
-    constexpr int LW = BW + D - 1, LH = BH + D - 1;                 // pixels tx0 - D + lx (box sums' support)
-    constexpr int SW = LW + 2, SH = LH + 2;                         // U from tx0 - D - 1 (the Laplacian's ring)
-    constexpr int AREA = (3 * SH * SW + 3 * LH * LW) > (6 * QH * QW + 6 * QH * TW) ? (3 * SH * SW + 3 * LH * LW)
-                                                                                    : (6 * QH * QW + 6 * QH * TW);
-    constexpr int NQT = (NQ + NT - 1) / NT, NST = (SH * SW + NT - 1) / NT;
-    __shared__ float sa[AREA];           // U and lap U, later the anchor residuals and their row sums
-    __shared__ float sb[6 * BH * BW];    // BU (3) and BL (3)
-    float *us = sa, *ls = sa + 3 * SH * SW;
-    float *qs = sa, *rs = sa + 6 * QH * QW;
-    float *bus = sb, *bls = sb + 3 * BH * BW;
-    const int t = xcd_tile(blockIdx.x, ntiles);
-    const int tx0 = (t % ntx) * TW, ty0 = (t / ntx) * TH;
+    using S = StDims<KS>;
+    constexpr int D = S::D, TW = S::TW, NT = S::NT, QW = S::QW, QH = S::QH, BW = S::BW, BH = S::BH, LW = S::LW,
+                  LH = S::LH, SH = S::SH, P = S::P, NQP = S::NQP, NQT = S::NQT, NST = S::NST;
+    float *us = sa, *ls = sa + 3 * SH * P;
+    float *qs = sa, *rs = sa + 6 * NQP;
+    float *bus = sb, *bls = sb + 3 * BH * P;
     const size_t N = (size_t)W * H;
     const bool grad = da.beta > 0.0f;
     const float *g9 = da.gw_type ? ops : ops + 8 * NA;  // couplings of the gradient term
@@ -273,8 +285,8 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
 #pragma unroll
     for (int k = 0; k < NST; k++) {
         const int i = tid + k * NT;
-        const int x = tx0 - D - 1 + i % SW, y = ty0 - D - 1 + i / SW;
-        const bool in = i < SH * SW && x >= 0 && x < W && y >= 0 && y < H;
+        const int x = tx0 - D - 1 + i % P, y = ty0 - D - 1 + i / P;
+        const bool in = i < SH * P && x >= 0 && x < W && y >= 0 && y < H;
         const size_t o = in ? (size_t)y * W + x : 0;
 #pragma unroll
         for (int c = 0; c < 3; c++) uv[k][c] = in ? U[c * N + o] : 0.0f;
@@ -283,20 +295,20 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
 #pragma unroll
     for (int k = 0; k < NQT; k++) {
         const int i = tid + k * NT;
-        const int xq = tx0 + i % QW, yq = ty0 + i / QW;
-        const bool in = i < NQ && xq < Wc && yq < Hc;
+        const int xq = tx0 + i % P, yq = ty0 + i / P;
+        const bool in = i < NQP && i % P < QW && xq < Wc && yq < Hc;
         const int xc = min(xq, Wc - 1), yc = min(yq, Hc - 1);
         const size_t o = (size_t)yc * Wc + xc;
         const bool xm = xc > 0, ym = yc > 0, xp = xc < Wc - 1;
         const size_t oE = xm ? o - 1 : o, oS = ym ? o - Wc : o, oSE = (xm && ym) ? o - Wc - 1 : o,
                      oSW = (xp && ym) ? o - Wc + 1 : o;
-        auto load9 = [&](const float *P, float(&f)[9]) {
-            f[0] = P[o];
-            f[1] = P[NA + o];
-            f[2] = P[2 * NA + o];
-            f[3] = P[3 * NA + o];
-            f[4] = P[4 * NA + o];
-            const float e = P[NA + oE], sn = P[2 * NA + oS], se = P[3 * NA + oSE], sw = P[4 * NA + oSW];
+        auto load9 = [&](const float *Q, float(&f)[9]) {
+            f[0] = Q[o];
+            f[1] = Q[NA + o];
+            f[2] = Q[2 * NA + o];
+            f[3] = Q[3 * NA + o];
+            f[4] = Q[4 * NA + o];
+            const float e = Q[NA + oE], sn = Q[2 * NA + oS], se = Q[3 * NA + oSE], sw = Q[4 * NA + oSW];
             f[5] = xm ? e : 0.0f;
             f[6] = ym ? sn : 0.0f;
             f[7] = (xm && ym) ? se : 0.0f;
@@ -317,8 +329,7 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
     const int px = tx0 + tid % TW, py = ty0 + tid / TW;
     const bool pin = px < W && py < H;
     const size_t po = pin ? (size_t)py * W + px : 0;
-    const float cov = pin ? coverage[po] : 0.0f;
-    float up[3];
+    cov = pin ? coverage[po] : 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; c++) up[c] = pin ? U[c * N + po] : 0.0f;
 
@@ -326,36 +337,45 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
 #pragma unroll
     for (int k = 0; k < NST; k++) {
         const int i = tid + k * NT;
-        if (i < SH * SW)
+        if (i < SH * P)
 #pragma unroll
-            for (int c = 0; c < 3; c++) us[c * SH * SW + i] = uv[k][c];
+            for (int c = 0; c < 3; c++) us[c * SH * P + i] = uv[k][c];
+        if (ur) {
+            // the ring of the primal-dual step: U tile coordinates (D, D) on
+            constexpr int RW = TW + 2, RH = S::TH + 2;
+            const int rx = i % P - D, ry = i / P - D;
+            if (i < SH * P && rx >= 0 && rx < RW && ry >= 0 && ry < RH)
+#pragma unroll
+                for (int c = 0; c < 3; c++) ur[(c * RH + ry) * RW + rx] = uv[k][c];
+        }
     }
     __syncthreads();
     if (grad) {
         // cuda_laplacian_device (vtv_sr_kernels_unstructured.cu:720-761), no hole gating
-        for (int i = tid; i < LH * LW; i += NT) {
-            const int lx = i % LW, ly = i / LW;
+        for (int i = tid; i < LH * P; i += NT) {
+            const int lx = i % P, ly = i / P;
+            if (lx >= LW) continue;
             const int x = tx0 - D + lx, y = ty0 - D + ly;
-            const int ui = (ly + 1) * SW + lx + 1;
+            const int ui = i + P + 1;  // the pixel in the U tile
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 float v = 0.0f;
                 if (x >= 0 && x < W && y >= 0 && y < H) {
-                    const float *u = us + c * SH * SW;
+                    const float *u = us + c * SH * P;
                     float sm = 0.0f, wsum = 0.0f;
                     if (x > 0) { sm += u[ui - 1]; wsum += 1.0f; }
-                    if (y > 0) { sm += u[ui - SW]; wsum += 1.0f; }
+                    if (y > 0) { sm += u[ui - P]; wsum += 1.0f; }
                     if (x < W - 1) { sm += u[ui + 1]; wsum += 1.0f; }
-                    if (y < H - 1) { sm += u[ui + SW]; wsum += 1.0f; }
+                    if (y < H - 1) { sm += u[ui + P]; wsum += 1.0f; }
                     v = sm / wsum - u[ui];
                 }
-                ls[c * LH * LW + i] = v;
+                ls[c * LH * P + i] = v;
             }
         }
         __syncthreads();
     }
-    for (int i = tid; i < BH * BW; i += NT) {
-        const int bx = i % BW, by = i / BW;
+    for (int i = tid; i < BH * P; i += NT) {
+        if (i % P >= BW) continue;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             float su = 0.0f, sl = 0.0f;
@@ -363,11 +383,11 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
             for (int l = 0; l < D; l++)
 #pragma unroll
                 for (int k = 0; k < D; k++) {
-                    su += us[c * SH * SW + (by + l + 1) * SW + bx + k + 1];
-                    if (grad) sl += ls[c * LH * LW + (by + l) * LW + bx + k];
+                    su += us[c * SH * P + i + (l + 1) * P + k + 1];
+                    if (grad) sl += ls[c * LH * P + i + l * P + k];
                 }
-            bus[c * BH * BW + i] = su;
-            bls[c * BH * BW + i] = sl;
+            bus[c * BH * P + i] = su;
+            bls[c * BH * P + i] = sl;
         }
     }
     __syncthreads();
@@ -376,36 +396,35 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
 #pragma unroll
     for (int k = 0; k < NQT; k++) {
         const int i = tid + k * NT;
-        if (i >= NQ) continue;
-        const int qx = i % QW, qy = i / QW;
-        const int b = (qy + 1) * BW + qx + 1;  // the point in the box-sum tile
+        if (i >= NQP || i % P >= QW) continue;
+        const int b = i + P + 1;  // the point in the box-sum tile
         auto st9 = [&](const float *B, const float(&f)[9]) {
-            return f[0] * B[b] + f[1] * B[b + 1] + f[5] * B[b - 1] + f[2] * B[b + BW] + f[6] * B[b - BW] +
-                   f[3] * B[b + BW + 1] + f[7] * B[b - BW - 1] + f[4] * B[b + BW - 1] + f[8] * B[b - BW + 1];
+            return f[0] * B[b] + f[1] * B[b + 1] + f[5] * B[b - 1] + f[2] * B[b + P] + f[6] * B[b - P] +
+                   f[3] * B[b + P + 1] + f[7] * B[b - P - 1] + f[4] * B[b + P - 1] + f[8] * B[b - P + 1];
         };
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            qs[c * NQ + i] = st9(bus + c * BH * BW, cf[k]) - bi[k][c];
-            if (grad) qs[(3 + c) * NQ + i] = da.dsf2 * st9(bls + c * BH * BW, cg[k]) - bg[k][c];
+            qs[c * NQP + i] = st9(bus + c * BH * P, cf[k]) - bi[k][c];
+            if (grad) qs[(3 + c) * NQP + i] = da.dsf2 * st9(bls + c * BH * P, cg[k]) - bg[k][c];
         }
     }
     __syncthreads();
-    // rows: r(x, qy) = sum_k q(x + k, qy)
-    for (int i = tid; i < QH * TW; i += NT) {
-        const int lx = i % TW, qy = i / TW;
+    // rows: r(x, qy) = sum_k q(x + k, qy), stored with pitch TW for the pixel pass
+    for (int i = tid; i < QH * P; i += NT) {
+        const int lx = i % P, qy = i / P;
+        if (lx >= TW) continue;
 #pragma unroll
         for (int c = 0; c < 6; c++) {
             if (c >= 3 && !grad) break;
             float sm = 0.0f;
 #pragma unroll
-            for (int k = 0; k < D; k++) sm += qs[c * NQ + qy * QW + lx + k];
+            for (int k = 0; k < D; k++) sm += qs[c * NQP + i + k];
             rs[(c * QH + qy) * TW + lx] = sm;
         }
     }
     __syncthreads();
 
     // ---- pixel: columns of the row sums, then the step of vtv_sr_ista_step_unstructured ----
-    if (!pin) return;
     const int lx = tid % TW, ly = tid / TW;
     float acc[6];
 #pragma unroll
@@ -435,11 +454,153 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
         G = fminf(1.0f, fmaxf(-1.0f, G));
         G *= da.prox_factor;
         G += up[c];
-        G = fminf(1.0f, fmaxf(0.0f, G));
-        Gout[c * N + po] = G;
+        F[c] = fminf(1.0f, fmaxf(0.0f, G));
+    }
+    __syncthreads();  // the LDS is the caller's again
+}
+
+template <int KS>
+__global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, int Wc, int Hc, int ntx, int ntiles,
+                                                                   DataTermArgs da, const float *__restrict__ U,
+                                                                   const float *__restrict__ ops, size_t NA,
+                                                                   const float *__restrict__ coverage,
+                                                                   float *__restrict__ Gout, float *__restrict__ Uq,
+                                                                   float *__restrict__ norm_mask, const int *halt)
+{
+    if (halt && *halt) return;  // a step enqueued behind a stopping rule that fired
+    __shared__ float sa[StDims<KS>::AREA];
+    __shared__ float sb[StDims<KS>::BAREA];
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const int tx0 = (t % ntx) * kStTW, ty0 = (t / ntx) * kStTH;
+    float F[3], up[3], cov;
+    sr_dataterm_tile<KS>(W, H, Wc, Hc, tx0, ty0, da, U, ops, NA, coverage, sa, sb, F, up, cov);
+    const int px = tx0 + threadIdx.x % kStTW, py = ty0 + threadIdx.x / kStTW;
+    if (px >= W || py >= H) return;
+    const size_t N = (size_t)W * H, po = (size_t)py * W + px;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        Gout[c * N + po] = F[c];
         Uq[c * N + po] = up[c];
     }
-    if (da.alpha > 0.0f || grad) norm_mask[po] = cov;
+    if (da.alpha > 0.0f || da.beta > 0.0f) norm_mask[po] = cov;
+}
+
+struct PdArgs {
+    float tau, lam, sigma, theta;
+};
+
+// One outer step for dsf >= 2 in one pass: the data term of the tile, then
+// the first primal-dual iteration on it (k_pd_step's update, with Uq = U as the
+// step sets it, optimization.cu:1765-1767).  U is only read; the new primal
+// goes to unew.  With uq_relax (the step's only inner iteration) the
+// over-relaxation of vtv_rof.cu:28-45 is fused too: uq_relax <- U + theta (new -
+// U).  Gout (nullable) receives F for further inner iterations.
+template <int KS>
+__global__ void __launch_bounds__(kStTW * kStTH) k_sr_step_st(int W, int H, int Wc, int Hc, int ntx, int ntiles,
+                                                               DataTermArgs da, PdArgs pa, const float *__restrict__ U,
+                                                               const float *__restrict__ ops, size_t NA,
+                                                               const float *__restrict__ coverage,
+                                                               const float *__restrict__ pxin,
+                                                               const float *__restrict__ pyin,
+                                                               const float *__restrict__ g, float *__restrict__ pxout,
+                                                               float *__restrict__ pyout, float *__restrict__ unew,
+                                                               float *__restrict__ uq_relax, float *__restrict__ Gout,
+                                                               float *__restrict__ norm_mask, const int *halt)
+{
+    if (halt && *halt) return;  // a step enqueued behind a stopping rule that fired
+    constexpr int TW = kStTW, TH = kStTH, NT = TW * TH;
+    constexpr int UW = TW + 2, UH = TH + 2;  // U region [x0-1, x0+TW] x [y0-1, y0+TH]
+    constexpr int PW = TW + 1, PH = TH + 1;  // p region [x0-1, x0+TW-1] x [y0-1, y0+TH-1]
+    constexpr int NPP = PH * UW;             // p slots with the U region's pitch (x < PW used)
+    constexpr int NPT = (NPP + NT - 1) / NT;
+    static_assert(6 * NPP <= StDims<KS>::AREA, "primal-dual tiles exceed the data-term LDS");
+    static_assert(UW == kStTW + 2 && UH == kStTH + 2, "the ring the data term leaves");
+    __shared__ float sa[StDims<KS>::AREA];
+    __shared__ float sb[StDims<KS>::BAREA];
+    __shared__ float us[3 * UH * UW];  // U on the primal-dual step's ring, left by the data term's staging
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const int tx0 = (t % ntx) * TW, ty0 = (t / ntx) * TH;
+    const size_t N = (size_t)W * H;
+    const int tid = threadIdx.x;
+    // the dual inputs of this thread's p points and the TV weight there, loaded
+    // first so that they arrive while the data term runs
+    float pa_in[NPT][6], gr[NPT];
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int i = tid + k * NT;
+        const int x = tx0 - 1 + i % UW, y = ty0 - 1 + i / UW;
+        const bool in = i < NPP && i % UW < PW && x >= 0 && x < W && y >= 0 && y < H;
+        const size_t o = in ? (size_t)y * W + x : 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            pa_in[k][c] = in ? pxin[c * N + o] : 0.0f;
+            pa_in[k][3 + c] = in ? pyin[c * N + o] : 0.0f;
+        }
+        gr[k] = (in && g) ? g[o] : 1.0f;
+    }
+    float F[3], up[3], cov;
+    sr_dataterm_tile<KS>(W, H, Wc, Hc, tx0, ty0, da, U, ops, NA, coverage, sa, sb, F, up, cov, us);
+
+    // ---- the primal-dual iteration (vtv_rof.cu:118-147, 280-321; cuda_kernels.cu:504-532) ----
+    float *ps = sa;
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int i = tid + k * NT;
+        if (i >= NPP || i % UW >= PW) continue;
+        const int lx = i % UW, ly = i / UW;
+        const int x = tx0 - 1 + lx, y = ty0 - 1 + ly;
+        float a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+        if (x >= 0 && x < W && y >= 0 && y < H) {
+            float nn = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                // the p region and the U region share the origin (x0-1, y0-1)
+                const float *u = us + c * UH * UW;
+                const float uc = u[i];
+                const float gx = (x < W - 1) ? u[i + 1] - uc : 0.0f;
+                const float gy = (y < H - 1) ? u[i + UW] - uc : 0.0f;
+                a[c] = pa_in[k][c] + pa.sigma * gx;
+                b[c] = pa_in[k][3 + c] + pa.sigma * gy;
+                nn += a[c] * a[c] + b[c] * b[c];
+            }
+            nn = sqrtf(nn);
+            if (nn > gr[k]) {
+                const float sc = gr[k] / nn;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    a[c] *= sc;
+                    b[c] *= sc;
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            ps[c * NPP + i] = a[c];
+            ps[(3 + c) * NPP + i] = b[c];
+        }
+    }
+    __syncthreads();
+    const int lx = tid % TW, ly = tid / TW;
+    const int x = tx0 + lx, y = ty0 + ly;
+    if (x >= W || y >= H) return;
+    const size_t o = (size_t)y * W + x;
+    const int pi = (ly + 1) * UW + lx + 1;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float *pxs = ps + c * NPP, *pys = ps + (3 + c) * NPP;
+        const float pxv = pxs[pi], pyv = pys[pi];
+        pxout[c * N + o] = pxv;
+        pyout[c * N + o] = pyv;
+        float step = pxv + pyv;
+        if (x > 0) step -= pxs[pi - 1];
+        if (y > 0) step -= pys[pi - UW];
+        const float un = up[c] + pa.tau * step;
+        const float v = (pa.tau * F[c] + pa.lam * un) / (pa.tau + pa.lam);
+        unew[c * N + o] = v;
+        if (uq_relax) uq_relax[c * N + o] = up[c] + pa.theta * (v - up[c]);
+        if (Gout) Gout[c * N + o] = F[c];
+    }
+    if (da.alpha > 0.0f || da.beta > 0.0f) norm_mask[o] = cov;
 }
 
 // theta over-relaxation (vtv_rof.cu:28-45): uq <- u + theta (uq - u); u <- uq_old
@@ -661,6 +822,13 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
     const float L = (float)(1.0 / (double)ctx->p.lambda);  // optimization.cu:1741
     da.prox_factor = (float)(-1.0 / ((double)ctx->p.lambda * (double)L));
     da.gw_type = ctx->p.gw_type;
+    // FISTA weights (optimization.cu:1744-1746): alpha_new is a cuflt (float)
+    // there, theta the float of the double quotient; the over-relaxation is
+    // fused into the last primal-dual step
+    const float an = (float)(0.5 * (1.0 + std::sqrt(1.0 + 4.0 * std::pow(ctx->fista_alpha, 2.0))));
+    const float theta = (float)((ctx->fista_alpha - 1.0) / (double)an);
+    const int inner = ctx->p.inner_iterations;
+    bool pd_done = false;  // the first primal-dual iteration ran with the data term
     const int HAL = ctx->ks - 1;
     if (ctx->ks == 2) {
         const size_t up = (size_t)(ctx->tile_w + 2 * HAL + 2) * (ctx->tile_h + 2 * HAL + 2);
@@ -700,23 +868,53 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
         }
         const int ntx = (ctx->W + kStTW - 1) / kStTW, ntiles = ntx * ((ctx->H + kStTH - 1) / kStTH);
         TimedRegion tr(ctx, 3);
+        if (inner > 0) {
+            // the data term and the first primal-dual iteration in one pass: U is
+            // read, the new primal goes to U2 (with one inner iteration the
+            // over-relaxed one to Uq), the duals to P2X / P2Y
+            PdArgs pa;
+            pa.tau = pa.sigma = (float)(0.3 / std::sqrt(8.0));
+            const float Lp = (float)(1.0 / (double)ctx->p.lambda);
+            pa.lam = (float)(1.0 / (double)Lp);
+            pa.theta = theta;
+            const float *gw = ctx->has_reg_g ? ctx->reg_g : nullptr;
+            float *uq_relax = inner == 1 ? ctx->Uq : nullptr;
+            float *gout = inner > 1 ? ctx->G : nullptr;
+#define ST(K) k_sr_step_st<K><<<ntiles, kStTW * kStTH, 0, ctx->stream>>>(ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, pa, ctx->U, ctx->sr_ops, NA, ctx->coverage, ctx->PX, ctx->PY, gw, ctx->P2X, ctx->P2Y, ctx->U2, uq_relax, gout, ctx->norm_mask, halt)
+            switch (ctx->ks) {
+            case 3: ST(3); break;
+            case 4: ST(4); break;
+            case 5: ST(5); break;
+            default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
+            }
+#undef ST
+            LFRV_LAUNCH(ctx, "data term + primal-dual step");
+            std::swap(ctx->PX, ctx->P2X);
+            std::swap(ctx->PY, ctx->P2Y);
+            if (inner == 1)
+                std::swap(ctx->U, ctx->U2);  // U the new primal, Uq the over-relaxed one
+            else
+                std::swap(ctx->Uq, ctx->U2);  // the remaining iterations start from Uq
+            pd_done = true;
+        } else {
 #define DT(K) k_sr_dataterm_st<K><<<ntiles, kStTW * kStTH, 0, ctx->stream>>>(ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, ctx->U, ctx->sr_ops, NA, ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask, halt)
-        switch (ctx->ks) {
-        case 3: DT(3); break;
-        case 4: DT(4); break;
-        case 5: DT(5); break;
-        default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
-        }
+            switch (ctx->ks) {
+            case 3: DT(3); break;
+            case 4: DT(4); break;
+            case 5: DT(5); break;
+            default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
+            }
 #undef DT
-        LFRV_LAUNCH(ctx, "data term");
+            LFRV_LAUNCH(ctx, "data term");
+        }
     }
-    // FISTA weights (optimization.cu:1744-1746): alpha_new is a cuflt (float)
-    // there, theta the float of the double quotient; the over-relaxation is
-    // fused into the last primal-dual step
-    const float an = (float)(0.5 * (1.0 + std::sqrt(1.0 + 4.0 * std::pow(ctx->fista_alpha, 2.0))));
-    const float theta = (float)((ctx->fista_alpha - 1.0) / (double)an);
-    if (ctx->p.inner_iterations > 0) {
-        st = run_pd_steps(ctx, ctx->p.inner_iterations, ctx->G, true, theta, halt);
+    if (pd_done) {
+        if (inner > 1) {
+            st = run_pd_steps(ctx, inner - 1, ctx->G, true, theta, halt);
+            if (st != LFRV_OK) return st;
+        }
+    } else if (inner > 0) {
+        st = run_pd_steps(ctx, inner, ctx->G, true, theta, halt);
         if (st != LFRV_OK) return st;
     } else {
         k_overrelax<<<blocks_for(3 * ctx->N), kBlock, 0, ctx->stream>>>(3 * ctx->N, theta, ctx->Uq, ctx->U, halt);
@@ -828,8 +1026,16 @@ lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *out
                                                                         ctx->partials + kViewPartials, Gv, ec,
                                                                         EPSILON_TOL);
         LFRV_LAUNCH(ctx, "sr loop begin");
-        // alpha before each enqueued step, to undo the steps the device skipped
-        std::vector<double> alphas{ctx->fista_alpha};
+        // alpha and the ping-pong buffers after each enqueued step, to undo the
+        // host-side effects of the steps the device skipped
+        struct StepState {
+            double alpha;
+            float *U, *Uq, *U2, *PX, *PY, *P2X, *P2Y;
+        };
+        auto snap = [ctx]() {
+            return StepState{ctx->fista_alpha, ctx->U, ctx->Uq, ctx->U2, ctx->PX, ctx->PY, ctx->P2X, ctx->P2Y};
+        };
+        std::vector<StepState> states{snap()};
         int enq = 0, polls = 0;
         bool halted = false;
         while (enq < niter && !halted) {
@@ -838,7 +1044,7 @@ lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *out
                 st = enqueue_fista(ctx, halt);
                 if (st == LFRV_OK) st = enqueue_sr_energy(ctx, halt);
                 if (st != LFRV_OK) return st;
-                alphas.push_back(ctx->fista_alpha);
+                states.push_back(snap());
                 k_sr_loop_energy<false><<<1, kFinalizeThreads, 0, ctx->stream>>>(
                     ctx->sr_loop, ctx->partials, Gtv, ctx->partials + kViewPartials, Gv, ec, EPSILON_TOL);
                 k_count_moving<<<Gm, 256, 0, ctx->stream>>>(N, ctx->U, prev, MAX_DIFF_THRESHOLD, ctx->partials, halt);
@@ -862,14 +1068,15 @@ lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *out
         LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
         const SrLoop L = ctx->sr_loop_host[2];
         if (L.steps < 0 || L.steps > enq) return fail(ctx, LFRV_E_STATE, "FISTA loop state out of range");
-        // the skipped steps swapped the primal-dual buffers on the host only
-        const int skipped = enq - L.steps;
-        if (ctx->p.inner_iterations > 0 && ((int64_t)skipped * ctx->p.inner_iterations) % 2 != 0) {
-            std::swap(ctx->Uq, ctx->U2);
-            std::swap(ctx->PX, ctx->P2X);
-            std::swap(ctx->PY, ctx->P2Y);
-        }
-        ctx->fista_alpha = alphas[L.steps];
+        const StepState &ss = states[L.steps];
+        ctx->fista_alpha = ss.alpha;
+        ctx->U = ss.U;
+        ctx->Uq = ss.Uq;
+        ctx->U2 = ss.U2;
+        ctx->PX = ss.PX;
+        ctx->PY = ss.PY;
+        ctx->P2X = ss.P2X;
+        ctx->P2Y = ss.P2Y;
         energy = L.energy;
         s.fista_steps += L.steps;
         s.last_stop = L.stop;

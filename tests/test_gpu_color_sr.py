@@ -24,11 +24,15 @@ def _scene(dsf=2, kind=1, w=48):
     return lfrv.synth(d)
 
 
-@pytest.mark.parametrize("dsf,beta,gw", [(2, 0.0, 0), (2, 0.5, 1), (3, 0.3, 0), (1, 0.5, 1)])
-def test_sr_steps_match_oracle(dsf, beta, gw):
+@pytest.mark.parametrize("dsf,beta,gw,inner", [(2, 0.0, 0, 10), (2, 0.5, 1, 10), (3, 0.3, 0, 10), (1, 0.5, 1, 10),
+                                              (2, 0.5, 0, 1), (4, 0.3, 0, 1), (4, 0.0, 1, 2), (3, 0.3, 0, 0)])
+def test_sr_steps_match_oracle(dsf, beta, gw, inner):
+    """inner = 1: for dsf >= 2 the data term and the only primal-dual iteration
+    (with the over-relaxation) run as one kernel; inner > 1 fuses the first
+    iteration; inner = 0 leaves the data term alone."""
     s = _scene(dsf=dsf, kind=2 if dsf == 1 else 1)
     V, h, w, H, W = s["V"], s["h"], s["w"], s["H"], s["W"]
-    sig, lam, lmf, inner = 1.0 / 255.0, 0.05, 100.0, 10
+    sig, lam, lmf = 1.0 / 255.0, 0.05, 100.0
     prm = pipelines.color_sr_params(s, lambda_=lam, lambda_max_factor=lmf, inner_iterations=inner, dt_beta=beta,
                                     gw_type=gw, gradient_step=1.0)
     with lfrv.Context(prm) as ctx:
@@ -78,7 +82,9 @@ def test_sr_steps_match_oracle(dsf, beta, gw):
                 oracle.fista_step(W, H, dsf, tx, ty, s["views"], om, sig, 1.0, beta, 1.0, gw, lam, inner, g, st)
             U = ctx.get_solution()
             assert np.max(np.abs(U - st.U)) < 2e-4, np.max(np.abs(U - st.U))
+            assert np.max(np.abs(ctx.get_buffer("UQ").reshape(3, H, W) - st.Uq)) < 4e-4
             assert np.max(np.abs(ctx.get_buffer("PX").reshape(3, H, W) - st.PX)) < 2e-4
+            assert np.max(np.abs(ctx.get_buffer("PY").reshape(3, H, W) - st.PY)) < 2e-4
             assert rel_err(ctx.get_buffer("NORM_MASK").reshape(H, W), st.norm_mask) < 1e-5
 
 
