@@ -254,7 +254,8 @@ struct StDims {
     static constexpr int P = SW;                            // the common row pitch
     static constexpr int A1 = 3 * SH * P + 3 * LH * P, A2 = 6 * QH * P + 6 * QH * TW;
     static constexpr int AREA = A1 > A2 ? A1 : A2;  // sa: U and lap U, later the anchor residuals and their row sums
-    static constexpr int BAREA = 6 * BH * P;        // sb: BU (3) and BL (3)
+    static constexpr int HR = BH + D - 1;           // rows of the horizontal box sums
+    static constexpr int BAREA = 6 * HR * P;        // sb: row sums of U, lap U, then BU (3) and BL (3)
     static constexpr int NQP = QH * P;              // anchor slots (qx < QW used)
     static constexpr int NQT = (NQP + NT - 1) / NT, NST = (SH * P + NT - 1) / NT;
 };
@@ -374,20 +375,50 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
         }
         __syncthreads();
     }
-    for (int i = tid; i < BH * P; i += NT) {
+    // D x D box sums, separably: the row sums of U (rows 1.. of the U tile) and
+    // lap U (rows 0..) into sb, then the column sums in registers, written
+    // over them once every thread has read its column
+    constexpr int HR = S::HR, NBT = (BH * P + NT - 1) / NT;
+    float *hsu = sb, *hsl = sb + 3 * HR * P;
+    for (int i = tid; i < HR * P; i += NT) {
         if (i % P >= BW) continue;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             float su = 0.0f, sl = 0.0f;
-#pragma unroll 1
-            for (int l = 0; l < D; l++)
 #pragma unroll
-                for (int k = 0; k < D; k++) {
-                    su += us[c * SH * P + i + (l + 1) * P + k + 1];
-                    if (grad) sl += ls[c * LH * P + i + l * P + k];
-                }
-            bus[c * BH * P + i] = su;
-            bls[c * BH * P + i] = sl;
+            for (int k = 0; k < D; k++) {
+                su += us[c * SH * P + i + P + k + 1];
+                if (grad) sl += ls[c * LH * P + i + k];
+            }
+            hsu[c * HR * P + i] = su;
+            if (grad) hsl[c * HR * P + i] = sl;
+        }
+    }
+    __syncthreads();
+    float bsum[NBT][6];
+#pragma unroll
+    for (int k = 0; k < NBT; k++) {
+        const int i = tid + k * NT;
+#pragma unroll
+        for (int c = 0; c < 6; c++) bsum[k][c] = 0.0f;
+        if (i >= BH * P || i % P >= BW) continue;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int l = 0; l < D; l++) {
+                bsum[k][c] += hsu[c * HR * P + i + l * P];
+                if (grad) bsum[k][3 + c] += hsl[c * HR * P + i + l * P];
+            }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NBT; k++) {
+        const int i = tid + k * NT;
+        if (i >= BH * P || i % P >= BW) continue;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            bus[c * BH * P + i] = bsum[k][c];
+            bls[c * BH * P + i] = bsum[k][3 + c];
         }
     }
     __syncthreads();
