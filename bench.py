@@ -30,14 +30,15 @@ from lfrv import pipelines  # noqa: E402
 METRIC = "ms per novel view to rel. residual 1e-5 (9 views, 1024²); achieved HBM GB/s"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
 TOL = 1e-5
-# rocprofv3 summary of the roofline kernel on this workload; it records the
-# HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE (copy-calibrated)
+# rocprofv3 summaries of the roofline kernels; each records the HBM bytes per
+# launch measured with FETCH_SIZE / WRITE_SIZE (copy-calibrated)
 PROFILE = os.path.join(ROOT, "profiles", "k_cgcg_st2_gradient_ibr_9views_1024.md")
+PROFILE_C3 = os.path.join(ROOT, "profiles", "k_sr_step_st_color_sr_25views_2048.md")
 
 
-def measured_traffic():
+def measured_traffic(path=PROFILE):
     try:
-        m = re.search(r"^traffic_bytes_per_launch: (\d+)", open(PROFILE).read(), re.M)
+        m = re.search(r"^traffic_bytes_per_launch: (\d+)", open(path).read(), re.M)
         return int(m.group(1)) if m else None
     except OSError:
         return None
@@ -207,8 +208,7 @@ def bench_color_sr(args, workload):
             irls.append(ctx.sr_solve(steps, nmeta))
         else:
             ctx.sr_init_regularizer_weight()
-            for _ in range(steps):
-                ctx.sr_iteration_fista()
+            ctx.sr_iterate(steps)
         ctx.sync()
 
     for _ in range(args.warmup):
@@ -313,7 +313,9 @@ def bench_color_sr(args, workload):
                    "seed": args.seed},
         "roofline": {"bound": "hbm", "kernel": dom,
                      "achieved": k["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(k["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(k["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(PROFILE_C3) if (workload == "color_sr_c3" and
+                                                                 dom == f"k_sr_step_st<{ks}>") else None,
                      "bytes_per_launch": k["bytes_per_launch"], "avg_launch_us": k["avg_launch_us"],
                      "launches": k["launches"]},
         "ms_per_step_median": round(float(np.median(per_view)), 3),

@@ -229,6 +229,11 @@ lfrv_status lfrv_sr_init_regularizer_weight(lfrv_ctx *ctx);
  * vtv_sr_ista_step_unstructured (:1751-1832). */
 lfrv_status lfrv_sr_iteration_fista(lfrv_ctx *ctx);
 
+/* `steps` outer steps in one call (queued back to back on the stream, no host
+ * wait): the loop body of super-resolution.cpp:218-227 without its energy and
+ * moving-pixel checks.  Same results as `steps` lfrv_sr_iteration_fista calls. */
+lfrv_status lfrv_sr_iterate(lfrv_ctx *ctx, int steps);
+
 /* Primal energy in FP64; parts (nullable) receives E_TV, E_int, E_grad.
  * Replaces coco_vtv_sr_primal_energy_unstructured (optimization.cu:1477-1717). */
 lfrv_status lfrv_sr_primal_energy(lfrv_ctx *ctx, double *energy, double *parts);

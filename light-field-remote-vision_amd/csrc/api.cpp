@@ -505,6 +505,19 @@ extern "C" lfrv_status lfrv_sr_iteration_fista(lfrv_ctx *ctx)
     return run_fista(ctx);
 }
 
+extern "C" lfrv_status lfrv_sr_iterate(lfrv_ctx *ctx, int steps)
+{
+    if (!ctx) return LFRV_E_ARG;
+    if (ctx->p.mode != LFRV_MODE_COLOR_SR) return fail(ctx, LFRV_E_STATE, "FISTA steps need LFRV_MODE_COLOR_SR");
+    if (steps < 0) return fail(ctx, LFRV_E_ARG, "negative step count");
+    ctx->u_zero = false;
+    for (int k = 0; k < steps; k++) {
+        const lfrv_status st = run_fista(ctx);
+        if (st != LFRV_OK) return st;
+    }
+    return LFRV_OK;
+}
+
 extern "C" lfrv_status lfrv_sr_primal_energy(lfrv_ctx *ctx, double *energy, double *parts)
 {
     if (!ctx || !energy) return LFRV_E_ARG;

@@ -122,6 +122,7 @@ def lib():
         "lfrv_gibr_energy": (st, [_vp, _dp]),
         "lfrv_sr_init_regularizer_weight": (st, [_vp]),
         "lfrv_sr_iteration_fista": (st, [_vp]),
+        "lfrv_sr_iterate": (st, [_vp, st]),
         "lfrv_sr_primal_energy": (st, [_vp, _dp, _dp]),
         "lfrv_sr_count_moving": (st, [_vp, _vp, ctypes.c_float, ctypes.POINTER(ctypes.c_int64)]),
         "lfrv_sr_solve": (st, [_vp, st, st, ctypes.POINTER(SrStats)]),
@@ -280,6 +281,10 @@ class Context:
         m = ctypes.c_int64()
         self._check(lib().lfrv_sr_count_moving(self._h, _ptr(_f32(prev)), max_diff, ctypes.byref(m)))
         return m.value
+
+    def sr_iterate(self, steps):
+        """`steps` FISTA steps queued in one call (lfrv_sr_iterate)."""
+        self._check(lib().lfrv_sr_iterate(self._h, steps))
 
     def sr_solve(self, niter, nmeta):
         """The re-weighted FISTA loop with the driver's stopping rules on the device (lfrv_sr_solve)."""

@@ -101,3 +101,25 @@ def test_sr_count_moving():
         prev.reshape(3, -1)[1, idx] += 2e-3
         prev.reshape(3, -1)[0, idx[:10]] += 5e-4
         assert ctx.sr_count_moving(prev, 1e-3) == 100
+
+
+@pytest.mark.parametrize("dsf,inner", [(2, 1), (1, 3)])
+def test_sr_iterate_equals_single_steps(dsf, inner):
+    """lfrv_sr_iterate(k) gives the solution and duals of k lfrv_sr_iteration_fista calls, bit for bit."""
+    s = _scene(dsf=dsf, kind=2 if dsf == 1 else 1)
+    prm = pipelines.color_sr_params(s, inner_iterations=inner, dt_beta=0.3, gradient_step=1.0)
+    out = []
+    for batched in (False, True):
+        with lfrv.Context(prm) as ctx:
+            pipelines.load_scene(ctx, s)
+            ctx.clear_solution()
+            pipelines.color_sr_setup(ctx)
+            ctx.sr_init_regularizer_weight()
+            if batched:
+                ctx.sr_iterate(7)
+            else:
+                for _ in range(7):
+                    ctx.sr_iteration_fista()
+            out.append((ctx.get_solution(), ctx.get_buffer("PX"), ctx.get_buffer("UQ")))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
