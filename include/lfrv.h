@@ -121,6 +121,12 @@ lfrv_status lfrv_set_depth(lfrv_ctx *ctx, int i, const float *depth, const doubl
 /* 3x4 row-major projection matrix of the target camera (import.cpp:255-260). */
 lfrv_status lfrv_set_target_camera(lfrv_ctx *ctx, const double *P);
 
+/* Depth gradients of view i (dz/dx, dz/dy per pixel, w*h each; both NULL: zero),
+ * the two normal-map channels the OpenGL producer reads with each depth map
+ * (InputView::setDepthAndNormals, view.cpp:608-636; zero when no normals,
+ * initDepthAndNormals :639-661).  Only lfrv_warps_from_target_depth uses them. */
+lfrv_status lfrv_set_depth_gradient(lfrv_ctx *ctx, int i, const float *dzdx, const float *dzdy);
+
 /* ---- warp and splat structure -------------------------------------------- */
 
 /* tau and sigma_z*dtau/dz of every view from its depth map and the cameras,
@@ -128,6 +134,23 @@ lfrv_status lfrv_set_target_camera(lfrv_ctx *ctx, const double *P);
  * (import.cpp:243-450) plus the partial-tau producer (import.cpp:658-680,
  * tauPartial.frag:23-47); runs on the GPU. */
 lfrv_status lfrv_warps_from_depth(lfrv_ctx *ctx);
+
+/* The OpenGL warp producer (the reference's fromMVE2coco path, dsf = 1 there),
+ * on the GPU: Scene::computeTargetDepthMap (scene.cpp:1506-1544) splats every
+ * view but skip_view (-1: none) as one quad per pixel of valid depth (z < 500),
+ * corners at the pixel centre +- 0.5 at depths z +- 0.5 dz/dx +- 0.5 dz/dy
+ * (depthSplatting.geom:21-61), into the target z-buffer; each covered target
+ * pixel centre keeps the target depth of the nearest quad's centre
+ * (depthSplatting.frag), 0 elsewhere (LFRV_BUF_TARGET_DEPTH).  Then
+ * InputView::exportWarp (view.cpp:663-721) for every view: tau = x_u, or -2
+ * outside (0, W) x (0, H) or at invalid depth, or -1 where the target depth map
+ * (bilinear, clamped) is nearer by more than 0.1 (tauWarp.frag:25-74); the
+ * deformation weight (LFRV_BUF_TAU_WEIGHT) and sigma_z dtau/dz
+ * (tauPartial.frag:23-47).  Rasterisation: pixel centres inside either
+ * triangle of the strip, edges owned by one triangle, depth test on 1/z
+ * interpolated in the image; quads spanning more than 64 target pixels or
+ * crossing the near plane z = 0.01 are skipped (GL would clip them). */
+lfrv_status lfrv_warps_from_target_depth(lfrv_ctx *ctx, int skip_view);
 
 /* Build the splat structure from the current warps: bins every cell-valid
  * source pixel of every view by target tile, computes the target coverage
@@ -293,7 +316,9 @@ typedef enum {
     LFRV_BUF_VISIBILITY = 16,  /* W*H, 0/1 as float */
     LFRV_BUF_TARGET_DEPTH = 17,/* W*H z-buffer of lfrv_warps_from_depth */
     LFRV_BUF_CG_B = 18,      /* 3*W*H right-hand side of the last CG solve */
-    LFRV_BUF_CG_MINV = 19    /* W*H Jacobi preconditioner of the last CG solve */
+    LFRV_BUF_CG_MINV = 19,   /* W*H Jacobi preconditioner of the last CG solve */
+    LFRV_BUF_TAU_WEIGHT = 20 /* V*w*h deformation weights of lfrv_warps_from_target_depth (the B channel of
+                                the tau PFMs InputView::exportWarp writes) */
 } lfrv_buffer;
 
 /* Copy an internal buffer to `out` (host or device, float); *count receives

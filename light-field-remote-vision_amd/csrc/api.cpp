@@ -261,7 +261,8 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
              ctx->cg_d, ctx->cg_q, ctx->cg_b, ctx->reg_g, ctx->norm_mask, ctx->coverage, ctx->u_grad_x,
              ctx->u_grad_y, ctx->target_mask, ctx->visibility, ctx->target_depth, ctx->tmpN, ctx->cg_minv,
              ctx->hole_list, ctx->pp_scratch, ctx->cg_state, ctx->counters, ctx->dcams, ctx->scal, ctx->partials,
-             ctx->cache_num, ctx->cache_sq, ctx->gram, ctx->cg_extra, ctx->sr_loop, ctx->sr_prev})
+             ctx->cache_num, ctx->cache_sq, ctx->gram, ctx->cg_extra, ctx->sr_loop, ctx->sr_prev, ctx->depth_grad,
+             ctx->tau_weight})
         if (p) (void)hipFree(p);
     if (ctx->cg_state_host) (void)hipHostFree(ctx->cg_state_host);
     if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
@@ -286,6 +287,12 @@ extern "C" lfrv_status lfrv_sync(lfrv_ctx *ctx)
 }
 
 // ---- inputs -------------------------------------------------------------------
+
+__global__ static void k_zero(size_t n, float *__restrict__ p)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0.0f;
+}
 
 static lfrv_status copy_in(lfrv_ctx *ctx, float *dst, const float *src, size_t count)
 {
@@ -349,6 +356,29 @@ extern "C" lfrv_status lfrv_set_depth(lfrv_ctx *ctx, int i, const float *depth, 
     return LFRV_OK;
 }
 
+extern "C" lfrv_status lfrv_set_depth_gradient(lfrv_ctx *ctx, int i, const float *dzdx, const float *dzdy)
+{
+    if (!ctx || i < 0 || i >= ctx->V) return ctx ? fail(ctx, LFRV_E_ARG, "bad lfrv_set_depth_gradient arguments") : LFRV_E_ARG;
+    if ((dzdx == nullptr) != (dzdy == nullptr)) return fail(ctx, LFRV_E_ARG, "lfrv_set_depth_gradient: both planes or neither");
+    const size_t Vn = (size_t)ctx->V * ctx->n;
+    if (!ctx->depth_grad) {
+        if (!dzdx) return LFRV_OK;  // no gradients anywhere: the kernels take them as zero
+        LFRV_HIP(ctx, hipMalloc((void **)&ctx->depth_grad, 2 * Vn * sizeof(float)));
+        k_zero<<<blocks_for(2 * Vn), kBlock, 0, ctx->stream>>>(2 * Vn, ctx->depth_grad);
+        LFRV_LAUNCH(ctx, "clear depth gradients");
+    }
+    float *gx = ctx->depth_grad + (size_t)i * ctx->n, *gy = gx + Vn;
+    if (!dzdx) {
+        k_zero<<<blocks_for(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, gx);
+        k_zero<<<blocks_for(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, gy);
+        LFRV_LAUNCH(ctx, "clear depth gradients");
+        return LFRV_OK;
+    }
+    lfrv_status st = copy_in(ctx, gx, dzdx, ctx->n);
+    if (st == LFRV_OK) st = copy_in(ctx, gy, dzdy, ctx->n);
+    return st;
+}
+
 extern "C" lfrv_status lfrv_set_target_camera(lfrv_ctx *ctx, const double *P)
 {
     if (!ctx || !P) return LFRV_E_ARG;
@@ -364,6 +394,13 @@ extern "C" lfrv_status lfrv_warps_from_depth(lfrv_ctx *ctx)
 {
     if (!ctx) return LFRV_E_ARG;
     return run_warps_from_depth(ctx);
+}
+
+extern "C" lfrv_status lfrv_warps_from_target_depth(lfrv_ctx *ctx, int skip_view)
+{
+    if (!ctx) return LFRV_E_ARG;
+    if (skip_view < -1 || skip_view >= ctx->V) return fail(ctx, LFRV_E_ARG, "lfrv_warps_from_target_depth: bad view");
+    return run_warps_from_target_depth(ctx, skip_view);
 }
 
 extern "C" lfrv_status lfrv_prepare(lfrv_ctx *ctx)
@@ -416,12 +453,6 @@ extern "C" lfrv_status lfrv_set_solution(lfrv_ctx *ctx, const float *rgb)
     if (!ctx) return LFRV_E_ARG;
     ctx->u_zero = false;
     return copy_in(ctx, ctx->U, rgb, 3 * ctx->N);
-}
-
-__global__ static void k_zero(size_t n, float *__restrict__ p)
-{
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) p[k] = 0.0f;
 }
 
 extern "C" lfrv_status lfrv_clear_solution(lfrv_ctx *ctx)
@@ -570,6 +601,7 @@ static float *buffer_ptr(lfrv_ctx *ctx, int which, size_t *count)
     case LFRV_BUF_TARGET_DEPTH: *count = N; return ctx->target_depth;
     case LFRV_BUF_CG_B: *count = 3 * N; return ctx->cg_b;
     case LFRV_BUF_CG_MINV: *count = N; return ctx->cg_minv;
+    case LFRV_BUF_TAU_WEIGHT: *count = ctx->tau_weight ? Vn : 0; return ctx->tau_weight;
     default: *count = 0; return nullptr;
     }
 }

@@ -146,6 +146,182 @@ __global__ void k_tau_visibility(int V, int w, int h, const float *__restrict__ 
     if (cell_valid(tux, tuy, W, H)) atomicAdd(cell_count + cell_of<KS>(tux, tuy, W + KS - 1), 1);
 }
 
+// ---- the OpenGL warp producer (Scene::computeTargetDepthMap + exportWarps) ----
+
+constexpr float kGlZNear = 1e-2f;  // near plane of the render camera, pinholeCamera.cpp:133
+constexpr int kGlMaxSpan = 64;     // quads wider or taller than this many target pixels are skipped
+
+__device__ __forceinline__ float gl_edge(const float *a, const float *b, float px, float py)
+{
+    return (b[0] - a[0]) * (py - a[1]) - (b[1] - a[1]) * (px - a[0]);
+}
+
+// A pixel centre on an edge belongs to the triangle owning the edge; of the two
+// directions a shared edge is walked in, exactly one owns it.
+__device__ __forceinline__ bool gl_owns(const float *a, const float *b)
+{
+    const float dx = b[0] - a[0], dy = b[1] - a[1];
+    return dy > 0.0f || (dy == 0.0f && dx < 0.0f);
+}
+
+// Triangle (a, b, c) = (x, y, depth) at pixel centre p: inside, and the depth
+// as the depth test interpolates it (1/z linear in the image).
+__device__ __forceinline__ bool gl_tri(const float *a, const float *b, const float *c, float px, float py, float &zi)
+{
+    float area = gl_edge(a, b, c[0], c[1]);
+    if (area == 0.0f) return false;
+    if (area < 0.0f) {
+        const float *t = b;
+        b = c;
+        c = t;
+        area = -area;
+    }
+    const float ea = gl_edge(b, c, px, py), eb = gl_edge(c, a, px, py), ec = gl_edge(a, b, px, py);
+    if (ea < 0.0f || (ea == 0.0f && !gl_owns(b, c))) return false;
+    if (eb < 0.0f || (eb == 0.0f && !gl_owns(c, a))) return false;
+    if (ec < 0.0f || (ec == 0.0f && !gl_owns(a, b))) return false;
+    zi = 1.0f / ((ea / area) / a[2] + (eb / area) / b[2] + (ec / area) / c[2]);
+    return true;
+}
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, size_t n, unsigned long long v)
+{
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o < n) p[o] = v;
+}
+
+// depthSplatting.vert / .geom / .frag (scene.cpp:1506-1544): every pixel of
+// valid depth of every view but `skip` is a quad, corners at the pixel centre
+// +- 0.5 with depths z +- 0.5 dz/dx +- 0.5 dz/dy, drawn as the strip BL, TL,
+// BR, TR into the target.  The depth test keeps the nearest quad at each
+// covered pixel centre: a 64-bit atomicMin of (interpolated depth, target
+// depth of the quad's centre).
+__global__ void k_gl_splat(int V, int w, int h, int W, int H, int skip, const float *__restrict__ depth,
+                           const float *__restrict__ grad, const WarpCam *__restrict__ cams,
+                           unsigned long long *__restrict__ key)
+{
+    SOURCE_INDEX()
+    if (v == skip) return;
+    const float z = depth[k];
+    if (z >= 0.5f * kInvalidDepth) return;
+    const float gx = grad ? grad[k] : 0.0f, gy = grad ? grad[(size_t)V * n + k] : 0.0f;
+    const WarpCam &cam = cams[v];
+    const float x = (float)mx + 0.5f, y = (float)my + 0.5f;
+    float q[4][3];
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float sx = c < 2 ? -0.5f : 0.5f, sy = (c & 1) ? 0.5f : -0.5f;  // BL, TL, BR, TR
+        const float zc = z + sx * gx + sy * gy, xc = x + sx, yc = y + sy;
+        const float h0 = zc * (cam.M[0] * xc + cam.M[1] * yc + cam.M[2]) + cam.c[0];
+        const float h1 = zc * (cam.M[3] * xc + cam.M[4] * yc + cam.M[5]) + cam.c[1];
+        const float h2 = zc * (cam.M[6] * xc + cam.M[7] * yc + cam.M[8]) + cam.c[2];
+        ok = ok && h2 > kGlZNear;
+        q[c][0] = h0 / h2;
+        q[c][1] = h1 / h2;
+        q[c][2] = h2;
+    }
+    if (!ok) return;
+    const float zval = z * (cam.M[6] * x + cam.M[7] * y + cam.M[8]) + cam.c[2];
+    float u0 = q[0][0], u1 = q[0][0], v0 = q[0][1], v1 = q[0][1];
+#pragma unroll
+    for (int c = 1; c < 4; c++) {
+        u0 = fminf(u0, q[c][0]);
+        u1 = fmaxf(u1, q[c][0]);
+        v0 = fminf(v0, q[c][1]);
+        v1 = fmaxf(v1, q[c][1]);
+    }
+    // the span test in float first: a quad near the camera plane can project far
+    // beyond the int range
+    if (!(u1 - u0 <= (float)kGlMaxSpan && v1 - v0 <= (float)kGlMaxSpan)) return;
+    if (u1 < 0.0f || v1 < 0.0f || u0 > (float)W || v0 > (float)H) return;
+    int px0 = (int)ceilf(u0 - 0.5f), px1 = (int)floorf(u1 - 0.5f);
+    int py0 = (int)ceilf(v0 - 0.5f), py1 = (int)floorf(v1 - 0.5f);
+    px0 = max(px0, 0);
+    py0 = max(py0, 0);
+    px1 = min(px1, W - 1);
+    py1 = min(py1, H - 1);
+    const unsigned long long lo = __float_as_uint(zval);
+    for (int py = py0; py <= py1; py++)
+        for (int px = px0; px <= px1; px++) {
+            float zi;
+            const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+            if (!gl_tri(q[0], q[1], q[2], cx, cy, zi) && !gl_tri(q[2], q[1], q[3], cx, cy, zi)) continue;
+            atomicMin(key + (size_t)py * W + px, ((unsigned long long)__float_as_uint(zi) << 32) | lo);
+        }
+}
+
+// The target depth map: the winning quad's centre depth, 0 where no quad
+// landed (the colour clear of scene.cpp:1515-1517).
+__global__ void k_gl_depth(const unsigned long long *__restrict__ key, float *__restrict__ depth, size_t N)
+{
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= N) return;
+    const unsigned long long kv = key[o];
+    depth[o] = kv == ~0ull ? 0.0f : __uint_as_float((unsigned int)(kv & 0xffffffffull));
+}
+
+// tauWarp.frag:25-74 and tauPartial.frag:23-47 for every pixel of every view:
+// tau = x_u, or (-2, -2) outside (0, W) x (0, H) or at invalid depth, or
+// (-1, -1) when the target depth map (bilinear, clamped edges, texel centres
+// at +0.5) is nearer than the point by more than 0.1; wdef the deformation
+// weight 1 / |det dtau/dx| capped at 30 (0 when invalid); dpart = sigma_z dtau/dz.
+__global__ void k_gl_tau(int V, int w, int h, int W, int H, const float *__restrict__ depth,
+                         const float *__restrict__ grad, const WarpCam *__restrict__ cams,
+                         const float *__restrict__ ud, float sigma_z, float *__restrict__ tx,
+                         float *__restrict__ ty, float *__restrict__ wdef, float *__restrict__ dpx,
+                         float *__restrict__ dpy)
+{
+    SOURCE_INDEX()
+    const WarpCam &cam = cams[v];
+    const float z = depth[k];
+    const float gx = grad ? grad[k] : 0.0f, gy = grad ? grad[(size_t)V * n + k] : 0.0f;
+    const float x = (float)mx + 0.5f, y = (float)my + 0.5f;
+    float Mx[3], hh[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        Mx[i] = cam.M[3 * i] * x + cam.M[3 * i + 1] * y + cam.M[3 * i + 2];
+        hh[i] = z * Mx[i] + cam.c[i];
+    }
+    float u = 0.0f, vv = 0.0f;
+    if (hh[2] != 0.0f) {
+        u = hh[0] / hh[2];
+        vv = hh[1] / hh[2];
+    }
+    dpx[k] = sigma_z * (Mx[0] - u * Mx[2]) / hh[2];
+    dpy[k] = sigma_z * (Mx[1] - vv * Mx[2]) / hh[2];
+    const float t0[3] = {x * gx + z, y * gx, gx}, t1[3] = {x * gy, y * gy + z, gy};
+    float a0[3], a1[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        a0[i] = cam.M[3 * i] * t0[0] + cam.M[3 * i + 1] * t0[1] + cam.M[3 * i + 2] * t0[2];
+        a1[i] = cam.M[3 * i] * t1[0] + cam.M[3 * i + 1] * t1[1] + cam.M[3 * i + 2] * t1[2];
+    }
+    const float j00 = (a0[0] - u * a0[2]) / hh[2], j10 = (a0[1] - vv * a0[2]) / hh[2];
+    const float j01 = (a1[0] - u * a1[2]) / hh[2], j11 = (a1[1] - vv * a1[2]) / hh[2];
+    const float wd = fminf(1.0f / fabsf(j00 * j11 - j10 * j01), 30.0f);
+    if (u <= 0.0f || (float)W <= u || vv <= 0.0f || (float)H <= vv || z >= 0.5f * kInvalidDepth) {
+        tx[k] = ty[k] = -2.0f;
+        wdef[k] = 0.0f;
+        return;
+    }
+    const float s = u - 0.5f, t = vv - 0.5f;
+    const int i0 = (int)floorf(s), j0 = (int)floorf(t);
+    const float fx = s - (float)i0, fy = t - (float)j0;
+    const int xa = min(max(i0, 0), W - 1), xb = min(max(i0 + 1, 0), W - 1);
+    const int ya = min(max(j0, 0), H - 1), yb = min(max(j0 + 1, 0), H - 1);
+    const float du = (1.0f - fy) * ((1.0f - fx) * ud[(size_t)ya * W + xa] + fx * ud[(size_t)ya * W + xb]) +
+                     fy * ((1.0f - fx) * ud[(size_t)yb * W + xa] + fx * ud[(size_t)yb * W + xb]);
+    if (hh[2] > du + 0.1f) {
+        tx[k] = ty[k] = -1.0f;
+        wdef[k] = 0.0f;
+        return;
+    }
+    tx[k] = u;
+    ty[k] = vv;
+    wdef[k] = wd;
+}
+
 // v = S u (gradientIBR_kernels.cuh:196-290), C <= 4 planes.
 template <int KS>
 __global__ void k_gather(int W, int H, int w, int h, const float *__restrict__ tx,
@@ -421,13 +597,10 @@ static void kinv_ref(const double *K, double *Ki)
     Ki[8] = 1.0;
 }
 
-lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
+// Camera algebra of the warps in FP64 on the host (import.cpp:301-320,
+// 412-430), uploaded to ctx->dcams; redone only when a camera changed.
+static lfrv_status upload_warp_cams(lfrv_ctx *ctx)
 {
-    if (!ctx->have_target_cam) return fail(ctx, LFRV_E_STATE, "lfrv_warps_from_depth: target camera not set");
-    for (int v = 0; v < ctx->V; v++)
-        if (!ctx->have_depth[v]) return fail(ctx, LFRV_E_STATE, "lfrv_warps_from_depth: depth of a view not set");
-    // camera algebra in FP64 on the host (import.cpp:301-320, 412-430), redone
-    // only when a camera changed
     if (!ctx->cams_valid) {
         std::vector<WarpCam> hc(ctx->V);
         double Ku[9], Ru[9], tu[3], Cu[3], Kui[9];
@@ -466,6 +639,22 @@ lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
                                      ctx->stream));
         ctx->cams_valid = true;
     }
+    return LFRV_OK;
+}
+
+static lfrv_status check_depth_inputs(lfrv_ctx *ctx, const char *who)
+{
+    if (!ctx->have_target_cam) return fail(ctx, LFRV_E_STATE, std::string(who) + ": target camera not set");
+    for (int v = 0; v < ctx->V; v++)
+        if (!ctx->have_depth[v]) return fail(ctx, LFRV_E_STATE, std::string(who) + ": depth of a view not set");
+    return LFRV_OK;
+}
+
+lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
+{
+    lfrv_status st = check_depth_inputs(ctx, "lfrv_warps_from_depth");
+    if (st == LFRV_OK) st = upload_warp_cams(ctx);
+    if (st != LFRV_OK) return st;
     const WarpCam *dcams = ctx->dcams;
     int *zkey = reinterpret_cast<int *>(ctx->tmpN);
     TimedRegion tr(ctx, 4);
@@ -494,6 +683,29 @@ lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
     ctx->prepared = false;
     ctx->epoch++;
     ctx->counts_valid = true;
+    return LFRV_OK;
+}
+
+lfrv_status run_warps_from_target_depth(lfrv_ctx *ctx, int skip)
+{
+    lfrv_status st = check_depth_inputs(ctx, "lfrv_warps_from_target_depth");
+    if (st == LFRV_OK) st = upload_warp_cams(ctx);
+    if (st != LFRV_OK) return st;
+    if (!ctx->tau_weight) LFRV_HIP(ctx, hipMalloc((void **)&ctx->tau_weight, (size_t)ctx->V * ctx->n * sizeof(float)));
+    // the depth-test keys in the 3N scratch (8 bytes per target pixel)
+    unsigned long long *key = reinterpret_cast<unsigned long long *>(ctx->tmp3N);
+    TimedRegion tr(ctx, 4);
+    k_fill_u64<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(key, ctx->N, ~0ull);
+    k_gl_splat<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, skip,
+                                                                   ctx->depth, ctx->depth_grad, ctx->dcams, key);
+    k_gl_depth<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(key, ctx->target_depth, ctx->N);
+    k_gl_tau<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
+        ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, ctx->depth, ctx->depth_grad, ctx->dcams, ctx->target_depth,
+        ctx->p.sigma_z, ctx->tau_x, ctx->tau_y, ctx->tau_weight, ctx->dpart_x, ctx->dpart_y);
+    LFRV_LAUNCH(ctx, "warps_from_target_depth kernels");
+    ctx->prepared = false;
+    ctx->epoch++;
+    ctx->counts_valid = false;  // the splat structure counts its cells itself
     return LFRV_OK;
 }
 

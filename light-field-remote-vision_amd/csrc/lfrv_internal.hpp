@@ -267,6 +267,8 @@ struct lfrv_ctx {
     float *dpart_x = nullptr, *dpart_y = nullptr;  // V*n
     float *omega = nullptr;    // V*n
     float *depth = nullptr;    // V*n
+    float *depth_grad = nullptr;  // 2*V*n dz/dx then dz/dy (lfrv_set_depth_gradient; null: zero)
+    float *tau_weight = nullptr;  // V*n deformation weights of the OpenGL warp producer (allocated on first use)
     float *view_lap = nullptr; // V*3n Laplacian of the views (IBR_color gradient term)
     float *view_grad = nullptr;// V*2n Sobel of the views' luminance
     std::vector<double> P_views;  // V*12
@@ -394,6 +396,7 @@ inline unsigned int blocks_for(size_t n, int bs = kBlock) { return (unsigned int
 
 // kernels-side entry points (defined in the .hip files)
 lfrv_status run_warps_from_depth(lfrv_ctx *ctx);
+lfrv_status run_warps_from_target_depth(lfrv_ctx *ctx, int skip);
 lfrv_status run_prepare(lfrv_ctx *ctx);
 lfrv_status run_refresh_entry_weights(lfrv_ctx *ctx, float scale);
 lfrv_status run_visibility(lfrv_ctx *ctx);

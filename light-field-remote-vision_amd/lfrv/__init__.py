@@ -22,6 +22,7 @@ STATUS_NAMES = {0: "OK", 1: "E_ARG", 2: "E_HIP", 3: "E_NONCONVERGED", 4: "E_NAN"
 BUF = dict(
     TAU_X=0, TAU_Y=1, DPART_X=2, DPART_Y=3, OMEGA=4, U=5, UQ=6, G=7, PX=8, PY=9, REG_G=10, NORM_MASK=11,
     COVERAGE=12, U_GRAD_X=13, U_GRAD_Y=14, TARGET_MASK=15, VISIBILITY=16, TARGET_DEPTH=17, CG_B=18, CG_MINV=19,
+    TAU_WEIGHT=20,
 )
 
 TIMING_CLASSES = dict(cg_op=0, cg_update=1, pd_step=2, dataterm=3, warp=4, splat=5)
@@ -108,6 +109,8 @@ def lib():
         "lfrv_set_depth": (st, [_vp, st, _vp, _dp]),
         "lfrv_set_target_camera": (st, [_vp, _dp]),
         "lfrv_warps_from_depth": (st, [_vp]),
+        "lfrv_set_depth_gradient": (st, [_vp, st, _vp, _vp]),
+        "lfrv_warps_from_target_depth": (st, [_vp, st]),
         "lfrv_prepare": (st, [_vp]),
         "lfrv_setup_visibility": (st, [_vp]),
         "lfrv_compute_weights": (st, [_vp]),
@@ -222,6 +225,17 @@ class Context:
     # steps
     def warps_from_depth(self):
         self._check(lib().lfrv_warps_from_depth(self._h))
+
+    def set_depth_gradient(self, i, dzdx=None, dzdy=None):
+        """dz/dx, dz/dy of view i for the OpenGL warp producer (None: zero)."""
+        if dzdx is None:
+            self._check(lib().lfrv_set_depth_gradient(self._h, i, None, None))
+        else:
+            self._check(lib().lfrv_set_depth_gradient(self._h, i, _ptr(_f32(dzdx)), _ptr(_f32(dzdy))))
+
+    def warps_from_target_depth(self, skip_view=-1):
+        """Target depth by depth splatting, then tau / dpart as the OpenGL exporter computes them."""
+        self._check(lib().lfrv_warps_from_target_depth(self._h, skip_view))
 
     def prepare(self):
         self._check(lib().lfrv_prepare(self._h))

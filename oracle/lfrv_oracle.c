@@ -556,6 +556,208 @@ void orc_warps_from_depth(int V, int w, int h, int dsf, const float *depth, cons
 }
 
 /* ------------------------------------------------------------------------- */
+/* The OpenGL warp producer: target depth by depth splatting, tau from it    */
+/* ------------------------------------------------------------------------- */
+
+#define ORC_GL_ZNEAR 1e-2  /* near plane of the render camera, pinholeCamera.cpp:133 */
+#define ORC_GL_MAX_SPAN 64 /* quads wider or taller than this many target pixels are skipped */
+
+/* M = P_u3 R_v^T K_v^-1 and c = P_u (C_v, 1): x_u ~ z M (x, y, 1) + c for a
+ * source pixel (x, y) at depth z (tauWarp.frag:30-32, depthSplatting.geom:34-57
+ * with depthSplatting.frag:14). */
+static void gl_cam(const double *Pv, const double *Pu, double *M, double *c)
+{
+    double Kv[9], Rv[9], tv[3], Cv[3], Kvi[9], RvT[9], Pu3[9];
+    orc_decompose_P(Pv, Kv, Rv, tv, Cv);
+    kinv_ref(Kv, Kvi);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            RvT[3 * i + j] = Rv[3 * j + i];
+            Pu3[3 * i + j] = Pu[4 * i + j];
+        }
+    mat3_mul(RvT, Kvi, M);
+    mat3_mul(Pu3, M, M);
+    for (int i = 0; i < 3; i++) c[i] = Pu[4 * i] * Cv[0] + Pu[4 * i + 1] * Cv[1] + Pu[4 * i + 2] * Cv[2] + Pu[4 * i + 3];
+}
+
+/* Edge function of a -> b at p; positive inside a positively oriented triangle. */
+static double gl_edge(const double *a, const double *b, double px, double py)
+{
+    return (b[0] - a[0]) * (py - a[1]) - (b[1] - a[1]) * (px - a[0]);
+}
+
+/* A pixel centre on an edge belongs to the triangle that owns the edge: of the
+ * two directions a shared edge is walked in, exactly one satisfies this. */
+static int gl_owns(const double *a, const double *b)
+{
+    const double dx = b[0] - a[0], dy = b[1] - a[1];
+    return dy > 0.0 || (dy == 0.0 && dx < 0.0);
+}
+
+/* Rasterises triangle (a, b, c) (x, y, depth) at pixel centre p: 1 and the
+ * depth interpolated as the depth test sees it (1/z linear in the image) when
+ * p is inside. */
+static int gl_tri(const double *a, const double *b, const double *c, double px, double py, double *zi)
+{
+    double area = gl_edge(a, b, c[0], c[1]);
+    if (area == 0.0) return 0;
+    if (area < 0.0) {
+        const double *t = b;
+        b = c;
+        c = t;
+        area = -area;
+    }
+    const double ea = gl_edge(b, c, px, py), eb = gl_edge(c, a, px, py), ec = gl_edge(a, b, px, py);
+    if (ea < 0.0 || (ea == 0.0 && !gl_owns(b, c))) return 0;
+    if (eb < 0.0 || (eb == 0.0 && !gl_owns(c, a))) return 0;
+    if (ec < 0.0 || (ec == 0.0 && !gl_owns(a, b))) return 0;
+    const double inv = (ea / area) / a[2] + (eb / area) / b[2] + (ec / area) / c[2];
+    *zi = 1.0 / inv;
+    return 1;
+}
+
+/* Scene::computeTargetDepthMap (scene.cpp:1506-1544; depthSplatting.vert,
+ * .geom, .frag): every view but `skip` is drawn as one quad per pixel of valid
+ * depth (z < INVALID_DEPTH / 2), corners at the pixel centre +- 0.5 with depths
+ * z +- 0.5 dz/dx +- 0.5 dz/dy, as the triangle strip BL, TL, BR, TR.  At each
+ * covered target pixel centre the nearest quad wins the depth test and writes
+ * the target depth of its centre; uncovered pixels keep the cleared 0
+ * (glClearColor 0, scene.cpp:1515-1517).  dzdx / dzdy may be NULL (zero). */
+void orc_gl_target_depth(int V, int w, int h, int W, int H, const float *depth, const float *dzdx,
+                         const float *dzdy, const double *P, const double *Pu, int skip, float *out)
+{
+    const size_t n = (size_t)w * h, N = (size_t)W * H;
+    double *zb = (double *)malloc(N * sizeof(double)), *val = (double *)malloc(N * sizeof(double));
+    for (size_t p = 0; p < N; p++) {
+        zb[p] = INFINITY;
+        val[p] = 0.0;
+    }
+    for (int v = 0; v < V; v++) {
+        if (v == skip) continue;
+        double M[9], c[3];
+        gl_cam(P + 12 * v, Pu, M, c);
+        for (size_t m = 0; m < n; m++) {
+            const size_t o = (size_t)v * n + m;
+            const double z = depth[o];
+            if (z >= 0.5 * ORC_INVALID_DEPTH) continue;
+            const double gx = dzdx ? dzdx[o] : 0.0, gy = dzdy ? dzdy[o] : 0.0;
+            const double x = (double)(m % w) + 0.5, y = (double)(m / w) + 0.5;
+            /* BL, TL, BR, TR */
+            const double sx[4] = {-0.5, -0.5, 0.5, 0.5}, sy[4] = {-0.5, 0.5, -0.5, 0.5};
+            double q[4][3];
+            int ok = 1;
+            for (int k = 0; k < 4; k++) {
+                const double zc = z + sx[k] * gx + sy[k] * gy, xc = x + sx[k], yc = y + sy[k];
+                double hh[3];
+                for (int i = 0; i < 3; i++) hh[i] = zc * (M[3 * i] * xc + M[3 * i + 1] * yc + M[3 * i + 2]) + c[i];
+                if (!(hh[2] > ORC_GL_ZNEAR)) ok = 0;
+                q[k][0] = hh[0] / hh[2];
+                q[k][1] = hh[1] / hh[2];
+                q[k][2] = hh[2];
+            }
+            if (!ok) continue;
+            const double zval = z * (M[6] * x + M[7] * y + M[8]) + c[2];
+            double u0 = q[0][0], u1 = q[0][0], v0 = q[0][1], v1 = q[0][1];
+            for (int k = 1; k < 4; k++) {
+                u0 = fmin(u0, q[k][0]);
+                u1 = fmax(u1, q[k][0]);
+                v0 = fmin(v0, q[k][1]);
+                v1 = fmax(v1, q[k][1]);
+            }
+            int px0 = (int)ceil(u0 - 0.5), px1 = (int)floor(u1 - 0.5);
+            int py0 = (int)ceil(v0 - 0.5), py1 = (int)floor(v1 - 0.5);
+            if (px1 - px0 + 1 > ORC_GL_MAX_SPAN || py1 - py0 + 1 > ORC_GL_MAX_SPAN) continue;
+            if (px0 < 0) px0 = 0;
+            if (py0 < 0) py0 = 0;
+            if (px1 > W - 1) px1 = W - 1;
+            if (py1 > H - 1) py1 = H - 1;
+            for (int py = py0; py <= py1; py++)
+                for (int px = px0; px <= px1; px++) {
+                    double zi;
+                    const double cx = px + 0.5, cy = py + 0.5;
+                    if (!gl_tri(q[0], q[1], q[2], cx, cy, &zi) && !gl_tri(q[2], q[1], q[3], cx, cy, &zi)) continue;
+                    const size_t p = (size_t)py * W + px;
+                    if (zi < zb[p] || (zi == zb[p] && zval < val[p])) {
+                        zb[p] = zi;
+                        val[p] = zval;
+                    }
+                }
+        }
+    }
+    for (size_t p = 0; p < N; p++) out[p] = zb[p] < INFINITY ? (float)val[p] : 0.0f;
+    free(zb);
+    free(val);
+}
+
+/* InputView::exportWarp's two passes (view.cpp:663-721): tauWarp.frag:25-74
+ * and tauPartial.frag:23-47 for every pixel of every view against the target
+ * depth map ud.  tau = x_u, or (-2, -2) outside (0, W) x (0, H) or at invalid
+ * depth, or (-1, -1) when the target depth map, sampled bilinearly with
+ * clamped edges at x_u, is nearer than the point by more than 0.1; wdef the
+ * deformation weight 1 / |det d tau / d x| capped at 30 (0 when invalid);
+ * dpart = sigma_z d tau / dz. */
+void orc_gl_warps(int V, int w, int h, int W, int H, const float *depth, const float *dzdx, const float *dzdy,
+                  const double *P, const double *Pu, const float *ud, double sigma_z, float *tx, float *ty,
+                  float *wdef, float *dpx, float *dpy)
+{
+    const size_t n = (size_t)w * h;
+    for (int v = 0; v < V; v++) {
+        double M[9], c[3];
+        gl_cam(P + 12 * v, Pu, M, c);
+        for (size_t m = 0; m < n; m++) {
+            const size_t o = (size_t)v * n + m;
+            const double z = depth[o];
+            const double gx = dzdx ? dzdx[o] : 0.0, gy = dzdy ? dzdy[o] : 0.0;
+            const double x = (double)(m % w) + 0.5, y = (double)(m / w) + 0.5;
+            double Mx[3], hh[3];
+            for (int i = 0; i < 3; i++) {
+                Mx[i] = M[3 * i] * x + M[3 * i + 1] * y + M[3 * i + 2];
+                hh[i] = z * Mx[i] + c[i];
+            }
+            double u = 0.0, vv = 0.0;
+            if (hh[2] != 0.0) {
+                u = hh[0] / hh[2];
+                vv = hh[1] / hh[2];
+            }
+            dpx[o] = (float)(sigma_z * (Mx[0] - u * Mx[2]) / hh[2]);
+            dpy[o] = (float)(sigma_z * (Mx[1] - vv * Mx[2]) / hh[2]);
+            /* d tau / d x = Je M temp, temp's columns d(z x_k) / dx and / dy */
+            const double t0[3] = {x * gx + z, y * gx, gx}, t1[3] = {x * gy, y * gy + z, gy};
+            double a0[3], a1[3];
+            for (int i = 0; i < 3; i++) {
+                a0[i] = M[3 * i] * t0[0] + M[3 * i + 1] * t0[1] + M[3 * i + 2] * t0[2];
+                a1[i] = M[3 * i] * t1[0] + M[3 * i + 1] * t1[1] + M[3 * i + 2] * t1[2];
+            }
+            const double j00 = (a0[0] - u * a0[2]) / hh[2], j10 = (a0[1] - vv * a0[2]) / hh[2];
+            const double j01 = (a1[0] - u * a1[2]) / hh[2], j11 = (a1[1] - vv * a1[2]) / hh[2];
+            double wd = 1.0 / fabs(j00 * j11 - j10 * j01);
+            if (wd >= 30.0) wd = 30.0;
+            if (u <= 0.0 || W <= u || vv <= 0.0 || H <= vv || z >= 0.5 * ORC_INVALID_DEPTH) {
+                tx[o] = ty[o] = -2.0f;
+                wdef[o] = 0.0f;
+                continue;
+            }
+            /* GL_LINEAR with GL_CLAMP_TO_EDGE (texture.cpp:182-185): texel centres at +0.5 */
+            const double s = u - 0.5, t = vv - 0.5;
+            const int i0 = (int)floor(s), j0 = (int)floor(t);
+            const double fx = s - i0, fy = t - j0;
+            const int xa = i0 < 0 ? 0 : (i0 > W - 1 ? W - 1 : i0), xb = i0 + 1 < 0 ? 0 : (i0 + 1 > W - 1 ? W - 1 : i0 + 1);
+            const int ya = j0 < 0 ? 0 : (j0 > H - 1 ? H - 1 : j0), yb = j0 + 1 < 0 ? 0 : (j0 + 1 > H - 1 ? H - 1 : j0 + 1);
+            const double du = (1.0 - fy) * ((1.0 - fx) * ud[(size_t)ya * W + xa] + fx * ud[(size_t)ya * W + xb]) +
+                              fy * ((1.0 - fx) * ud[(size_t)yb * W + xa] + fx * ud[(size_t)yb * W + xb]);
+            if (hh[2] > du + 0.1) {
+                tx[o] = ty[o] = -1.0f;
+                wdef[o] = 0.0f;
+                continue;
+            }
+            tx[o] = (float)u;
+            ty[o] = (float)vv;
+            wdef[o] = (float)wd;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Gradient-domain IBR (IBR_gradient)                                        */
 /* ------------------------------------------------------------------------- */
 

@@ -53,6 +53,8 @@ def lib():
         "orc_deform_weights": (None, [_i, _i, _i, _i, _fp, _fp, _dp]),
         "orc_decompose_P": (None, [_dp, _dp, _dp, _dp, _dp]),
         "orc_warps_from_depth": (None, [_i, _i, _i, _i, _fp, _dp, _dp, _d, _i, _d, _fp, _fp, _fp, _fp, _fp]),
+        "orc_gl_target_depth": (None, [_i, _i, _i, _i, _i, _fp, _fp, _fp, _dp, _dp, _i, _fp]),
+        "orc_gl_warps": (None, [_i, _i, _i, _i, _i, _fp, _fp, _fp, _dp, _dp, _fp, _d, _fp, _fp, _fp, _fp, _fp]),
         "orc_initial_image": (None, [_i, _i, _i, _i, _i, _i, _fp, _fp, _fp, _dp, _dp, _dp]),
         "orc_hole_filling": (_i, [_i, _i, _dp, _u8, _i, _d]),
         "orc_u_gradient": (None, [_i, _i, _i, _i, _i, _i, _fp, _fp, _fp, _dp, _d, _dp, _dp, _dp, _dp]),
@@ -233,6 +235,33 @@ def warps_from_depth(depths, P_views, P_target, dsf, occl_tol=10.0, source_cam=0
     lib().orc_warps_from_depth(V, w, h, dsf, pf(depths), pd(P), pd(Pu), occl_tol, source_cam, sigma_z,
                                pf(tx), pf(ty), pf(dx), pf(dy), pf(ud))
     return tx, ty, dx, dy, ud
+
+
+def gl_target_depth(depths, P_views, P_target, W, H, skip=-1, dzdx=None, dzdy=None):
+    """The target depth map of the OpenGL producer (depth splatting, scene.cpp:1506-1544)."""
+    depths = _F(depths)
+    V, h, w = depths.shape
+    gx = None if dzdx is None else _F(dzdx)
+    gy = None if dzdy is None else _F(dzdy)
+    out = np.zeros((H, W), np.float32)
+    lib().orc_gl_target_depth(V, w, h, W, H, pf(depths), None if gx is None else pf(gx), None if gy is None else pf(gy),
+                              pd(_D(P_views).reshape(V * 12)), pd(_D(P_target).reshape(12)), skip, pf(out))
+    return out
+
+
+def gl_warps(depths, P_views, P_target, ud, sigma_z=10.0, dzdx=None, dzdy=None):
+    """tau, deformation weight and sigma_z dtau/dz of the OpenGL producer
+    (tauWarp.frag, tauPartial.frag) against the target depth map ud."""
+    depths = _F(depths)
+    V, h, w = depths.shape
+    H, W = ud.shape
+    gx = None if dzdx is None else _F(dzdx)
+    gy = None if dzdy is None else _F(dzdy)
+    tx, ty, wd, dx, dy = (np.zeros((V, h, w), np.float32) for _ in range(5))
+    lib().orc_gl_warps(V, w, h, W, H, pf(depths), None if gx is None else pf(gx), None if gy is None else pf(gy),
+                       pd(_D(P_views).reshape(V * 12)), pd(_D(P_target).reshape(12)), pf(_F(ud)), sigma_z,
+                       pf(tx), pf(ty), pf(wd), pf(dx), pf(dy))
+    return tx, ty, wd, dx, dy
 
 
 def initial_image(W, H, dsf, tx, ty, views, omega):
