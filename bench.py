@@ -44,6 +44,26 @@ def measured_traffic(path=PROFILE):
         return None
 
 
+def warp_producer_times(ctx, reps=5):
+    """Device time of the two depth -> tau producers on the bench scene, after
+    the timed region: the import path (lfrv_warps_from_depth, the one the solve
+    uses) and the OpenGL path (lfrv_warps_from_target_depth: target depth by
+    quad splatting, then tau with its markers)."""
+    out = {}
+    for name, run in (("import_warps_from_depth", ctx.warps_from_depth),
+                      ("opengl_warps_from_target_depth", lambda: ctx.warps_from_target_depth(-1))):
+        run()
+        ctx.sync()
+        ctx.set_timing(True)
+        for _ in range(reps):
+            run()
+        ctx.sync()
+        ms, n = ctx.get_timing("warp")
+        ctx.set_timing(False)
+        out[name] = round(ms / reps, 4)
+    return out
+
+
 def cell_valid_count(tx, ty, W, H):
     ok = (tx >= 0) & (ty >= 0) & (tx <= W) & (ty <= H)
     px = np.floor(tx - np.float32(0.5))
@@ -417,6 +437,7 @@ def main():
     up_avg_s = up_ms / max(up_n, 1) / 1e3 if up_n else None
     psnr = lfrv.psnr(U, scene["target"])
     tile = [ctx.params.tile_w or 32, ctx.params.tile_h or 8]  # 0 = library default 32x8
+    producers = warp_producer_times(ctx)
     ctx.close()
     calib_copy(args)
 
@@ -449,6 +470,7 @@ def main():
         "true_rel_residual": [float("%.3g" % r) for r in st_last["true_rel_residual"]],
         "psnr_vs_ground_truth_db": round(psnr, 3),
         "kernels": {"binned_entries": e_binned, "unique_entries": e_unique, "tiles": ntiles},
+        "warp_producers_ms": producers,
     }
     if up_n:
         line["kernels"]["cg_update"] = {"avg_launch_us": round(up_avg_s * 1e6, 2),
