@@ -53,10 +53,13 @@ TimedRegion::~TimedRegion()
     if (pending(ctx).size() > 4096) flush_timing(ctx);
 }
 
-KernelEvents kernel_events(lfrv_ctx *ctx)
+KernelEvents kernel_events(lfrv_ctx *ctx, int k)
 {
     KernelEvents ev;
     if (!ctx->timing) return ev;
+    // a timestamped dispatch waits for its completion signal (several us on a
+    // short kernel), so only one launch in timing_stride of each class carries one
+    if (ctx->timing_seq[k]++ % ctx->timing_stride != 0) return ev;
     ev.a = get_event(ctx);
     ev.b = ev.a ? get_event(ctx) : nullptr;
     if (!ev.b) ev.a = nullptr;
@@ -643,9 +646,12 @@ extern "C" lfrv_status lfrv_set_buffer(lfrv_ctx *ctx, int which, const float *in
 extern "C" lfrv_status lfrv_set_timing(lfrv_ctx *ctx, int enable)
 {
     if (!ctx) return LFRV_E_ARG;
+    if (enable < 0) return fail(ctx, LFRV_E_ARG, "lfrv_set_timing: negative stride");
     flush_timing(ctx);
     ctx->timing = enable != 0;
+    ctx->timing_stride = enable > 0 ? enable : 1;
     for (auto &s : ctx->tslot) s = TimingSlot();
+    for (int64_t &q : ctx->timing_seq) q = 0;
     return LFRV_OK;
 }
 

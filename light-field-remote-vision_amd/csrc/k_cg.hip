@@ -972,7 +972,7 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
                 a.w_out = wbuf[cur ^ 1];
                 a.s_out = sbuf[cur ^ 1];
                 {
-                    const KernelEvents ev = kernel_events(ctx);
+                    const KernelEvents ev = kernel_events(ctx, 0);
                     hipExtLaunchKernelGGL(k_cgcg_st2, dim3(st2_tiles), dim3(kSt2W * kSt2H), 0, ctx->stream,
                                           ev.a, ev.b, 0, a, (const CgState *)stt, part);
                     record_kernel_events(ctx, 0, ev);

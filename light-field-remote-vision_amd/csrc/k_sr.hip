@@ -2,6 +2,8 @@
 // gradient, the outer proximal step, the primal energy.
 #include "tile_common.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -867,10 +869,12 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
         const size_t tiles = sizeof(float) * (3 * up + 3 * lpp);
         const int cap = pull_cap(ctx, 6, tiles, ctx->pull_budget, PullKind::Cell);
         const size_t lds = tiles + pull_lds(ctx, 6, cap, PullKind::Cell);
-        TimedRegion tr(ctx, 3);
-        k_sr_dataterm<2><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, ctx->n, da, ctx->U, ctx->views,
-                                                                          ctx->view_lap, ctx->coverage, ctx->G, ctx->Uq,
-                                                                          ctx->norm_mask, halt);
+        const KernelEvents ev = kernel_events(ctx, 3);
+        hipExtLaunchKernelGGL(k_sr_dataterm<2>, dim3(ctx->ntiles), dim3(ctx->pull_bs), lds, ctx->stream, ev.a, ev.b, 0,
+                              g, te, cap, ctx->n, da, (const float *)ctx->U, (const float *)ctx->views,
+                              (const float *)ctx->view_lap, (const float *)ctx->coverage, ctx->G, ctx->Uq,
+                              ctx->norm_mask, halt);
+        record_kernel_events(ctx, 3, ev);
         LFRV_LAUNCH(ctx, "data term");
     } else {
         // assembled operator on the anchor grid, rebuilt when the weights or
@@ -898,7 +902,7 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
             ctx->sr_ops_epoch = ctx->epoch;
         }
         const int ntx = (ctx->W + kStTW - 1) / kStTW, ntiles = ntx * ((ctx->H + kStTH - 1) / kStTH);
-        TimedRegion tr(ctx, 3);
+        const KernelEvents ev = kernel_events(ctx, 3);  // stamped by the dispatch itself
         if (inner > 0) {
             // the data term and the first primal-dual iteration in one pass: U is
             // read, the new primal goes to U2 (with one inner iteration the
@@ -911,7 +915,7 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
             const float *gw = ctx->has_reg_g ? ctx->reg_g : nullptr;
             float *uq_relax = inner == 1 ? ctx->Uq : nullptr;
             float *gout = inner > 1 ? ctx->G : nullptr;
-#define ST(K) k_sr_step_st<K><<<ntiles, kStTW * kStTH, 0, ctx->stream>>>(ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, pa, ctx->U, ctx->sr_ops, NA, ctx->coverage, ctx->PX, ctx->PY, gw, ctx->P2X, ctx->P2Y, ctx->U2, uq_relax, gout, ctx->norm_mask, halt)
+#define ST(K) hipExtLaunchKernelGGL(k_sr_step_st<K>, dim3(ntiles), dim3(kStTW * kStTH), 0, ctx->stream, ev.a, ev.b, 0, ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, pa, (const float *)ctx->U, (const float *)ctx->sr_ops, NA, (const float *)ctx->coverage, (const float *)ctx->PX, (const float *)ctx->PY, gw, ctx->P2X, ctx->P2Y, ctx->U2, uq_relax, gout, ctx->norm_mask, halt)
             switch (ctx->ks) {
             case 3: ST(3); break;
             case 4: ST(4); break;
@@ -919,6 +923,7 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
             default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
             }
 #undef ST
+            record_kernel_events(ctx, 3, ev);
             LFRV_LAUNCH(ctx, "data term + primal-dual step");
             std::swap(ctx->PX, ctx->P2X);
             std::swap(ctx->PY, ctx->P2Y);
@@ -928,7 +933,7 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
                 std::swap(ctx->Uq, ctx->U2);  // the remaining iterations start from Uq
             pd_done = true;
         } else {
-#define DT(K) k_sr_dataterm_st<K><<<ntiles, kStTW * kStTH, 0, ctx->stream>>>(ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, ctx->U, ctx->sr_ops, NA, ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask, halt)
+#define DT(K) hipExtLaunchKernelGGL(k_sr_dataterm_st<K>, dim3(ntiles), dim3(kStTW * kStTH), 0, ctx->stream, ev.a, ev.b, 0, ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, (const float *)ctx->U, (const float *)ctx->sr_ops, NA, (const float *)ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask, halt)
             switch (ctx->ks) {
             case 3: DT(3); break;
             case 4: DT(4); break;
@@ -936,6 +941,7 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
             default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
             }
 #undef DT
+            record_kernel_events(ctx, 3, ev);
             LFRV_LAUNCH(ctx, "data term");
         }
     }

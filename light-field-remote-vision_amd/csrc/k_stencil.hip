@@ -2,6 +2,8 @@
 // smoothing of the TV weight, and the fused primal-dual ROF step.
 #include "tile_common.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <cmath>
 
 namespace lfrv {
@@ -444,13 +446,15 @@ lfrv_status run_pd_steps(lfrv_ctx *ctx, int iters, const float *F, bool overrela
     const float lam = (float)(1.0 / (double)L);
     dim3 gs((W + PD_TW - 1) / PD_TW, (H + PD_TH - 1) / PD_TH);
     const float *g = ctx->has_reg_g ? ctx->reg_g : nullptr;
-    TimedRegion tr(ctx, 2);
     for (int k = 0; k < iters; k++) {
         float *uin = ctx->Uq, *uout = ctx->U2;
         float *pxin = ctx->PX, *pyin = ctx->PY, *pxout = ctx->P2X, *pyout = ctx->P2Y;
         float *ufin = (overrelax && k == iters - 1) ? ctx->U : nullptr;
-        k_pd_step<<<gs, 256, 0, ctx->stream>>>(W, H, tau, lam, sigma, uin, uout, pxin, pyin, pxout, pyout, F, g,
-                                               ufin, theta, halt);
+        const KernelEvents ev = kernel_events(ctx, 2);  // stamped by the dispatch itself
+        hipExtLaunchKernelGGL(k_pd_step, gs, dim3(256), 0, ctx->stream, ev.a, ev.b, 0, W, H, tau, lam, sigma,
+                              (const float *)uin, uout, (const float *)pxin, (const float *)pyin, pxout, pyout, F, g,
+                              ufin, theta, halt);
+        record_kernel_events(ctx, 2, ev);
         std::swap(ctx->Uq, ctx->U2);
         std::swap(ctx->PX, ctx->P2X);
         std::swap(ctx->PY, ctx->P2Y);

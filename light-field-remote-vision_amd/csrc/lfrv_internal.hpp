@@ -352,6 +352,8 @@ struct lfrv_ctx {
 
     // timing
     bool timing = false;
+    int timing_stride = 1;          // kernel-event classes: one launch in timing_stride is timed
+    int64_t timing_seq[6] = {};     // launches seen per class while timing
     lfrv::TimingSlot tslot[6];
     std::vector<hipEvent_t> ev_pool;
 };
@@ -389,7 +391,7 @@ void flush_timing(lfrv_ctx *ctx);
 struct KernelEvents {
     hipEvent_t a = nullptr, b = nullptr;
 };
-KernelEvents kernel_events(lfrv_ctx *ctx);          // empty pair unless timing is on
+KernelEvents kernel_events(lfrv_ctx *ctx, int k);   // empty pair unless timing is on and launch k's turn
 void record_kernel_events(lfrv_ctx *ctx, int k, const KernelEvents &ev);
 
 inline unsigned int blocks_for(size_t n, int bs = kBlock) { return (unsigned int)((n + bs - 1) / bs); }
