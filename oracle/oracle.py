@@ -142,6 +142,44 @@ def gather(W, H, dsf, tx, ty, u):
     return out
 
 
+def structured_warp_down(u, dmap, dx, dy, vis, dsf):
+    """The structured (disparity) model of cocolib's vtv_sr (not on the ULF
+    path; the cross-check of the tau-footprint model for pure translations):
+    vtv_sr_warp_view_device (vtv_sr_kernels.cu:72-135) samples u (C, H, W)
+    bilinearly at (x + d dx, y + d dy), integer-origin pixels, renormalised
+    where the right / bottom taps leave the image and 0 where the sample starts
+    outside or the pixel is not visible; vtv_sr_downsample_view_device
+    (:767-802) then averages the visible pixels of each dsf x dsf block.
+    dmap, vis: (H, W) on the hi-res view grid.  Returns (C, H / dsf, W / dsf)."""
+    u = _D(u)
+    C, H, W = u.shape
+    d = _D(dmap)
+    vis = np.asarray(vis).astype(bool)
+    oy, ox = np.mgrid[0:H, 0:W].astype(np.float64)
+    ux = ox + d * dx
+    uy = oy + d * dy
+    px = np.floor(ux).astype(np.int64)
+    py = np.floor(uy).astype(np.int64)
+    ax, ay = ux - px, uy - py
+    out_img = (px < 0) | (py < 0) | (px > W - 1) | (py > H - 1)
+    pxc, pyc = np.clip(px, 0, W - 1), np.clip(py, 0, H - 1)
+    xr, yr = px < W - 1, py < H - 1
+    pxr, pyr = np.clip(px + 1, 0, W - 1), np.clip(py + 1, 0, H - 1)
+    wmm, wpm, wmp, wpp = (1 - ax) * (1 - ay), ax * (1 - ay), (1 - ax) * ay, ax * ay
+    weight = 1.0 - np.where(xr, 0.0, wpm) - np.where(yr, 0.0, wmp) - np.where(xr & yr, 0.0, wpp)
+    v = np.zeros((C, H, W))
+    for c in range(C):
+        r = u[c][pyc, pxc] * wmm
+        r = r + np.where(xr, u[c][pyc, pxr] * wpm, 0.0)
+        r = r + np.where(yr, u[c][pyr, pxc] * wmp, 0.0)
+        r = r + np.where(xr & yr, u[c][pyr, pxr] * wpp, 0.0)
+        v[c] = np.where(vis & ~out_img, r / weight, 0.0)
+    h, w = H // dsf, W // dsf
+    vv = (v[:, :h * dsf, :w * dsf] * vis[None, :h * dsf, :w * dsf]).reshape(C, h, dsf, w, dsf).sum(axis=(2, 4))
+    cnt = vis[:h * dsf, :w * dsf].reshape(h, dsf, w, dsf).sum(axis=(1, 3))
+    return np.where(cnt > 0, vv / np.maximum(cnt, 1), 0.0)
+
+
 def splat(W, H, dsf, tx, ty, vals, wgt=None, out=None, wsum=None):
     """vals: (C, h, w); accumulates into out (C, H, W) and wsum (H, W)."""
     tx, ty = _F(tx), _F(ty)
