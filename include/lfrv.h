@@ -331,10 +331,11 @@ lfrv_status lfrv_set_buffer(lfrv_ctx *ctx, int which, const float *in);
 /* Per-kernel-class GPU time recorded with HIP events on the context stream
  * while timing is enabled: class 0 = CG operator, 1 = CG vector update,
  * 2 = PD step, 3 = data term, 4 = warp from depth, 5 = splat/setup.  enable = 0
- * turns it off; enable = n >= 1 times one launch in n of classes 0-3 with
- * events the dispatch itself stamps (a stamped dispatch waits for its
- * completion signal, several microseconds on a short kernel), every call of
- * classes 4-5.  lfrv_get_timing's launch count is the number timed. */
+ * turns it off; enable = n >= 1 times one launch (classes 0-3, with events the
+ * dispatch itself stamps) or one call (classes 4-5, events recorded around it)
+ * in n of each class: a timed dispatch or an event record waits for its
+ * completion signal, several microseconds each.  lfrv_get_timing's launch
+ * count is the number timed. */
 lfrv_status lfrv_set_timing(lfrv_ctx *ctx, int enable);
 lfrv_status lfrv_get_timing(lfrv_ctx *ctx, int kclass, double *total_ms, int64_t *launches);
 

@@ -39,6 +39,9 @@ static hipEvent_t get_event(lfrv_ctx *ctx)
 TimedRegion::TimedRegion(lfrv_ctx *c, int kk) : ctx(c), k(kk)
 {
     if (!ctx->timing) return;
+    // event records are queue packets the stream waits on: sampled like the
+    // kernel events, one region in timing_stride of each class
+    if (ctx->timing_seq[k]++ % ctx->timing_stride != 0) return;
     a = get_event(ctx);
     if (a) (void)hipEventRecord(a, ctx->stream);
 }

@@ -989,11 +989,9 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
                 k_st2_residual<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(
                     ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, (float)beta, gco, x, ctx->cg_b, part);
             }
-            k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, st2_tiles, ctx->scal);
+            k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, st2_tiles, stt->resid);
             LFRV_LAUNCH(ctx, "cg residual");
             LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
-            LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, 3 * sizeof(double), hipMemcpyDeviceToHost,
-                                         ctx->stream));
             LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
             done = ctx->cg_state_host->done != 0;
             final_ready = true;
@@ -1037,11 +1035,9 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
             }
         }
         k_resid<<<rb, 256, 0, ctx->stream>>>(N, ctx->cg_b, q, part);
-        k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, (int)rb, ctx->scal);
+        k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, (int)rb, stt->resid);
         LFRV_LAUNCH(ctx, "cg residual");
         LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
-        LFRV_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, 3 * sizeof(double), hipMemcpyDeviceToHost,
-                                     ctx->stream));
         LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
 #undef OPK
@@ -1054,8 +1050,7 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
             st->iterations[c] = hs.iter[c];
             st->converged[c] = conv ? 1 : 0;
             st->rel_residual[c] = hs.bb[c] > 0 ? std::sqrt(hs.rr[c] / hs.bb[c]) : std::sqrt(hs.rr[c]);
-            st->true_rel_residual[c] =
-                hs.bb[c] > 0 ? std::sqrt(ctx->scal_host[c] / hs.bb[c]) : std::sqrt(ctx->scal_host[c]);
+            st->true_rel_residual[c] = hs.bb[c] > 0 ? std::sqrt(hs.resid[c] / hs.bb[c]) : std::sqrt(hs.resid[c]);
         }
         if (hs.breakdown[c]) status = LFRV_E_BREAKDOWN;
         else if (!conv && status == LFRV_OK) status = LFRV_E_NONCONVERGED;

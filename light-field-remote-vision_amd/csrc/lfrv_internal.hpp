@@ -234,6 +234,7 @@ struct CgState {
     int done;
     int maxit;
     int breakdown[3];
+    double resid[3];  // true residual sums of the final iterate (filled with each poll)
 };
 
 // Device state of the IBR_color FISTA loop (the stopping rules of
