@@ -1,5 +1,8 @@
-// IBR_color (TV super-resolution with warped data terms): fused data-term
-// gradient, the outer proximal step, the primal energy.
+// IBR_color (TV super-resolution with warped data terms): the data-term
+// gradient and the outer proximal step (dsf = 1: an entry pull; dsf >= 2: the
+// assembled anchor-grid operator as a stencil, fused with the first
+// primal-dual iteration), the primal energy, and the re-weighting loop with
+// its stopping rules on the device.
 #include "tile_common.hpp"
 
 #include <hip/hip_ext.h>

@@ -1,4 +1,5 @@
-// Warps from depth, gather (warp) and per-view weight kernels.
+// Warps from depth (the import path and the OpenGL path), gather (warp) and
+// per-view weight kernels.
 #include "lfrv_internal.hpp"
 
 #include <cstring>
