@@ -3,7 +3,7 @@
  *
  * The file formats and configuration semantics the reference's drivers read
  * and write, for callers that drive the solver through lfrv.h directly and
- * for tests.  No GPU work happens here.
+ * for tests.  No GPU work happens here except in lfrv_io_export_warps.
  *
  *   images: PFM (gsl_image.cpp:416-502, import.cpp:695-775) and 8-bit PNG
  *           (gsl_image.cpp:539-637), planar float RGB, rows top to bottom
@@ -31,6 +31,13 @@ int lfrv_io_save_image(const char *path, size_t w, size_t h, const float *rgb);
  * Config_data and print its fields as "name value" lines into out (cap
  * bytes, NUL-terminated).  Returns the length needed, or -1. */
 int lfrv_io_config_data(int argc, const char *const *argv, char *out, size_t cap);
+
+/* lfrv_ref::export_warps for a reference command line (argv[0] the program
+ * name; -config file and switches as lfrv_cli takes them): the OpenGL path's
+ * target depth map and warps computed on the GPU and written as PFMs.  The one
+ * entry point here that uses the GPU.  Returns 0, or -1 with the reason on
+ * stderr. */
+int lfrv_io_export_warps(int argc, const char *const *argv);
 
 #ifdef __cplusplus
 }

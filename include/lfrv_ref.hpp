@@ -97,6 +97,15 @@ bool coco_vtv_sr_write_pfm_solution(coco_vtv_data *data);
  * TV weight, FISTA steps with the energy / moving-pixel stopping rules). */
 void sr_synthesize_view(ConfigData *config_data);
 
+/* The warp export of the reference's OpenGL path, Scene::computeTargetDepthMap
+ * + Scene::exportWarps (scene.cpp:1506-1585, view.cpp:663-721), on the GPU
+ * (lfrv_warps_from_target_depth) from depth maps (depth_name) and cameras
+ * (cam_name) at every index s_min..s_max, the target being view s_rmv (its own
+ * depth map not splatted), dsf 1: writes tau_name (R tau_x, G tau_y, B the
+ * deformation weight; -1 occluded, -2 outside), dpart_name (R sigma_z, G / B
+ * dtau / dz) and uDepth_name (R the target depth map) where those are set. */
+void export_warps(ConfigData *config_data);
+
 /* ---- front end helpers ------------------------------------------------------ */
 
 /* import_lightfield (import.cpp:116-170): views s_min..s_max except s_rmv of

@@ -1,11 +1,14 @@
 // C-ABI of the front end (include/lfrv_io.h).
 #include "../../include/lfrv_io.h"
 
+#include <cstdio>
 #include <cstring>
+#include <exception>
 #include <sstream>
 #include <vector>
 
 #include "../../include/lfrv_ref_config.hpp"
+#include "../../include/lfrv_ref.hpp"
 #include "../../include/lfrv_ref_image.hpp"
 
 using namespace lfrv_ref;
@@ -62,4 +65,23 @@ extern "C" int lfrv_io_config_data(int argc, const char *const *argv, char *out,
         out[n] = 0;
     }
     return (int)s.size() + 1;
+}
+
+extern "C" int lfrv_io_export_warps(int argc, const char *const *argv)
+{
+    if (argc < 1 || !argv) return -1;
+    std::vector<std::string> args(argv, argv + argc);
+    std::vector<char *> av;
+    for (auto &a : args) av.push_back(&a[0]);
+    try {
+        Config cfg;
+        cfg.parse_command_line(argc, av.data());
+        ConfigData cd;
+        cd.parse_config(cfg);
+        export_warps(&cd);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "lfrv_io_export_warps: %s\n", e.what());
+        return -1;
+    }
+    return 0;
 }

@@ -31,6 +31,8 @@ def lib():
     L.lfrv_io_save_image.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p]
     L.lfrv_io_config_data.restype = ctypes.c_int
     L.lfrv_io_config_data.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p, ctypes.c_size_t]
+    L.lfrv_io_export_warps.restype = ctypes.c_int
+    L.lfrv_io_export_warps.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
     _lib = L
     return L
 
@@ -70,3 +72,12 @@ def config_data(argv):
         k, _, v = line.partition(" ")
         out[k] = v
     return out
+
+
+def export_warps(argv):
+    """lfrv_ref::export_warps for a reference command line (argv[0] the program
+    name): the OpenGL path's target depth map and warps, computed on the GPU
+    and written as the tau / dpart / uDepth PFMs the config names."""
+    args = (ctypes.c_char_p * len(argv))(*[os.fsencode(a) for a in argv])
+    if lib().lfrv_io_export_warps(len(argv), args) != 0:
+        raise RuntimeError("export_warps failed (reason on stderr)")

@@ -86,3 +86,76 @@ def test_gradient_ibr_on_gl_warps_matches_oracle():
     assert all(st["converged"])
     assert rel_err(U, r["U"]) < 1e-4
     assert abs(oracle.psnr(U, s["target"]) - oracle.psnr(r["U"], s["target"])) < 0.01
+
+
+def test_export_warps_files_feed_the_cli(tmp_path):
+    """export_warps (Scene::computeTargetDepthMap + exportWarps) writes the tau /
+    dpart / uDepth PFMs of the OpenGL path; the command-line IBR_gradient reads
+    them back (read_tau, read_partial_tau) and solves as the in-process sequence
+    on the same warps does."""
+    import subprocess
+    import sys
+    from lfrv import io as lio
+    s = _scene(40, 32, seed=8)
+    V, h, w = s["V"], s["h"], s["w"]
+    rmv = 4                                  # the target is the centre view of the 3 x 3 grid
+    for v in range(V):
+        lio.save_image(str(tmp_path / f"v_{v:02d}.pfm"), s["views"][v])
+        lio.save_image(str(tmp_path / f"d_{v:02d}.pfm"), np.repeat(s["depths"][v][None], 3, 0))
+        with open(tmp_path / f"cam_{v:02d}.txt", "w") as f:
+            f.write("CONTOUR\n" + "\n".join(" ".join(f"{x:.17g}" for x in r) for r in s["P_views"][v]) + "\n")
+    lio.export_warps(["export", "-depth_name", f"{tmp_path}/d_%02d.pfm", "-cam_name", f"{tmp_path}/cam_%02d.txt",
+                      "-tau_name", f"{tmp_path}/t_%02d.pfm", "-dpart_name", f"{tmp_path}/p_%02d.pfm",
+                      "-uDepth_name", f"{tmp_path}/ud.pfm", "-s_min", "0", "-s_max", str(V - 1), "-s_rmv", str(rmv),
+                      "-dsf", "1", "-outdir", str(tmp_path)])
+    # the files hold the producer's buffers
+    with lfrv.Context(pipelines.gradient_ibr_params(s)) as ctx:
+        pipelines.load_scene(ctx, s, with_depth=False)
+        for v in range(V):
+            ctx.set_depth(v, s["depths"][v], s["P_views"][v])
+        ctx.set_target_camera(s["P_views"][rmv])
+        ctx.warps_from_target_depth(rmv)
+        tx = ctx.get_buffer("TAU_X").reshape(V, h, w)
+        wd = ctx.get_buffer("TAU_WEIGHT").reshape(V, h, w)
+        dpy = ctx.get_buffer("DPART_Y").reshape(V, h, w)
+        ud = ctx.get_buffer("TARGET_DEPTH").reshape(h, w)
+    t3 = lio.load_image(str(tmp_path / "t_03.pfm"))
+    assert np.array_equal(t3[0], tx[3]) and np.array_equal(t3[2], wd[3])
+    p3 = lio.load_image(str(tmp_path / "p_03.pfm"))
+    assert np.all(p3[0] == 10.0) and np.allclose(p3[2] * 10.0, dpy[3], rtol=1e-6, atol=1e-9)
+    assert np.array_equal(lio.load_image(str(tmp_path / "ud.pfm"))[0], ud)
+    # the target view's own warp is the identity where it sees itself
+    X = np.arange(w) + 0.5
+    ok = tx[rmv] > 0
+    assert ok.mean() > 0.8 and np.max(np.abs(tx[rmv][ok] - np.broadcast_to(X, (h, w))[ok])) < 1e-3
+    # the command line reads those files back (import_warps + read_tau / read_partial_tau) for every view
+    # but the target and solves as the same calls made in this process
+    cfg = tmp_path / "gibr.cfg"
+    cfg.write_text("\n".join([
+        "algorithm IBR_gradient", f"lf_name {tmp_path}/v_%02d.pfm", f"tau_name {tmp_path}/t_%02d.pfm",
+        f"dpart_name {tmp_path}/p_%02d.pfm", f"outdir {tmp_path}/out", "outfile u.pfm", "s_min 0",
+        f"s_max {V - 1}", f"s_rmv {rmv}", "dsf 1", "nchannels 3", "sigma_sensor 1", "dt_beta 0.5"]) + "\n")
+    r = subprocess.run([lio.CLI_PATH, "-config", str(cfg)], capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout)
+    assert r.returncode == 0, r.stderr
+    U_cli = lio.load_image(str(tmp_path / "out" / "u.pfm"))
+    keep = [v for v in range(V) if v != rmv]
+    s2 = dict(s, V=len(keep), views=s["views"][keep])
+    prm = pipelines.gradient_ibr_params(s2, dt_beta=0.5, sigma_sensor=np.float32(1.0) / np.float32(255))
+    with lfrv.Context(prm) as ctx:
+        for i, v in enumerate(keep):
+            ctx.set_view(i, s["views"][v])
+            t = lio.load_image(str(tmp_path / f"t_{v:02d}.pfm"))
+            p = lio.load_image(str(tmp_path / f"p_{v:02d}.pfm"))
+            ctx.set_tau(i, t[0], t[1])
+            ctx.set_partial_tau(i, p[0], p[1], p[2])
+        ctx.clear_solution()
+        ctx.prepare()
+        ctx.setup_visibility()
+        ctx.compute_weights()
+        ctx.compute_initial_image()
+        ctx.init_u_gradient()
+        st = ctx.cg_solve(1e-5, 5000)
+        U = ctx.get_solution()
+    assert all(st["converged"])
+    assert np.max(np.abs(U_cli - U)) <= 1e-6 * max(1.0, np.abs(U).max())
