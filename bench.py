@@ -68,6 +68,38 @@ def warp_producer_times(ctx, reps=5):
     return out
 
 
+def hole_filling_times(scene, reps=3):
+    """The Poisson hole filling (gradientIBR.cu:87-152) on the bench scene with
+    a 256 x 256 block of invalid depth in every view: lfrv_compute_initial_image
+    timed with and without the holes, after the timed region."""
+    out = {}
+    for name, holed in (("without_holes", False), ("with_holes", True)):
+        s = dict(scene)
+        if holed:
+            d = scene["depths"].copy()
+            d[:, 384:640, 384:640] = 0.0   # depth < 1e-3: invalid (import.cpp:292-296)
+            s["depths"] = d
+        ctx = lfrv.Context(pipelines.gradient_ibr_params(s))
+        pipelines.load_scene(ctx, s)
+        ms = []
+        for r in range(reps + 1):
+            ctx.clear_solution()
+            ctx.warps_from_depth()
+            ctx.prepare()
+            ctx.setup_visibility()
+            ctx.compute_weights()
+            ctx.sync()
+            t = time.perf_counter()
+            ctx.compute_initial_image()
+            ctx.sync()
+            if r:
+                ms.append(1e3 * (time.perf_counter() - t))
+        holes, sweeps = ctx.hole_stats()
+        ctx.close()
+        out[name] = dict(initial_image_ms=round(float(np.median(ms)), 3), holes=holes, sweeps=sweeps)
+    return out
+
+
 def cell_valid_count(tx, ty, W, H):
     ok = (tx >= 0) & (ty >= 0) & (tx <= W) & (ty <= H)
     px = np.floor(tx - np.float32(0.5))
@@ -443,6 +475,7 @@ def main():
     tile = [ctx.params.tile_w or 32, ctx.params.tile_h or 8]  # 0 = library default 32x8
     producers = warp_producer_times(ctx)
     ctx.close()
+    holes = hole_filling_times(scene)
     calib_copy(args)
 
     line = {
@@ -475,6 +508,7 @@ def main():
         "psnr_vs_ground_truth_db": round(psnr, 3),
         "kernels": {"binned_entries": e_binned, "unique_entries": e_unique, "tiles": ntiles},
         "warp_producers_ms": producers,
+        "hole_filling": holes,
     }
     if up_n:
         line["kernels"]["cg_update"] = {"avg_launch_us": round(up_avg_s * 1e6, 2),

@@ -182,6 +182,12 @@ lfrv_status lfrv_compute_weights(lfrv_ctx *ctx);
  * coco_vtv_sr_compute_initial_image (optimization.cu:1259-1360). */
 lfrv_status lfrv_compute_initial_image(lfrv_ctx *ctx, const float *u0);
 
+/* The last Poisson hole filling of lfrv_compute_initial_image: the number of
+ * hole pixels (not visible, setup_target_visibility) and of diffusion sweeps
+ * run before its stopping rule (gradientIBR.cu:87-152: 500 at most, relative
+ * change < 1e-3). */
+lfrv_status lfrv_get_hole_stats(lfrv_ctx *ctx, int *holes, int *sweeps);
+
 /* Current solution, planar RGB W*H.  Replaces get_solution
  * (gradientIBR.cu:20-37) and coco_vtv_get_solution (vtv.cu:268-287). */
 lfrv_status lfrv_get_solution(lfrv_ctx *ctx, float *rgb);

@@ -446,6 +446,14 @@ extern "C" lfrv_status lfrv_compute_initial_image(lfrv_ctx *ctx, const float *u0
     return run_push_pull(ctx, 10);  // optimization.cu:574
 }
 
+extern "C" lfrv_status lfrv_get_hole_stats(lfrv_ctx *ctx, int *holes, int *sweeps)
+{
+    if (!ctx) return LFRV_E_ARG;
+    if (holes) *holes = ctx->n_holes;
+    if (sweeps) *sweeps = ctx->hole_iters;
+    return LFRV_OK;
+}
+
 extern "C" lfrv_status lfrv_get_solution(lfrv_ctx *ctx, float *rgb)
 {
     if (!ctx || !rgb) return LFRV_E_ARG;

@@ -116,6 +116,7 @@ def lib():
         "lfrv_compute_weights": (st, [_vp]),
         "lfrv_compute_initial_image": (st, [_vp, _vp]),
         "lfrv_get_solution": (st, [_vp, _vp]),
+        "lfrv_get_hole_stats": (st, [_vp, ctypes.POINTER(st), ctypes.POINTER(st)]),
         "lfrv_set_solution": (st, [_vp, _vp]),
         "lfrv_clear_solution": (st, [_vp]),
         "lfrv_init_u_gradient": (st, [_vp]),
@@ -305,6 +306,12 @@ class Context:
         s = SrStats()
         self._check(lib().lfrv_sr_solve(self._h, niter, nmeta, ctypes.byref(s)))
         return s.as_dict()
+
+    def hole_stats(self):
+        """(hole pixels, diffusion sweeps) of the last Poisson hole filling."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().lfrv_get_hole_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def sync(self):
         self._check(lib().lfrv_sync(self._h))

@@ -208,6 +208,7 @@ def test_hole_filling_matches_oracle(small_scene):
         ctx.compute_weights()
         ctx.compute_initial_image()
         U0 = ctx.get_solution()
+        nholes, sweeps = ctx.hole_stats()
         tx = ctx.get_buffer("TAU_X").reshape(V, h, w)
         ty = ctx.get_buffer("TAU_Y").reshape(V, h, w)
         om = ctx.get_buffer("OMEGA").reshape(V, h, w)
@@ -220,6 +221,7 @@ def test_hole_filling_matches_oracle(small_scene):
     oU, _ = oracle.initial_image(W, H, s["dsf"], tx, ty, s["views"], om)
     oU, k = oracle.hole_filling(oU, vis)
     assert k > 1
+    assert nholes == int((vis == 0).sum()) and abs(sweeps - k) <= 1
     # Jacobi sweeps to an energy-change test: FP32 and FP64 may stop one sweep apart
     assert np.max(np.abs(U0 - oU)) < 2e-3
     assert np.max(np.abs(U0[:, vis == 1] - oU[:, vis == 1])) < 1e-5
