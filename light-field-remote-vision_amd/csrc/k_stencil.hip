@@ -21,25 +21,75 @@ __global__ void k_hole_list(size_t N, const float *__restrict__ vis, int *__rest
     list[k] = (int)o;
 }
 
-struct HoleState {
-    double e_prev;
+struct HoleState {  // read by the host after each chunk of sweeps
     double e;
     int iters;
     int done;
-    int maxit;
-    double eps;
 };
 
-// One Jacobi sweep u <- mean of the 4-neighbours (the reference's laplacian
-// then add_to), on the hole pixels only, all three channels; the energy
-// sum(lap * u_new) of the last channel is reduced and compared with the
-// previous sweep's (the stopping rule of gradientIBR.cu:139-143).
+// The diffusion's bookkeeping between sweeps: eh[j] holds the energy of sweep
+// j (eh[0] = 0, the energy the first sweep is compared with); *stopped the
+// number of sweeps run once the stopping rule has fired.
+struct HoleLoop {
+    int maxit;
+    double eps;
+    double *eh;      // maxit + 1
+    int *stopped;
+    int G;           // blocks of a sweep (its partials per sweep)
+};
+
+__global__ void k_hole_init(HoleLoop hl)
+{
+    if (threadIdx.x == 0) {
+        hl.eh[0] = 0.0;
+        *hl.stopped = 0;
+    }
+}
+
+// The stopping rule of gradientIBR.cu:139-143 after sweep j (>= 1), from that
+// sweep's block partials: |E_{j-1} - E_j| < eps, or j >= maxit.  Every block
+// evaluates it the same way (a deterministic reduction of G partials).
+__device__ __forceinline__ bool hole_rule(const HoleLoop &hl, int j, const double *pj, double *red, double &e)
+{
+    double v[1];
+    reduce_partials<1>(pj, hl.G, v, red);
+    __shared__ double eb;
+    if (threadIdx.x == 0) eb = v[0];  // the sum is valid in thread 0
+    __syncthreads();
+    e = eb;
+    return fabs(hl.eh[j - 1] - e) < hl.eps || j >= hl.maxit;
+}
+
+// *hl.stopped read once per block (every thread then takes the same branch)
+__device__ __forceinline__ int hole_stopped(const HoleLoop &hl)
+{
+    __shared__ int sb;
+    if (threadIdx.x == 0) sb = *hl.stopped;
+    __syncthreads();
+    return sb;
+}
+
+// Sweep k (1-based) of the Poisson diffusion: first the stopping rule of sweep
+// k - 1 (so the rule needs no kernel of its own), then one Jacobi sweep
+// u <- mean of the 4-neighbours (the reference's laplacian then add_to) on the
+// hole pixels, all three channels, with the energy sum(lap * u_new) of the
+// last channel as this sweep's block partials.  pin / pout: the partials of
+// sweeps k - 1 and k.
 __global__ void __launch_bounds__(256) k_hole_sweep(int W, int H, int nholes, const int *__restrict__ list,
                                                     const float *__restrict__ uin, float *__restrict__ uout,
-                                                    const HoleState *st, double *partials)
+                                                    HoleLoop hl, int k, const double *pin, double *pout)
 {
     __shared__ double red[16];
-    if (st->done) return;
+    if (hole_stopped(hl)) return;
+    if (k >= 2) {
+        double e;
+        const bool done = hole_rule(hl, k - 1, pin, red, e);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            hl.eh[k - 1] = e;
+            if (done) *hl.stopped = k - 1;
+        }
+        if (done) return;
+    }
     const size_t N = (size_t)W * H;
     double v[1] = {0.0};
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nholes; k += gridDim.x * blockDim.x) {
@@ -58,21 +108,25 @@ __global__ void __launch_bounds__(256) k_hole_sweep(int W, int H, int nholes, co
             if (c == 2) v[0] += (double)lap * (double)un;
         }
     }
-    block_partials<1>(v, partials, red);
+    block_partials<1>(v, pout, red);
 }
 
-// Energy bookkeeping and stopping rule of one sweep (one block).
-__global__ void __launch_bounds__(kFinalizeThreads) k_hole_state(HoleState *st, const double *__restrict__ partials, int G)
+// The state after the last sweep `last` of a chunk, for the host: the sweeps
+// run and whether the rule has fired (one block).
+__global__ void __launch_bounds__(kFinalizeThreads) k_hole_state(HoleLoop hl, int last, const double *plast,
+                                                                 HoleState *st)
 {
     __shared__ double red[16];
-    if (st->done) return;
-    double v[1];
-    reduce_partials<1>(partials, G, v, red);
+    int iters = hole_stopped(hl), done = iters != 0;
+    double e = 0.0;
+    if (!done) {
+        done = hole_rule(hl, last, plast, red, e);
+        iters = last;
+    }
     if (threadIdx.x == 0) {
-        st->e_prev = st->e;
-        st->e = v[0];
-        st->iters += 1;
-        if (fabs(st->e_prev - st->e) < st->eps || st->iters >= st->maxit) st->done = 1;
+        st->iters = iters;
+        st->done = done;
+        st->e = done && *hl.stopped ? hl.eh[iters] : e;
     }
 }
 
@@ -337,26 +391,32 @@ lfrv_status run_hole_filling(lfrv_ctx *ctx, int iters, double eps, int *ran)
     ctx->n_holes = (int)nh;
     if (ran) *ran = 0;
     if (nh == 0 || iters <= 0) return LFRV_OK;
-    HoleState hs{};
-    hs.maxit = iters;
-    hs.eps = eps;
     HoleState *dst = reinterpret_cast<HoleState *>(ctx->scal + 16);
-    LFRV_HIP(ctx, hipMemcpyAsync(dst, &hs, sizeof hs, hipMemcpyHostToDevice, ctx->stream));
     float *bufs[2] = {ctx->U, ctx->tmp3N};
     LFRV_HIP(ctx, hipMemcpyAsync(ctx->tmp3N, ctx->U, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
     unsigned int nb = blocks_for(nh);
     if (nb > 1024) nb = 1024;
+    // partials of odd / even sweeps, the energy history and the stop flag
+    HoleLoop hl;
+    hl.maxit = iters;
+    hl.eps = eps;
+    hl.G = (int)nb;
+    double *psw[2] = {ctx->partials, ctx->partials + 1024};
+    hl.eh = ctx->partials + 2048;  // iters + 1 <= partials_cap - 2048, checked below
+    hl.stopped = reinterpret_cast<int *>(ctx->counters + 12);
+    if ((size_t)iters + 1 > ctx->partials_cap - 2048) return fail(ctx, LFRV_E_ARG, "hole filling: too many sweeps");
+    k_hole_init<<<1, 64, 0, ctx->stream>>>(hl);
     int launched = 0;
     HoleState host{};
     while (launched < iters) {
         const int chunk = std::min(16, iters - launched);
-        for (int k = 0; k < chunk; k++) {
-            const int cur = (launched + k) & 1;
-            k_hole_sweep<<<nb, 256, 0, ctx->stream>>>(ctx->W, ctx->H, (int)nh, ctx->hole_list, bufs[cur],
-                                                      bufs[cur ^ 1], dst, ctx->partials);
-            k_hole_state<<<1, kFinalizeThreads, 0, ctx->stream>>>(dst, ctx->partials, (int)nb);
+        for (int j = 0; j < chunk; j++) {
+            const int k = launched + j + 1;  // sweep k reads bufs[(k - 1) & 1]
+            k_hole_sweep<<<nb, 256, 0, ctx->stream>>>(ctx->W, ctx->H, (int)nh, ctx->hole_list, bufs[(k - 1) & 1],
+                                                      bufs[k & 1], hl, k, psw[(k - 1) & 1], psw[k & 1]);
         }
         launched += chunk;
+        k_hole_state<<<1, kFinalizeThreads, 0, ctx->stream>>>(hl, launched, psw[launched & 1], dst);
         LFRV_LAUNCH(ctx, "hole filling");
         LFRV_HIP(ctx, hipMemcpyAsync(&host, dst, sizeof host, hipMemcpyDeviceToHost, ctx->stream));
         LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
