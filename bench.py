@@ -48,23 +48,30 @@ def measured_traffic(path=PROFILE):
         return None
 
 
-def warp_producer_times(ctx, reps=5):
+def warp_producer_times(ctx, scene, reps=5):
     """Device time of the two depth -> tau producers on the bench scene, after
     the timed region: the import path (lfrv_warps_from_depth, the one the solve
     uses) and the OpenGL path (lfrv_warps_from_target_depth: target depth by
-    quad splatting, then tau with its markers)."""
+    quad splatting, then tau with its markers).  Algorithmic bytes per source
+    pixel: the import path reads the depth (4 B) and writes tau_x, tau_y,
+    dpart_x, dpart_y (16 B); the OpenGL path also writes the deformation
+    weight (4 B) and, per target pixel, the target depth (4 B)."""
+    n = scene["V"] * scene["w"] * scene["h"]
+    N = scene["W"] * scene["H"]
     out = {}
-    for name, run in (("import_warps_from_depth", ctx.warps_from_depth),
-                      ("opengl_warps_from_target_depth", lambda: ctx.warps_from_target_depth(-1))):
+    for name, run, nbytes in (("import_warps_from_depth", ctx.warps_from_depth, 20 * n),
+                              ("opengl_warps_from_target_depth", lambda: ctx.warps_from_target_depth(-1),
+                               24 * n + 4 * N)):
         run()
         ctx.sync()
         ctx.set_timing(True)
         for _ in range(reps):
             run()
         ctx.sync()
-        ms, n = ctx.get_timing("warp")
+        ms, _ = ctx.get_timing("warp")
         ctx.set_timing(False)
-        out[name] = round(ms / reps, 4)
+        ms /= reps
+        out[name] = dict(ms=round(ms, 4), bytes=nbytes, achieved_gbs=round(nbytes / (ms * 1e6), 1))
     return out
 
 
@@ -293,6 +300,7 @@ def bench_color_sr(args, workload):
     dt_ms, dt_n = ctx.get_timing("dataterm")
     pd_ms, pd_n = ctx.get_timing("pd_step")
     ctx.set_timing(False)
+    producers = warp_producer_times(ctx, scene)
     host_loop = None
     if nmeta:
         # the same view with the loop driven from the host, for comparison
@@ -379,6 +387,7 @@ def bench_color_sr(args, workload):
         "ms_per_step_p90": round(float(np.percentile(per_view, 90)), 3),
         "psnr_vs_ground_truth_db": round(psnr, 3),
         "kernels": dict(kern, binned_entries=e_binned, tiles=ntiles),
+        "warp_producers": producers,
     }
     if nmeta:
         line["irls"] = dict(meta_iterations=last["meta_iterations"], fista_steps=last["fista_steps"],
@@ -473,7 +482,7 @@ def main():
     up_avg_s = up_ms / max(up_n, 1) / 1e3 if up_n else None
     psnr = lfrv.psnr(U, scene["target"])
     tile = [ctx.params.tile_w or 32, ctx.params.tile_h or 8]  # 0 = library default 32x8
-    producers = warp_producer_times(ctx)
+    producers = warp_producer_times(ctx, scene)
     ctx.close()
     holes = hole_filling_times(scene)
     calib_copy(args)
@@ -507,7 +516,7 @@ def main():
         "true_rel_residual": [float("%.3g" % r) for r in st_last["true_rel_residual"]],
         "psnr_vs_ground_truth_db": round(psnr, 3),
         "kernels": {"binned_entries": e_binned, "unique_entries": e_unique, "tiles": ntiles},
-        "warp_producers_ms": producers,
+        "warp_producers": producers,
         "hole_filling": holes,
     }
     if up_n:
