@@ -32,41 +32,71 @@ __device__ __forceinline__ float key2f(int k) { return __int_as_float(k >= 0 ? k
 
 // Pass 1 of import_warps_from_depth (import.cpp:288-339) for every view:
 // tau = pi(d M x + c), partial tau (tauPartial.frag:38-47, import.cpp:658-680)
-// and the target z-buffer over a dsf x dsf footprint.
+// and the target z-buffer over a dsf x dsf footprint.  The block is 64 x 4:
+// one wave per row of 64 source pixels.  For dsf >= 2 the wave writes its
+// footprints row by row, dsf consecutive lanes on one source pixel's dsf
+// consecutive target pixels (64 / dsf source pixels per instruction), so an
+// atomic instruction touches 2-3 cache lines instead of one per source
+// pixel: device-scope atomics are paid per cache line.  The set of
+// (pixel, key) minima is the same in either order.
 __global__ void k_tau_from_depth(int V, int w, int h, int dsf, const float *__restrict__ depth,
                                  const WarpCam *__restrict__ cams, float sigma_z,
                                  float *__restrict__ tx, float *__restrict__ ty,
                                  float *__restrict__ dpx, float *__restrict__ dpy,
                                  int *__restrict__ zkey)
 {
-    SOURCE_INDEX()
+    const int mx = blockIdx.x * blockDim.x + threadIdx.x, my = blockIdx.y * blockDim.y + threadIdx.y;
+    const int v = blockIdx.z;
     const int W = w * dsf, H = h * dsf;
-    const WarpCam &cam = cams[v];
-    const float d = depth[k];
-    if (d < 1.0f / kInvalidDepth || kInvalidDepth < d) {
-        tx[k] = -(float)w;
-        ty[k] = -(float)h;
-        dpx[k] = 0.0f;
-        dpy[k] = 0.0f;
+    float tux = 0.0f, tuy = 0.0f;
+    int key = 0, live = 0;
+    if (mx < w && my < h) {
+        const size_t k = (size_t)v * w * h + (size_t)my * w + mx;
+        const WarpCam &cam = cams[v];
+        const float d = depth[k];
+        if (d < 1.0f / kInvalidDepth || kInvalidDepth < d) {
+            tx[k] = -(float)w;
+            ty[k] = -(float)h;
+            dpx[k] = 0.0f;
+            dpy[k] = 0.0f;
+        } else {
+            const float xx = (float)mx + 0.5f, yy = (float)my + 0.5f;
+            const float Mx0 = cam.M[0] * xx + cam.M[1] * yy + cam.M[2];
+            const float Mx1 = cam.M[3] * xx + cam.M[4] * yy + cam.M[5];
+            const float Mx2 = cam.M[6] * xx + cam.M[7] * yy + cam.M[8];
+            const float X0 = d * Mx0 + cam.c[0], X1 = d * Mx1 + cam.c[1], X2 = d * Mx2 + cam.c[2];
+            tux = X0 / X2;
+            tuy = X1 / X2;
+            tx[k] = tux;
+            ty[k] = tuy;
+            dpx[k] = sigma_z * (Mx0 - tux * Mx2) / X2;
+            dpy[k] = sigma_z * (Mx1 - tuy * Mx2) / X2;
+            key = f2key(X2);
+            live = 1;
+        }
+    }
+    if (dsf == 1) {
+        const int px = (int)tux, py = (int)tuy;
+        if (live && 0 <= px && px <= W - 1 && 0 <= py && py <= H - 1) atomicMin(zkey + (size_t)py * W + px, key);
         return;
     }
-    const float xx = (float)mx + 0.5f, yy = (float)my + 0.5f;
-    const float Mx0 = cam.M[0] * xx + cam.M[1] * yy + cam.M[2];
-    const float Mx1 = cam.M[3] * xx + cam.M[4] * yy + cam.M[5];
-    const float Mx2 = cam.M[6] * xx + cam.M[7] * yy + cam.M[8];
-    const float X0 = d * Mx0 + cam.c[0], X1 = d * Mx1 + cam.c[1], X2 = d * Mx2 + cam.c[2];
-    const float tux = X0 / X2, tuy = X1 / X2;
-    tx[k] = tux;
-    ty[k] = tuy;
-    dpx[k] = sigma_z * (Mx0 - tux * Mx2) / X2;
-    dpy[k] = sigma_z * (Mx1 - tuy * Mx2) / X2;
-    const int key = f2key(X2);
-    for (int i = 0; i < dsf; i++)
-        for (int j = 0; j < dsf; j++) {
-            const int px = (int)tux - dsf / 2 + j;
-            const int py = (int)tuy - dsf / 2 + i;
-            if (0 <= px && px <= W - 1 && 0 <= py && py <= H - 1) atomicMin(zkey + (size_t)py * W + px, key);
+    // every lane of the wave takes part in the shuffles below
+    const int lane = threadIdx.x;  // blockDim.x == 64
+    const int per = 64 / dsf, s_off = lane / dsf, j = lane - s_off * dsf;
+    for (int g0 = 0; g0 < 64; g0 += per) {
+        const int src = g0 + s_off;
+        const bool act = s_off < per && src < 64;
+        const int sl = act ? src : lane;
+        const float gx = __shfl(tux, sl, 64), gy = __shfl(tuy, sl, 64);
+        const int gk = __shfl(key, sl, 64), gl = __shfl(live, sl, 64);
+        if (!act || !gl) continue;
+        const int px = (int)gx - dsf / 2 + j;
+        if (px < 0 || px > W - 1) continue;
+        for (int i = 0; i < dsf; i++) {
+            const int py = (int)gy - dsf / 2 + i;
+            if (0 <= py && py <= H - 1) atomicMin(zkey + (size_t)py * W + px, gk);
         }
+    }
 }
 
 // Fills the z-buffer keys; also clears the ncnt cell counters.
