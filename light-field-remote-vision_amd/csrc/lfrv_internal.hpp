@@ -129,7 +129,9 @@ __device__ __forceinline__ void block_sum(double (&v)[NQ], double *lds /* >= NQ 
 
 // Grid-wide reductions without device-scope fences.  The L2 caches of the
 // XCDs are not coherent with each other, so a device-scope release after
-// every block (the classic last-block pattern) writes back an L2 per block.
+// every block (the classic last-block pattern) writes back an L2 per block:
+// measured on the 1024 x 1024 PCG iteration (2048 blocks), that pattern made
+// the pass 80 us long, against 32 us plus an 11 us finalize launch.
 // Instead each block stores its NQ partial sums (layout [q][block], plain
 // stores) and a one-block finalize kernel launched after it on the same
 // stream (the kernel boundary orders the stores) reduces them in block order.
