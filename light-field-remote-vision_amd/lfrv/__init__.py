@@ -60,6 +60,14 @@ class CgStats(ctypes.Structure):
                     status=self.status, ms=self.ms)
 
 
+def _with_status(stats, status):
+    """The stats as a dict whose status is the call's own: a call rejected
+    before the solve (bad arguments, wrong state) leaves the struct unfilled."""
+    d = stats.as_dict()
+    d["status"] = status
+    return d
+
+
 class SrStats(ctypes.Structure):
     _fields_ = [
         ("meta_iterations", ctypes.c_int), ("fista_steps", ctypes.c_int), ("last_stop", ctypes.c_int),
@@ -258,7 +266,7 @@ class Context:
         st = lib().lfrv_cg_solve(self._h, tol, maxit, ctypes.byref(s))
         if check:
             self._check(st)
-        return s.as_dict()
+        return _with_status(s, st)
 
     def ibr_gradient(self, tol=1e-5, maxit=5000, check=True):
         """The whole device-resident IBR_gradient sequence in one call (lfrv_ibr_gradient)."""
@@ -266,14 +274,14 @@ class Context:
         st = lib().lfrv_ibr_gradient(self._h, tol, maxit, ctypes.byref(s))
         if check:
             self._check(st)
-        return s.as_dict()
+        return _with_status(s, st)
 
     def poisson_cg(self, lam=1.0, tol=1e-5, maxit=1000, precond=False, check=True):
         s = CgStats()
         st = lib().lfrv_poisson_cg(self._h, lam, tol, maxit, int(precond), ctypes.byref(s))
         if check:
             self._check(st)
-        return s.as_dict()
+        return _with_status(s, st)
 
     def gibr_energy(self):
         e = ctypes.c_double()
