@@ -151,3 +151,31 @@ def test_product_path_fails_loudly_without_a_device():
     from lfrv import pipelines
     with pytest.raises(lfrv.LfrvError):
         lfrv.Context(pipelines.gradient_ibr_params(s))
+
+
+BAD_PARAMS = [
+    ("nviews", 0), ("w", 2), ("h", 2), ("dsf", 0), ("dsf", 5), ("mode", 7),
+    ("dw_type", 1),            # the reference asserts on the other deformation weights (gradientIBR.cu:636-641)
+    ("pull_lds_kb", 200),      # more LDS than a CU has
+    ("tile_w", 1),             # narrower than a footprint
+    ("tile_h", 64),            # window rows beyond one wave
+]
+
+
+@pytest.mark.parametrize("field,value", BAD_PARAMS)
+def test_create_rejects_bad_parameters(field, value):
+    """lfrv_create checks its parameters before it touches the device: each
+    bad field alone gives LFRV_E_ARG and no context."""
+    p = lfrv.default_params(lfrv.MODE_GRADIENT_IBR, 9, 64, 48, 1)
+    setattr(p, field, value)
+    h = ctypes.c_void_p(1)
+    assert lfrv.lib().lfrv_create(ctypes.byref(p), ctypes.byref(h)) == lfrv.E_ARG
+    assert not h.value
+
+
+def test_create_rejects_a_colour_problem_without_tv_weight():
+    p = lfrv.default_params(lfrv.MODE_COLOR_SR, 9, 64, 48, 2, lambda_=0.0)
+    h = ctypes.c_void_p(1)
+    assert lfrv.lib().lfrv_create(ctypes.byref(p), ctypes.byref(h)) == lfrv.E_ARG
+    assert not h.value
+    assert lfrv.lib().lfrv_create(None, ctypes.byref(h)) == lfrv.E_ARG
