@@ -227,18 +227,20 @@ __global__ void k_fill_u64(unsigned long long *__restrict__ p, size_t n, unsigne
 // BR, TR into the target.  The depth test keeps the nearest quad at each
 // covered pixel centre: a 64-bit atomicMin of (interpolated depth, target
 // depth of the quad's centre).
-__global__ void k_gl_splat(int V, int w, int h, int W, int H, int skip, const float *__restrict__ depth,
-                           const float *__restrict__ grad, const WarpCam *__restrict__ cams,
-                           unsigned long long *__restrict__ key)
+//
+// The quad of source pixel k = (v, mx, my): its corners (x, y, h2) in the
+// target, the low word of its depth-test key and the box of target pixels it
+// may cover, [px0, px1] x [py0, py1] (empty when px0 > px1 or py0 > py1);
+// false when the quad is not drawn.
+__device__ __forceinline__ bool gl_quad(int V, int W, int H, size_t n, size_t k, int mx, int my,
+                                        const float *__restrict__ depth, const float *__restrict__ grad,
+                                        const WarpCam &cam, float (&q)[4][3], unsigned int &lo, int &px0,
+                                        int &px1, int &py0, int &py1)
 {
-    SOURCE_INDEX()
-    if (v == skip) return;
     const float z = depth[k];
-    if (z >= 0.5f * kInvalidDepth) return;
+    if (z >= 0.5f * kInvalidDepth) return false;
     const float gx = grad ? grad[k] : 0.0f, gy = grad ? grad[(size_t)V * n + k] : 0.0f;
-    const WarpCam &cam = cams[v];
     const float x = (float)mx + 0.5f, y = (float)my + 0.5f;
-    float q[4][3];
     bool ok = true;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
@@ -252,7 +254,7 @@ __global__ void k_gl_splat(int V, int w, int h, int W, int H, int skip, const fl
         q[c][1] = h1 / h2;
         q[c][2] = h2;
     }
-    if (!ok) return;
+    if (!ok) return false;
     const float zval = z * (cam.M[6] * x + cam.M[7] * y + cam.M[8]) + cam.c[2];
     float u0 = q[0][0], u1 = q[0][0], v0 = q[0][1], v1 = q[0][1];
 #pragma unroll
@@ -264,22 +266,91 @@ __global__ void k_gl_splat(int V, int w, int h, int W, int H, int skip, const fl
     }
     // the span test in float first: a quad near the camera plane can project far
     // beyond the int range
-    if (!(u1 - u0 <= (float)kGlMaxSpan && v1 - v0 <= (float)kGlMaxSpan)) return;
-    if (u1 < 0.0f || v1 < 0.0f || u0 > (float)W || v0 > (float)H) return;
-    int px0 = (int)ceilf(u0 - 0.5f), px1 = (int)floorf(u1 - 0.5f);
-    int py0 = (int)ceilf(v0 - 0.5f), py1 = (int)floorf(v1 - 0.5f);
-    px0 = max(px0, 0);
-    py0 = max(py0, 0);
-    px1 = min(px1, W - 1);
-    py1 = min(py1, H - 1);
-    const unsigned long long lo = __float_as_uint(zval);
+    if (!(u1 - u0 <= (float)kGlMaxSpan && v1 - v0 <= (float)kGlMaxSpan)) return false;
+    if (u1 < 0.0f || v1 < 0.0f || u0 > (float)W || v0 > (float)H) return false;
+    px0 = max((int)ceilf(u0 - 0.5f), 0);
+    px1 = min((int)floorf(u1 - 0.5f), W - 1);
+    py0 = max((int)ceilf(v0 - 0.5f), 0);
+    py1 = min((int)floorf(v1 - 0.5f), H - 1);
+    lo = __float_as_uint(zval);
+    return true;
+}
+
+// Target pixel (px, py) against the quad's two triangles; the depth test.
+__device__ __forceinline__ void gl_cover(const float (&q)[4][3], unsigned int lo, int px, int py, int W,
+                                         unsigned long long *__restrict__ key)
+{
+    float zi;
+    const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+    if (!gl_tri(q[0], q[1], q[2], cx, cy, zi) && !gl_tri(q[2], q[1], q[3], cx, cy, zi)) return;
+    atomicMin(key + (size_t)py * W + px, ((unsigned long long)__float_as_uint(zi) << 32) | lo);
+}
+
+// One quad per thread (dsf 1: a quad covers one or two target pixels, and
+// the lanes of a wave already write neighbouring pixels).
+__global__ void k_gl_splat(int V, int w, int h, int W, int H, int skip, const float *__restrict__ depth,
+                           const float *__restrict__ grad, const WarpCam *__restrict__ cams,
+                           unsigned long long *__restrict__ key)
+{
+    SOURCE_INDEX()
+    if (v == skip) return;
+    float q[4][3];
+    unsigned int lo;
+    int px0, px1, py0, py1;
+    if (!gl_quad(V, W, H, n, k, mx, my, depth, grad, cams[v], q, lo, px0, px1, py0, py1)) return;
     for (int py = py0; py <= py1; py++)
-        for (int px = px0; px <= px1; px++) {
-            float zi;
-            const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
-            if (!gl_tri(q[0], q[1], q[2], cx, cy, zi) && !gl_tri(q[2], q[1], q[3], cx, cy, zi)) continue;
-            atomicMin(key + (size_t)py * W + px, ((unsigned long long)__float_as_uint(zi) << 32) | lo);
-        }
+        for (int px = px0; px <= px1; px++) gl_cover(q, lo, px, py, W, key);
+}
+
+// dsf >= 2: a quad's box is about dsf + 1 pixels wide.  The columns of the 64
+// boxes of a wave (one row of source pixels, blockDim.x == 64) are dealt to
+// the lanes in order, 64 at a time, and each lane walks its column's rows, so
+// an atomic instruction writes consecutive target pixels of a row: a few
+// cache lines instead of one per quad (device-scope atomics are paid per
+// line).  Every (quad, pixel) pair of k_gl_splat is tested once, with the
+// same arithmetic: the same minima.
+__global__ void k_gl_splat_wave(int V, int w, int h, int W, int H, int skip, const float *__restrict__ depth,
+                                const float *__restrict__ grad, const WarpCam *__restrict__ cams,
+                                unsigned long long *__restrict__ key)
+{
+    const int mx = blockIdx.x * blockDim.x + threadIdx.x, my = blockIdx.y * blockDim.y + threadIdx.y;
+    const int v = blockIdx.z, lane = threadIdx.x;
+    const size_t n = (size_t)w * h;
+    float q[4][3] = {};
+    unsigned int lo = 0;
+    int px0 = 0, px1 = -1, py0 = 0, py1 = -1;
+    bool ok = mx < w && my < h && v != skip;
+    if (ok)
+        ok = gl_quad(V, W, H, n, (size_t)v * n + (size_t)my * w + mx, mx, my, depth, grad, cams[v], q, lo, px0,
+                     px1, py0, py1);
+    const int bw = ok && py0 <= py1 ? max(px1 - px0 + 1, 0) : 0;
+    // exclusive offsets of the boxes' columns in lane order (every lane takes
+    // part in the shuffles below)
+    int inc = bw;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += u;
+    }
+    const int exc = inc - bw, total = __shfl(inc, 63, 64);
+    for (int base = 0; base < total; base += 64) {
+        const int f = base + lane;
+        // the owner: the last lane whose offset is <= f (it has columns, and f is one of them)
+        int o = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if (__shfl(exc, o + step, 64) <= f) o += step;
+        float oq[4][3];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) oq[c][d] = __shfl(q[c][d], o, 64);
+        const unsigned int olo = (unsigned int)__shfl((int)lo, o, 64);
+        const int px = __shfl(px0, o, 64) + (f - __shfl(exc, o, 64));
+        const int oy0 = __shfl(py0, o, 64), oy1 = __shfl(py1, o, 64);
+        if (f >= total) continue;
+        for (int py = oy0; py <= oy1; py++) gl_cover(oq, olo, px, py, W, key);
+    }
 }
 
 // The target depth map: the winning quad's centre depth, 0 where no quad
@@ -727,8 +798,12 @@ lfrv_status run_warps_from_target_depth(lfrv_ctx *ctx, int skip)
     unsigned long long *key = reinterpret_cast<unsigned long long *>(ctx->tmp3N);
     TimedRegion tr(ctx, 4);
     k_fill_u64<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(key, ctx->N, ~0ull);
-    k_gl_splat<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, skip,
-                                                                   ctx->depth, ctx->depth_grad, ctx->dcams, key);
+    if (ctx->dsf == 1)
+        k_gl_splat<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, skip,
+                                                                       ctx->depth, ctx->depth_grad, ctx->dcams, key);
+    else
+        k_gl_splat_wave<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, skip,
+                                                                       ctx->depth, ctx->depth_grad, ctx->dcams, key);
     k_gl_depth<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(key, ctx->target_depth, ctx->N);
     k_gl_tau<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
         ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, ctx->depth, ctx->depth_grad, ctx->dcams, ctx->target_depth,

@@ -13,10 +13,11 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _scene(w=48, h=40, seed=4):
+def _scene(w=48, h=40, seed=4, dsf=1):
     d = lfrv.synth_desc(2, seed)
     d.w, d.h = w, h
     d.focal = 1.5 * w      # disparities up to a few pixels, with occlusions between the planes
+    d.dsf = d.render_scale = dsf
     return lfrv.synth(d)
 
 
@@ -26,13 +27,16 @@ def _grads(s, seed):
              r.uniform(-0.02, 0.02, (s["h"], s["w"])).astype(np.float32)) for _ in range(s["V"])]
 
 
-@pytest.mark.parametrize("with_grad,skip", [(False, -1), (True, -1), (False, 4)])
-def test_gl_producer_matches_oracle(with_grad, skip):
-    s = _scene()
+# dsf 1 draws one quad per thread (k_gl_splat); dsf >= 2 deals the quads'
+# columns to the lanes of a wave (k_gl_splat_wave)
+@pytest.mark.parametrize("with_grad,skip,dsf", [(False, -1, 1), (True, -1, 1), (False, 4, 1), (False, -1, 2),
+                                                (True, 4, 3)])
+def test_gl_producer_matches_oracle(with_grad, skip, dsf):
+    s = _scene(dsf=dsf)
     V, h, w = s["V"], s["h"], s["w"]
     W, H = s["W"], s["H"]
     g = _grads(s, 9) if with_grad else None
-    print(f"gl producer grad={with_grad} skip={skip}", flush=True)
+    print(f"gl producer grad={with_grad} skip={skip} dsf={dsf}", flush=True)
     with lfrv.Context(pipelines.gradient_ibr_params(s)) as ctx:
         pipelines.load_scene(ctx, s)
         if g:
