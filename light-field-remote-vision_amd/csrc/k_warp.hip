@@ -556,9 +556,9 @@ __global__ void k_view_stencils(int V, int w, int h, const float *__restrict__ v
 }
 
 // Sobel/8 of the luminance of every view (gradientIBR_kernels.cuh:57-99),
-// LDS-tiled: a 64 x 4 tile computes Y once per pixel of the tile and its
+// LDS-tiled: a 64 x 16 tile computes Y once per pixel of the tile and its
 // 1-pixel halo, then the 3 x 3 stencil reads LDS.  grid = (tiles x, tiles y, V).
-constexpr int kVsW = 64, kVsH = 4;
+constexpr int kVsW = 64, kVsH = 16;
 
 __global__ void __launch_bounds__(kVsW * kVsH) k_view_sobel(int w, int h, const float *__restrict__ views,
                                                            float *__restrict__ vgrad)
