@@ -772,34 +772,59 @@ struct EnergyCoef {
 
 // The energy of the driver, E = lambda E_TV + dt_alpha E_int + dt_beta E_grad
 // (optimization.cu:1710), from the block partials; the driver keeps it in
-// float (super-resolution.cpp:181, 230).  BEGIN: the pass's starting energy
-// (:207-209), which also opens the pass.  Otherwise, after a FISTA step
-// (:230-244): E < 1e-4 stops; |(E - E_prev) / E| < 1e-4 stops; else E_prev = E.
-template <bool BEGIN>
-__global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_energy(SrLoop *L, const double *__restrict__ ptv, int Gtv,
-                                                                   const double *__restrict__ pviews, int Gv,
-                                                                   EnergyCoef c, float eps)
+// float (super-resolution.cpp:181, 230).  The sum in the host's order,
+// without contraction into FMAs.
+__device__ __forceinline__ float sr_loop_energy(const double (&tv)[1], const double (&dv)[2], const EnergyCoef &c)
 {
-    if (!BEGIN && L->halt) return;
+    return (float)__dadd_rn(__dadd_rn(__dmul_rn(c.lambda, tv[0]), __dmul_rn(c.alpha, dv[0])),
+                            __dmul_rn(c.beta, dv[1]));
+}
+
+// The pass's starting energy (super-resolution.cpp:207-209), which also opens
+// the pass.
+__global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_begin(SrLoop *L, const double *__restrict__ ptv, int Gtv,
+                                                                  const double *__restrict__ pviews, int Gv,
+                                                                  EnergyCoef c)
+{
     __shared__ double red[2 * 16];
     double tv[1], dv[2];
     reduce_partials<1>(ptv, Gtv, tv, red);
     reduce_partials<2>(pviews, Gv, dv, red);
     if (threadIdx.x != 0) return;
-    // the host's sum order, without contraction into FMAs
-    const float E = (float)__dadd_rn(__dadd_rn(__dmul_rn(c.lambda, tv[0]), __dmul_rn(c.alpha, dv[0])),
-                                     __dmul_rn(c.beta, dv[1]));
+    const float E = sr_loop_energy(tv, dv, c);
     L->parts[0] = tv[0];
     L->parts[1] = dv[0];
     L->parts[2] = dv[1];
     L->energy = E;
-    if (BEGIN) {
-        L->halt = 0;
-        L->steps = 0;
-        L->stop = LFRV_SR_STOP_MAXIT;
-        L->prev_energy = E;
-        return;
-    }
+    L->halt = 0;
+    L->steps = 0;
+    L->stop = LFRV_SR_STOP_MAXIT;
+    L->prev_energy = E;
+}
+
+// The rules after a FISTA step, in the driver's order: E < 1e-4 stops;
+// |(E - E_prev) / E| < 1e-4 stops; else E_prev = E (super-resolution.cpp:
+// 230-244), and fewer than pct of the pixels moved against the previous
+// step's solution stops (:246-257), on the count k_count_moving took before
+// this launch (its block partials at pmove).
+__global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_step(SrLoop *L, const double *__restrict__ ptv, int Gtv,
+                                                                 const double *__restrict__ pviews, int Gv,
+                                                                 EnergyCoef c, float eps,
+                                                                 const double *__restrict__ pmove, int Gm,
+                                                                 float npix, float pct)
+{
+    if (L->halt) return;
+    __shared__ double red[2 * 16];
+    double tv[1], dv[2], m[1];
+    reduce_partials<1>(ptv, Gtv, tv, red);
+    reduce_partials<2>(pviews, Gv, dv, red);
+    reduce_partials<1>(pmove, Gm, m, red);
+    if (threadIdx.x != 0) return;
+    const float E = sr_loop_energy(tv, dv, c);
+    L->parts[0] = tv[0];
+    L->parts[1] = dv[0];
+    L->parts[2] = dv[1];
+    L->energy = E;
     L->steps += 1;
     if (E < eps) {
         L->halt = 1;
@@ -809,21 +834,10 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_energy(SrLoop *L, 
         L->stop = LFRV_SR_STOP_RELATIVE;
     } else {
         L->prev_energy = E;
-    }
-}
-
-// The moving-pixel rule after a step (super-resolution.cpp:246-257): fewer
-// than pct of the pixels moved against the previous step's solution stops.
-__global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_moving(SrLoop *L, const double *__restrict__ partials,
-                                                                   int G, float npix, float pct)
-{
-    if (L->halt) return;
-    __shared__ double red[16];
-    double m[1];
-    reduce_partials<1>(partials, G, m, red);
-    if (threadIdx.x == 0 && (float)m[0] / npix < pct) {
-        L->halt = 1;
-        L->stop = LFRV_SR_STOP_MOVING;
+        if ((float)m[0] / npix < pct) {
+            L->halt = 1;
+            L->stop = LFRV_SR_STOP_MOVING;
+        }
     }
 }
 
@@ -1054,6 +1068,9 @@ lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *out
     const EnergyCoef ec = energy_coef(ctx);
     const int Gtv = (int)rblocks(N), Gv = (int)rblocks((size_t)ctx->V * ctx->n), Gm = (int)rblocks(N);
     const float npix = (float)N;
+    // the moving count's block partials, after the energy's (E_TV: 2048, views: 2 x 2048)
+    double *pmove = ctx->partials + 3 * kViewPartials;
+    if (3 * kViewPartials + (size_t)Gm > ctx->partials_cap) return fail(ctx, LFRV_E_STATE, "partials buffer too small");
     lfrv_sr_stats s{};
     float energy = 0.0f, previous_energy_reweight = 0.0f;
     int r;
@@ -1062,9 +1079,8 @@ lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *out
         if (st == LFRV_OK) st = run_regularizer_weight(ctx);
         if (st == LFRV_OK) st = enqueue_sr_energy(ctx, nullptr);
         if (st != LFRV_OK) return st;
-        k_sr_loop_energy<true><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->sr_loop, ctx->partials, Gtv,
-                                                                        ctx->partials + kViewPartials, Gv, ec,
-                                                                        EPSILON_TOL);
+        k_sr_loop_begin<<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->sr_loop, ctx->partials, Gtv,
+                                                                 ctx->partials + kViewPartials, Gv, ec);
         LFRV_LAUNCH(ctx, "sr loop begin");
         // alpha and the ping-pong buffers after each enqueued step, to undo the
         // host-side effects of the steps the device skipped
@@ -1085,11 +1101,10 @@ lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *out
                 if (st == LFRV_OK) st = enqueue_sr_energy(ctx, halt);
                 if (st != LFRV_OK) return st;
                 states.push_back(snap());
-                k_sr_loop_energy<false><<<1, kFinalizeThreads, 0, ctx->stream>>>(
-                    ctx->sr_loop, ctx->partials, Gtv, ctx->partials + kViewPartials, Gv, ec, EPSILON_TOL);
-                k_count_moving<<<Gm, 256, 0, ctx->stream>>>(N, ctx->U, prev, MAX_DIFF_THRESHOLD, ctx->partials, halt);
-                k_sr_loop_moving<<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->sr_loop, ctx->partials, Gm, npix,
-                                                                         PERCENT_MOVING);
+                k_count_moving<<<Gm, 256, 0, ctx->stream>>>(N, ctx->U, prev, MAX_DIFF_THRESHOLD, pmove, halt);
+                k_sr_loop_step<<<1, kFinalizeThreads, 0, ctx->stream>>>(
+                    ctx->sr_loop, ctx->partials, Gtv, ctx->partials + kViewPartials, Gv, ec, EPSILON_TOL, pmove, Gm,
+                    npix, PERCENT_MOVING);
                 k_copy_unless_halted<<<blocks_for(n3), kBlock, 0, ctx->stream>>>(n3, ctx->U, prev, halt);
                 LFRV_LAUNCH(ctx, "sr loop step");
             }
