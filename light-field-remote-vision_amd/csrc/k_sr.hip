@@ -652,15 +652,23 @@ __global__ void k_overrelax(size_t n3, float theta, float *__restrict__ uq, floa
 
 // E_TV over pixels whose 3x3 neighbourhood is visible, and the ungated Sobel
 // of Y(U) for the gradient term (optimization.cu:1505-1534, 1622-1625).
+// With prev, the same pass also counts the pixels that moved against prev
+// (the count of k_count_moving, block partials at pmove).
 __global__ void __launch_bounds__(256) k_sr_energy_tv(int W, int H, const float *__restrict__ U,
                                                       const float *__restrict__ vis, float *__restrict__ ugx,
-                                                      float *__restrict__ ugy, double *partials, const int *halt)
+                                                      float *__restrict__ ugy, double *partials, const int *halt,
+                                                      const float *__restrict__ prev, float thr, double *pmove)
 {
     if (halt && *halt) return;
-    __shared__ double red[16];
+    __shared__ double red[16], red_m[16];
     const size_t N = (size_t)W * H;
-    double v[1] = {0.0};
+    double v[1] = {0.0}, vm[1] = {0.0};
     for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < N; o += (size_t)gridDim.x * blockDim.x) {
+        if (prev) {
+            double mx = 0.0;
+            for (int c = 0; c < 3; c++) mx = fmax(mx, fabs((double)U[c * N + o] - (double)prev[c * N + o]));
+            if (!(mx < (double)thr)) vm[0] += 1.0;
+        }
         const int ox = (int)(o % W), oy = (int)(o / W);
         float GX = 0.0f, GY = 0.0f;
         bool ok = false;
@@ -682,6 +690,7 @@ __global__ void __launch_bounds__(256) k_sr_energy_tv(int W, int H, const float 
         if (ok) v[0] += fabs((double)GX) + fabs((double)GY);
     }
     block_partials<1>(v, partials, red);
+    if (prev) block_partials<1>(vm, pmove, red_m);
 }
 
 // E_int and E_grad over every (view, pixel) with gather validity
@@ -805,8 +814,8 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_begin(SrLoop *L, c
 // The rules after a FISTA step, in the driver's order: E < 1e-4 stops;
 // |(E - E_prev) / E| < 1e-4 stops; else E_prev = E (super-resolution.cpp:
 // 230-244), and fewer than pct of the pixels moved against the previous
-// step's solution stops (:246-257), on the count k_count_moving took before
-// this launch (its block partials at pmove).
+// step's solution stops (:246-257), on the count the step's energy pass took
+// (k_sr_energy_tv with prev; its block partials at pmove).
 __global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_step(SrLoop *L, const double *__restrict__ ptv, int Gtv,
                                                                  const double *__restrict__ pviews, int Gv,
                                                                  EnergyCoef c, float eps,
@@ -980,12 +989,13 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
 
 // The energy's block partials (tv at partials[0], views at
 // partials + kViewPartials); ugx / ugy in tmp3N.
-lfrv_status enqueue_sr_energy(lfrv_ctx *ctx, const int *halt)
+lfrv_status enqueue_sr_energy(lfrv_ctx *ctx, const int *halt, const float *prev = nullptr, float thr = 0.0f,
+                              double *pmove = nullptr)
 {
     const size_t N = ctx->N;
     float *ugx = ctx->tmp3N, *ugy = ctx->tmp3N + N;
     k_sr_energy_tv<<<rblocks(N), 256, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->visibility, ugx, ugy,
-                                                        ctx->partials, halt);
+                                                        ctx->partials, halt, prev, thr, pmove);
     const size_t tot = (size_t)ctx->V * ctx->n;
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
     double *pv = ctx->partials + kViewPartials;
@@ -1098,10 +1108,10 @@ lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *out
             const int chunk = std::min(kSrPollSteps, niter - enq);
             for (int j = 0; j < chunk; j++) {
                 st = enqueue_fista(ctx, halt);
-                if (st == LFRV_OK) st = enqueue_sr_energy(ctx, halt);
+                // the energy pass also takes the moving count (rblocks(N) == Gm block partials)
+                if (st == LFRV_OK) st = enqueue_sr_energy(ctx, halt, prev, MAX_DIFF_THRESHOLD, pmove);
                 if (st != LFRV_OK) return st;
                 states.push_back(snap());
-                k_count_moving<<<Gm, 256, 0, ctx->stream>>>(N, ctx->U, prev, MAX_DIFF_THRESHOLD, pmove, halt);
                 k_sr_loop_step<<<1, kFinalizeThreads, 0, ctx->stream>>>(
                     ctx->sr_loop, ctx->partials, Gtv, ctx->partials + kViewPartials, Gv, ec, EPSILON_TOL, pmove, Gm,
                     npix, PERCENT_MOVING);
