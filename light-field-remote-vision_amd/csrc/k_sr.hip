@@ -652,21 +652,32 @@ __global__ void k_overrelax(size_t n3, float theta, float *__restrict__ uq, floa
 
 // E_TV over pixels whose 3x3 neighbourhood is visible, and the ungated Sobel
 // of Y(U) for the gradient term (optimization.cu:1505-1534, 1622-1625).
-// With prev, the same pass also counts the pixels that moved against prev
-// (the count of k_count_moving, block partials at pmove).
+// With prev2 (two 3N buffers, the loop state L saying which one holds the
+// previous solution), the same pass also counts the pixels of U that moved
+// against it (the count of k_count_moving, block partials at pmove) and
+// writes U into the other buffer: if the step's rules keep it, that buffer
+// becomes the previous solution (k_sr_loop_step flips L->cur).
 __global__ void __launch_bounds__(256) k_sr_energy_tv(int W, int H, const float *__restrict__ U,
                                                       const float *__restrict__ vis, float *__restrict__ ugx,
                                                       float *__restrict__ ugy, double *partials, const int *halt,
-                                                      const float *__restrict__ prev, float thr, double *pmove)
+                                                      float *__restrict__ prev2, const SrLoop *L, float thr,
+                                                      double *pmove)
 {
     if (halt && *halt) return;
     __shared__ double red[16], red_m[16];
     const size_t N = (size_t)W * H;
+    const int cur = prev2 ? L->cur : 0;
+    const float *prev = prev2 ? prev2 + (size_t)cur * 3 * N : nullptr;
+    float *next = prev2 ? prev2 + (size_t)(cur ^ 1) * 3 * N : nullptr;
     double v[1] = {0.0}, vm[1] = {0.0};
     for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < N; o += (size_t)gridDim.x * blockDim.x) {
-        if (prev) {
+        if (prev2) {
             double mx = 0.0;
-            for (int c = 0; c < 3; c++) mx = fmax(mx, fabs((double)U[c * N + o] - (double)prev[c * N + o]));
+            for (int c = 0; c < 3; c++) {
+                const float u = U[c * N + o];
+                mx = fmax(mx, fabs((double)u - (double)prev[c * N + o]));
+                next[c * N + o] = u;
+            }
             if (!(mx < (double)thr)) vm[0] += 1.0;
         }
         const int ox = (int)(o % W), oy = (int)(o / W);
@@ -690,7 +701,7 @@ __global__ void __launch_bounds__(256) k_sr_energy_tv(int W, int H, const float 
         if (ok) v[0] += fabs((double)GX) + fabs((double)GY);
     }
     block_partials<1>(v, partials, red);
-    if (prev) block_partials<1>(vm, pmove, red_m);
+    if (prev2) block_partials<1>(vm, pmove, red_m);
 }
 
 // E_int and E_grad over every (view, pixel) with gather validity
@@ -815,7 +826,8 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_begin(SrLoop *L, c
 // |(E - E_prev) / E| < 1e-4 stops; else E_prev = E (super-resolution.cpp:
 // 230-244), and fewer than pct of the pixels moved against the previous
 // step's solution stops (:246-257), on the count the step's energy pass took
-// (k_sr_energy_tv with prev; its block partials at pmove).
+// (k_sr_energy_tv with prev2; its block partials at pmove); a step the rules
+// keep becomes the previous solution.
 __global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_step(SrLoop *L, const double *__restrict__ ptv, int Gtv,
                                                                  const double *__restrict__ pviews, int Gv,
                                                                  EnergyCoef c, float eps,
@@ -846,17 +858,10 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_step(SrLoop *L, co
         if ((float)m[0] / npix < pct) {
             L->halt = 1;
             L->stop = LFRV_SR_STOP_MOVING;
+        } else {
+            L->cur ^= 1;  // prev <- U (super-resolution.cpp:255): the energy pass wrote it
         }
     }
-}
-
-// prev <- U unless a rule fired in this step (super-resolution.cpp:255).
-__global__ void k_copy_unless_halted(size_t n, const float *__restrict__ src, float *__restrict__ dst,
-                                     const int *halt)
-{
-    if (*halt) return;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) dst[k] = src[k];
 }
 
 }  // namespace
@@ -989,13 +994,13 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
 
 // The energy's block partials (tv at partials[0], views at
 // partials + kViewPartials); ugx / ugy in tmp3N.
-lfrv_status enqueue_sr_energy(lfrv_ctx *ctx, const int *halt, const float *prev = nullptr, float thr = 0.0f,
+lfrv_status enqueue_sr_energy(lfrv_ctx *ctx, const int *halt, float *prev2 = nullptr, float thr = 0.0f,
                               double *pmove = nullptr)
 {
     const size_t N = ctx->N;
     float *ugx = ctx->tmp3N, *ugy = ctx->tmp3N + N;
     k_sr_energy_tv<<<rblocks(N), 256, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->visibility, ugx, ugy,
-                                                        ctx->partials, halt, prev, thr, pmove);
+                                                        ctx->partials, halt, prev2, ctx->sr_loop, thr, pmove);
     const size_t tot = (size_t)ctx->V * ctx->n;
     const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
     double *pv = ctx->partials + kViewPartials;
@@ -1070,9 +1075,11 @@ lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *out
     const float EPSILON_TOL = 1e-4f, MAX_DIFF_THRESHOLD = 1e-3f, PERCENT_MOVING = 0.01f;
     const auto t0 = std::chrono::steady_clock::now();
     const size_t N = ctx->N, n3 = 3 * N;
-    if (!ctx->sr_prev) LFRV_HIP(ctx, hipMalloc((void **)&ctx->sr_prev, 2 * n3 * sizeof(float)));
-    float *prev = ctx->sr_prev, *prev_meta = ctx->sr_prev + n3;
-    LFRV_HIP(ctx, hipMemcpyAsync(prev, ctx->U, n3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    // the previous step's solution (two buffers, SrLoop::cur picks) and the previous pass's
+    if (!ctx->sr_prev) LFRV_HIP(ctx, hipMalloc((void **)&ctx->sr_prev, 3 * n3 * sizeof(float)));
+    float *prev2 = ctx->sr_prev, *prev_meta = ctx->sr_prev + 2 * n3;
+    LFRV_HIP(ctx, hipMemcpyAsync(prev2, ctx->U, n3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    LFRV_HIP(ctx, hipMemsetAsync(&ctx->sr_loop->cur, 0, sizeof(int), ctx->stream));
     LFRV_HIP(ctx, hipMemcpyAsync(prev_meta, ctx->U, n3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
     const int *halt = &ctx->sr_loop->halt;
     const EnergyCoef ec = energy_coef(ctx);
@@ -1109,13 +1116,12 @@ lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *out
             for (int j = 0; j < chunk; j++) {
                 st = enqueue_fista(ctx, halt);
                 // the energy pass also takes the moving count (rblocks(N) == Gm block partials)
-                if (st == LFRV_OK) st = enqueue_sr_energy(ctx, halt, prev, MAX_DIFF_THRESHOLD, pmove);
+                if (st == LFRV_OK) st = enqueue_sr_energy(ctx, halt, prev2, MAX_DIFF_THRESHOLD, pmove);
                 if (st != LFRV_OK) return st;
                 states.push_back(snap());
                 k_sr_loop_step<<<1, kFinalizeThreads, 0, ctx->stream>>>(
                     ctx->sr_loop, ctx->partials, Gtv, ctx->partials + kViewPartials, Gv, ec, EPSILON_TOL, pmove, Gm,
                     npix, PERCENT_MOVING);
-                k_copy_unless_halted<<<blocks_for(n3), kBlock, 0, ctx->stream>>>(n3, ctx->U, prev, halt);
                 LFRV_LAUNCH(ctx, "sr loop step");
             }
             enq += chunk;

@@ -245,7 +245,7 @@ struct SrLoop {
     int halt;    // a rule fired: the steps enqueued after it return at once
     int steps;   // FISTA steps run in this pass
     int stop;    // LFRV_SR_STOP_* of the pass
-    int pad;
+    int cur;     // which of the two previous-solution buffers holds the last step the rules kept
     float energy, prev_energy;  // float, as the driver keeps them
     double parts[3];            // E_TV, E_int, E_grad of the last energy
 };
