@@ -22,8 +22,6 @@ lfrv_status hip_fail(lfrv_ctx *ctx, hipError_t e, const char *what)
 
 // ---- event timing ------------------------------------------------------------
 
-static std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> &pending(lfrv_ctx *ctx);
-
 static hipEvent_t get_event(lfrv_ctx *ctx)
 {
     if (!ctx->ev_pool.empty()) {
@@ -52,8 +50,8 @@ TimedRegion::~TimedRegion()
     b = get_event(ctx);
     if (!b) return;
     (void)hipEventRecord(b, ctx->stream);
-    pending(ctx).emplace_back(k, a, b);
-    if (pending(ctx).size() > 4096) flush_timing(ctx);
+    ctx->ev_pending.emplace_back(k, a, b);
+    if (ctx->ev_pending.size() > 4096) flush_timing(ctx);
 }
 
 KernelEvents kernel_events(lfrv_ctx *ctx, int k)
@@ -72,24 +70,19 @@ KernelEvents kernel_events(lfrv_ctx *ctx, int k)
 void record_kernel_events(lfrv_ctx *ctx, int k, const KernelEvents &ev)
 {
     if (!ev.a || !ev.b) return;
-    pending(ctx).emplace_back(k, ev.a, ev.b);
-    if (pending(ctx).size() > 4096) flush_timing(ctx);
+    ctx->ev_pending.emplace_back(k, ev.a, ev.b);
+    if (ctx->ev_pending.size() > 4096) flush_timing(ctx);
 }
 
 }  // namespace lfrv
 
-// pending event pairs live beside the context (kept out of the header)
-#include <unordered_map>
 namespace lfrv {
-static std::unordered_map<const lfrv_ctx *, std::vector<std::tuple<int, hipEvent_t, hipEvent_t>>> g_pending;
-static std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> &pending(lfrv_ctx *ctx) { return g_pending[ctx]; }
 
 void flush_timing(lfrv_ctx *ctx)
 {
-    auto it = g_pending.find(ctx);
-    if (it == g_pending.end() || it->second.empty()) return;
+    if (ctx->ev_pending.empty()) return;
     (void)hipStreamSynchronize(ctx->stream);
-    for (auto &t : it->second) {
+    for (auto &t : ctx->ev_pending) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, std::get<1>(t), std::get<2>(t)) == hipSuccess) {
             ctx->tslot[std::get<0>(t)].ms += ms;
@@ -98,7 +91,7 @@ void flush_timing(lfrv_ctx *ctx)
         ctx->ev_pool.push_back(std::get<1>(t));
         ctx->ev_pool.push_back(std::get<2>(t));
     }
-    it->second.clear();
+    ctx->ev_pending.clear();
 }
 
 }  // namespace lfrv
@@ -259,7 +252,6 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     flush_timing(ctx);
-    g_pending.erase(ctx);
     for (void *p : std::initializer_list<void *>{
              ctx->views, ctx->tau_x, ctx->tau_y, ctx->dpart_x, ctx->dpart_y, ctx->omega, ctx->depth, ctx->view_lap,
              ctx->view_grad, ctx->cell_count, ctx->cell_start, ctx->cell_cursor, ctx->scan_tmp, ctx->e_src, ctx->e_tx, ctx->e_ty,

@@ -957,6 +957,20 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         a.p = ctx->cg_d;
         a.x = x;
         a.g = gco;
+        // true residual ||b - A x||^2 of the current x, then the state to the host
+        auto st2_poll = [&]() -> lfrv_status {
+            {
+                TimedRegion tr(ctx, 5);
+                k_st2_residual<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(
+                    ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, (float)beta, gco, x, ctx->cg_b, part);
+            }
+            k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, st2_tiles, stt->resid);
+            LFRV_LAUNCH(ctx, "cg residual");
+            LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+            LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            final_ready = true;
+            return LFRV_OK;
+        };
         LFRV_LAUNCH(ctx, "cg start");
         // launch as many iterations as the last solve needed before the first
         // look at the convergence flag, then small chunks
@@ -984,20 +998,19 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
             it += nk;
             // the true residual of the current x goes out with the poll: if the
             // chunk converged it is the answer, and the host waits once
-            {
-                TimedRegion tr(ctx, 5);
-                k_st2_residual<<<st2_tiles, kSt2W * kSt2H, 0, ctx->stream>>>(
-                    ctx->W, ctx->H, st2_ntx, st2_tiles, wscale, (float)c0, (float)beta, gco, x, ctx->cg_b, part);
-            }
-            k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, st2_tiles, stt->resid);
-            LFRV_LAUNCH(ctx, "cg residual");
-            LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
-            LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            lfrv_status ps = st2_poll();
+            if (ps != LFRV_OK) return ps;
             done = ctx->cg_state_host->done != 0;
-            final_ready = true;
         }
-        const CgState &hs = *ctx->cg_state_host;
-        ctx->cg_iter_hint = std::max(hs.iter[0], std::max(hs.iter[1], hs.iter[2]));
+        if (final_ready) {
+            const CgState &hs = *ctx->cg_state_host;
+            ctx->cg_iter_hint = std::max(hs.iter[0], std::max(hs.iter[1], hs.iter[2]));
+        } else {
+            // maxit == 0: no iteration ran; the state and true residual of x_0
+            // come from the same stencil residual pass
+            lfrv_status ps = st2_poll();
+            if (ps != LFRV_OK) return ps;
+        }
         done = true;
     }
     while (it < maxit && !done) {

@@ -1063,10 +1063,12 @@ lfrv_status run_count_moving(lfrv_ctx *ctx, const float *prev_dev, float max_dif
 }
 
 // The re-weighted least-squares loop of super-resolution.cpp:177-300.  Each
-// FISTA step is followed on the stream by its energy, the energy rules, the
-// moving-pixel count against the previous step's solution, the moving rule
-// and the copy to the previous solution; once a rule fires, the launches
-// enqueued after it return at once.  The host enqueues kSrPollSteps steps at
+// FISTA step is followed on the stream by its energy pass (which also counts
+// the moving pixels against the previous step's solution and writes U into
+// the spare one of the two previous-step buffers) and one rule kernel (the
+// energy rules, then the moving rule; when the step is kept it flips
+// SrLoop::cur, so the spare buffer becomes the previous solution without a
+// copy); once a rule fires, the launches enqueued after it return at once.  The host enqueues kSrPollSteps steps at
 // a time and reads the loop state of the chunk before the one it has just
 // enqueued, so the device runs ahead of the poll.
 lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *out)

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/lfrv.h"
@@ -322,7 +323,10 @@ struct lfrv_ctx {
     lfrv::SrLoop *sr_loop = nullptr;       // device loop state of lfrv_sr_solve
     lfrv::SrLoop *sr_loop_host = nullptr;  // pinned, 3 slots: two alternating polls, the final read
     hipEvent_t sr_ev[2] = {nullptr, nullptr};
-    float *sr_prev = nullptr;              // 6N: the previous step's and pass's solutions (allocated on first use)
+    // 9N (allocated on first use): two 3N buffers of the previous step's solution, double-buffered
+    // (SrLoop::cur names the one the rules kept; the energy pass writes U into the other), then the
+    // previous pass's 3N solution
+    float *sr_prev = nullptr;
 
     // CG
     float *cg_r = nullptr, *cg_d = nullptr, *cg_q = nullptr, *cg_b = nullptr;  // 3N
@@ -359,6 +363,9 @@ struct lfrv_ctx {
     int64_t timing_seq[6] = {};     // launches seen per class while timing
     lfrv::TimingSlot tslot[6];
     std::vector<hipEvent_t> ev_pool;
+    // recorded (class, start, stop) event pairs not yet read back; owned by the
+    // context, so contexts driven from different host threads share nothing
+    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> ev_pending;
 };
 
 namespace lfrv {

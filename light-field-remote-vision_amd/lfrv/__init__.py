@@ -165,11 +165,23 @@ def lib():
 
 
 def _ptr(a):
+    """Address of an array the caller keeps referenced for the whole C call."""
     return None if a is None else ctypes.c_void_p(a.ctypes.data)
 
 
 def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _in(a):
+    """(array, address) of `a` as contiguous float32.  When `a` needs a
+    conversion the array is a new temporary: the caller holds the returned
+    pair until the C call has returned, so the address stays valid (a bare
+    address of a temporary would point at freed memory during the call)."""
+    if a is None:
+        return None, None
+    arr = _f32(a)
+    return arr, ctypes.c_void_p(arr.ctypes.data)
 
 
 def default_params(mode, nviews, w, h, dsf, **kw):
@@ -215,17 +227,24 @@ class Context:
 
     # inputs
     def set_view(self, i, rgb):
-        self._check(lib().lfrv_set_view(self._h, i, _ptr(_f32(rgb))))
+        a, pa = _in(rgb)
+        self._check(lib().lfrv_set_view(self._h, i, pa))
 
     def set_tau(self, i, tx, ty):
-        self._check(lib().lfrv_set_tau(self._h, i, _ptr(_f32(tx)), _ptr(_f32(ty))))
+        a, pa = _in(tx)
+        b, pb = _in(ty)
+        self._check(lib().lfrv_set_tau(self._h, i, pa, pb))
 
     def set_partial_tau(self, i, sigma_z, dx, dy):
-        self._check(lib().lfrv_set_partial_tau(self._h, i, _ptr(_f32(sigma_z)), _ptr(_f32(dx)), _ptr(_f32(dy))))
+        a, pa = _in(sigma_z)
+        b, pb = _in(dx)
+        c, pc = _in(dy)
+        self._check(lib().lfrv_set_partial_tau(self._h, i, pa, pb, pc))
 
     def set_depth(self, i, depth, P):
         Pd = np.ascontiguousarray(P, dtype=np.float64).reshape(12)
-        self._check(lib().lfrv_set_depth(self._h, i, _ptr(_f32(depth)), Pd.ctypes.data_as(_dp)))
+        a, pa = _in(depth)
+        self._check(lib().lfrv_set_depth(self._h, i, pa, Pd.ctypes.data_as(_dp)))
 
     def set_target_camera(self, P):
         Pd = np.ascontiguousarray(P, dtype=np.float64).reshape(12)
@@ -240,7 +259,9 @@ class Context:
         if dzdx is None:
             self._check(lib().lfrv_set_depth_gradient(self._h, i, None, None))
         else:
-            self._check(lib().lfrv_set_depth_gradient(self._h, i, _ptr(_f32(dzdx)), _ptr(_f32(dzdy))))
+            a, pa = _in(dzdx)
+            b, pb = _in(dzdy)
+            self._check(lib().lfrv_set_depth_gradient(self._h, i, pa, pb))
 
     def warps_from_target_depth(self, skip_view=-1):
         """Target depth by depth splatting, then tau / dpart as the OpenGL exporter computes them."""
@@ -256,7 +277,8 @@ class Context:
         self._check(lib().lfrv_compute_weights(self._h))
 
     def compute_initial_image(self, u0=None):
-        self._check(lib().lfrv_compute_initial_image(self._h, _ptr(None if u0 is None else _f32(u0))))
+        a, pa = _in(u0)
+        self._check(lib().lfrv_compute_initial_image(self._h, pa))
 
     def init_u_gradient(self):
         self._check(lib().lfrv_init_u_gradient(self._h))
@@ -302,7 +324,8 @@ class Context:
 
     def sr_count_moving(self, prev, max_diff=1e-3):
         m = ctypes.c_int64()
-        self._check(lib().lfrv_sr_count_moving(self._h, _ptr(_f32(prev)), max_diff, ctypes.byref(m)))
+        a, pa = _in(prev)
+        self._check(lib().lfrv_sr_count_moving(self._h, pa, max_diff, ctypes.byref(m)))
         return m.value
 
     def sr_iterate(self, steps):
@@ -331,7 +354,8 @@ class Context:
         return out.reshape(3, self.H, self.W)
 
     def set_solution(self, rgb):
-        self._check(lib().lfrv_set_solution(self._h, _ptr(_f32(rgb))))
+        a, pa = _in(rgb)
+        self._check(lib().lfrv_set_solution(self._h, pa))
 
     def clear_solution(self):
         self._check(lib().lfrv_clear_solution(self._h))
@@ -346,7 +370,8 @@ class Context:
 
     def set_buffer(self, name, arr):
         which = BUF[name] if isinstance(name, str) else name
-        self._check(lib().lfrv_set_buffer(self._h, which, _ptr(_f32(arr))))
+        a, pa = _in(arr)
+        self._check(lib().lfrv_set_buffer(self._h, which, pa))
 
     def set_timing(self, enable=True):
         self._check(lib().lfrv_set_timing(self._h, int(enable)))

@@ -179,3 +179,58 @@ def test_create_rejects_a_colour_problem_without_tv_weight():
     assert lfrv.lib().lfrv_create(ctypes.byref(p), ctypes.byref(h)) == lfrv.E_ARG
     assert not h.value
     assert lfrv.lib().lfrv_create(None, ctypes.byref(h)) == lfrv.E_ARG
+
+
+class _RecordingLib:
+    """Stands in for liblfrv.so: each input entry point reads the floats at
+    the addresses it is given, at call time, as the library's copy would."""
+
+    def __init__(self, nfloats):
+        self.n = nfloats
+        self.seen = {}
+
+    def _read(self, name, *ptrs):
+        self.seen[name] = [np.frombuffer(ctypes.string_at(p.value, 4 * self.n), np.float32).copy()
+                           for p in ptrs if p is not None]
+        return lfrv.OK
+
+    def __getattr__(self, name):
+        if name.startswith("lfrv_set_") or name in ("lfrv_compute_initial_image", "lfrv_sr_count_moving"):
+            def f(h, *args):
+                ptrs = [a for a in args if isinstance(a, ctypes.c_void_p)]
+                return self._read(name, *ptrs)
+            return f
+        raise AttributeError(name)
+
+
+@pytest.mark.parametrize("kind", ["float64", "strided", "float32"])
+def test_converted_inputs_stay_alive_during_the_call(monkeypatch, kind):
+    """Inputs that need a conversion to contiguous float32 (float64, strided
+    views) are converted into a temporary; that temporary must still hold the
+    data when the C function reads it.  4 MiB arrays: glibc returns such
+    blocks to the system on free, so a dangling address would not read back
+    the values."""
+    n = 1 << 20
+    rng = np.random.default_rng(1)
+    base = rng.random(2 * n)
+    x = {"float64": base[:n], "strided": base.astype(np.float32)[::2],
+         "float32": base[:n].astype(np.float32)}[kind]
+    assert x.size == n and (kind == "float32") == (x.flags["C_CONTIGUOUS"] and x.dtype == np.float32)
+    want = np.asarray(x, np.float32).ravel()
+    fake = _RecordingLib(n)
+    monkeypatch.setattr(lfrv, "_lib", fake)
+    ctx = object.__new__(lfrv.Context)
+    ctx._h = ctypes.c_void_p(1)
+    ctx.set_view(0, x)
+    ctx.set_tau(0, x, x)
+    ctx.set_partial_tau(0, x, x, x)
+    ctx.set_depth(0, x, np.zeros(12))
+    ctx.set_depth_gradient(0, x, x)
+    ctx.compute_initial_image(x)
+    ctx.set_solution(x)
+    ctx.set_buffer("U", x)
+    for name, arrays in fake.seen.items():
+        assert arrays, name
+        for a in arrays:
+            assert np.array_equal(a, want), name
+    ctx._h = ctypes.c_void_p()

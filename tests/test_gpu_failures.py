@@ -123,6 +123,37 @@ def test_iteration_cap_is_reported(small_scene):
         assert all(st["converged"]) and max(st["true_rel_residual"]) < 1e-5
 
 
+def test_zero_iteration_cap(small_scene):
+    """maxit = 0 runs no iteration: the starting point comes back unchanged
+    with its own residual (recursive and true agree: r_0 = b - A x_0), the
+    status says it did not converge, and the next solve in the context is
+    the same as in a fresh one."""
+    print("IBR_gradient with maxit 0 after a full solve", flush=True)
+    with _ctx(small_scene) as ctx:
+        ctx.ibr_gradient(1e-5, 2000)           # the context has a previous solve behind it
+        ctx.clear_solution()
+        ctx.warps_from_depth()
+        ctx.prepare()
+        ctx.setup_visibility()
+        ctx.compute_weights()
+        ctx.compute_initial_image()
+        ctx.init_u_gradient()
+        U0 = ctx.get_solution()
+        st = ctx.cg_solve(1e-5, 0, check=False)
+        assert st["status"] == lfrv.E_NONCONVERGED
+        assert st["iterations"] == [0, 0, 0] and not any(st["converged"])
+        assert np.array_equal(ctx.get_solution(), U0)
+        for c in range(3):
+            assert st["rel_residual"][c] > 1e-5
+            assert abs(st["true_rel_residual"][c] - st["rel_residual"][c]) <= 1e-3 * st["rel_residual"][c]
+        st = ctx.cg_solve(1e-5, 2000)
+        U1 = ctx.get_solution()
+    with _ctx(small_scene) as ctx:
+        ctx.ibr_gradient(1e-5, 2000)
+        assert np.array_equal(ctx.get_solution(), U1)
+    assert all(st["converged"])
+
+
 def test_calls_out_of_order_and_bad_arguments(small_scene):
     s = small_scene
     print("state and argument errors", flush=True)
@@ -170,7 +201,7 @@ def _png(U):
 def test_cli_init_name(tmp_path, small_scene):
     s = small_scene
     s_min, s_max = _write_scene(tmp_path, s)
-    base = ["dt_alpha 1.0", "dt_beta 0.5"]
+    base = ["dt_alpha 1.0", "dt_beta 0.5", "ibr_system normal_equations"]
     runs = {}
     rng = np.random.default_rng(5)
     init = rng.uniform(0.0, 1.0, (3, s["H"], s["W"])).astype(np.float32)
