@@ -40,6 +40,7 @@ struct Data {
     int _dw_type = 0, _gw_type = 0;
     float _dt_alpha = 1.0f, _dt_beta = 0.0f, _gradient_step = 0.1f;
     std::string _outdir, _init_name;
+    int _normal_equations = 0;  /* ConfigData::_ibr_system == "normal_equations" (an extension) */
     lfrv_ctx *ctx = nullptr;
     lfrv_cg_stats cg{};  /* statistics of the last possion_conj_grad */
 };
@@ -56,7 +57,9 @@ void compute_weights(Data *data);                               /* :558-648 */
 void compute_initial_image(Data *data);                         /* :724-819, hole filling included */
 void hole_filling(Data *data);                                  /* :87-152 (run by compute_initial_image) */
 void init_u_gradient(Data *data);                               /* :833-920 */
-void possion_conj_grad(Data *data);                             /* solver.cu:526-699, corrected PCG */
+void possion_conj_grad(Data *data);                             /* solver.cu:526-699: (I + D^T D) u = u + D^T g by a
+                                                                   corrected CG to 1e-5; the normal equations
+                                                                   when _normal_equations */
 void get_solution(Data *data, std::vector<std::vector<float>> &U); /* :20-37, U[c] of W*H */
 void write_pfm_solution(Data *data);                            /* :822-830, <outdir>/output.pfm */
 

@@ -62,6 +62,12 @@ struct ConfigData {
     float _dt_alpha = 1.0f, _dt_beta = 0.0f;
     int _niter = 0, _nmeta_iter = 1;
     float _gradient_step = 0.1f;
+    // Not in the reference: which system possion_conj_grad solves.
+    // "poisson" (default, the reference's): (lambda I + D^T D) u = lambda u0 + D^T g
+    // with lambda = 1 (solver.cu:361-505, 541).  "normal_equations": the
+    // gradient-constraint normal equations with the warped data term
+    // (dt_alpha sum S^T W S + dt_beta D^T D), lfrv_cg_solve.
+    std::string _ibr_system = "poisson";
 
     // Config_data::parse_config (config.cpp:67-128)
     void parse_config(const Config &cfg);

@@ -186,6 +186,7 @@ void ConfigData::parse_config(const Config &cfg)
     cfg.get_switch("niter", _niter);
     cfg.get_switch("nmeta_iter", _nmeta_iter);
     cfg.get_switch("gradient_step", _gradient_step);
+    cfg.get_switch("ibr_system", _ibr_system);  // extension switch (not read by the reference)
 }
 
 }  // namespace lfrv_ref

@@ -180,6 +180,9 @@ Data *init_data(ConfigData *cd)
     d->_gradient_step = cd->_gradient_step;
     d->_outdir = cd->_outdir;
     d->_init_name = cd->_init_name;
+    if (cd->_ibr_system == "normal_equations") d->_normal_equations = 1;
+    else if (cd->_ibr_system != "poisson")
+        throw Error("ibr_system must be poisson or normal_equations, not '" + cd->_ibr_system + "'");
     check(nullptr, lfrv_create(&p, &d->ctx), "lfrv_create");
     check(d->ctx, lfrv_clear_solution(d->ctx), "lfrv_clear_solution");  // U = 0 (gradientIBR.cu:281-287)
     return d.release();
@@ -233,8 +236,20 @@ void init_u_gradient(Data *d) { check(d->ctx, lfrv_init_u_gradient(d->ctx), "ini
 
 void possion_conj_grad(Data *d)
 {
-    const lfrv_status st = lfrv_cg_solve(d->ctx, 1e-5, 5000, &d->cg);
-    if (st != LFRV_OK && st != LFRV_E_NONCONVERGED) check(d->ctx, st, "possion_conj_grad");
+    // The reference's system: lambda = 1 (solver.cu:541), u0 the current
+    // (hole-filled) solution, the luminance gradient target for every channel,
+    // plain CG as in solver.cu:526-699 (its bugs fixed), run to a tolerance
+    // instead of its 5 iterations.  dt_alpha / dt_beta play no part in it.
+    const lfrv_status st = d->_normal_equations ? lfrv_cg_solve(d->ctx, 1e-5, 5000, &d->cg)
+                                                : lfrv_poisson_cg(d->ctx, 1.0, 1e-5, 5000, 0, &d->cg);
+    if (st == LFRV_E_NONCONVERGED) {
+        // the iterate is kept (the caller may still write it); the driver reports it
+        std::fprintf(stderr, "possion_conj_grad: CG did not reach 1e-5 in 5000 iterations (true rel. residual "
+                             "%.3g %.3g %.3g)\n", d->cg.true_rel_residual[0], d->cg.true_rel_residual[1],
+                     d->cg.true_rel_residual[2]);
+        return;
+    }
+    check(d->ctx, st, "possion_conj_grad");
 }
 
 void get_solution(Data *d, std::vector<std::vector<float>> &U)
@@ -273,10 +288,13 @@ void IBR_gradient(ConfigData *cd)
         possion_conj_grad(data);
         save_image(data->_outdir + "/" + cd->_outfile, solution_image(data->ctx, data->_W, data->_H));
         write_pfm_solution(data);
-        std::printf("IBR_gradient: %zu views %ux%u -> %zux%zu, CG iterations %d %d %d, rel. residual %.3g %.3g %.3g\n",
-                    data->_nviews, cd->_w, cd->_h, data->_W, data->_H, data->cg.iterations[0], data->cg.iterations[1],
+        std::printf("IBR_gradient (%s): %zu views %ux%u -> %zux%zu, CG iterations %d %d %d, rel. residual %.3g %.3g "
+                    "%.3g\n", data->_normal_equations ? "normal equations" : "screened Poisson", data->_nviews,
+                    cd->_w, cd->_h, data->_W, data->_H, data->cg.iterations[0], data->cg.iterations[1],
                     data->cg.iterations[2], data->cg.true_rel_residual[0], data->cg.true_rel_residual[1],
                     data->cg.true_rel_residual[2]);
+        if (data->cg.status == LFRV_E_NONCONVERGED)
+            throw Error("possion_conj_grad did not converge (outputs written from the last iterate)");
     } catch (...) {
         free_data(data);
         throw;

@@ -55,7 +55,7 @@ extern "C" int lfrv_io_config_data(int argc, const char *const *argv, char *out,
       << cd._ugrad_threshold << "\nsigma_sensor " << cd._sigma_sensor << "\nlambda " << cd._lambda
       << "\nlambda_max_factor " << cd._lambda_max_factor << "\ndw_type " << cd._dw_type << "\ngw_type " << cd._gw_type
       << "\ndt_alpha " << cd._dt_alpha << "\ndt_beta " << cd._dt_beta << "\nniter " << cd._niter << "\nnmeta_iter "
-      << cd._nmeta_iter << "\ngradient_step " << cd._gradient_step << "\n";
+      << cd._nmeta_iter << "\ngradient_step " << cd._gradient_step << "\nibr_system " << cd._ibr_system << "\n";
     std::string algo;
     if (cfg.get_switch("algorithm", algo)) o << "algorithm " << algo << "\n";
     const std::string s = o.str();

@@ -143,6 +143,14 @@ def test_config_file_and_command_line_order(tmp_path):
     assert os.path.isdir(out)           # outdir is created
 
 
+def test_ibr_system_switch():
+    """possion_conj_grad's system: the reference's screened Poisson system
+    unless the extension switch asks for the normal equations."""
+    assert lio.config_data(["prog"])["ibr_system"] == "poisson"
+    d = lio.config_data(["prog", "-ibr_system", "normal_equations"])
+    assert d["ibr_system"] == "normal_equations"
+
+
 def test_config_defaults_match_config_data():
     d = lio.config_data(["prog"])
     want = dict(s_min="-1", s_max="-1", s_rmv="-1", t_min="1", t_max="1", t_rmv="-1", dsf="0", width="0",

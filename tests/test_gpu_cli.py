@@ -64,7 +64,8 @@ def _run_cli(cfg):
 def test_cli_ibr_gradient_matches_abi_and_oracle(tmp_path, small_scene, s_rmv):
     s = small_scene
     s_min, s_max = _write_scene(tmp_path, s, s_rmv)
-    cfg = _config(tmp_path, s, s_min, s_max, s_rmv, "IBR_gradient", ["dt_alpha 1.0", "dt_beta 0.5"])
+    cfg = _config(tmp_path, s, s_min, s_max, s_rmv, "IBR_gradient",
+                  ["dt_alpha 1.0", "dt_beta 0.5", "ibr_system normal_equations"])
     _run_cli(cfg)
     out = tmp_path / "out"
     U_cli = lio.load_image(str(out / "output.pfm"))
@@ -101,6 +102,52 @@ def test_cli_ibr_gradient_matches_abi_and_oracle(tmp_path, small_scene, s_rmv):
     assert rel_err(U_cli, r["U"]) < 1e-3
 
 
+def test_cli_ibr_gradient_default_is_the_screened_poisson_system(tmp_path, small_scene):
+    """A reference config without dt_beta (Config_data's default 0,
+    config.cpp:61): possion_conj_grad solves the reference's own system
+    (I + D^T D) u = u0 + D^T g (solver.cu:361-505, lambda = 1 at :541), u0 the
+    hole-filled initial image and g the splatted luminance gradient, whatever
+    dt_alpha / dt_beta say."""
+    s = small_scene
+    s_min, s_max = _write_scene(tmp_path, s)
+    cfg = _config(tmp_path, s, s_min, s_max, -1, "IBR_gradient", [])
+    _run_cli(cfg)
+    U_cli = lio.load_image(str(tmp_path / "out" / "output.pfm"))
+    H, W, V, h, w = s["H"], s["W"], s["V"], s["h"], s["w"]
+    prm = pipelines.gradient_ibr_params(s, dt_alpha=1.0, dt_beta=0.0, sigma_sensor=np.float32(1.0) / np.float32(255))
+    with lfrv.Context(prm) as ctx:
+        pipelines.load_scene(ctx, s)
+        ctx.clear_solution()
+        ctx.warps_from_depth()
+        ctx.prepare()
+        ctx.setup_visibility()
+        ctx.compute_weights()
+        ctx.compute_initial_image()
+        U0 = ctx.get_solution().astype(np.float64)
+        ctx.init_u_gradient()
+        gx = ctx.get_buffer("U_GRAD_X").reshape(H, W)
+        gy = ctx.get_buffer("U_GRAD_Y").reshape(H, W)
+        tx = ctx.get_buffer("TAU_X").reshape(V, h, w)
+        ty = ctx.get_buffer("TAU_Y").reshape(V, h, w)
+        st = ctx.poisson_cg(1.0, 1e-5, 5000, precond=False)
+        U = ctx.get_solution()
+    assert all(st["converged"])
+    assert np.max(np.abs(U_cli - U)) <= 1e-6 * max(1.0, np.abs(U).max())
+    # the FP64 oracle's CG on (I + D^T D) from the same u0 and g
+    wz = np.zeros(tx.shape)
+    ob = oracle.rhs(W, H, s["dsf"], tx, ty, s["views"], wz, 0.0, 1.0, U0, 1.0, gx, gy)
+    oX, ost = oracle.pcg(W, H, s["dsf"], tx, ty, wz, 0.0, 1.0, 1.0, ob, U0, 1e-8, 5000, False)
+    assert all(ost["converged"])
+    assert rel_err(U_cli, oX) < 1e-4
+    # the gradient term is really in it: the result is not u0, and it is not
+    # the normal-equations solution either
+    assert rel_err(U_cli, U0) > 1e-3
+    with lfrv.Context(pipelines.gradient_ibr_params(s, dt_alpha=1.0, dt_beta=0.5)) as ctx:
+        pipelines.load_scene(ctx, s)
+        ctx.ibr_gradient(1e-5, 5000)
+        assert rel_err(U_cli, ctx.get_solution()) > 1e-3
+
+
 def test_cli_ibr_gradient_from_tau_files(tmp_path, small_scene):
     """Warps read from tau / dpart PFMs (import_warps) instead of depth maps."""
     s = small_scene
@@ -128,7 +175,8 @@ def test_cli_ibr_gradient_from_tau_files(tmp_path, small_scene):
     cfg.write_text("\n".join([
         "algorithm IBR_gradient", f"lf_name {tmp_path}/v_%02d.pfm", f"tau_name {tmp_path}/t_%02d.pfm",
         f"dpart_name {tmp_path}/p_%02d.pfm", f"outdir {tmp_path}/out", "outfile u.pfm", "s_min 0",
-        f"s_max {V - 1}", f"dsf {s['dsf']}", "nchannels 3", "sigma_sensor 1", "dt_beta 0.5"]) + "\n")
+        f"s_max {V - 1}", f"dsf {s['dsf']}", "nchannels 3", "sigma_sensor 1", "dt_beta 0.5",
+        "ibr_system normal_equations"]) + "\n")
     _run_cli(cfg)
     U_cli = lio.load_image(str(tmp_path / "out" / "u.pfm"))
     assert np.max(np.abs(U_cli - U)) <= 1e-6 * max(1.0, np.abs(U).max())

@@ -138,7 +138,8 @@ def test_export_warps_files_feed_the_cli(tmp_path):
     cfg.write_text("\n".join([
         "algorithm IBR_gradient", f"lf_name {tmp_path}/v_%02d.pfm", f"tau_name {tmp_path}/t_%02d.pfm",
         f"dpart_name {tmp_path}/p_%02d.pfm", f"outdir {tmp_path}/out", "outfile u.pfm", "s_min 0",
-        f"s_max {V - 1}", f"s_rmv {rmv}", "dsf 1", "nchannels 3", "sigma_sensor 1", "dt_beta 0.5"]) + "\n")
+        f"s_max {V - 1}", f"s_rmv {rmv}", "dsf 1", "nchannels 3", "sigma_sensor 1", "dt_beta 0.5",
+        "ibr_system normal_equations"]) + "\n")
     r = subprocess.run([lio.CLI_PATH, "-config", str(cfg)], capture_output=True, text=True, timeout=120)
     sys.stdout.write(r.stdout)
     assert r.returncode == 0, r.stderr
