@@ -635,3 +635,399 @@ def test_fista_step_closed_form_with_zero_tv_radius():
     # first step: a_1 = (1 + sqrt 5) / 2 and theta = 0, so Uq holds the previous U
     assert abs(st.fista_alpha.value - 0.5 * (1 + np.sqrt(5))) < 1e-15
     np.testing.assert_allclose(st.Uq, U0, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# Gradient-constraint data term (vtv_sr_dataterm_gradients,
+# optimization.cu:1969-2116) and its energy (E_grad, :1615-1717)
+# ---------------------------------------------------------------------------
+
+def mean_laplacian_of_quadratic(a, b):
+    """cuda_laplacian_device (vtv_sr_kernels_unstructured.cu:720-760) away from
+    the border: the mean of the 4 neighbours minus the centre.  For
+    u = a x^2 + b y^2 + (linear) this is (2a + 2b) / 4."""
+    return 0.5 * (a + b)
+
+
+def quadratic(H, W, a, b, c0, cx, cy):
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
+    return c0 + a * (xs - cx) ** 2 + b * (ys - cy) ** 2
+
+
+def block_warps(V, h, w, dsf, shifts=None):
+    """Source pixel (x, y) of view v warped to the centre of the target block
+    [dsf x, dsf x + dsf) (shifted by an integer number of blocks): the
+    footprint is exactly that block."""
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.float64)
+    shifts = shifts or [(0, 0)] * V
+    tx = np.stack([dsf * (xs + sx) + 0.5 * dsf for sx, _ in shifts]).astype(np.float32)
+    ty = np.stack([dsf * (ys + sy) + 0.5 * dsf for _, sy in shifts]).astype(np.float32)
+    return tx, ty
+
+
+@pytest.mark.parametrize("dsf,gw_type,shifted", [(1, 0, False), (2, 0, False), (1, 1, False), (2, 1, True)])
+def test_gradient_term_closed_form(dsf, gw_type, shifted):
+    """Quadratic U (constant mean-Laplacian c_u per channel) and quadratic views
+    (constant c_v): wherever every footprint involved is interior, the warped
+    residual is r = dsf^2 c_u - c_v (the dsf^2 warp-Laplacian approximation,
+    :2035-2037), the splat normalised by the footprint weights returns r, and
+    G = -dt_beta * gradient_step * r, times omega sigma^2 when gw_type != 0
+    (:2078-2113).  F = clamp(U - clamp(G, -1, 1), 0, 1) (:1785-1811).  A wrong
+    sign of gradient_step, a lost dsf^2 or a lost omega sigma^2 changes G."""
+    V, h, w = 3, 14, 16
+    W, H = w * dsf, h * dsf
+    shifts = [(0, 0), (1, 0), (0, -1)] if shifted else None
+    tx, ty = block_warps(V, h, w, dsf, shifts)
+    coeff_u = [(0.001, 0.0005), (-0.0008, 0.0004), (0.0006, 0.0009)]
+    coeff_v = [(0.03, -0.01), (0.008, 0.016), (-0.02, -0.006)]
+    U = np.stack([quadratic(H, W, a, b, 0.5, W / 2.3, H / 1.7) for a, b in coeff_u])
+    views = np.stack([np.stack([quadratic(h, w, a, b, 0.4, w / 2.1, h / 2.6) for a, b in coeff_v])] * V)
+    views = views.astype(np.float32)
+    om_v = np.array([0.7, 1.9, 1.3])
+    om = np.stack([np.full((h, w), o) for o in om_v])
+    sig, beta, gstep = 0.8, 0.35, 0.9
+    st = oracle.SrState(W, H, U)
+    oracle.fista_step(W, H, dsf, tx, ty, views, om, sig, 0.0, beta, gstep, gw_type, 0.05, 0, None, st)
+    F = st.G
+    # target pixels all of whose contributions come from view pixels at least
+    # 2 away from the view border, whose footprints are at least 2 away from
+    # the target border
+    ok = np.zeros((H, W), bool)
+    for v in range(V):
+        sx, sy = shifts[v] if shifted else (0, 0)
+        for y in range(2, h - 2):
+            for x in range(2, w - 2):
+                X, Y = dsf * (x + sx), dsf * (y + sy)
+                if 2 <= X and X + dsf <= W - 2 and 2 <= Y and Y + dsf <= H - 2:
+                    ok[Y:Y + dsf, X:X + dsf] = True
+    # ... and no contribution from a view pixel outside that interior
+    bad = np.zeros((H, W), bool)
+    for v in range(V):
+        sx, sy = shifts[v] if shifted else (0, 0)
+        for y in range(h):
+            for x in range(w):
+                if 2 <= y < h - 2 and 2 <= x < w - 2:
+                    continue
+                X, Y = dsf * (x + sx), dsf * (y + sy)
+                bad[max(Y, 0):max(Y + dsf, 0), max(X, 0):max(X + dsf, 0)] = True
+    ok &= ~bad
+    assert ok.sum() > 40
+    scale = sig * sig * om_v.mean() if gw_type else 1.0
+    for c in range(3):
+        r = dsf * dsf * mean_laplacian_of_quadratic(*coeff_u[c]) - mean_laplacian_of_quadratic(*coeff_v[c])
+        G = -beta * gstep * scale * r
+        assert 1e-3 < abs(G) < 1.0
+        np.testing.assert_allclose(F[c][ok], np.clip(U[c][ok] - G, 0, 1), atol=1e-12)
+
+
+def test_gradient_term_sign_and_clamp():
+    """The clamp of G to [-1, 1] (:1785-1789) and of F to [0, 1] (:1809-1811),
+    with the closed form of the test above at dsf 1: a large curvature mismatch
+    saturates G, and the sign of G follows -gradient_step."""
+    V, h, w = 1, 10, 10
+    tx, ty = block_warps(V, h, w, 1)
+    U = np.stack([quadratic(h, w, 0.0, 0.0, 0.5, 5, 5)] * 3)
+    views = np.stack([np.stack([quadratic(h, w, a, a, 0.5, 5, 5) for a in (4.0, -4.0, 0.5)])]).astype(np.float32)
+    st = oracle.SrState(w, h, U)
+    oracle.fista_step(w, h, 1, tx, ty, views, np.ones((V, h, w)), 1.0, 0.0, 1.0, 1.0, 0, 0.05, 0, None, st)
+    inner = (slice(3, 7), slice(3, 7))
+    # r = -c_v: G = +c_v * gradient_step; c_v = 4 -> G = 1 (clamped) -> F = 0 (clamped);
+    # c_v = -4 -> G = -1 -> F = 1; c_v = 0.5 -> G = 0.5 -> F = 0
+    assert np.all(st.G[0][inner] == 0.0) and np.all(st.G[1][inner] == 1.0)
+    np.testing.assert_allclose(st.G[2][inner], 0.0, atol=1e-12)
+    st = oracle.SrState(w, h, U)
+    oracle.fista_step(w, h, 1, tx, ty, views, np.ones((V, h, w)), 1.0, 0.0, 1.0, 0.25, 0, 0.05, 0, None, st)
+    np.testing.assert_allclose(st.G[2][inner], 0.5 - 0.25 * 0.5, atol=1e-12)
+
+
+def luminance_ramp(H, W, slopes_x, slopes_y, base=(0.3, 0.4, 0.5)):
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
+    return np.stack([base[c] + slopes_x[c] * xs + slopes_y[c] * ys for c in range(3)])
+
+
+REC709 = np.array([0.2126, 0.7152, 0.0722])
+
+
+@pytest.mark.parametrize("gw_type", [0, 1])
+def test_gradient_energy_closed_form(gw_type):
+    """E_grad = sum_i sum_m |S_i Sobel(Y U) - Sobel(Y v_i)|^2 (x omega sigma^2):
+    with identity warps and colour ramps, Sobel/8 of the Rec. 709 luminance
+    is the luminance slope inside the 1-px border and 0 on it (both images
+    share the border), so each view contributes (w-2)(h-2) |dU - dv|^2."""
+    V, H, W = 3, 9, 12
+    tx, ty = identity_warps(V, H, W)
+    ux, uy = (0.01, -0.02, 0.03), (0.005, 0.015, -0.01)
+    U = luminance_ramp(H, W, ux, uy)
+    vslopes = [((0.02, 0.0, 0.01), (0.0, 0.01, 0.0)), ((-0.01, 0.01, 0.0), (0.01, 0.0, 0.02)),
+               ((0.0, -0.02, 0.05), (0.02, 0.02, -0.03))]
+    views = np.stack([luminance_ramp(H, W, sx, sy) for sx, sy in vslopes]).astype(np.float32)
+    om_v = np.array([0.5, 2.0, 1.25])
+    om = np.stack([np.full((H, W), o) for o in om_v])
+    sig = 0.6
+    gUx, gUy = REC709 @ np.array(ux), REC709 @ np.array(uy)
+    want = 0.0
+    for v, (sx, sy) in enumerate(vslopes):
+        d2 = (gUx - REC709 @ np.array(sx)) ** 2 + (gUy - REC709 @ np.array(sy)) ** 2
+        want += (W - 2) * (H - 2) * d2 * (om_v[v] * sig * sig if gw_type else 1.0)
+    vis = np.ones((H, W), np.uint8)
+    e, parts = oracle.sr_energy(W, H, 1, tx, ty, views, om, sig, 0.0, 0.0, 1.7, gw_type, vis, U)
+    assert want > 1e-6
+    # the views are float32 copies of the ramps: 1e-6 relative
+    np.testing.assert_allclose(parts[2], want, rtol=1e-6)
+    np.testing.assert_allclose(e, 1.7 * parts[2], rtol=1e-14)
+
+
+def test_gradient_energy_with_a_translation():
+    """A view shifted by t target pixels: its pixels whose tau leaves [0, W]
+    gather 0, so there the term is |Sobel(Y v)|^2; the rest as above."""
+    H, W, t = 8, 14, 3
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
+    tx = (xs + 0.5 + t)[None].astype(np.float32)
+    ty = (ys + 0.5)[None].astype(np.float32)
+    ux, uy = (0.02, 0.01, 0.0), (0.0, -0.01, 0.04)
+    vx, vy = (0.01, 0.0, 0.02), (0.03, 0.0, 0.0)
+    U = luminance_ramp(H, W, ux, uy)
+    views = luminance_ramp(H, W, vx, vy)[None].astype(np.float32)
+    gU = np.array([REC709 @ np.array(ux), REC709 @ np.array(uy)])
+    gv = np.array([REC709 @ np.array(vx), REC709 @ np.array(vy)])
+    want = 0.0
+    for y in range(1, H - 1):
+        for x in range(1, W - 1):  # the view's own Sobel is zero on its border
+            X = x + t  # the target pixel the view pixel lands on (tau = X + .5)
+            if X + 0.5 > W:
+                want += np.sum(gv ** 2)              # gather invalid: S u = 0
+            elif 1 <= X <= W - 2:
+                want += np.sum((gU - gv) ** 2)       # interior target Sobel
+            else:
+                want += np.sum(gv ** 2)              # the target's border Sobel is 0
+    # border rows / columns of the view: its Sobel is 0, the gathered one is not
+    for y in range(H):
+        for x in range(W):
+            if 1 <= y <= H - 2 and 1 <= x <= W - 2:
+                continue
+            X = x + t
+            if X + 0.5 <= W and 1 <= X <= W - 2 and 1 <= y <= H - 2:
+                want += np.sum(gU ** 2)
+    _, parts = oracle.sr_energy(W, H, 1, tx, ty, views, np.ones((1, H, W)), 1.0, 0.0, 0.0, 1.0, 0,
+                                np.ones((H, W), np.uint8), U)
+    np.testing.assert_allclose(parts[2], want, rtol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# Push-pull (optimization.cu:210-651) against dense pyramid matrices
+# ---------------------------------------------------------------------------
+
+PP_K = np.array([0.25 - 0.2, 0.25, 0.4, 0.25, 0.25 - 0.2])
+
+
+def pp_reduce_matrix(n):
+    """n -> n // 2 along one axis: row i takes taps 2i-2 .. 2i+2 with weights
+    PP_K, indices below 0 read 0 and above n-1 read n-1 (reduce, :209-352)."""
+    m = n // 2
+    R = np.zeros((m, n))
+    for i in range(m):
+        for q in range(5):
+            R[i, min(max(2 * i + q - 2, 0), n - 1)] += PP_K[q]
+    return R
+
+
+def pp_expand_matrix(n, m):
+    """m -> n along one axis (expand, :401-517): even i from coarse rows
+    i/2 - 1, i/2, i/2 + 1 with 2 PP_K[0], 2 PP_K[2], 2 PP_K[4]; odd i from
+    (i-1)/2 and (i+1)/2 with 2 PP_K[1], 2 PP_K[3]; coarse indices clamped to
+    [0, m-1] (the centre tap included, where the reference reads past it)."""
+    E = np.zeros((n, m))
+    c = lambda k: min(max(k, 0), m - 1)  # noqa: E731
+    for i in range(n):
+        if i % 2 == 0:
+            for k, q in ((i // 2 - 1, 0), (i // 2, 2), (i // 2 + 1, 4)):
+                E[i, c(k)] += 2 * PP_K[q]
+        else:
+            for k, q in (((i - 1) // 2, 1), ((i + 1) // 2, 3)):
+                E[i, c(k)] += 2 * PP_K[q]
+    return E
+
+
+def pp_reduce_dense(img, R_r, R_c):
+    """The zero-skipping normalised reduce as masked matrix products: rows
+    first, then columns; a sample equal to 0 carries no weight."""
+    M = (img != 0).astype(np.float64)
+    num, den = R_r @ (img * M), R_r @ M
+    T = np.where(den > 0, num / np.where(den > 0, den, 1), 0.0)
+    M2 = (T != 0).astype(np.float64)
+    num, den = (T * M2) @ R_c.T, M2 @ R_c.T
+    return np.where(den > 0, num / np.where(den > 0, den, 1), 0.0)
+
+
+def pp_dense(U, vis, levels):
+    out = np.empty_like(U)
+    H, W = vis.shape
+    for ch in range(U.shape[0]):
+        imgs, viss, sizes = [U[ch].copy()], [vis.astype(bool)], [(H, W)]
+        for s in range(1, levels + 1):
+            Hs, Ws = H >> (s - 1), W >> (s - 1)
+            hs, ws = H >> s, W >> s
+            if hs == 0 or ws == 0:
+                break
+            Rr, Rc = pp_reduce_matrix(Hs), pp_reduce_matrix(Ws)
+            imgs.append(pp_reduce_dense(imgs[-1], Rr, Rc))
+            viss.append(((Rr > 0).astype(int) @ viss[-1].astype(int) @ (Rc > 0).astype(int).T) > 0)
+            sizes.append((hs, ws))
+        for s in range(len(imgs) - 1, 0, -1):
+            (Hs, Ws), (hs, ws) = sizes[s - 1], sizes[s]
+            up = pp_expand_matrix(Hs, hs) @ imgs[s] @ pp_expand_matrix(Ws, ws).T
+            imgs[s - 1] = np.where(viss[s - 1], imgs[s - 1], up)
+        out[ch] = imgs[0] - U[ch]
+    return out
+
+
+@pytest.mark.parametrize("shape,levels", [((20, 24), 1), ((20, 24), 3), ((37, 29), 10), ((16, 16), 10)])
+def test_push_pull_matches_dense_pyramid(shape, levels):
+    """Random values (zeros included, which the reduce skips), random
+    visibility with a large hole block, odd and even sizes."""
+    H, W = shape
+    U = RNG.uniform(0.1, 1.0, (3, H, W))
+    U[RNG.random((3, H, W)) < 0.1] = 0.0
+    vis = (RNG.random((H, W)) > 0.3).astype(np.uint8)
+    vis[4:14, 5:17] = 0
+    U[:, ~vis.astype(bool)] = 0.0
+    got = oracle.push_pull(U, vis, levels)
+    want = pp_dense(U, vis, levels)
+    np.testing.assert_allclose(got, want, atol=1e-13)
+    assert np.all(got[:, vis.astype(bool)] == 0.0)
+
+
+def test_push_pull_fills_a_hole_in_a_ramp():
+    """A hole in a linear ramp comes back close to the ramp: the symmetric
+    5-tap kernels reproduce linear functions wherever no sample is skipped,
+    so what is left is the bias of the normalised zero-skip near the hole;
+    a swapped tap or even/odd rule leaves errors of the order of the slope
+    times the hole size."""
+    H, W = 32, 32
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float64)
+    ramp = 0.2 + 0.015 * xs + 0.01 * ys
+    vis = np.ones((H, W), np.uint8)
+    vis[12:20, 10:18] = 0
+    U = np.stack([ramp] * 3)
+    U[:, ~vis.astype(bool)] = 0.0
+    filled = oracle.push_pull(U, vis, 10) + U
+    hole = ~vis.astype(bool)
+    err = np.abs(filled[0][hole] - ramp[hole])
+    assert err.max() < 0.03 and err.mean() < 0.012
+    np.testing.assert_allclose(filled[:, ~hole], np.stack([ramp] * 3)[:, ~hole], atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# Hole filling stop rule (gradientIBR.cu:87-152)
+# ---------------------------------------------------------------------------
+
+def test_hole_filling_stop_rule_single_hole():
+    """One hole pixel with four visible neighbours of mean m: sweep 1 sets it to
+    m with energy E1 = (m - u0) m (only the last channel's energy is kept,
+    the variable is overwritten per channel), sweep 2 has E2 = 0, sweep 3 E3 =
+    0.  The loop stops after the first sweep whose |E_prev - E| < eps (E_prev
+    = 0 before the first)."""
+    H = W = 5
+    vis = np.ones((H, W), np.uint8)
+    vis[2, 2] = 0
+    U = np.zeros((3, H, W))
+    U[:, 1, 2], U[:, 3, 2], U[:, 2, 1], U[:, 2, 3] = 0.2, 0.4, 0.6, 0.8  # m = 0.5
+    U[0] += 0.3  # channel 0: m = 0.8, large energy, not looked at
+    out, k = oracle.hole_filling(U, vis, iters=500, eps=1e-3)
+    assert k == 3                      # E1 = 0.25 >= eps, |E1 - E2| = 0.25, |E2 - E3| = 0
+    np.testing.assert_allclose(out[2, 2, 2], 0.5, atol=1e-15)
+    np.testing.assert_allclose(out[0, 2, 2], 0.8, atol=1e-15)
+    # u0 such that E1 = (m - u0) m < eps on the last channel: one sweep only
+    U[2, 2, 2] = 0.5 - 0.5e-3          # E1 = 2.5e-4
+    _, k = oracle.hole_filling(U, vis, iters=500, eps=1e-3)
+    assert k == 1
+
+
+def test_hole_filling_stop_rule_geometric_decay():
+    """A 1 x 4 row with visible ends at 1 and two holes starting at 0: each
+    Jacobi sweep halves the error, u_k = 1 - 2^-k, the sweep's laplacian is
+    2^-k, so E_k = 2 * 2^-k (1 - 2^-k).  With eps = 1e-3 the first k with
+    |E_{k-1} - E_k| < eps (E_0 = 0) is the number of sweeps run."""
+    vis = np.array([[1, 0, 0, 1]], np.uint8)
+    U = np.zeros((3, 1, 4))
+    U[:, 0, 0] = U[:, 0, 3] = 1.0
+    E = lambda k: 0.0 if k == 0 else 2.0 * 2.0 ** -k * (1 - 2.0 ** -k)  # noqa: E731
+    want = next(k for k in range(1, 100) if abs(E(k - 1) - E(k)) < 1e-3)
+    assert want == 11
+    out, k = oracle.hole_filling(U, vis, iters=500, eps=1e-3)
+    assert k == want
+    np.testing.assert_allclose(out[:, 0, 1:3], 1 - 2.0 ** -want, atol=1e-15)
+    # an iteration cap below it: every sweep runs
+    _, k = oracle.hole_filling(U, vis, iters=7, eps=1e-3)
+    assert k == 7
+
+
+# ---------------------------------------------------------------------------
+# Occlusion round trip of import_warps_from_depth (import.cpp:363-442)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dsf", [1, 2])
+def test_occlusion_round_trip_foreground_square(dsf):
+    """A fronto-parallel background at Zb with a foreground square at Zf in
+    front of it, views on a horizontal line, the target between them.  A
+    background source pixel that lands inside the target's projection of the
+    square reads the square's depth there; back-projected at that depth
+    through the target camera and projected into its own view it moves by
+    f dB (1/Zb - 1/Zf) px, beyond the 10 px tolerance for every view here,
+    so it is rejected (-2w, -2h).  Background pixels landing outside the
+    square, and every foreground pixel, round-trip to themselves and keep
+    their tau.  Pixels within 3 px of the square's edge are not judged."""
+    h, w = 40, 160
+    W, H = w * dsf, h * dsf
+    f, cx, cy = 100.0, w / 2.0, h / 2.0
+    Zb, Zf = 10.0, 2.0
+    X0, X1, Y0, Y1 = 0.15, 0.45, -0.12, 0.12            # the square in world units at Zf
+    bu = 0.3
+    bvs = [-0.45, -0.1, 0.7, 1.05]                       # |dB| >= 0.4: the round trip moves >= 16 px
+    def P(fx, c_x, c_y, b):
+        return np.array([[fx, 0, c_x, -fx * b], [0, fx, c_y, 0], [0, 0, 1, 0]], np.float64)
+    Pv = np.stack([P(f, cx, cy, b) for b in bvs])
+    Pu = P(f * dsf, cx * dsf, cy * dsf, bu)
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.float64)
+    depths = np.empty((len(bvs), h, w), np.float32)
+    fg = []
+    for v, b in enumerate(bvs):
+        Xw = (xs + 0.5 - cx) * Zf / f + b
+        Yw = (ys + 0.5 - cy) * Zf / f
+        m = (Xw >= X0) & (Xw <= X1) & (Yw >= Y0) & (Yw <= Y1)
+        depths[v] = np.where(m, Zf, Zb)
+        fg.append(m)
+    tx, ty, _, _, _ = oracle.warps_from_depth(depths, Pv, Pu, dsf)
+    # the square's projection in the target (target pixel units)
+    sx0, sx1 = dsf * (f * (X0 - bu) / Zf + cx), dsf * (f * (X1 - bu) / Zf + cx)
+    sy0, sy1 = dsf * (f * Y0 / Zf + cy), dsf * (f * Y1 / Zf + cy)
+    margin = 3.0 + dsf
+    judged = rejected = kept = 0
+    for v, b in enumerate(bvs):
+        assert abs(f * (b - bu) * (1 / Zb - 1 / Zf)) > 12      # the round trip moves it beyond 10 px
+        for y in range(h):
+            for x in range(w):
+                d = depths[v, y, x]
+                ex = dsf * (x + 0.5) + dsf * f * (b - bu) / d
+                ey = dsf * (y + 0.5)
+                if not (margin <= ex <= W - margin and margin <= ey <= H - margin):
+                    continue
+                if fg[v][y, x]:
+                    # foreground pixels near the square's own edge mix depths too
+                    if not (sx0 + margin <= ex <= sx1 - margin and sy0 + margin <= ey <= sy1 - margin):
+                        continue
+                    assert tx[v, y, x] >= 0, (v, y, x)
+                    kept += 1
+                    judged += 1
+                    continue
+                inside = sx0 + margin <= ex <= sx1 - margin and sy0 + margin <= ey <= sy1 - margin
+                outside = ex < sx0 - margin or ex > sx1 + margin or ey < sy0 - margin or ey > sy1 + margin
+                if inside:
+                    assert tx[v, y, x] == -2 * w and ty[v, y, x] == -2 * h, (v, y, x)
+                    rejected += 1
+                    judged += 1
+                elif outside:
+                    assert abs(tx[v, y, x] - ex) < 1e-3 and abs(ty[v, y, x] - ey) < 1e-3, (v, y, x)
+                    kept += 1
+                    judged += 1
+    assert rejected > 50 and kept > 500
