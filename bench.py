@@ -115,38 +115,79 @@ def cell_valid_count(tx, ty, W, H):
     return int(ok.sum())
 
 
-def cpu_baseline(scene, gpu_iters, threads, cg_sample_iters=8):
-    """Time the FP64 oracle on the host: the full set-up of the step plus a
-    bounded number of PCG iterations per channel, scaled to the iteration
-    count the solve needs."""
+def cpu_model():
+    """CPU model name of the host (/proc/cpuinfo), else the platform's."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def host_threads():
+    """Threads the oracle may use here: the box's share (OMP_NUM_THREADS when
+    set, else the cores this process may run on), at most 16."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, min(16, n))
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    oracle.set_threads(threads)
+    return oracle
+
+
+def cpu_baseline(scene, threads):
+    """The FP64 oracle on the host, measured: the whole novel view (its own
+    warps from depth, visibility, weights, initial image with hole filling,
+    target gradient, Jacobi-PCG to relative residual 1e-5) on all `threads`
+    cores; then the same view's set-up plus 2 PCG iterations on one core."""
+    oracle = _oracle()
     s = scene
-    dsf, W, H = s["dsf"], s["W"], s["H"]
-    sig = 1.0 / 255.0
+    args = (s["views"], s["depths"], s["P_views"], s["P_target"], s["dsf"], 1.0 / 255.0, 1.0, 1.0, 0.5, TOL, 5000)
+    oracle.set_threads(threads)
     t0 = time.perf_counter()
-    tx, ty, dpx, dpy, _ = oracle.warps_from_depth(s["depths"], s["P_views"], s["P_target"], dsf)
-    vis = oracle.target_visibility(W, H, dsf, list(zip(tx, ty)))
-    om = oracle.gibr_weights(W, H, dsf, tx, ty, dpx, dpy, np.zeros((3, H, W)), sig, 1.0)
-    U0, _ = oracle.initial_image(W, H, dsf, tx, ty, s["views"], om)
-    U0, _ = oracle.hole_filling(U0, vis)
-    gx, gy, _, _ = oracle.u_gradient(W, H, dsf, tx, ty, s["views"], om, sig)
-    wd = om * sig * sig
-    b = oracle.rhs(W, H, dsf, tx, ty, s["views"], wd, 1.0, 0.0, None, 0.5, gx, gy)
-    t_setup = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    _, st = oracle.pcg(W, H, dsf, tx, ty, wd, 1.0, 0.0, 0.5, b, U0, 1e-30, cg_sample_iters)
-    t_cg = time.perf_counter() - t1
-    done = max(1, sum(st["iterations"]))
-    per_iter = t_cg / done
-    total_iters = int(sum(gpu_iters))
-    est_ms = 1e3 * (t_setup + per_iter * total_iters)
-    sample = (f"FP64 oracle on the host: full set-up of one view ({t_setup:.1f} s) plus {done} PCG "
-              f"iterations ({per_iter * 1e3:.0f} ms each), scaled to the {total_iters} iterations "
-              f"(3 channels) of the GPU solve")
-    return dict(value=round(est_ms, 1), unit="ms per novel view", cores=threads, kind="port", sample=sample,
-                cpu=platform.processor() or platform.machine())
+    r = oracle.gibr_pipeline(*args)
+    t_all = time.perf_counter() - t0
+    # one core: the set-up and 2 PCG iterations of the same view
+    oracle.set_threads(1)
+    t0 = time.perf_counter()
+    r1 = oracle.gibr_pipeline(*args[:-1], 2, taus=None)
+    t_one = time.perf_counter() - t0
+    oracle.set_threads(threads)
+    iters = sum(r["stats"]["iterations"])
+    one_core = dict(ms=round(1e3 * t_one, 1), pcg_iterations=sum(r1["stats"]["iterations"]),
+                    sample="set-up of the view plus 2 PCG iterations per channel, FP64 oracle on 1 core")
+    return dict(value=round(1e3 * t_all, 1), unit="ms per novel view", cores=threads, kind="port",
+                sample=(f"the whole view measured (not scaled): FP64 oracle from depth maps and cameras to the "
+                        f"PCG solution at rel. residual 1e-5 ({iters} iterations over 3 channels), OpenMP over "
+                        f"rows on {threads} threads"),
+                cpu=cpu_model(), nproc=os.cpu_count(), one_core=one_core, pcg_iterations=r["stats"]["iterations"])
+
+
+def error_vs_oracle(scene, gpu, threads):
+    """The GPU solution against the FP64 oracle run on the GPU's own warps
+    (the same update equations on the same tau / dtau fields): relative
+    solution error, relative energy error (each energy of its own solution)
+    and PSNR difference to the ground truth."""
+    oracle = _oracle()
+    s = scene
+    oracle.set_threads(threads)
+    r = oracle.gibr_pipeline(s["views"], s["depths"], s["P_views"], s["P_target"], s["dsf"], 1.0 / 255.0, 1.0, 1.0,
+                             0.5, TOL, 5000, taus=gpu["taus"])
+    U = gpu["U"].astype(np.float64)
+    rel = float(np.linalg.norm(U - r["U"]) / np.linalg.norm(r["U"]))
+    e_rel = abs(gpu["energy"] - r["energy"]) / abs(r["energy"])
+    p_orc = oracle.psnr(r["U"], s["target"])
+    return dict(rel_err_vs_oracle=float("%.3g" % rel), rel_energy_err=float("%.3g" % e_rel),
+                dpsnr_vs_oracle_db=round(gpu["psnr"] - p_orc, 5), oracle_psnr_db=round(p_orc, 3),
+                oracle_energy=r["energy"], gpu_energy=gpu["energy"],
+                oracle_pcg_iterations=r["stats"]["iterations"])
 
 
 # IBR_color workloads: (synthetic scene preset, FISTA steps per view, dt_beta, description)
@@ -196,18 +237,20 @@ def host_driven_sr_loop(ctx, W, H, niter, nmeta):
     return steps
 
 
-def cpu_baseline_sr(scene, prm, steps, threads, sample_steps=2):
-    """The FP64 oracle on the host for an IBR_color view: warps, weights,
-    visibility, initial image (push-pull), TV weight, then `sample_steps`
-    FISTA steps, scaled to the `steps` of the GPU run."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    oracle.set_threads(threads)
+def oracle_sr_view(scene, prm, steps, taus=None, budget_s=None):
+    """One IBR_color view by the FP64 oracle: warps (its own, or `taus` =
+    (tx, ty, dpx, dpy)), weights, visibility, push-pull initial image, TV
+    weight, then FISTA steps.  With `budget_s` it stops after the first two
+    steps when the rest would not fit in that many seconds."""
+    oracle = _oracle()
     s = scene
     dsf, W, H = s["dsf"], s["W"], s["H"]
     sig = float(prm.sigma_sensor)
     t0 = time.perf_counter()
-    tx, ty, dpx, dpy, _ = oracle.warps_from_depth(s["depths"], s["P_views"], s["P_target"], dsf)
+    if taus is None:
+        tx, ty, dpx, dpy, _ = oracle.warps_from_depth(s["depths"], s["P_views"], s["P_target"], dsf)
+    else:
+        tx, ty, dpx, dpy = taus
     st = oracle.SrState(W, H, np.zeros((3, H, W)))
     om, gx, gy = oracle.sr_weights(W, H, dsf, tx, ty, dpx, dpy, st.U, np.zeros((H, W), np.uint8), st.PX[0],
                                    st.PY[0], sig, 1.0)
@@ -215,23 +258,62 @@ def cpu_baseline_sr(scene, prm, steps, threads, sample_steps=2):
     cov = oracle.coverage(W, H, dsf, tx, ty)
     vis = (cov >= 1.0).astype(np.uint8)
     st.norm_mask += cov
-    U0, nm = oracle.initial_image(W, H, dsf, tx, ty, s["views"], om)
-    st.norm_mask = nm
+    U0, st.norm_mask = oracle.initial_image(W, H, dsf, tx, ty, s["views"], om)
     st.U = oracle.push_pull(U0, vis)
     st.Uq = st.U.copy()
     g, st.norm_mask = oracle.regularizer_weight(W, H, dsf, tx, ty, om, sig, prm.lambda_ * prm.lambda_max_factor,
                                                 prm.lambda_, st.norm_mask)
     t_setup = time.perf_counter() - t0
     t1 = time.perf_counter()
-    for _ in range(sample_steps):
+    done = 0
+    while done < steps:
         oracle.fista_step(W, H, dsf, tx, ty, s["views"], om, sig, prm.dt_alpha, prm.dt_beta, prm.gradient_step,
                           prm.gw_type, prm.lambda_, prm.inner_iterations, g, st)
-    per_step = (time.perf_counter() - t1) / sample_steps
-    est_ms = 1e3 * (t_setup + per_step * steps)
-    sample = (f"FP64 oracle on the host: set-up of one view ({t_setup:.1f} s) plus {sample_steps} FISTA steps "
-              f"({per_step * 1e3:.0f} ms each), scaled to the {steps} steps of the GPU run")
-    return dict(value=round(est_ms, 1), unit="ms per novel view", cores=threads, kind="port", sample=sample,
-                cpu=platform.processor() or platform.machine())
+        done += 1
+        if budget_s is not None and done == 2:
+            per = (time.perf_counter() - t1) / 2
+            if t_setup + per * steps > budget_s:
+                break
+    t_steps = time.perf_counter() - t1
+    e, _ = oracle.sr_energy(W, H, dsf, tx, ty, s["views"], om, sig, prm.lambda_, prm.dt_alpha, prm.dt_beta,
+                            prm.gw_type, vis, st.U)
+    return dict(U=st.U, energy=e, steps=done, t_setup=t_setup, t_steps=t_steps)
+
+
+def cpu_baseline_sr(scene, prm, steps, threads, budget_s=30.0):
+    """The FP64 oracle on the host for an IBR_color view (its own warps): the
+    whole view measured when it fits in `budget_s`, else its set-up and two
+    FISTA steps, scaled to the `steps` of the GPU run (said in `sample`)."""
+    oracle = _oracle()
+    oracle.set_threads(threads)
+    r = oracle_sr_view(scene, prm, steps, budget_s=budget_s)
+    if r["steps"] == steps:
+        ms = 1e3 * (r["t_setup"] + r["t_steps"])
+        sample = (f"the whole view measured (not scaled): FP64 oracle set-up ({r['t_setup']:.1f} s) and all "
+                  f"{steps} FISTA steps, OpenMP on {threads} threads")
+    else:
+        per = r["t_steps"] / r["steps"]
+        ms = 1e3 * (r["t_setup"] + per * steps)
+        sample = (f"FP64 oracle set-up of one view ({r['t_setup']:.1f} s, measured) plus {r['steps']} FISTA steps "
+                  f"({per * 1e3:.0f} ms each, measured), scaled to the {steps} steps of the GPU run; OpenMP on "
+                  f"{threads} threads")
+    return dict(value=round(ms, 1), unit="ms per novel view", cores=threads, kind="port", sample=sample,
+                cpu=cpu_model(), nproc=os.cpu_count())
+
+
+def error_vs_oracle_sr(scene, prm, taus, steps, U_gpu, e_gpu, psnr_gpu, threads):
+    """The GPU view after `steps` FISTA steps against the FP64 oracle's, on
+    the GPU's warps."""
+    oracle = _oracle()
+    oracle.set_threads(threads)
+    r = oracle_sr_view(scene, prm, steps, taus=taus)
+    U = U_gpu.astype(np.float64)
+    p_orc = oracle.psnr(r["U"], scene["target"])
+    return dict(steps_compared=steps,
+                rel_err_vs_oracle=float("%.3g" % (np.linalg.norm(U - r["U"]) / np.linalg.norm(r["U"]))),
+                max_abs_err_vs_oracle=float("%.3g" % np.max(np.abs(U - r["U"]))),
+                rel_energy_err=float("%.3g" % (abs(e_gpu - r["energy"]) / abs(r["energy"]))),
+                dpsnr_vs_oracle_db=round(psnr_gpu - p_orc, 5), oracle_psnr_db=round(p_orc, 3))
 
 
 def calib_copy(args):
@@ -394,7 +476,24 @@ def bench_color_sr(args, workload):
                             last_stop=last["last_stop"], converged=last["converged"],
                             energy=round(float(last["energy"]), 6), host_driven_loop=host_loop)
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
+        if not nmeta:
+            # error against the oracle: the whole view when the oracle runs it in
+            # seconds, else the first steps of a view run again on the GPU
+            cmp_steps = steps if scene["W"] * scene["H"] * scene["V"] * steps <= 9 * 256 * 256 * 300 else 2
+            ctx = lfrv.Context(prm)
+            pipelines.load_scene(ctx, scene)
+            ctx.clear_solution()
+            pipelines.color_sr_setup(ctx)
+            ctx.sr_init_regularizer_weight()
+            ctx.sr_iterate(cmp_steps)
+            Ug = ctx.get_solution()
+            eg, _ = ctx.sr_primal_energy()
+            shp = (scene["V"], scene["h"], scene["w"])
+            taus = tuple(ctx.get_buffer(b).reshape(shp) for b in ("TAU_X", "TAU_Y", "DPART_X", "DPART_Y"))
+            ctx.close()
+            line.update(error_vs_oracle_sr(scene, prm, taus, cmp_steps, Ug, eg, lfrv.psnr(Ug, scene["target"]),
+                                           threads))
         line["cpu_baseline"] = cpu_baseline_sr(scene, prm, irls[-1]["fista_steps"] if nmeta else steps, threads)
     print(json.dumps(line))
 
@@ -458,10 +557,13 @@ def main():
     up_ms, up_n = ctx.get_timing("cg_update")
     ctx.set_timing(False)
     U = ctx.get_solution()
+    e_gpu = ctx.gibr_energy()
     st_last = st
     N = scene["W"] * scene["H"]
     tx = ctx.get_buffer("TAU_X")
     ty = ctx.get_buffer("TAU_Y")
+    shp = (scene["V"], scene["h"], scene["w"])
+    taus = tuple(ctx.get_buffer(b).reshape(shp) for b in ("TAU_X", "TAU_Y", "DPART_X", "DPART_Y"))
     e_unique = cell_valid_count(tx, ty, scene["W"], scene["H"])
     e_binned, ntiles = ctx.bin_stats()
     if scene["dsf"] == 1:
@@ -523,8 +625,9 @@ def main():
         line["kernels"]["cg_update"] = {"avg_launch_us": round(up_avg_s * 1e6, 2),
                                         "achieved_gbs": round(up_bytes / up_avg_s / 1e9, 1), "launches": up_n}
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(scene, st_last["iterations"], threads)
+        threads = host_threads()
+        line.update(error_vs_oracle(scene, dict(U=U, energy=e_gpu, psnr=psnr, taus=taus), threads))
+        line["cpu_baseline"] = cpu_baseline(scene, threads)
     print(json.dumps(line))
 
 

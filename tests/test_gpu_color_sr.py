@@ -30,7 +30,43 @@ def test_sr_steps_match_oracle(dsf, beta, gw, inner):
     """inner = 1: for dsf >= 2 the data term and the only primal-dual iteration
     (with the over-relaxation) run as one kernel; inner > 1 fuses the first
     iteration; inner = 0 leaves the data term alone."""
-    s = _scene(dsf=dsf, kind=2 if dsf == 1 else 1)
+    _steps_match_oracle(_scene(dsf=dsf, kind=2 if dsf == 1 else 1), beta, gw, inner)
+
+
+def _c3_scene(w=96, seed=7):
+    """The 5x5-view generator of the 512^2 -> 2048^2 workload (8 tilted planes,
+    disparity -2..+2 px per grid step through the principal-point shift,
+    views box-averaged x4) at w x w views: the focal length scales with w and
+    f * baseline stays 10, so the disparities in view pixels are the same."""
+    d = lfrv.synth_desc(3, seed)
+    d.w = d.h = w
+    d.focal = float(w)
+    d.baseline = 10.0 / w
+    return lfrv.synth(d)
+
+
+@pytest.mark.parametrize("beta,inner", [(1.0, 1), (0.3, 2)])
+def test_sr_steps_match_oracle_25_views_dsf4(beta, inner):
+    """25 views and dsf 4 (the 512^2 -> 2048^2 configuration's generator, at
+    96^2 -> 384^2): the pull kernels' per-window caps and the assembled
+    anchor-grid operator at 25 views."""
+    s = _c3_scene()
+    assert s["V"] == 25 and s["dsf"] == 4
+    _steps_match_oracle(s, beta, 0, inner, metas=2, steps=3)
+
+
+def test_sr_full_size_128_to_256_matches_oracle():
+    """The 3x3 128^2 -> 256^2 configuration at its full size: 20 FISTA steps of
+    the benchmark's parameters (one primal-dual iteration, lambda 0.05,
+    lambda_max_factor 100) against the oracle, U within 5e-4 and the energy
+    within 1e-4 relative at the end."""
+    s = lfrv.synth(1, seed=2024)
+    _steps_match_oracle(s, 0.0, 0, 1, metas=1, steps=20, u_tol=5e-4, dual_tol=5e-4)
+
+
+def _steps_match_oracle(s, beta, gw, inner, metas=2, steps=3, u_tol=2e-4, dual_tol=2e-4):
+    dsf = s["dsf"]
+    print(f"IBR_color V={s['V']} {s['w']}x{s['h']} dsf={dsf} beta={beta} gw={gw} inner={inner}", flush=True)
     V, h, w, H, W = s["V"], s["h"], s["w"], s["H"], s["W"]
     sig, lam, lmf = 1.0 / 255.0, 0.05, 100.0
     prm = pipelines.color_sr_params(s, lambda_=lam, lambda_max_factor=lmf, inner_iterations=inner, dt_beta=beta,
@@ -65,7 +101,7 @@ def test_sr_steps_match_oracle(dsf, beta, gw, inner):
         st.U = U0.copy()
         st.Uq = U0.copy()
         assert np.max(np.abs(ctx.get_solution() - U0)) < 1e-5
-        for meta in range(2):
+        for meta in range(metas):
             ctx.compute_weights()
             om, gx, gy = oracle.sr_weights(W, H, dsf, tx, ty, dpx, dpy, st.U, vis, st.PX[0], st.PY[0], sig, 1.0)
             st.PX[0], st.PY[0] = gx, gy
@@ -77,15 +113,19 @@ def test_sr_steps_match_oracle(dsf, beta, gw, inner):
             e, parts = ctx.sr_primal_energy()
             oe, oparts = oracle.sr_energy(W, H, dsf, tx, ty, s["views"], om, sig, lam, 1.0, beta, gw, vis, st.U)
             assert abs(e - oe) / oe < 1e-4
-            for k in range(3):
+            for k in range(steps):
                 ctx.sr_iteration_fista()
                 oracle.fista_step(W, H, dsf, tx, ty, s["views"], om, sig, 1.0, beta, 1.0, gw, lam, inner, g, st)
             U = ctx.get_solution()
-            assert np.max(np.abs(U - st.U)) < 2e-4, np.max(np.abs(U - st.U))
-            assert np.max(np.abs(ctx.get_buffer("UQ").reshape(3, H, W) - st.Uq)) < 4e-4
-            assert np.max(np.abs(ctx.get_buffer("PX").reshape(3, H, W) - st.PX)) < 2e-4
-            assert np.max(np.abs(ctx.get_buffer("PY").reshape(3, H, W) - st.PY)) < 2e-4
+            assert np.max(np.abs(U - st.U)) < u_tol, np.max(np.abs(U - st.U))
+            assert np.max(np.abs(ctx.get_buffer("UQ").reshape(3, H, W) - st.Uq)) < 2 * u_tol
+            assert np.max(np.abs(ctx.get_buffer("PX").reshape(3, H, W) - st.PX)) < dual_tol
+            assert np.max(np.abs(ctx.get_buffer("PY").reshape(3, H, W) - st.PY)) < dual_tol
             assert rel_err(ctx.get_buffer("NORM_MASK").reshape(H, W), st.norm_mask) < 1e-5
+        # the energy of the final iterate
+        e, _ = ctx.sr_primal_energy()
+        oe, _ = oracle.sr_energy(W, H, dsf, tx, ty, s["views"], om, sig, lam, 1.0, beta, gw, vis, st.U)
+        assert abs(e - oe) / oe < 1e-4, (e, oe)
 
 
 def test_sr_count_moving():
