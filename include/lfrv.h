@@ -385,11 +385,29 @@ lfrv_status lfrv_dev_copy_kernel(void *dst, const void *src, size_t bytes, void 
 /* Wait for the default stream (the stateless operators' NULL stream). */
 lfrv_status lfrv_device_sync(void);
 
-/* ---- quality metric (host) ------------------------------------------------- */
+/* ---- quality metrics ------------------------------------------------------- */
 
-/* PSNR, vmax = 1, 2-px border excluded, RGB-averaged MSE
- * (gsl_image.cpp:991-1025, psnr_main.cpp).  Host pointers, planar RGB. */
+/* PSNR, vmax = 1, 2-px border excluded, RGB-averaged MSE; DBL_MAX for equal
+ * images (gsl_image_psnr, gsl_image.cpp:991-1025, psnr_main.cpp).  Host
+ * pointers, planar RGB. */
 double lfrv_psnr(int W, int H, const float *a, const float *b);
+
+/* SSIM as the reference computes it (gsl_image_ssim, gsl_image.cpp:1027-1037,
+ * gsl_matrix_ssim, gsl_matrix_helper.cpp:799-914): the mean over R, G, B of
+ * the mean over all pixels of ((2 mu1 mu2 + C1) + (2 s12 + C2)) /
+ * ((mu1^2 + mu2^2 + C1) + (s1^2 + s2^2 + C2)), local statistics by an 11-tap
+ * Gaussian of sigma 1.5 renormalised at the borders, C1 = (0.01 range)^2,
+ * C2 = (0.03 range)^2; the factors are added, not multiplied, as in the
+ * reference.  Host pointers, planar RGB, FP64. */
+double lfrv_ssim(int W, int H, const float *a, const float *b, double dynamic_range);
+
+/* PSNR of gsl_image_psnr (as lfrv_psnr) of two device-resident planar RGB
+ * images, written as a double to the device pointer `out` on `stream`
+ * (FP64 sums, one partial per 4096 interior pixels, summed in a fixed
+ * order).  Its scratch is stream-ordered memory it allocates and frees on
+ * that stream.  W, H > 4; W*H <= 2^31.  No counterpart kernel in the
+ * reference (its tool reads two image files, psnr_main.cpp:12-20). */
+lfrv_status lfrv_op_psnr(int W, int H, const float *a, const float *b, double *out, void *stream);
 
 #ifdef __cplusplus
 }

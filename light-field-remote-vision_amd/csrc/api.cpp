@@ -1,5 +1,6 @@
 // C-ABI of the lfrv library: context lifecycle, inputs, and the solver steps.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <tuple>
@@ -772,6 +773,6 @@ extern "C" double lfrv_psnr(int W, int H, const float *a, const float *b)
         mse_t += mse / ((double)(W - 4) * (H - 4));
     }
     mse_t /= 3.0;
-    if (mse_t == 0.0) return 1.0e308;
+    if (mse_t == 0.0) return DBL_MAX;  // coco::psnr (gsl_image.cpp:1009-1015)
     return 10.0 * std::log(1.0 / mse_t) / std::log(10.0);
 }

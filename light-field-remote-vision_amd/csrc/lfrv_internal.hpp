@@ -426,6 +426,13 @@ lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts);
 lfrv_status run_count_moving(lfrv_ctx *ctx, const float *prev_dev, float max_diff, int64_t *moving);
 lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *st);
 
+// device PSNR (k_quality.hip): interior pixels per partial-sum block, the
+// number of blocks for a W x H image, and the two launches over `partials`
+constexpr int kPsnrChunk = 4096;
+int psnr_blocks(int W, int H);
+lfrv_status psnr_launch(int W, int H, const float *a, const float *b, double *out, double *partials, int G,
+                        hipStream_t s);
+
 // stateless kernels
 hipError_t launch_gather(int W, int H, int w, int h, int dsf, const float *tx, const float *ty,
                          const float *u, int C, float *out, hipStream_t s);

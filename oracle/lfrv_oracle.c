@@ -19,6 +19,7 @@
  * Layout: images are planar, channel c of an image of N pixels starts at c*N;
  * rows are contiguous, pixel (x,y) is at y*W + x, top-left origin.
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -1506,7 +1507,7 @@ double orc_psnr(int W, int H, const double *A, const double *B)
                 cnt++;
             }
     mse /= (double)cnt;
-    if (mse == 0.0) return 1.0e308; /* coco::psnr returns DBL_MAX */
+    if (mse == 0.0) return DBL_MAX; /* coco::psnr (gsl_image.cpp:1009-1015) */
     return 10.0 * log10(1.0 / mse);
 }
 

@@ -6,6 +6,7 @@ import ctypes
 import glob
 import os
 import re
+import sys
 
 import numpy as np
 import pytest
@@ -106,7 +107,7 @@ def test_psnr_host_entry_matches_oracle():
     a = rng.random((3, 20, 30)).astype(np.float32)
     b = (a + rng.normal(0, 0.02, a.shape)).astype(np.float32)
     assert abs(lfrv.psnr(a, b) - oracle.psnr(a, b)) < 1e-9
-    assert lfrv.psnr(a, a) == 1e308
+    assert lfrv.psnr(a, a) == sys.float_info.max   # DBL_MAX, as coco::psnr
 
 
 @pytest.mark.parametrize("config", [0, 1, 2])
@@ -234,3 +235,88 @@ def test_converted_inputs_stay_alive_during_the_call(monkeypatch, kind):
         for a in arrays:
             assert np.array_equal(a, want), name
     ctx._h = ctypes.c_void_p()
+
+
+# ---------------------------------------------------------------------------
+# Quality harness: SSIM (gsl_matrix_ssim) and the psnr tool's line
+# ---------------------------------------------------------------------------
+
+def _ssim_lib(a, b, rng=1.0):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    L = lfrv.lib()
+    L.lfrv_ssim.restype = ctypes.c_double
+    L.lfrv_ssim.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
+    return L.lfrv_ssim(a.shape[2], a.shape[1], a.ctypes.data, b.ctypes.data, rng)
+
+
+def _normalised_gauss(img, sigma=1.5, taps=11):
+    """Separable Gaussian as normalised convolution: (g * (x, 1)) / (g * 1)
+    along x, then along y, on zero-padded data."""
+    o = taps // 2
+    k = np.exp(-0.5 * (np.arange(-o, o + 1) / sigma) ** 2)
+    k /= k.sum()
+
+    def along(x, axis):
+        num = np.apply_along_axis(lambda r: np.convolve(r, k, mode="same"), axis, x)
+        den = np.apply_along_axis(lambda r: np.convolve(r, k, mode="same"), axis, np.ones_like(x))
+        return num / den
+    return along(along(img, 1), 0)
+
+
+def test_ssim_identical_and_constant_images():
+    rng = np.random.default_rng(3)
+    a = rng.random((3, 20, 23)).astype(np.float32)
+    assert abs(_ssim_lib(a, a) - 1.0) < 1e-12
+    # constant images: mu = the constants, every sigma 0
+    ca, cb = np.float32(0.25), np.float32(0.75)
+    A, B = np.full((3, 9, 12), ca, np.float32), np.full((3, 9, 12), cb, np.float32)
+    C1, C2 = 0.01 ** 2, 0.03 ** 2
+    want = ((2 * 0.25 * 0.75 + C1) + C2) / ((0.25 ** 2 + 0.75 ** 2 + C1) + C2)
+    assert abs(_ssim_lib(A, B) - want) < 1e-12
+    # the dynamic range scales C1 and C2
+    C1, C2 = 2.55 ** 2, 7.65 ** 2
+    want = ((2 * 0.25 * 0.75 + C1) + C2) / ((0.25 ** 2 + 0.75 ** 2 + C1) + C2)
+    assert abs(_ssim_lib(A, B, 255.0) - want) < 1e-12
+
+
+def test_ssim_matches_a_normalised_convolution_restatement():
+    """Random images against the reference's formula written with normalised
+    convolutions: the factors of numerator and denominator added (as
+    gsl_matrix_add_scaled adds them), the map averaged over every pixel and
+    the three channels.  The standard (multiplied) SSIM differs."""
+    rng = np.random.default_rng(4)
+    a = rng.random((3, 17, 26)).astype(np.float32)
+    b = np.clip(a + 0.1 * rng.standard_normal(a.shape), 0, 1).astype(np.float32)
+    C1, C2 = 0.01 ** 2, 0.03 ** 2
+    s_add = s_mul = 0.0
+    for c in range(3):
+        A, B = a[c].astype(np.float64), b[c].astype(np.float64)
+        m1, m2 = _normalised_gauss(A), _normalised_gauss(B)
+        v1 = _normalised_gauss(A * A) - m1 * m1
+        v2 = _normalised_gauss(B * B) - m2 * m2
+        v12 = _normalised_gauss(A * B) - m1 * m2
+        s_add += np.mean(((2 * m1 * m2 + C1) + (2 * v12 + C2)) / ((m1 * m1 + m2 * m2 + C1) + (v1 + v2 + C2)))
+        s_mul += np.mean(((2 * m1 * m2 + C1) * (2 * v12 + C2)) / ((m1 * m1 + m2 * m2 + C1) * (v1 + v2 + C2)))
+    got = _ssim_lib(a, b)
+    assert abs(got - s_add / 3) < 1e-12
+    assert abs(got - s_mul / 3) > 1e-3
+
+
+def test_psnr_tool_prints_the_reference_line(tmp_path):
+    """lfrv_psnr_tool a b prints '& PSNR & 10000 (1 - SSIM) / 2' (psnr_main.cpp:17-20)."""
+    import subprocess
+    from lfrv import io as lio
+    rng = np.random.default_rng(5)
+    a = rng.random((3, 24, 30)).astype(np.float32)
+    b = np.clip(a + 0.05 * rng.standard_normal(a.shape), 0, 1).astype(np.float32)
+    lio.save_image(str(tmp_path / "a.pfm"), a)
+    lio.save_image(str(tmp_path / "b.pfm"), b)
+    tool = os.path.join(ROOT, "light-field-remote-vision_amd", "lfrv_psnr_tool")
+    r = subprocess.run([tool, str(tmp_path / "a.pfm"), str(tmp_path / "b.pfm")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    p = lfrv.psnr(a, b)
+    s = _ssim_lib(a, b)
+    assert r.stdout == "& %.4g & %d\n" % (p, int(10000 * (1 - s) / 2))
+    assert abs(p - oracle.psnr(a, b)) < 1e-9

@@ -10,6 +10,8 @@ idempotence, fixed points).  A plausible slip in the oracle (a dropped edge
 term, a swapped tap index, a sign, a transposed camera) fails at least one of
 these.
 """
+import sys
+
 import numpy as np
 import pytest
 
@@ -555,7 +557,7 @@ def test_psnr_hand_cases():
     C = A.copy()
     C[:, :2, :] = 1.0  # differences only inside the 2-px border
     C[:, :, -2:] = 1.0
-    assert oracle.psnr(A, C) == 1e308
+    assert oracle.psnr(A, C) == sys.float_info.max  # coco::psnr returns DBL_MAX
     D = A.copy()
     D[0, 5, 5] = 1.0  # one pixel of one channel: mse = 1 / (3 * 48)
     assert abs(oracle.psnr(A, D) - 10 * np.log10(3 * 6 * 8)) < 1e-12
