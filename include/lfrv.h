@@ -409,6 +409,15 @@ double lfrv_ssim(int W, int H, const float *a, const float *b, double dynamic_ra
  * reference (its tool reads two image files, psnr_main.cpp:12-20). */
 lfrv_status lfrv_op_psnr(int W, int H, const float *a, const float *b, double *out, void *stream);
 
+/* lfrv_op_psnr with its partial sums in a fixed device array (128 KiB per
+ * GPU, held until the process ends) instead of memory it allocates: no
+ * counterpart in the reference; for callers that cannot allocate (code under
+ * HIP graph capture, serving loops that forbid hipMalloc).  The same kernels
+ * and result as lfrv_op_psnr.  Images whose interior (W-4)(H-4) exceeds
+ * 16384 * 4096 pixels are refused (LFRV_E_ARG).  Calls that overlap on one
+ * GPU (two streams or two host threads) share the array and race. */
+lfrv_status lfrv_op_psnr_noalloc(int W, int H, const float *a, const float *b, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
