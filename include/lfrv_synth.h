@@ -60,6 +60,16 @@ int lfrv_synth_nviews(const lfrv_synth_desc *d);
 int lfrv_synth_generate(const lfrv_synth_desc *d, float *views, float *depths, double *P_views,
                         double *P_target, float *target_rgb, float *target_depth);
 
+/* The same scene rendered on the GPU: views, depths, target_rgb and
+ * target_depth are device pointers (each nullable), P_views / P_target host
+ * pointers as above; work is queued on `stream` and finished when the call
+ * returns.  The ray tracer is the host one in FP64, so the images agree with
+ * lfrv_synth_generate's to float rounding.  Noise is not supported (returns 1
+ * when noise_sigma > 0).  Returns 0, 1 for a bad description, 2 for a HIP
+ * error. */
+int lfrv_synth_generate_device(const lfrv_synth_desc *d, float *views, float *depths, double *P_views,
+                               double *P_target, float *target_rgb, float *target_depth, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -226,19 +226,6 @@ constexpr int kSt2W = 64, kSt2H = 8;  // tile of the stencil kernels, one thread
 //
 // The five coefficient planes of the assembled stencil: centre and the E, S,
 // SE, SW couplings of each pixel (the W, N, NW, NE ones are its neighbours').
-struct St2Coeffs {
-    const float *gD, *gE, *gS, *gSE, *gSW;
-};
-
-struct CgcgArgs {
-    int W, H, ntx, ntiles;
-    float wscale, c0, beta_reg;
-    const float *r_in, *w_in, *s_in;
-    float *r_out, *w_out, *s_out, *p, *x;
-    const float *minv;
-    St2Coeffs g;
-};
-
 // w = A u at pixel (x, y) from the LDS tile `up` (row stride LW, centre li):
 // the assembled 9-point stencil cf = {D, E, S, SE, SW, E(x-1), S(y-1),
 // SE(x-1,y-1), SW(x+1,y-1)} (zero where the neighbour is outside), the
@@ -290,14 +277,16 @@ __device__ __forceinline__ void st2_coeffs(const St2Coeffs &g, int W, size_t o, 
 // s_{i-1} (never written).  All global loads of the pass (the pixel's
 // vectors, its halo cell's, the stencil coefficients) are issued before the
 // one barrier.
-__global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const CgState *st, double *partials)
+__device__ void cgcg_scalars(CgState &s, const double (&pv)[9], bool after_start);
+
+// The pass on tile t; its partial sums go to slot `slot` of G per quantity.
+__device__ __forceinline__ void cgcg_st2_tile(const CgcgArgs &a, const CgState *st, double *partials, int t, int slot,
+                                              int G)
 {
     constexpr int LW = kSt2W + 2, LH = kSt2H + 2, LP = LW * LH, NH = 2 * LW + 2 * kSt2H;
     static_assert(NH <= kSt2W * kSt2H, "one thread per halo cell");
     __shared__ float ul[3 * LP];
     __shared__ double red[9 * 16];
-    if (st->done) return;
-    const int t = xcd_tile(blockIdx.x, a.ntiles);
     const int tx0 = (t % a.ntx) * kSt2W, ty0 = (t / a.ntx) * kSt2H;
     const int W = a.W, H = a.H;
     const size_t N = (size_t)W * H;
@@ -400,7 +389,42 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const Cg
             v[6 + c] += (double)rn[c] * (double)rn[c];
         }
     }
-    block_partials<9>(v, partials, red);
+    block_partials_at<9>(v, partials, red, slot, G);
+}
+
+__global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const CgState *st, double *partials)
+{
+    if (st->done) return;
+    cgcg_st2_tile(a, st, partials, xcd_tile(blockIdx.x, a.ntiles), blockIdx.x, gridDim.x);
+}
+
+// Many solves in one launch: block b is block work[b].y of solve work[b].x
+// in that solve's own launch (the same tile and partial-sum slot), with its
+// arguments of this iteration's parity and its scalar state.
+__global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2_batch(const CgBatchItem *__restrict__ items,
+                                                                   const int2 *__restrict__ work, int parity)
+{
+    const int2 wk = work[blockIdx.x];
+    const CgBatchItem &it = items[wk.x];
+    if (it.st->done) return;
+    const int G = it.a[parity].ntiles;
+    cgcg_st2_tile(it.a[parity], it.st, it.partials, xcd_tile(wk.y, G), wk.y, G);
+}
+
+// The scalar step of every solve of the batch: block q sums solve q's tile
+// partials in tile order and updates its state (k_cgcg_step per solve).
+__global__ void __launch_bounds__(kFinalizeThreads) k_cgcg_step_batch(const CgBatchItem *__restrict__ items)
+{
+    __shared__ double red[9 * 16];
+    const CgBatchItem &it = items[blockIdx.x];
+    CgState *st = it.st;
+    if (st->done) return;
+    double v[9];
+    reduce_partials<9>(it.partials, it.a[0].ntiles, v, red);
+    if (threadIdx.x == 0) {
+        cgcg_scalars(*st, v, false);
+        st->first = 0;
+    }
 }
 
 // The true residual of a stencil-path solve: partials ||b - A x||^2 per
@@ -477,7 +501,6 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_st2_residual(int W, int H, in
     block_partials<NQ>(v, partials, red);
 }
 
-__device__ void cgcg_scalars(CgState &s, const double (&pv)[9], bool after_start);
 
 // The start of a stencil-path solve in one pass: r_0 = b - A x_0 and
 // u_0 = Minv r_0 on the tile and a 1-pixel ring (x_0 staged with a 2-pixel
@@ -860,6 +883,8 @@ unsigned int reduce_blocks(size_t N)
 
 }  // namespace
 
+static lfrv_status cg_finish_stats(lfrv_ctx *ctx, lfrv_cg_stats *st, std::chrono::steady_clock::time_point t_start);
+
 lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use_views, bool use_precond,
                    bool poisson_rhs, double tol, int maxit, lfrv_cg_stats *st)
 {
@@ -1054,6 +1079,13 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
 #undef OPK
+    return cg_finish_stats(ctx, st, t_start);
+}
+
+// Statistics of a finished solve from the state copied to the host.
+static lfrv_status cg_finish_stats(lfrv_ctx *ctx, lfrv_cg_stats *st,
+                                   std::chrono::steady_clock::time_point t_start)
+{
     flush_timing(ctx);
     const CgState &hs = *ctx->cg_state_host;
     lfrv_status status = LFRV_OK;
@@ -1080,6 +1112,105 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         return fail(ctx, status, msg);
     }
     return LFRV_OK;
+}
+
+// ---- batches of dsf = 1 solves ------------------------------------------------
+
+lfrv_status cg_batch_begin(lfrv_ctx *ctx, double tol, int maxit, CgBatchItem *item)
+{
+    if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "batched CG before lfrv_prepare");
+    const double alpha = ctx->p.dt_alpha, beta = ctx->p.dt_beta;
+    const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
+    const float wscale = (float)(alpha * s2);
+    if (ctx->ks != 2 || !ctx->gram || wscale == 0.0f)
+        return fail(ctx, LFRV_E_ARG, "the batched CG takes dsf 1 solves with the data term");
+    ctx->u_zero = false;
+    if (!(ctx->gram_epoch == ctx->epoch)) {
+        lfrv_status s = run_refresh_entry_weights(ctx, 1.0f);
+        if (s != LFRV_OK) return s;
+    }
+    lfrv_status s = run_rhs_and_diag(ctx, alpha, 0.0, beta, true, true, ctx->tmp3N);
+    if (s != LFRV_OK) return s;
+    const size_t N = ctx->N;
+    const int ntx = (ctx->W + kSt2W - 1) / kSt2W, ntiles = ntx * ((ctx->H + kSt2H - 1) / kSt2H);
+    float *gD = ctx->gram, *gE = gD + N, *gS = gE + N, *gSE = gS + N, *gSW = gSE + N;
+    if (ctx->gram_epoch != ctx->epoch) {
+        const TileGeom g = tile_geom(ctx);
+        TileEntries te = tile_entries(ctx);
+        te.active = 1;
+        const int gcap = pull_cap(ctx, 3, 0, ctx->pull_budget, PullKind::Cell);
+        const size_t glds = pull_lds(ctx, 3, gcap, PullKind::Cell);
+        k_gram2<<<ctx->ntiles, ctx->pull_bs, glds, ctx->stream>>>(g, te, gcap, gD, gE, gS, gSE, gSW);
+        LFRV_LAUNCH(ctx, "stencil assembly");
+        ctx->gram_epoch = ctx->epoch;
+    }
+    const St2Coeffs gco{gD, gE, gS, gSE, gSW};
+    float *x = ctx->U, *minv = ctx->cg_minv;
+    k_cgcg_begin_st2<<<ntiles, kSt2W * kSt2H, 0, ctx->stream>>>(ctx->W, ctx->H, ntx, ntiles, wscale, 0.0f,
+                                                                 (float)beta, gco, x, ctx->cg_b, minv, ctx->cg_r,
+                                                                 ctx->cg_q, ctx->partials);
+    k_cgcg_begin<<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->cg_state, ctx->partials, ntiles, tol * tol, maxit);
+    LFRV_LAUNCH(ctx, "batched cg start");
+    float *rbuf[2] = {ctx->cg_r, ctx->cg_extra};
+    float *sbuf[2] = {ctx->cg_extra + 3 * N, ctx->tmp3N};
+    float *wbuf[2] = {ctx->cg_q, ctx->cg_extra + 6 * N};
+    for (int par = 0; par < 2; par++) {
+        CgcgArgs &a = item->a[par];
+        a = CgcgArgs{};
+        a.W = ctx->W;
+        a.H = ctx->H;
+        a.ntx = ntx;
+        a.ntiles = ntiles;
+        a.wscale = wscale;
+        a.c0 = 0.0f;
+        a.beta_reg = (float)beta;
+        a.minv = minv;
+        a.p = ctx->cg_d;
+        a.x = x;
+        a.g = gco;
+        a.r_in = rbuf[par];
+        a.w_in = wbuf[par];
+        a.s_in = sbuf[par];
+        a.r_out = rbuf[par ^ 1];
+        a.w_out = wbuf[par ^ 1];
+        a.s_out = sbuf[par ^ 1];
+    }
+    item->st = ctx->cg_state;
+    item->partials = ctx->partials;
+    return LFRV_OK;
+}
+
+hipError_t launch_cg_batch_iteration(const CgBatchItem *items, const int2 *work, int nblocks, int parity, hipStream_t s)
+{
+    k_cgcg_st2_batch<<<nblocks, kSt2W * kSt2H, 0, s>>>(items, work, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_batch_step(const CgBatchItem *items, int nitems, hipStream_t s)
+{
+    k_cgcg_step_batch<<<nitems, kFinalizeThreads, 0, s>>>(items);
+    return hipGetLastError();
+}
+
+lfrv_status cg_batch_end(lfrv_ctx *ctx, lfrv_cg_stats *st)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t N = ctx->N;
+    const float s2 = ctx->p.sigma_sensor * ctx->p.sigma_sensor;
+    const float wscale = (float)(ctx->p.dt_alpha * s2);
+    const int ntx = (ctx->W + kSt2W - 1) / kSt2W, ntiles = ntx * ((ctx->H + kSt2H - 1) / kSt2H);
+    float *gD = ctx->gram, *gE = gD + N, *gS = gE + N, *gSE = gS + N, *gSW = gSE + N;
+    const St2Coeffs gco{gD, gE, gS, gSE, gSW};
+    k_st2_residual<<<ntiles, kSt2W * kSt2H, 0, ctx->stream>>>(ctx->W, ctx->H, ntx, ntiles, wscale, 0.0f,
+                                                               (float)ctx->p.dt_beta, gco, ctx->U, ctx->cg_b,
+                                                               ctx->partials);
+    k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(ctx->partials, ntiles, ctx->cg_state->resid);
+    LFRV_LAUNCH(ctx, "batched cg residual");
+    LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, ctx->cg_state, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+    LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const CgState &hs = *ctx->cg_state_host;
+    ctx->cg_iter_hint = std::max(hs.iter[0], std::max(hs.iter[1], hs.iter[2]));
+    return cg_finish_stats(ctx, st, t0);
 }
 
 lfrv_status run_gibr_energy(lfrv_ctx *ctx, double *e)

@@ -157,6 +157,27 @@ __device__ __forceinline__ void block_partials(double (&v)[NQ], double *partials
     }
 }
 
+// block_partials with an explicit slot and stride: value q of this block goes
+// to partials[q * G + slot].
+template <int NQ>
+__device__ __forceinline__ void block_partials_at(double (&v)[NQ], double *partials, double *lds, int slot, int G)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const double s = wave_sum(v[q]);
+        if (lane == 0) lds[q * 16 + wid] = s;
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if (q < NQ) {
+        double s = 0.0;
+        for (int k = 0; k < nw; k++) s += lds[q * 16 + k];
+        partials[(size_t)q * G + slot] = s;
+    }
+}
+
 // Deterministic sum of G block partials (layout [q][block]) by one block;
 // the result is valid in thread 0.  Up to 2 x blocks-size partials, all
 // loads of a thread are independent and in flight at once (the partials come
@@ -249,6 +270,30 @@ struct SrLoop {
     int cur;     // which of the two previous-solution buffers holds the last step the rules kept
     float energy, prev_energy;  // float, as the driver keeps them
     double parts[3];            // E_TV, E_int, E_grad of the last energy
+};
+
+// The five coefficient planes of the assembled dsf = 1 stencil and the
+// arguments of one single-reduction PCG pass over it (k_cg.hip).
+struct St2Coeffs {
+    const float *gD, *gE, *gS, *gSE, *gSW;
+};
+
+struct CgcgArgs {
+    int W, H, ntx, ntiles;
+    float wscale, c0, beta_reg;
+    const float *r_in, *w_in, *s_in;
+    float *r_out, *w_out, *s_out, *p, *x;
+    const float *minv;
+    St2Coeffs g;
+};
+
+
+// One solve of a batch of PCG solves (lfrv_batch): its iteration arguments for even and odd iterations
+// (r, w, s swap buffers), its device scalars and its partial sums.
+struct CgBatchItem {
+    CgcgArgs a[2];
+    CgState *st;
+    double *partials;
 };
 
 struct TimingSlot {
@@ -425,6 +470,16 @@ lfrv_status run_fista(lfrv_ctx *ctx);
 lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts);
 lfrv_status run_count_moving(lfrv_ctx *ctx, const float *prev_dev, float max_diff, int64_t *moving);
 lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *st);
+
+// batched stencil PCG (k_cg.hip): the part of lfrv_cg_solve before the
+// iterations for one dsf = 1 solve (its CgBatchItem filled, work queued on the
+// context's stream), the iterations of many solves (block b on tile
+// work[b].y of solve work[b].x), the scalar step of every solve, and the
+// part after the iterations (true residual, statistics; waits for the stream)
+lfrv_status cg_batch_begin(lfrv_ctx *ctx, double tol, int maxit, CgBatchItem *item);
+hipError_t launch_cg_batch_iteration(const CgBatchItem *items, const int2 *work, int nblocks, int parity, hipStream_t s);
+hipError_t launch_cg_batch_step(const CgBatchItem *items, int nitems, hipStream_t s);
+lfrv_status cg_batch_end(lfrv_ctx *ctx, lfrv_cg_stats *st);
 
 // device PSNR (k_quality.hip): interior pixels per partial-sum block, the
 // number of blocks for a W x H image, and the two launches over `partials`

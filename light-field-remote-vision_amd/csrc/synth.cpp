@@ -1,5 +1,6 @@
 // Seeded synthetic light fields (host code).  See include/lfrv_synth.h.
 #include "../../include/lfrv_synth.h"
+#include "synth_scene.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -8,6 +9,8 @@
 #include <vector>
 
 namespace {
+
+using namespace lfrv_synth;
 
 struct Rng {
     uint64_t s;
@@ -27,40 +30,6 @@ struct Rng {
         if (u1 < 1e-300) u1 = 1e-300;
         return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
     }
-};
-
-constexpr int kSin = 32;
-constexpr int kRect = 16;
-
-struct Texture {
-    double base[3];
-    double amp[kSin][3], kx[kSin], ky[kSin], ph[kSin];
-    double rc[kRect][2], rh[kRect][2], col[kRect][3];
-
-    void eval(double u, double v, float *rgb) const
-    {
-        double c[3] = {base[0], base[1], base[2]};
-        for (int k = 0; k < kSin; k++) {
-            const double s = std::sin(kx[k] * u + ky[k] * v + ph[k]);
-            c[0] += amp[k][0] * s;
-            c[1] += amp[k][1] * s;
-            c[2] += amp[k][2] * s;
-        }
-        for (int k = 0; k < kRect; k++) {
-            if (std::fabs(u - rc[k][0]) <= rh[k][0] && std::fabs(v - rc[k][1]) <= rh[k][1]) {
-                for (int q = 0; q < 3; q++) c[q] = 0.5 * c[q] + 0.5 * col[k][q];
-            }
-        }
-        for (int q = 0; q < 3; q++) rgb[q] = (float)std::min(1.0, std::max(0.0, c[q]));
-    }
-};
-
-struct Plane {
-    double n[3], d;          // n . X = d
-    double o[3], e1[3], e2[3];
-    double h1, h2;
-    bool bounded;
-    Texture tex;
 };
 
 void make_texture(Texture &t, Rng &rng, double world_per_px, double pmin, double pmax, double extent)
@@ -97,46 +66,6 @@ void cross3(const double *a, const double *b, double *c)
     c[0] = a[1] * b[2] - a[2] * b[1];
     c[1] = a[2] * b[0] - a[0] * b[2];
     c[2] = a[0] * b[1] - a[1] * b[0];
-}
-
-struct Cam {
-    double f, cx, cy;  // K = [[f,0,cx],[0,f,cy],[0,0,1]], R = I
-    double C[3];
-};
-
-// Nearest surface along the ray through image point (x, y) (corner-origin pixel
-// coordinates).  Returns false when nothing is hit.
-bool trace(const std::vector<Plane> &planes, const Cam &cam, double x, double y, double *depth,
-           float *rgb)
-{
-    const double r[3] = {(x - cam.cx) / cam.f, (y - cam.cy) / cam.f, 1.0};
-    double best = 1e30;
-    int bi = -1;
-    double bu = 0, bv = 0;
-    for (size_t i = 0; i < planes.size(); i++) {
-        const Plane &p = planes[i];
-        const double nr = p.n[0] * r[0] + p.n[1] * r[1] + p.n[2] * r[2];
-        if (std::fabs(nr) < 1e-12) continue;
-        const double t = (p.d - (p.n[0] * cam.C[0] + p.n[1] * cam.C[1] + p.n[2] * cam.C[2])) / nr;
-        if (!(t > 1e-6) || t >= best) continue;
-        const double X[3] = {cam.C[0] + t * r[0] - p.o[0], cam.C[1] + t * r[1] - p.o[1],
-                             cam.C[2] + t * r[2] - p.o[2]};
-        const double u = X[0] * p.e1[0] + X[1] * p.e1[1] + X[2] * p.e1[2];
-        const double v = X[0] * p.e2[0] + X[1] * p.e2[1] + X[2] * p.e2[2];
-        if (p.bounded && (std::fabs(u) > p.h1 || std::fabs(v) > p.h2)) continue;
-        best = t;  // r_z = 1 and R = I, so t is the z-depth
-        bi = (int)i;
-        bu = u;
-        bv = v;
-    }
-    if (bi < 0) {
-        *depth = 0.0;
-        if (rgb) rgb[0] = rgb[1] = rgb[2] = 0.0f;
-        return false;
-    }
-    *depth = best;
-    if (rgb) planes[bi].tex.eval(bu, bv, rgb);
-    return true;
 }
 
 template <class F>
@@ -215,15 +144,15 @@ extern "C" void lfrv_synth_preset(lfrv_synth_desc *d, int config, uint64_t seed)
 
 extern "C" int lfrv_synth_nviews(const lfrv_synth_desc *d) { return d->grid * d->grid; }
 
-extern "C" int lfrv_synth_generate(const lfrv_synth_desc *d, float *views, float *depths,
-                                   double *P_views, double *P_target, float *target_rgb,
-                                   float *target_depth)
+namespace lfrv_synth {
+
+int build_scene(const lfrv_synth_desc *d, Scene *sc, double *P_views, double *P_target)
 {
-    if (!d || d->grid < 1 || d->w < 1 || d->h < 1 || d->dsf < 1 || d->render_scale < 1 ||
-        d->focal <= 0)
+    if (!d || d->grid < 1 || d->grid * d->grid > kMaxViews || d->w < 1 || d->h < 1 || d->dsf < 1 ||
+        d->render_scale < 1 || d->focal <= 0 || d->nplanes > kMaxPlanes)
         return 1;
     Rng rng(d->seed);
-    const int g = d->grid, V = g * g, w = d->w, h = d->h, rs = d->render_scale;
+    const int g = d->grid, V = g * g, w = d->w, h = d->h;
     const int W = w * d->dsf, H = h * d->dsf;
     const double ft = d->focal * d->dsf;  // target focal, target pixels
     const double half = 0.5 * (g - 1);
@@ -236,22 +165,23 @@ extern "C" int lfrv_synth_generate(const lfrv_synth_desc *d, float *views, float
     }
 
     // scene
-    std::vector<Plane> planes;
+    sc->nplanes = 0;
     if (d->kind == 1) {
-        Plane p{};
+        Plane &p = sc->planes[sc->nplanes++];
+        p = Plane{};
         p.n[0] = 0; p.n[1] = 0; p.n[2] = 1;
         p.d = d->plane_depth;
         p.o[0] = 0; p.o[1] = 0; p.o[2] = d->plane_depth;
         p.e1[0] = 1; p.e1[1] = 0; p.e1[2] = 0;
         p.e2[0] = 0; p.e2[1] = 1; p.e2[2] = 0;
-        p.bounded = false;
+        p.bounded = 0;
         const double wpp = d->plane_depth / ft;
         make_texture(p.tex, rng, wpp, d->tex_min_period, d->tex_max_period, 0.5 * W * wpp);
-        planes.push_back(p);
     } else {
         const int np = std::max(1, d->nplanes);
         for (int k = 0; k < np; k++) {
-            Plane p{};
+            Plane &p = sc->planes[sc->nplanes++];
+            p = Plane{};
             double z0, tilt, az;
             double ix, iy;
             if (k == 0) {
@@ -282,25 +212,24 @@ extern "C" int lfrv_synth_generate(const lfrv_synth_desc *d, float *views, float
             normalize3(p.e2);
             const double wpp = z0 / ft;
             if (k == 0) {
-                p.bounded = false;
+                p.bounded = 0;
                 p.h1 = p.h2 = 0.0;
             } else {
-                p.bounded = true;
+                p.bounded = 1;
                 p.h1 = rng.uni(0.08, 0.25) * W * wpp;
                 p.h2 = rng.uni(0.08, 0.25) * H * wpp;
             }
             make_texture(p.tex, rng, wpp, d->tex_min_period, d->tex_max_period,
                          p.bounded ? std::max(p.h1, p.h2) : 0.6 * W * wpp);
-            planes.push_back(p);
         }
     }
 
     // cameras (low-res views, target at W x H)
-    std::vector<Cam> cams(V);
+    sc->V = V;
     for (int t = 0; t < g; t++)
         for (int s = 0; s < g; s++) {
             const double ss = s - half, tt = t - half;
-            Cam &c = cams[t * g + s];
+            Cam &c = sc->cams[t * g + s];
             c.f = d->focal;
             c.cx = 0.5 * w + ss * d->pp_shift;
             c.cy = 0.5 * h + tt * d->pp_shift;
@@ -309,7 +238,7 @@ extern "C" int lfrv_synth_generate(const lfrv_synth_desc *d, float *views, float
             c.C[2] = 0.0;
             if (P_views) cam_P(c, P_views + 12 * (t * g + s));
         }
-    Cam tc;
+    Cam &tc = sc->target;
     tc.f = ft;
     tc.cx = (0.5 * w + ta * d->pp_shift) * d->dsf;
     tc.cy = (0.5 * h + tb * d->pp_shift) * d->dsf;
@@ -317,10 +246,25 @@ extern "C" int lfrv_synth_generate(const lfrv_synth_desc *d, float *views, float
     tc.C[1] = d->baseline * tb;
     tc.C[2] = 0.0;
     if (P_target) cam_P(tc, P_target);
+    return 0;
+}
 
+}  // namespace lfrv_synth
+
+using namespace lfrv_synth;
+
+extern "C" int lfrv_synth_generate(const lfrv_synth_desc *d, float *views, float *depths,
+                                   double *P_views, double *P_target, float *target_rgb,
+                                   float *target_depth)
+{
+    std::vector<Scene> hold(1);
+    Scene &sc = hold[0];
+    if (lfrv_synth::build_scene(d, &sc, P_views, P_target) != 0) return 1;
+    const int V = sc.V, w = d->w, h = d->h, rs = d->render_scale;
+    const int W = w * d->dsf, H = h * d->dsf;
     const size_t n = (size_t)w * h;
     for (int v = 0; v < V; v++) {
-        const Cam &c = cams[v];
+        const Cam &c = sc.cams[v];
         float *img = views ? views + (size_t)v * 3 * n : nullptr;
         float *dep = depths ? depths + (size_t)v * n : nullptr;
         parallel_rows(h, [&](int y) {
@@ -328,7 +272,7 @@ extern "C" int lfrv_synth_generate(const lfrv_synth_desc *d, float *views, float
                 const size_t m = (size_t)y * w + x;
                 double dd;
                 if (dep) {
-                    trace(planes, c, x + 0.5, y + 0.5, &dd, nullptr);
+                    trace(sc, c, x + 0.5, y + 0.5, &dd, nullptr);
                     dep[m] = (float)dd;
                 }
                 if (img) {
@@ -336,7 +280,7 @@ extern "C" int lfrv_synth_generate(const lfrv_synth_desc *d, float *views, float
                     for (int sy = 0; sy < rs; sy++)
                         for (int sx = 0; sx < rs; sx++) {
                             float rgb[3];
-                            trace(planes, c, x + (sx + 0.5) / rs, y + (sy + 0.5) / rs, &dd, rgb);
+                            trace(sc, c, x + (sx + 0.5) / rs, y + (sy + 0.5) / rs, &dd, rgb);
                             acc[0] += rgb[0];
                             acc[1] += rgb[1];
                             acc[2] += rgb[2];
@@ -360,7 +304,7 @@ extern "C" int lfrv_synth_generate(const lfrv_synth_desc *d, float *views, float
                 const size_t o = (size_t)y * W + x;
                 double dd;
                 float rgb[3];
-                trace(planes, tc, x + 0.5, y + 0.5, &dd, rgb);
+                trace(sc, sc.target, x + 0.5, y + 0.5, &dd, rgb);
                 if (target_rgb)
                     for (int q = 0; q < 3; q++) target_rgb[q * N + o] = rgb[q];
                 if (target_depth) target_depth[o] = (float)dd;
