@@ -1,0 +1,513 @@
+// Kernels of the warm-start CNN: 3x3 convolutions on the bf16 MFMA units of
+// CDNA4 with FP32 accumulation, their backward passes, and the elementwise
+// pieces of a U-Net (2x2 average pooling, nearest 2x upsampling, ReLU, the
+// squared-error loss, Adam).  No counterpart in the reference (its CNN
+// training, sources/CNN/, is a separate TensorFlow tool outside this path).
+//
+// Layouts.  Activations are NHWC bf16: [B][H][W][C], channels contiguous.
+// Weights are [Cout][9][Cin] bf16 (tap t = 3 (dy + 1) + (dx + 1)), so one
+// output channel's weights are contiguous along K = 9 Cin.  Gradients of
+// activations and weights are FP32.  Input channel counts are multiples of
+// 32 (the first layer's are zero-padded), so a K chunk of 32 never straddles
+// a tap.
+//
+// The convolution is an implicit GEMM: C[pixel][cout] = sum_k A[pixel][k]
+// W[cout][k] with A[p][(t, ci)] = X[p + offset(t)][ci] (0 outside the
+// image).  A block of 256 threads (4 waves) owns 64 pixels x 64 output
+// channels; per K chunk of 32 it stages the A and W tiles in LDS (one 16-byte
+// load per thread each) and each wave runs 2 x 2 v_mfma_f32_16x16x16_bf16 on
+// its 32 x 32 sub-tile.  The backward pass with respect to the input is the
+// same kernel on the output gradient with the weights flipped in space and
+// transposed in channels; the weight gradient is a second MFMA kernel whose
+// K runs over pixels.
+#include "lfrv_internal.hpp"
+
+namespace lfrv {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                      // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+constexpr int kCvBM = 64, kCvBN = 64, kCvBK = 32, kCvPitch = 40;  // LDS rows of 40 bf16 (80 B)
+
+enum : int { kEpiRelu = 1, kEpiSkip = 2, kEpiF32 = 4, kEpiAccum = 8, kEpiBias = 16 };
+
+// Y[p][co] = act(sum_{t,ci} W[co][t][ci] X[p + off(t)][ci] + bias[co] + skip[p][co]).
+// EPI: kEpiRelu, kEpiSkip (bf16 skip), kEpiBias, and the output kind: bf16,
+// or FP32 (kEpiF32), optionally added to what Y holds (kEpiAccum).
+template <int EPI>
+__global__ void __launch_bounds__(256)
+k_conv3x3(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
+          const uint16_t *__restrict__ skip, void *__restrict__ Y, int B, int H, int W, int Cin, int Cout)
+{
+    __shared__ __align__(16) uint16_t As[kCvBM * kCvPitch];
+    __shared__ __align__(16) uint16_t Bs[kCvBN * kCvPitch];
+    const int HW = H * W, P = B * HW;
+    const int p0 = blockIdx.x * kCvBM, co0 = blockIdx.y * kCvBN;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, wm = wv >> 1, wn = wv & 1;
+    const int sr = tid >> 2, sc = (tid & 3) * 8;  // staging row and its 8-element column chunk
+    const int sp = p0 + sr;
+    const bool prow = sp < P;
+    const int sb = prow ? sp / HW : 0, rem = prow ? sp - sb * HW : 0;
+    const int sy = rem / W, sx = rem - sy * W;
+    const int sco = co0 + sr;
+    f4v acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+    const uint4 zero = make_uint4(0, 0, 0, 0);
+    for (int t = 0; t < 9; t++) {
+        const int yy = sy + t / 3 - 1, xx = sx + t % 3 - 1;
+        const bool valid = prow && yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const uint16_t *xrow = X + ((size_t)(sb * H + (valid ? yy : 0)) * W + (valid ? xx : 0)) * Cin;
+        const uint16_t *wrow = Wt + ((size_t)(sco < Cout ? sco : 0) * 9 + t) * Cin;
+        for (int ci0 = 0; ci0 < Cin; ci0 += kCvBK) {
+            const uint4 av = valid ? *(const uint4 *)(xrow + ci0 + sc) : zero;
+            const uint4 bv = sco < Cout ? *(const uint4 *)(wrow + ci0 + sc) : zero;
+            __syncthreads();
+            *(uint4 *)(As + sr * kCvPitch + sc) = av;
+            *(uint4 *)(Bs + sr * kCvPitch + sc) = bv;
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < kCvBK; kk += 16) {
+                s4v a[2], b[2];
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+                    a[i] = *(const s4v *)(As + (wm * 32 + i * 16 + (lane & 15)) * kCvPitch + kk + 4 * (lane >> 4));
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    b[j] = *(const s4v *)(Bs + (wn * 32 + j * 16 + (lane & 15)) * kCvPitch + kk + 4 * (lane >> 4));
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+    // lane holds rows 4 (lane / 16) + r, column lane % 16 of each 16 x 16 tile
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int co = co0 + wn * 32 + j * 16 + (lane & 15);
+        if (co >= Cout) continue;
+        const float bco = (EPI & kEpiBias) ? bias[co] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int p = p0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+                if (p >= P) continue;
+                const size_t o = (size_t)p * Cout + co;
+                float v = acc[i][j][r] + bco;
+                if (EPI & kEpiSkip) v += bf2f(skip[o]);
+                if (EPI & kEpiRelu) v = fmaxf(v, 0.0f);
+                if (EPI & kEpiF32) {
+                    float *yf = (float *)Y;
+                    yf[o] = (EPI & kEpiAccum) ? yf[o] + v : v;
+                } else {
+                    ((uint16_t *)Y)[o] = f2bf(v);
+                }
+            }
+    }
+}
+
+// Weight gradient, split over pixel chunks:
+//   dW[co][t][ci] += sum_{p in chunk} dU[p][co] X[p + off(t)][ci].
+// GEMM with M = co (64 per block), N = ci of one tap (32 per block), K =
+// pixels (32 per LDS stage, kWgChunk per block).  Both tiles are stored
+// transposed in LDS ([co][p], [ci][p]) so an MFMA fragment's 4 consecutive K
+// values are contiguous.  Each wave owns 16 rows of co and both 16-column
+// tiles of ci; the block adds its 64 x 32 partial to dW with FP32 atomics.
+constexpr int kWgChunk = 512;
+constexpr int kWgPitch = 36;  // bf16 per LDS row of 32 pixels (+4: rows spread over the banks)
+
+__global__ void __launch_bounds__(256)
+k_conv3x3_wgrad(const uint16_t *__restrict__ dU, const uint16_t *__restrict__ X, float *__restrict__ dW, int B,
+                int H, int W, int Cin, int Cout)
+{
+    __shared__ __align__(16) uint16_t Ds[64 * kWgPitch];
+    __shared__ __align__(16) uint16_t Xs[32 * kWgPitch];
+    const int HW = H * W, P = B * HW;
+    const int pc0 = blockIdx.x * kWgChunk;
+    const int co0 = blockIdx.y * 64;
+    const int t = blockIdx.z / (Cin / 32), ci0 = (blockIdx.z % (Cin / 32)) * 32;
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    f4v acc[2] = {f4v{0.f, 0.f, 0.f, 0.f}, f4v{0.f, 0.f, 0.f, 0.f}};
+    // staging: dU tile 32 pixels x 64 co = 256 x 8 bf16 (one 16-byte load per
+    // thread); X tile 32 pixels x 32 ci = 128 x 8 bf16 (threads 0..127)
+    const int dp = tid >> 3, dc = (tid & 7) * 8;
+    const int xp = (tid & 127) >> 2, xc = (tid & 3) * 8;
+    const uint4 zero = make_uint4(0, 0, 0, 0);
+    const int pend = min(pc0 + kWgChunk, P);
+    for (int pb = pc0; pb < pend; pb += 32) {
+        uint4 dv = zero, xv = zero;
+        {
+            const int p = pb + dp;
+            if (p < pend && co0 + dc < Cout) dv = *(const uint4 *)(dU + (size_t)p * Cout + co0 + dc);
+        }
+        if (tid < 128) {
+            const int p = pb + xp;
+            if (p < pend) {
+                const int b = p / HW, r = p - b * HW, y = r / W + dy, x = r % W + dx;
+                if (y >= 0 && y < H && x >= 0 && x < W)
+                    xv = *(const uint4 *)(X + ((size_t)(b * H + y) * W + x) * Cin + ci0 + xc);
+            }
+        }
+        __syncthreads();
+        {
+            const uint16_t *d = (const uint16_t *)&dv;
+#pragma unroll
+            for (int k = 0; k < 8; k++) Ds[(dc + k) * kWgPitch + dp] = d[k];
+        }
+        if (tid < 128) {
+            const uint16_t *d = (const uint16_t *)&xv;
+#pragma unroll
+            for (int k = 0; k < 8; k++) Xs[(xc + k) * kWgPitch + xp] = d[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 32; kk += 16) {
+            const s4v a = *(const s4v *)(Ds + (wv * 16 + (lane & 15)) * kWgPitch + kk + 4 * (lane >> 4));
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const s4v b = *(const s4v *)(Xs + (j * 16 + (lane & 15)) * kWgPitch + kk + 4 * (lane >> 4));
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int ci = ci0 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int co = co0 + wv * 16 + 4 * (lane >> 4) + r;
+            if (co < Cout) atomicAdd(dW + ((size_t)co * 9 + t) * Cin + ci, acc[j][r]);
+        }
+    }
+}
+
+// db[c] += sum_p dU[p][c] over a chunk of pixels per block (C <= 256).
+__global__ void __launch_bounds__(256)
+k_bias_grad(const uint16_t *__restrict__ dU, float *__restrict__ db, int P, int C)
+{
+    const int c = threadIdx.x;
+    if (c >= C) return;
+    const int p0 = blockIdx.x * 256, p1 = min(p0 + 256, P);
+    float s = 0.0f;
+    for (int p = p0; p < p1; p++) s += bf2f(dU[(size_t)p * C + c]);
+    atomicAdd(db + c, s);
+}
+
+// dU = bf16(dY * [y > 0]) (y: the layer's ReLU output; relu = 0: no mask);
+// with dskip: dskip += the masked gradient (the skip input's share).
+__global__ void k_relu_bwd(const float *__restrict__ dY, const uint16_t *__restrict__ y, uint16_t *__restrict__ dU,
+                           float *__restrict__ dskip, size_t n, int relu)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float g = dY[i];
+        if (relu && !(bf2f(y[i]) > 0.0f)) g = 0.0f;
+        dU[i] = f2bf(g);
+        if (dskip) dskip[i] += g;
+    }
+}
+
+// 2x2 average pooling of NHWC bf16 (H, W even): out [B][H/2][W/2][C].
+__global__ void k_pool2(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, int B, int H, int W, int C)
+{
+    const int Ho = H / 2, Wo = W / 2;
+    const size_t n = (size_t)B * Ho * Wo * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        size_t q = i / C;
+        const int xo = (int)(q % Wo);
+        q /= Wo;
+        const int yo = (int)(q % Ho);
+        const int b = (int)(q / Ho);
+        const size_t r0 = ((size_t)(b * H + 2 * yo) * W + 2 * xo) * C + c, r1 = r0 + (size_t)W * C;
+        y[i] = f2bf(0.25f * ((bf2f(x[r0]) + bf2f(x[r0 + C])) + (bf2f(x[r1]) + bf2f(x[r1 + C]))));
+    }
+}
+
+// Gradient of the pooling: dx[4 children] += dy / 4 (FP32).
+__global__ void k_pool2_bwd(const float *__restrict__ dy, float *__restrict__ dx, int B, int H, int W, int C)
+{
+    const int Ho = H / 2, Wo = W / 2;
+    const size_t n = (size_t)B * Ho * Wo * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        size_t q = i / C;
+        const int xo = (int)(q % Wo);
+        q /= Wo;
+        const int yo = (int)(q % Ho);
+        const int b = (int)(q / Ho);
+        const float g = 0.25f * dy[i];
+        const size_t r0 = ((size_t)(b * H + 2 * yo) * W + 2 * xo) * C + c, r1 = r0 + (size_t)W * C;
+        dx[r0] += g;
+        dx[r0 + C] += g;
+        dx[r1] += g;
+        dx[r1 + C] += g;
+    }
+}
+
+// Nearest 2x upsampling of NHWC bf16: out [B][2H][2W][C].
+__global__ void k_up2(const uint16_t *__restrict__ x, uint16_t *__restrict__ y, int B, int H, int W, int C)
+{
+    const int Ho = 2 * H, Wo = 2 * W;
+    const size_t n = (size_t)B * Ho * Wo * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        size_t q = i / C;
+        const int xo = (int)(q % Wo);
+        q /= Wo;
+        const int yo = (int)(q % Ho);
+        const int b = (int)(q / Ho);
+        y[i] = x[((size_t)(b * H + yo / 2) * W + xo / 2) * C + c];
+    }
+}
+
+// Gradient of the upsampling: dx += the sum of the 4 children of dy (FP32).
+__global__ void k_up2_bwd(const float *__restrict__ dy, float *__restrict__ dx, int B, int H, int W, int C)
+{
+    const size_t n = (size_t)B * H * W * C;
+    const int Wo = 2 * W;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        size_t q = i / C;
+        const int x = (int)(q % W);
+        q /= W;
+        const int y = (int)(q % H);
+        const int b = (int)(q / H);
+        const size_t r0 = ((size_t)(b * 2 * H + 2 * y) * Wo + 2 * x) * C + c, r1 = r0 + (size_t)Wo * C;
+        dx[i] += (dy[r0] + dy[r0 + C]) + (dy[r1] + dy[r1 + C]);
+    }
+}
+
+__global__ void k_fill_f32(float *__restrict__ a, size_t n, float v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+
+__global__ void k_f32_to_bf16(const float *__restrict__ a, uint16_t *__restrict__ b, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = f2bf(a[i]);
+}
+
+// Wf[ci][t][co] = W[co][8 - t][ci]: the weights of the input-gradient convolution.
+__global__ void k_flip_weights(const uint16_t *__restrict__ Wt, uint16_t *__restrict__ Wf, int Cin, int Cout)
+{
+    const size_t n = (size_t)Cout * 9 * Cin;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int ci = (int)(i % Cin);
+        const size_t q = i / Cin;
+        const int t = (int)(q % 9);
+        const int co = (int)(q / 9);
+        Wf[((size_t)ci * 9 + (8 - t)) * Cout + co] = Wt[i];
+    }
+}
+
+// Loss of the prediction base + out[., 0..2] against the target (planar FP32
+// [B][3][H][W] for base and target; out NHWC FP32 with `co` channels):
+// per-block partial sums of the squared error (FP64) and the gradient
+// d loss / d out = 2 (pred - target) / count (0 on the padding channels).
+__global__ void __launch_bounds__(256)
+k_sq_loss(const float *__restrict__ out, const float *__restrict__ base, const float *__restrict__ target,
+          float *__restrict__ dout, double *__restrict__ partials, int B, int HW, int co, float scale)
+{
+    double s = 0.0;
+    const size_t n = (size_t)B * HW;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+        const size_t b = p / HW, o = p - b * HW;
+        for (int c = 0; c < co; c++) {
+            float g = 0.0f;
+            if (c < 3) {
+                const size_t q = (b * 3 + c) * HW + o;
+                const float d = base[q] + out[p * co + c] - target[q];
+                s += (double)d * d;
+                g = scale * d;
+            }
+            if (dout) dout[p * co + c] = g;
+        }
+    }
+    __shared__ double red[16];
+    double v[1] = {s};
+    block_partials<1>(v, partials, red);
+}
+
+// pred (planar FP32 [B][3][H][W]) = base + out[., 0..2].
+__global__ void k_add_prediction(const float *__restrict__ out, const float *__restrict__ base, float *__restrict__ pred,
+                                 int B, int HW, int co)
+{
+    const size_t n = (size_t)B * 3 * HW;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t o = i % HW, bc = i / HW, b = bc / 3, c = bc % 3;
+        pred[i] = base[i] + out[(b * HW + o) * co + c];
+    }
+}
+
+// Adam on FP32 master weights (Kingma & Ba, bias-corrected), then the bf16
+// copy the convolutions read.  mask (nullable) zeroes the update of padding
+// weights.
+__global__ void k_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v, const float *__restrict__ g,
+                       uint16_t *__restrict__ wb, size_t n, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        if (wb) wb[i] = f2bf(wi);
+    }
+}
+
+// One view of the warped stack: channels chan0 .. chan0+3 of out (bf16 NHWC
+// [N][cpad]) = the splatted colour sums normalised by the splatted weight,
+// and min(1, weight); 0 where no footprint reached the pixel.
+__global__ void k_stack_pack(const float *__restrict__ acc, const float *__restrict__ wsum, uint16_t *__restrict__ out,
+                             size_t N, int cpad, int chan0)
+{
+    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < N; o += (size_t)gridDim.x * blockDim.x) {
+        const float w = wsum[o];
+        const float iw = w > 0.0f ? 1.0f / w : 0.0f;
+        uint16_t *q = out + o * cpad + chan0;
+        q[0] = f2bf(acc[o] * iw);
+        q[1] = f2bf(acc[N + o] * iw);
+        q[2] = f2bf(acc[2 * N + o] * iw);
+        q[3] = f2bf(fminf(w, 1.0f));
+    }
+}
+
+// ---- launchers --------------------------------------------------------------
+
+inline unsigned int ew_blocks(size_t n) { return (unsigned int)std::min<size_t>((n + 255) / 256, 16384); }
+
+hipError_t cnn_conv_fwd(const uint16_t *X, const uint16_t *Wt, const float *bias, const uint16_t *skip, void *Y,
+                        int B, int H, int W, int Cin, int Cout, int epi, hipStream_t s)
+{
+    if (Cin % kCvBK != 0) return hipErrorInvalidValue;
+    const dim3 grid((unsigned int)(((size_t)B * H * W + kCvBM - 1) / kCvBM), (unsigned int)((Cout + kCvBN - 1) / kCvBN));
+#define CV(E) k_conv3x3<E><<<grid, 256, 0, s>>>(X, Wt, bias, skip, Y, B, H, W, Cin, Cout)
+    switch (epi) {
+    case kEpiBias | kEpiRelu: CV(kEpiBias | kEpiRelu); break;
+    case kEpiBias | kEpiRelu | kEpiSkip: CV(kEpiBias | kEpiRelu | kEpiSkip); break;
+    case kEpiBias: CV(kEpiBias); break;
+    case kEpiBias | kEpiF32: CV(kEpiBias | kEpiF32); break;
+    case kEpiF32: CV(kEpiF32); break;
+    case kEpiF32 | kEpiAccum: CV(kEpiF32 | kEpiAccum); break;
+    case 0: CV(0); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef CV
+    return hipGetLastError();
+}
+
+hipError_t cnn_conv_wgrad(const uint16_t *dU, const uint16_t *X, float *dW, int B, int H, int W, int Cin, int Cout,
+                          hipStream_t s)
+{
+    if (Cin % 32 != 0 || Cout % 8 != 0) return hipErrorInvalidValue;
+    const size_t P = (size_t)B * H * W;
+    const dim3 grid((unsigned int)((P + kWgChunk - 1) / kWgChunk), (unsigned int)((Cout + 63) / 64),
+                    (unsigned int)(9 * (Cin / 32)));
+    k_conv3x3_wgrad<<<grid, 256, 0, s>>>(dU, X, dW, B, H, W, Cin, Cout);
+    return hipGetLastError();
+}
+
+hipError_t cnn_bias_grad(const uint16_t *dU, float *db, int P, int C, hipStream_t s)
+{
+    if (C > 256) return hipErrorInvalidValue;
+    k_bias_grad<<<(P + 255) / 256, 256, 0, s>>>(dU, db, P, C);
+    return hipGetLastError();
+}
+
+hipError_t cnn_relu_bwd(const float *dY, const uint16_t *y, uint16_t *dU, float *dskip, size_t n, int relu,
+                        hipStream_t s)
+{
+    k_relu_bwd<<<ew_blocks(n), 256, 0, s>>>(dY, y, dU, dskip, n, relu);
+    return hipGetLastError();
+}
+
+hipError_t cnn_pool2(const uint16_t *x, uint16_t *y, int B, int H, int W, int C, hipStream_t s)
+{
+    k_pool2<<<ew_blocks((size_t)B * H * W * C / 4), 256, 0, s>>>(x, y, B, H, W, C);
+    return hipGetLastError();
+}
+
+hipError_t cnn_pool2_bwd(const float *dy, float *dx, int B, int H, int W, int C, hipStream_t s)
+{
+    k_pool2_bwd<<<ew_blocks((size_t)B * H * W * C / 4), 256, 0, s>>>(dy, dx, B, H, W, C);
+    return hipGetLastError();
+}
+
+hipError_t cnn_up2(const uint16_t *x, uint16_t *y, int B, int H, int W, int C, hipStream_t s)
+{
+    k_up2<<<ew_blocks((size_t)B * H * W * C * 4), 256, 0, s>>>(x, y, B, H, W, C);
+    return hipGetLastError();
+}
+
+hipError_t cnn_up2_bwd(const float *dy, float *dx, int B, int H, int W, int C, hipStream_t s)
+{
+    k_up2_bwd<<<ew_blocks((size_t)B * H * W * C), 256, 0, s>>>(dy, dx, B, H, W, C);
+    return hipGetLastError();
+}
+
+hipError_t cnn_fill(float *a, size_t n, float v, hipStream_t s)
+{
+    k_fill_f32<<<ew_blocks(n), 256, 0, s>>>(a, n, v);
+    return hipGetLastError();
+}
+
+hipError_t cnn_to_bf16(const float *a, uint16_t *b, size_t n, hipStream_t s)
+{
+    k_f32_to_bf16<<<ew_blocks(n), 256, 0, s>>>(a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t cnn_flip_weights(const uint16_t *Wt, uint16_t *Wf, int Cin, int Cout, hipStream_t s)
+{
+    k_flip_weights<<<ew_blocks((size_t)Cout * 9 * Cin), 256, 0, s>>>(Wt, Wf, Cin, Cout);
+    return hipGetLastError();
+}
+
+hipError_t cnn_sq_loss(const float *out, const float *base, const float *target, float *dout, double *partials,
+                       int nblocks, int B, int HW, int co, float scale, hipStream_t s)
+{
+    k_sq_loss<<<nblocks, 256, 0, s>>>(out, base, target, dout, partials, B, HW, co, scale);
+    return hipGetLastError();
+}
+
+hipError_t cnn_add_prediction(const float *out, const float *base, float *pred, int B, int HW, int co, hipStream_t s)
+{
+    k_add_prediction<<<ew_blocks((size_t)B * 3 * HW), 256, 0, s>>>(out, base, pred, B, HW, co);
+    return hipGetLastError();
+}
+
+hipError_t cnn_stack_pack(const float *acc, const float *wsum, uint16_t *out, size_t N, int cpad, int chan0,
+                          hipStream_t s)
+{
+    k_stack_pack<<<ew_blocks(N), 256, 0, s>>>(acc, wsum, out, N, cpad, chan0);
+    return hipGetLastError();
+}
+
+hipError_t cnn_adam(float *w, float *m, float *v, const float *g, uint16_t *wb, size_t n, float lr, float b1,
+                    float b2, float eps, int step, hipStream_t s)
+{
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    k_adam<<<ew_blocks(n), 256, 0, s>>>(w, m, v, g, wb, n, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+}  // namespace lfrv

@@ -1,4 +1,4 @@
-"""ctypes bindings of the FP64 oracle (oracle/lfrv_oracle.c).
+"""ctypes bindings of the FP64 oracle (oracle/lfrv_oracle.c, oracle/lfrv_oracle_cnn.c).
 
 TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg may import this module.  It is the checker of the HIP path,
@@ -72,6 +72,9 @@ def lib():
         "orc_sr_energy": (_d, [_i, _i, _i, _i, _i, _i, _fp, _fp, _fp, _dp, _d, _d, _d, _d, _i, _u8, _dp, _dp]),
         "orc_push_pull": (None, [_i, _i, _i, _dp, _u8]),
         "orc_psnr": (_d, [_i, _i, _dp, _dp]),
+        "orc_conv3x3": (None, [_i, _i, _i, _i, _i, _dp, _dp, _dp, _dp]),
+        "orc_cnn_num_params": (ctypes.c_size_t, [_i, _i, _i]),
+        "orc_cnn_forward_backward": (_d, [_i, _i, _i, _i, _i, _i, _dp, _dp, _dp, _dp, _i, _dp, _dp]),
         "orc_num_threads": (_i, []),
         "orc_set_num_threads": (None, [_i]),
     }
@@ -512,3 +515,34 @@ def coverage(W, H, dsf, tx, ty):
     for v in range(tx.shape[0]):
         splat(W, H, dsf, tx[v], ty[v], None, wsum=cov)
     return cov
+
+
+# --------------------------------------------------------------------------
+# The warm-start U-Net (lfrv_oracle_cnn.c)
+# --------------------------------------------------------------------------
+
+def conv3x3(X, Wt, b=None):
+    """Y[n, y, x, co] = sum_{t, ci} Wt[co, t, ci] X[n, y+dy, x+dx, ci] (+ b), zero padding;
+    X (B, H, W, Cin), Wt (Cout, 9, Cin)."""
+    X, Wt = _D(X), _D(Wt)
+    B, H, W, Cin = X.shape
+    Cout = Wt.shape[0]
+    Y = np.zeros((B, H, W, Cout))
+    lib().orc_conv3x3(B, H, W, Cin, Cout, pd(X), pd(Wt), pd(None if b is None else _D(b)), pd(Y))
+    return Y
+
+
+def cnn_num_params(levels, base, cpad):
+    return int(lib().orc_cnn_num_params(levels, base, cpad))
+
+
+def cnn_forward_backward(levels, base_ch, x, params, base, target, round_bf16=False, grads=True):
+    """x (B, H, W, cpad); base, target (B, 3, H, W).  Returns (loss, pred, grads or None)."""
+    x = _D(x)
+    B, H, W, cpad = x.shape
+    params = _D(params)
+    pred = np.zeros((B, 3, H, W))
+    g = np.zeros(params.size) if grads else None
+    loss = lib().orc_cnn_forward_backward(levels, base_ch, cpad, B, H, W, pd(x), pd(params), pd(_D(base)),
+                                          pd(_D(target)), int(round_bf16), pd(pred), pd(g))
+    return loss, pred, g
