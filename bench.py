@@ -498,6 +498,281 @@ def bench_color_sr(args, workload):
     print(json.dumps(line))
 
 
+# ---- batched serving: many uneven solves at once -------------------------------
+
+def c4_problems(seed, n):
+    """n solves with seeded uneven sizes and view counts: target side uniform
+    in [256, 1536] (a multiple of 16), 2x2 .. 5x5 views (4 .. 25), half
+    gradient-domain PCG solves (dsf 1, to rel. residual 1e-5) and half
+    IBR_color solves (dsf 2, 100 FISTA steps), all scenes from the tilted-plane
+    generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        side = int(rng.integers(256, 1537)) // 16 * 16
+        grid = int(rng.integers(2, 6))
+        kind = "cg" if rng.random() < 0.5 else "pd"
+        out.append(dict(kind=kind, side=side, grid=grid, seed=int(seed * 1000 + i)))
+    return out
+
+
+def c4_desc(pr):
+    if pr["kind"] == "cg":
+        d = lfrv.synth_desc(2, pr["seed"])
+        d.grid = pr["grid"]
+        d.w = d.h = pr["side"]
+        d.focal = 1000.0 * pr["side"] / 1024.0
+        return d
+    w = pr["side"] // 2
+    d = lfrv.synth_desc(3, pr["seed"])
+    d.grid = pr["grid"]
+    d.w = d.h = w
+    d.dsf = 2
+    d.render_scale = 2
+    d.focal = float(w)
+    d.baseline = 10.0 / w
+    return d
+
+
+PD_STEPS = 100
+
+
+def c4_contexts(problems, cfg):
+    """One context per solve with its scene's views, depths and cameras in
+    device memory (rendered on the GPU)."""
+    from lfrv.warmstart import SceneOnDevice
+    ctxs = []
+    for pr in problems:
+        d = c4_desc(pr)
+        sc = SceneOnDevice(d)
+        scene = dict(V=sc.V, w=d.w, h=d.h, dsf=d.dsf)
+        kw = dict(tile_w=cfg["tile_w"], tile_h=cfg["tile_h"], pull_lds_kb=cfg["pull_lds_kb"])
+        if pr["kind"] == "cg":
+            prm = pipelines.gradient_ibr_params(scene, **kw)
+        else:
+            prm = pipelines.color_sr_params(scene, lambda_=0.05, lambda_max_factor=100.0, inner_iterations=1,
+                                            dt_beta=0.3, gradient_step=1.0, **kw)
+        ctx = lfrv.Context(prm)
+        sc.load(ctx)
+        sc.free()
+        ctxs.append(ctx)
+    return ctxs
+
+
+def c4_serial(problems, ctxs):
+    """The plain form: the solves one after another, each finished before the next."""
+    lat = []
+    sols = []
+    t0 = time.perf_counter()
+    for pr, ctx in zip(problems, ctxs):
+        if pr["kind"] == "cg":
+            ctx.ibr_gradient(TOL, 5000)
+        else:
+            ctx.clear_solution()
+            pipelines.color_sr_setup(ctx)
+            ctx.sr_init_regularizer_weight()
+            ctx.sr_iterate(PD_STEPS)
+        ctx.sync()
+        lat.append(1e3 * (time.perf_counter() - t0))
+    total = time.perf_counter() - t0
+    for ctx in ctxs:
+        sols.append(ctx.get_solution())
+    return total, lat, sols
+
+
+def c4_batched(problems, ctxs, chunk):
+    from lfrv.batch import Batch
+    b = Batch(chunk=chunk, use_graph=True)
+    for pr, ctx in zip(problems, ctxs):
+        if pr["kind"] == "cg":
+            b.add_cg(ctx, TOL, 5000)
+        else:
+            b.add_pd(ctx, PD_STEPS)
+    return b
+
+
+def bench_batch(args):
+    """One step = one batch of 64 uneven solves, from device-resident views,
+    depths and cameras to the device-resident solutions."""
+    from lfrv import autotune
+    n = 64
+    problems = c4_problems(args.seed, n)
+    # autotuner on a calibration batch of 8 of the solves
+    calib = problems[:8]
+
+    def run_calib(cfg):
+        ctxs = c4_contexts(calib, cfg)
+        b = c4_batched(calib, ctxs, cfg["chunk"])
+        b.run()
+        lfrv.device_sync()
+        t = time.perf_counter()
+        b.run()
+        lfrv.device_sync()
+        dt = time.perf_counter() - t
+        b.close()
+        for c in ctxs:
+            c.close()
+        return dt
+
+    sig = "batch_c4:" + ",".join(f"{p['kind']}{p['side']}x{p['grid']}" for p in calib)
+    t_tune = time.perf_counter()
+    cfg, trials = autotune.tune(run_calib, sig, reps=2, use_cache=not args.no_tune_cache,
+                                log=lambda m: print(m, file=sys.stderr, flush=True))
+    t_tune = time.perf_counter() - t_tune
+    print(f"autotune: {cfg} in {t_tune:.1f} s ({len(trials)} trials)", file=sys.stderr, flush=True)
+    ctxs = c4_contexts(problems, cfg)
+    lfrv.device_sync()
+    # the plain form: one solve after another (warm-up pass first)
+    c4_serial(problems, ctxs)
+    t_serial, lat_serial, sols_serial = c4_serial(problems, ctxs)
+    # the batched form
+    b = c4_batched(problems, ctxs, cfg["chunk"])
+    for _ in range(max(1, args.warmup)):
+        b.run()
+    lfrv.device_sync()
+    walls, lats = [], []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        res = b.run()
+        lfrv.device_sync()
+        walls.append(time.perf_counter() - t)
+        lats += [r["done_ms"] for r in res]
+    identical = all(np.array_equal(ctx.get_solution(), s) for ctx, s in zip(ctxs, sols_serial))
+    iters = [r["iterations"] for r in res]
+    b.close()
+    for c in ctxs:
+        c.close()
+    wall = float(np.median(walls))
+    vps = n / wall
+    vps_serial = n / t_serial
+    line = {
+        "metric": "novel views per second, batch of 64 uneven solves (target 256²-1536², 4-25 views, "
+                  "PCG to rel. residual 1e-5 / 100 FISTA steps)",
+        "value": round(vps, 2),
+        "unit": "views/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched serving: 64 seeded solves, target side uniform in [256, 1536], 4-25 views, "
+                               "half gradient-IBR PCG (dsf 1) and half IBR_color (dsf 2) from the tilted-plane "
+                               "generator; one batched PCG launch per iteration replayed as a hipGraph",
+                   "solves": n, "cg_solves": sum(p["kind"] == "cg" for p in problems),
+                   "pd_solves": sum(p["kind"] == "pd" for p in problems), "seed": args.seed,
+                   "autotuned": cfg},
+        "batched": {"views_per_s": round(vps, 2), "wall_ms_median": round(1e3 * wall, 3),
+                    "latency_p50_ms": round(float(np.percentile(lats, 50)), 3),
+                    "latency_p99_ms": round(float(np.percentile(lats, 99)), 3)},
+        "serial": {"views_per_s": round(vps_serial, 2), "wall_ms": round(1e3 * t_serial, 3),
+                   "latency_p50_ms": round(float(np.percentile(lat_serial, 50)), 3),
+                   "latency_p99_ms": round(float(np.percentile(lat_serial, 99)), 3)},
+        "speedup_vs_serial": round(vps / vps_serial, 3),
+        "solutions_identical_to_serial": bool(identical),
+        "cg_iterations": {"min": int(min(i for i, p in zip(iters, problems) if p["kind"] == "cg")),
+                          "max": int(max(i for i, p in zip(iters, problems) if p["kind"] == "cg"))},
+        "autotune": {"seconds": round(t_tune, 1), "trials": len(trials),
+                     "calibration": "the first 8 solves, 2 timed runs per configuration"},
+    }
+    print(json.dumps(line))
+
+
+# ---- the learned warm start ----------------------------------------------------------
+
+def bench_warm_start(args):
+    """The learned warm start: 2000 scenes of the 5 x 5-view generator (128^2
+    views, 256^2 targets) solved to rel. residual 1e-5 by the plain PCG, the
+    U-Net trained on them on the GPU, 200 held-out scenes solved both ways,
+    then end-to-end timing at the full size (512^2 views, 100 input channels
+    at 1024^2).  One step = one full-size novel view, warm-started."""
+    from lfrv import warmstart as wsm
+    from lfrv.cnn import Cnn
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    n_train, n_test = args.train_scenes, args.test_scenes
+    w_small, w_full = 128, 512
+    t0 = time.perf_counter()
+    data = wsm.TrainingSet(list(range(100000, 100000 + n_train)), w_small, log=log)
+    t_data = time.perf_counter() - t0
+    log(f"training set: {n_train} scenes in {t_data:.1f} s, plain PCG iterations mean {data.iters.mean():.1f}")
+    net = Cnn(data.in_channels, levels=4, base=32, max_batch=8, max_h=data.H, max_w=data.W, seed=7)
+    t0 = time.perf_counter()
+    losses = wsm.train(net, data, epochs=args.epochs, batch=8, lr=1e-3, log=log)
+    lfrv.device_sync()
+    t_train = time.perf_counter() - t0
+    data.free()
+    params = net.get_params()
+    res = wsm.compare(net, list(range(900000, 900000 + n_test)), w_small)
+    net.close()
+    ip = np.array([r["iters_plain"] for r in res], np.float64)
+    iw = np.array([r["iters_warm"] for r in res], np.float64)
+    used = np.array([r["used_net"] for r in res])
+    rel = max(r["rel_diff"] for r in res)
+    # full size: the trained weights on a 1024^2 network
+    big = Cnn(data.in_channels, levels=4, base=32, max_batch=1, max_h=2 * w_full, max_w=2 * w_full, seed=7)
+    big.set_params(params)
+    seeds_full = list(range(990000, 990000 + args.steps + args.warmup))
+    full = wsm.compare(big, seeds_full, w_full)[args.warmup:]
+    # MFMA rate of the convolutions on one full-size forward
+    big.set_timing(True)
+    import ctypes
+    from lfrv.cnn import Raw
+    xin = Raw(4 * w_full * w_full * big.cpad * 2)
+    base = Raw(3 * 4 * w_full * w_full * 4)
+    big.set_batch(1, 2 * w_full, 2 * w_full, [xin.ptr], [base.ptr])
+    for _ in range(3):
+        big.forward()
+    conv_ms, conv_n, conv_flops = big.get_timing()
+    big.set_timing(False)
+    fwd_flops = big.forward_flops()
+    xin.free()
+    base.free()
+    big.close()
+    del ctypes
+    tflops = conv_flops / (conv_ms * 1e-3) / 1e12
+    ms_plain = float(np.median([r["ms_plain"] for r in full]))
+    ms_warm = float(np.median([r["ms_warm"] for r in full]))
+    line = {
+        "metric": "ms per novel view, learned warm start + PCG to rel. residual 1e-5 (25 views 512² -> 1024²)",
+        "value": round(ms_warm, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_warm, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 network (FP32 accumulation), f32 solver",
+        "data": "synthetic",
+        "config": {"workload": "learned warm start: 4-level U-Net (32-256 channels, 3x3 convs, bf16 MFMA) on 25 "
+                               "warped views + validity (100 channels) predicting the PCG start; trained on "
+                               f"{n_train} solved scenes (128² views, 256² targets), tested on {n_test} held-out "
+                               "scenes and end to end at 512² views -> 1024²",
+                   "train_scenes": n_train, "test_scenes": n_test, "epochs": args.epochs, "batch": 8},
+        "held_out": {"scenes": n_test, "cg_iterations_plain_mean": round(float(ip.mean()), 3),
+                     "cg_iterations_warm_mean": round(float(iw.mean()), 3),
+                     "iterations_saved_mean": round(float((ip - iw).mean()), 3),
+                     "iterations_saved_fraction": round(float(1 - iw.sum() / ip.sum()), 4),
+                     "net_start_used": int(used.sum()), "max_rel_diff_vs_plain_solution": float("%.3g" % rel)},
+        "full_size": {"views": len(full), "ms_plain_median": round(ms_plain, 3), "ms_warm_median": round(ms_warm, 3),
+                      "speedup_vs_plain": round(ms_plain / ms_warm, 3),
+                      "cg_iterations_plain": [r["iters_plain"] for r in full],
+                      "cg_iterations_warm": [r["iters_warm"] for r in full],
+                      "net_start_used": [r["used_net"] for r in full]},
+        "training": {"seconds_data": round(t_data, 1), "seconds_train": round(t_train, 1),
+                     "loss_first_epoch": float("%.4g" % losses[0]), "loss_last_epoch": float("%.4g" % losses[-1])},
+        "roofline": {"bound": "mfma", "kernel": "k_conv3x3 (implicit-GEMM 3x3 conv, bf16 MFMA 16x16x16)",
+                     "achieved": round(tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tflops / 2500.0, 4), "traffic": None,
+                     "flops_per_forward": fwd_flops, "conv_launches": conv_n},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -510,8 +785,14 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the timed region, run the library's plain copy kernel once over MIB MiB "
                          "(reference dispatch for the HBM byte counters)")
-    ap.add_argument("--workload", default="gradient_ibr_c2", choices=["gradient_ibr_c2"] + sorted(COLOR_WORKLOADS),
-                    help="gradient_ibr_c2 (the headline metric, default) or an IBR_color configuration")
+    ap.add_argument("--workload", default="gradient_ibr_c2",
+                    choices=["gradient_ibr_c2", "batch_c4", "warm_start_c5"] + sorted(COLOR_WORKLOADS),
+                    help="gradient_ibr_c2 (the headline metric, default), batch_c4 (64 uneven solves at once) "
+                         "or an IBR_color configuration")
+    ap.add_argument("--no-tune-cache", action="store_true", help="batch_c4: autotune afresh, ignore the cache")
+    ap.add_argument("--train-scenes", type=int, default=2000, help="warm_start_c5: training scenes")
+    ap.add_argument("--test-scenes", type=int, default=200, help="warm_start_c5: held-out scenes")
+    ap.add_argument("--epochs", type=int, default=4, help="warm_start_c5: training epochs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs one novel-view solve per GPU "
@@ -520,6 +801,12 @@ def main():
         sys.exit(2)
     if args.workload in COLOR_WORKLOADS:
         bench_color_sr(args, args.workload)
+        return
+    if args.workload == "batch_c4":
+        bench_batch(args)
+        return
+    if args.workload == "warm_start_c5":
+        bench_warm_start(args)
         return
 
     scene = lfrv.synth(2, seed=args.seed)
