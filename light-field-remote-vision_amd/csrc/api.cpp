@@ -261,7 +261,7 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
              ctx->u_grad_y, ctx->target_mask, ctx->visibility, ctx->target_depth, ctx->tmpN, ctx->cg_minv,
              ctx->hole_list, ctx->pp_scratch, ctx->cg_state, ctx->counters, ctx->dcams, ctx->scal, ctx->partials,
              ctx->cache_num, ctx->cache_sq, ctx->gram, ctx->cg_extra, ctx->sr_loop, ctx->sr_prev, ctx->depth_grad,
-             ctx->tau_weight})
+             ctx->tau_weight, ctx->gl_big})
         if (p) (void)hipFree(p);
     if (ctx->cg_state_host) (void)hipHostFree(ctx->cg_state_host);
     if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);

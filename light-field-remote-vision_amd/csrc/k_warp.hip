@@ -180,7 +180,7 @@ __global__ void k_tau_visibility(int V, int w, int h, const float *__restrict__ 
 // ---- the OpenGL warp producer (Scene::computeTargetDepthMap + exportWarps) ----
 
 constexpr float kGlZNear = 1e-2f;  // near plane of the render camera, pinholeCamera.cpp:133
-constexpr int kGlMaxSpan = 64;     // quads wider or taller than this many target pixels are skipped
+constexpr int kGlMaxSpan = 64;     // quads wider or taller than this go to the overflow list (k_gl_splat_big)
 
 __device__ __forceinline__ float gl_edge(const float *a, const float *b, float px, float py)
 {
@@ -215,6 +215,8 @@ __device__ __forceinline__ bool gl_tri(const float *a, const float *b, const flo
     return true;
 }
 
+__global__ void k_zero_u32(unsigned int *__restrict__ p) { *p = 0u; }
+
 __global__ void k_fill_u64(unsigned long long *__restrict__ p, size_t n, unsigned long long v)
 {
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -230,12 +232,15 @@ __global__ void k_fill_u64(unsigned long long *__restrict__ p, size_t n, unsigne
 //
 // The quad of source pixel k = (v, mx, my): its corners (x, y, h2) in the
 // target, the low word of its depth-test key and the box of target pixels it
-// may cover, [px0, px1] x [py0, py1] (empty when px0 > px1 or py0 > py1);
-// false when the quad is not drawn.
+// may cover, [px0, px1] x [py0, py1] clipped to the target (empty when
+// px0 > px1 or py0 > py1); false when the quad is not drawn (a corner at or
+// behind the near plane, a non-finite corner, or a box outside the target).
+// `big`: the quad spans more than kGlMaxSpan target pixels in x or y; such
+// quads are rasterised by a block each (k_gl_splat_big), not by one lane.
 __device__ __forceinline__ bool gl_quad(int V, int W, int H, size_t n, size_t k, int mx, int my,
                                         const float *__restrict__ depth, const float *__restrict__ grad,
                                         const WarpCam &cam, float (&q)[4][3], unsigned int &lo, int &px0,
-                                        int &px1, int &py0, int &py1)
+                                        int &px1, int &py0, int &py1, bool &big)
 {
     const float z = depth[k];
     if (z >= 0.5f * kInvalidDepth) return false;
@@ -264,16 +269,24 @@ __device__ __forceinline__ bool gl_quad(int V, int W, int H, size_t n, size_t k,
         v0 = fminf(v0, q[c][1]);
         v1 = fmaxf(v1, q[c][1]);
     }
-    // the span test in float first: a quad near the camera plane can project far
-    // beyond the int range
-    if (!(u1 - u0 <= (float)kGlMaxSpan && v1 - v0 <= (float)kGlMaxSpan)) return false;
+    if (!(isfinite(u0) && isfinite(u1) && isfinite(v0) && isfinite(v1))) return false;
     if (u1 < 0.0f || v1 < 0.0f || u0 > (float)W || v0 > (float)H) return false;
-    px0 = max((int)ceilf(u0 - 0.5f), 0);
-    px1 = min((int)floorf(u1 - 0.5f), W - 1);
-    py0 = max((int)ceilf(v0 - 0.5f), 0);
-    py1 = min((int)floorf(v1 - 0.5f), H - 1);
+    big = !(u1 - u0 <= (float)kGlMaxSpan && v1 - v0 <= (float)kGlMaxSpan);
+    // clipped in float: a quad near the camera plane can project far beyond the int range
+    px0 = (int)fmaxf(ceilf(u0 - 0.5f), 0.0f);
+    px1 = (int)fminf(floorf(u1 - 0.5f), (float)(W - 1));
+    py0 = (int)fmaxf(ceilf(v0 - 0.5f), 0.0f);
+    py1 = (int)fminf(floorf(v1 - 0.5f), (float)(H - 1));
     lo = __float_as_uint(zval);
     return true;
+}
+
+// A big quad waits in the overflow list for k_gl_splat_big.
+__device__ __forceinline__ void gl_defer(size_t k, unsigned int *__restrict__ big_count, unsigned int *__restrict__ big_list,
+                                         unsigned int cap)
+{
+    const unsigned int i = atomicAdd(big_count, 1u);
+    if (i < cap) big_list[i] = (unsigned int)k;
 }
 
 // Target pixel (px, py) against the quad's two triangles; the depth test.
@@ -290,14 +303,20 @@ __device__ __forceinline__ void gl_cover(const float (&q)[4][3], unsigned int lo
 // the lanes of a wave already write neighbouring pixels).
 __global__ void k_gl_splat(int V, int w, int h, int W, int H, int skip, const float *__restrict__ depth,
                            const float *__restrict__ grad, const WarpCam *__restrict__ cams,
-                           unsigned long long *__restrict__ key)
+                           unsigned long long *__restrict__ key, unsigned int *__restrict__ big_count,
+                           unsigned int *__restrict__ big_list, unsigned int big_cap)
 {
     SOURCE_INDEX()
     if (v == skip) return;
     float q[4][3];
     unsigned int lo;
     int px0, px1, py0, py1;
-    if (!gl_quad(V, W, H, n, k, mx, my, depth, grad, cams[v], q, lo, px0, px1, py0, py1)) return;
+    bool big = false;
+    if (!gl_quad(V, W, H, n, k, mx, my, depth, grad, cams[v], q, lo, px0, px1, py0, py1, big)) return;
+    if (big) {
+        gl_defer(k, big_count, big_list, big_cap);
+        return;
+    }
     for (int py = py0; py <= py1; py++)
         for (int px = px0; px <= px1; px++) gl_cover(q, lo, px, py, W, key);
 }
@@ -311,7 +330,8 @@ __global__ void k_gl_splat(int V, int w, int h, int W, int H, int skip, const fl
 // same arithmetic: the same minima.
 __global__ void k_gl_splat_wave(int V, int w, int h, int W, int H, int skip, const float *__restrict__ depth,
                                 const float *__restrict__ grad, const WarpCam *__restrict__ cams,
-                                unsigned long long *__restrict__ key)
+                                unsigned long long *__restrict__ key, unsigned int *__restrict__ big_count,
+                                unsigned int *__restrict__ big_list, unsigned int big_cap)
 {
     const int mx = blockIdx.x * blockDim.x + threadIdx.x, my = blockIdx.y * blockDim.y + threadIdx.y;
     const int v = blockIdx.z, lane = threadIdx.x;
@@ -319,10 +339,14 @@ __global__ void k_gl_splat_wave(int V, int w, int h, int W, int H, int skip, con
     float q[4][3] = {};
     unsigned int lo = 0;
     int px0 = 0, px1 = -1, py0 = 0, py1 = -1;
-    bool ok = mx < w && my < h && v != skip;
+    bool ok = mx < w && my < h && v != skip, big = false;
     if (ok)
         ok = gl_quad(V, W, H, n, (size_t)v * n + (size_t)my * w + mx, mx, my, depth, grad, cams[v], q, lo, px0,
-                     px1, py0, py1);
+                     px1, py0, py1, big);
+    if (ok && big) {
+        gl_defer((size_t)v * n + (size_t)my * w + mx, big_count, big_list, big_cap);
+        ok = false;
+    }
     const int bw = ok && py0 <= py1 ? max(px1 - px0 + 1, 0) : 0;
     // exclusive offsets of the boxes' columns in lane order (every lane takes
     // part in the shuffles below)
@@ -350,6 +374,35 @@ __global__ void k_gl_splat_wave(int V, int w, int h, int W, int H, int skip, con
         const int oy0 = __shfl(py0, o, 64), oy1 = __shfl(py1, o, 64);
         if (f >= total) continue;
         for (int py = oy0; py <= oy1; py++) gl_cover(oq, olo, px, py, W, key);
+    }
+}
+
+// The quads of the overflow list, a block each: the threads walk the quad's
+// box (clipped to the target) in row-major steps, so an atomic instruction
+// writes consecutive target pixels.  Same arithmetic as k_gl_splat.
+__global__ void __launch_bounds__(256)
+k_gl_splat_big(int V, int w, int h, int W, int H, const float *__restrict__ depth, const float *__restrict__ grad,
+               const WarpCam *__restrict__ cams, unsigned long long *__restrict__ key,
+               const unsigned int *__restrict__ big_count, const unsigned int *__restrict__ big_list,
+               unsigned int big_cap)
+{
+    const size_t n = (size_t)w * h;
+    const unsigned int cnt = min(*big_count, big_cap);
+    for (unsigned int e = blockIdx.x; e < cnt; e += gridDim.x) {
+        const size_t k = big_list[e];
+        const int v = (int)(k / n);
+        const size_t m = k - (size_t)v * n;
+        const int mx = (int)(m % w), my = (int)(m / w);
+        float q[4][3];
+        unsigned int lo;
+        int px0, px1, py0, py1;
+        bool big = false;
+        if (!gl_quad(V, W, H, n, k, mx, my, depth, grad, cams[v], q, lo, px0, px1, py0, py1, big)) continue;
+        if (px0 > px1 || py0 > py1) continue;
+        const int bw = px1 - px0 + 1;
+        const long long area = (long long)bw * (py1 - py0 + 1);
+        for (long long i = threadIdx.x; i < area; i += blockDim.x)
+            gl_cover(q, lo, px0 + (int)(i % bw), py0 + (int)(i / bw), W, key);
     }
 }
 
@@ -797,13 +850,23 @@ lfrv_status run_warps_from_target_depth(lfrv_ctx *ctx, int skip)
     // the depth-test keys in the 3N scratch (8 bytes per target pixel)
     unsigned long long *key = reinterpret_cast<unsigned long long *>(ctx->tmp3N);
     TimedRegion tr(ctx, 4);
+    // the overflow list of big quads: one slot per source pixel, and its count
+    const size_t vn = (size_t)ctx->V * ctx->n;
+    if (!ctx->gl_big) LFRV_HIP(ctx, hipMalloc((void **)&ctx->gl_big, (vn + 1) * sizeof(unsigned int)));
+    unsigned int *big_count = ctx->gl_big, *big_list = ctx->gl_big + 1;
+    const unsigned int big_cap = (unsigned int)std::min<size_t>(vn, 0xffffffffu);
     k_fill_u64<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(key, ctx->N, ~0ull);
+    k_zero_u32<<<1, 1, 0, ctx->stream>>>(big_count);
     if (ctx->dsf == 1)
         k_gl_splat<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, skip,
-                                                                       ctx->depth, ctx->depth_grad, ctx->dcams, key);
+                                                                       ctx->depth, ctx->depth_grad, ctx->dcams, key,
+                                                                       big_count, big_list, big_cap);
     else
         k_gl_splat_wave<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, skip,
-                                                                       ctx->depth, ctx->depth_grad, ctx->dcams, key);
+                                                                       ctx->depth, ctx->depth_grad, ctx->dcams, key,
+                                                                       big_count, big_list, big_cap);
+    k_gl_splat_big<<<1024, 256, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, ctx->depth, ctx->depth_grad,
+                                                  ctx->dcams, key, big_count, big_list, big_cap);
     k_gl_depth<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(key, ctx->target_depth, ctx->N);
     k_gl_tau<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
         ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, ctx->depth, ctx->depth_grad, ctx->dcams, ctx->target_depth,

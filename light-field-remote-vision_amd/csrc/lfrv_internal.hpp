@@ -318,6 +318,7 @@ struct lfrv_ctx {
     float *depth = nullptr;    // V*n
     float *depth_grad = nullptr;  // 2*V*n dz/dx then dz/dy (lfrv_set_depth_gradient; null: zero)
     float *tau_weight = nullptr;  // V*n deformation weights of the OpenGL warp producer (allocated on first use)
+    unsigned int *gl_big = nullptr;  // 1 + V*n: the OpenGL producer's count and list of big quads (first use)
     float *view_lap = nullptr; // V*3n Laplacian of the views (IBR_color gradient term)
     float *view_grad = nullptr;// V*2n Sobel of the views' luminance
     std::vector<double> P_views;  // V*12

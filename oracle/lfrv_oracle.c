@@ -561,7 +561,6 @@ void orc_warps_from_depth(int V, int w, int h, int dsf, const float *depth, cons
 /* ------------------------------------------------------------------------- */
 
 #define ORC_GL_ZNEAR 1e-2  /* near plane of the render camera, pinholeCamera.cpp:133 */
-#define ORC_GL_MAX_SPAN 64 /* quads wider or taller than this many target pixels are skipped */
 
 /* M = P_u3 R_v^T K_v^-1 and c = P_u (C_v, 1): x_u ~ z M (x, y, 1) + c for a
  * source pixel (x, y) at depth z (tauWarp.frag:30-32, depthSplatting.geom:34-57
@@ -665,13 +664,12 @@ void orc_gl_target_depth(int V, int w, int h, int W, int H, const float *depth, 
                 v0 = fmin(v0, q[k][1]);
                 v1 = fmax(v1, q[k][1]);
             }
-            int px0 = (int)ceil(u0 - 0.5), px1 = (int)floor(u1 - 0.5);
-            int py0 = (int)ceil(v0 - 0.5), py1 = (int)floor(v1 - 0.5);
-            if (px1 - px0 + 1 > ORC_GL_MAX_SPAN || py1 - py0 + 1 > ORC_GL_MAX_SPAN) continue;
-            if (px0 < 0) px0 = 0;
-            if (py0 < 0) py0 = 0;
-            if (px1 > W - 1) px1 = W - 1;
-            if (py1 > H - 1) py1 = H - 1;
+            if (!(isfinite(u0) && isfinite(u1) && isfinite(v0) && isfinite(v1))) continue;
+            /* every quad is rasterised, over its box clipped to the target (GL
+             * clips primitives to the viewport, not by size) */
+            if (u1 < 0.0 || v1 < 0.0 || u0 > W || v0 > H) continue;
+            const int px0 = (int)fmax(ceil(u0 - 0.5), 0.0), px1 = (int)fmin(floor(u1 - 0.5), W - 1.0);
+            const int py0 = (int)fmax(ceil(v0 - 0.5), 0.0), py1 = (int)fmin(floor(v1 - 0.5), H - 1.0);
             for (int py = py0; py <= py1; py++)
                 for (int px = px0; px <= px1; px++) {
                     double zi;

@@ -164,3 +164,27 @@ def test_export_warps_files_feed_the_cli(tmp_path):
         U = ctx.get_solution()
     assert all(st["converged"])
     assert np.max(np.abs(U_cli - U)) <= 1e-6 * max(1.0, np.abs(U).max())
+
+
+@pytest.mark.parametrize("dsf,zoom", [(1, 90.0), (4, 30.0)])
+def test_gl_producer_rasterises_big_quads(dsf, zoom):
+    """A target camera magnifying `zoom` times beyond the grid's: each source
+    pixel's quad spans more target pixels than the per-thread walk takes (64),
+    so it goes through the overflow pass; the target depth still matches the
+    oracle (which rasterises every quad)."""
+    s = _scene(w=40, h=32, seed=6, dsf=dsf)
+    W, H = s["W"], s["H"]
+    Pu = s["P_target"].copy()
+    Pu[:2, :] *= zoom                   # K_u scaled about the principal point: x_u - c = zoom (x - c)
+    Pu[0, :] += (1 - zoom) * Pu[2, :] * (W / 2)
+    Pu[1, :] += (1 - zoom) * Pu[2, :] * (H / 2)
+    print(f"GL producer, big quads: dsf {dsf}, zoom {zoom}", flush=True)
+    with lfrv.Context(pipelines.gradient_ibr_params(s)) as ctx:
+        pipelines.load_scene(ctx, s)
+        ctx.set_target_camera(Pu)
+        ctx.warps_from_target_depth(-1)
+        ud = ctx.get_buffer("TARGET_DEPTH").reshape(H, W)
+    oud = oracle.gl_target_depth(s["depths"], s["P_views"], Pu, W, H)
+    assert (oud > 0).mean() > 0.5
+    bad = np.abs(ud - oud) > 1e-4 * np.maximum(np.abs(oud), 1.0)
+    assert bad.mean() <= 5e-3, bad.mean()

@@ -135,3 +135,41 @@ def test_target_depth_matches_a_brute_force_rasteriser():
     ok = ~amb
     assert ok.mean() > 0.8
     np.testing.assert_allclose(ud[ok], want[ok], rtol=1e-6)
+
+
+def test_big_quads_are_rasterised():
+    """A target camera that magnifies 80x: every source pixel's quad spans 80
+    target pixels, beyond any per-thread walk, and is still rasterised (GL
+    clips by the viewport, not by size).  Brute force: source pixel (x, y) at
+    depth z covers [k (x - cxv) + cxu + s, k (x + 1 - cxv) + cxu + s], k = F / f,
+    s = F b / z."""
+    h, w, f, k = 4, 5, 4.0, 80.0
+    F = k * f
+    W, H = 200, 160
+    Pu = _P(F, W / 2, H / 2, [0, 0, 0])
+    shifts = [(0.0, 0.0), (0.004, -0.003)]
+    Pv = np.stack([_P(f, w / 2, h / 2, [sx, sy, 0]) for sx, sy in shifts])
+    depth = np.stack([np.full((h, w), 2.0), RNG.choice([1.5, 2.5], size=(h, w))]).astype(np.float32)
+    ud = oracle.gl_target_depth(depth, Pv, Pu, W, H)
+    best = np.full((H, W), np.inf)
+    amb = np.zeros((H, W), bool)
+    cx, cy = np.arange(W) + 0.5, np.arange(H) + 0.5
+    for v, (bx, by) in enumerate(shifts):
+        for y in range(h):
+            for x in range(w):
+                z = float(depth[v, y, x])
+                x0 = k * (x - w / 2) + W / 2 + F * bx / z
+                y0 = k * (y - h / 2) + H / 2 + F * by / z
+                x1, y1 = x0 + k, y0 + k
+                inx = (cx > x0 + 1e-6) & (cx < x1 - 1e-6)
+                iny = (cy > y0 + 1e-6) & (cy < y1 - 1e-6)
+                onx = (np.abs(cx - x0) <= 1e-6) | (np.abs(cx - x1) <= 1e-6)
+                ony = (np.abs(cy - y0) <= 1e-6) | (np.abs(cy - y1) <= 1e-6)
+                cover = iny[:, None] & inx[None, :]
+                amb |= (iny[:, None] | ony[:, None]) & (inx[None, :] | onx[None, :]) & ~cover
+                upd = cover & (z < best)
+                best[upd] = z
+    want = np.where(np.isinf(best), 0.0, best)
+    ok = ~amb
+    assert (want[ok] > 0).sum() > 0.5 * W * H      # most of the target is covered by big quads
+    np.testing.assert_allclose(ud[ok], want[ok], rtol=1e-6)
