@@ -10,7 +10,9 @@ import json
 import os
 import time
 
-TILES = [(32, 8), (64, 4), (32, 16), (16, 16), (64, 8)]
+from . import LfrvError
+
+TILES = [(32, 8), (64, 4), (32, 16), (16, 16), (16, 8)]
 LDS_KB = [32, 48, 64]
 CHUNKS = [4, 8, 16]
 DEFAULT = dict(tile_w=32, tile_h=8, pull_lds_kb=48, chunk=8)
@@ -49,8 +51,14 @@ def tune(run_batch, signature, reps=3, use_cache=True, log=None):
 
     def measure(cfg):
         ts = []
-        for _ in range(reps):
-            ts.append(run_batch(cfg))
+        try:
+            for _ in range(reps):
+                ts.append(run_batch(cfg))
+        except LfrvError as e:  # a shape the library refuses for these solves
+            trials.append(dict(config=dict(cfg), seconds=None, refused=str(e)))
+            if log:
+                log(f"autotune {cfg}: refused ({e})")
+            return float("inf")
         t = min(ts)
         trials.append(dict(config=dict(cfg), seconds=t))
         if log:

@@ -125,13 +125,13 @@ class TrainingSet:
         self.seconds = time.perf_counter() - t0
 
     def x_ptr(self, k):
-        return ctypes.c_void_p(self.x.ptr.value + k * self.W * self.H * self.cpad * 2)
+        return ctypes.c_void_p(self.x.ptr.value + int(k) * self.W * self.H * self.cpad * 2)
 
     def u0_ptr(self, k):
-        return ctypes.c_void_p(self.u0.ptr.value + k * 3 * self.W * self.H * 4)
+        return ctypes.c_void_p(self.u0.ptr.value + int(k) * 3 * self.W * self.H * 4)
 
     def ustar_ptr(self, k):
-        return ctypes.c_void_p(self.ustar.ptr.value + k * 3 * self.W * self.H * 4)
+        return ctypes.c_void_p(self.ustar.ptr.value + int(k) * 3 * self.W * self.H * 4)
 
     def free(self):
         for b in (self.x, self.u0, self.ustar):
