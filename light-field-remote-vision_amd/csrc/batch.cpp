@@ -138,7 +138,11 @@ extern "C" lfrv_status lfrv_batch_run(lfrv_batch *b, lfrv_batch_result *res)
     std::vector<lfrv_batch_result> out(ne);
     std::vector<int> cg_index;  // entry of each CG item
     std::vector<CgBatchItem> items;
-    // 1. every solve's set-up on its own stream; the PD solves' steps queued behind it
+    // 1. every solve's set-up on its own stream, in two rounds over the solves:
+    // the hole filling of an IBR_gradient solve waits on the host for its
+    // visibility pass's hole count, so every solve's warps, splat structure,
+    // visibility and weights are queued first, and that wait then finds them
+    // running (or done) on the other streams
     for (size_t i = 0; i < ne; i++) {
         auto &e = b->entries[i];
         lfrv_ctx *ctx = e.ctx;
@@ -149,16 +153,23 @@ extern "C" lfrv_status lfrv_batch_run(lfrv_batch *b, lfrv_batch_result *res)
         if (e.kind == LFRV_BATCH_CG) {
             B_CHECK(b, ctx, lfrv_setup_visibility(ctx));
             B_CHECK(b, ctx, lfrv_compute_weights(ctx));
-            B_CHECK(b, ctx, lfrv_compute_initial_image(ctx, nullptr));
+        } else {
+            B_CHECK(b, ctx, lfrv_compute_weights(ctx));
+            B_CHECK(b, ctx, lfrv_setup_visibility(ctx));
+        }
+    }
+    // the rest of the set-up; the PD solves' steps queued behind it
+    for (size_t i = 0; i < ne; i++) {
+        auto &e = b->entries[i];
+        lfrv_ctx *ctx = e.ctx;
+        B_CHECK(b, ctx, lfrv_compute_initial_image(ctx, nullptr));
+        if (e.kind == LFRV_BATCH_CG) {
             B_CHECK(b, ctx, lfrv_init_u_gradient(ctx));
             CgBatchItem it{};
             B_CHECK(b, ctx, cg_batch_begin(ctx, e.tol, e.maxit, &it));
             items.push_back(it);
             cg_index.push_back((int)i);
         } else {
-            B_CHECK(b, ctx, lfrv_compute_weights(ctx));
-            B_CHECK(b, ctx, lfrv_setup_visibility(ctx));
-            B_CHECK(b, ctx, lfrv_compute_initial_image(ctx, nullptr));
             B_CHECK(b, ctx, lfrv_sr_init_regularizer_weight(ctx));
             B_CHECK(b, ctx, lfrv_sr_iterate(ctx, e.steps));
             out[i].iterations = e.steps;

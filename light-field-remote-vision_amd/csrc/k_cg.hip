@@ -279,6 +279,34 @@ __device__ __forceinline__ void st2_coeffs(const St2Coeffs &g, int W, size_t o, 
 // one barrier.
 __device__ void cgcg_scalars(CgState &s, const double (&pv)[9], bool after_start);
 
+// The scalar state is moved through LDS by one-block kernels: its words are
+// loaded by as many threads at once, together with the partial sums, and
+// written back the same way (thread 0 working on the global copy would pay
+// the memory latency once per field it reads after a write).
+constexpr int kCgStateWords = (int)(sizeof(CgState) / 4);
+static_assert(sizeof(CgState) % 4 == 0 && kCgStateWords <= kFinalizeThreads, "CgState moved by words");
+
+// Scalars after iteration i: convergence on r_{i+1}.r_{i+1}, then beta and
+// alpha of the next iteration (a solve already done is left as it is).
+__device__ __forceinline__ void cgcg_step_body(CgState *st, const double *partials, int G)
+{
+    __shared__ double red[9 * 16];
+    __shared__ CgState sst;
+    const int t = threadIdx.x;
+    const uint32_t word = t < kCgStateWords ? reinterpret_cast<const uint32_t *>(st)[t] : 0u;
+    double v[9];
+    reduce_partials<9>(partials, G, v, red);
+    if (t < kCgStateWords) reinterpret_cast<uint32_t *>(&sst)[t] = word;
+    __syncthreads();
+    if (sst.done) return;
+    if (t == 0) {
+        cgcg_scalars(sst, v, false);
+        sst.first = 0;
+    }
+    __syncthreads();
+    if (t < kCgStateWords) reinterpret_cast<uint32_t *>(st)[t] = reinterpret_cast<const uint32_t *>(&sst)[t];
+}
+
 // The pass on tile t; its partial sums go to slot `slot` of G per quantity.
 __device__ __forceinline__ void cgcg_st2_tile(const CgcgArgs &a, const CgState *st, double *partials, int t, int slot,
                                               int G)
@@ -290,12 +318,15 @@ __device__ __forceinline__ void cgcg_st2_tile(const CgcgArgs &a, const CgState *
     const int tx0 = (t % a.ntx) * kSt2W, ty0 = (t / a.ntx) * kSt2H;
     const int W = a.W, H = a.H;
     const size_t N = (size_t)W * H;
+    // every scalar in one batch of loads, then the done test
     float al[3], be[3];
+    const int done = st->done;
     const int first = st->first;
     for (int c = 0; c < 3; c++) {
         al[c] = (float)st->alpha[c];
         be[c] = (float)st->beta[c];
     }
+    if (done) return;
     // this thread's pixel
     const int lx = threadIdx.x % kSt2W, ly = threadIdx.x / kSt2W;
     const int x = tx0 + lx, y = ty0 + ly;
@@ -394,7 +425,6 @@ __device__ __forceinline__ void cgcg_st2_tile(const CgcgArgs &a, const CgState *
 
 __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2(CgcgArgs a, const CgState *st, double *partials)
 {
-    if (st->done) return;
     cgcg_st2_tile(a, st, partials, xcd_tile(blockIdx.x, a.ntiles), blockIdx.x, gridDim.x);
 }
 
@@ -406,7 +436,6 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2_batch(const CgBatchI
 {
     const int2 wk = work[blockIdx.x];
     const CgBatchItem &it = items[wk.x];
-    if (it.st->done) return;
     const int G = it.a[parity].ntiles;
     cgcg_st2_tile(it.a[parity], it.st, it.partials, xcd_tile(wk.y, G), wk.y, G);
 }
@@ -415,16 +444,8 @@ __global__ void __launch_bounds__(kSt2W * kSt2H) k_cgcg_st2_batch(const CgBatchI
 // partials in tile order and updates its state (k_cgcg_step per solve).
 __global__ void __launch_bounds__(kFinalizeThreads) k_cgcg_step_batch(const CgBatchItem *__restrict__ items)
 {
-    __shared__ double red[9 * 16];
     const CgBatchItem &it = items[blockIdx.x];
-    CgState *st = it.st;
-    if (st->done) return;
-    double v[9];
-    reduce_partials<9>(it.partials, it.a[0].ntiles, v, red);
-    if (threadIdx.x == 0) {
-        cgcg_scalars(*st, v, false);
-        st->first = 0;
-    }
+    cgcg_step_body(it.st, it.partials, it.a[0].ntiles);
 }
 
 // The true residual of a stencil-path solve: partials ||b - A x||^2 per
@@ -614,28 +635,37 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_cgcg_begin(CgState *st, co
                                                                  int G, double tol2, int maxit)
 {
     __shared__ double red[12 * 16];
+    __shared__ CgState sst;
+    const int t = threadIdx.x;
+    // the fields this kernel does not set (resid) keep their values
+    const uint32_t word = t < kCgStateWords ? reinterpret_cast<const uint32_t *>(st)[t] : 0u;
     double v[12];
     reduce_partials<12>(partials, G, v, red);
-    if (threadIdx.x == 0) {
+    if (t < kCgStateWords) reinterpret_cast<uint32_t *>(&sst)[t] = word;
+    __syncthreads();
+    if (t == 0) {
+        CgState &s = sst;
         int any = 0;
         for (int c = 0; c < 3; c++) {
-            st->rr[c] = v[c];
-            st->rz[c] = v[3 + c];
-            st->bb[c] = v[6 + c];
-            st->alpha[c] = 0.0;
-            st->beta[c] = 0.0;
-            st->iter[c] = 0;
-            st->breakdown[c] = 0;
-            st->active[c] = (v[c] > tol2 * v[6 + c]) && maxit > 0;
-            any |= st->active[c];
+            s.rr[c] = v[c];
+            s.rz[c] = v[3 + c];
+            s.bb[c] = v[6 + c];
+            s.alpha[c] = 0.0;
+            s.beta[c] = 0.0;
+            s.iter[c] = 0;
+            s.breakdown[c] = 0;
+            s.active[c] = (v[c] > tol2 * v[6 + c]) && maxit > 0;
+            any |= s.active[c];
         }
-        st->tol2 = tol2;
-        st->maxit = maxit;
-        st->first = 1;
-        st->done = !any;
+        s.tol2 = tol2;
+        s.maxit = maxit;
+        s.first = 1;
+        s.done = !any;
         const double pv[9] = {v[3], v[4], v[5], v[9], v[10], v[11], v[0], v[1], v[2]};
-        cgcg_scalars(*st, pv, true);
+        cgcg_scalars(s, pv, true);
     }
+    __syncthreads();
+    if (t < kCgStateWords) reinterpret_cast<uint32_t *>(st)[t] = reinterpret_cast<const uint32_t *>(&sst)[t];
 }
 
 // The scalar step between two passes (one thread).  After the start pass:
@@ -687,19 +717,11 @@ __device__ void cgcg_scalars(CgState &s, const double (&pv)[9], bool after_start
     s.done = !any;
 }
 
-// Scalars after iteration i: convergence on r_{i+1}.r_{i+1}, then beta and
-// alpha of the next iteration.
+// The scalar step of one solve between two passes.
 __global__ void __launch_bounds__(kFinalizeThreads) k_cgcg_step(CgState *st, const double *__restrict__ partials,
                                                                 int G)
 {
-    __shared__ double red[9 * 16];
-    if (st->done) return;
-    double v[9];
-    reduce_partials<9>(partials, G, v, red);
-    if (threadIdx.x == 0) {
-        cgcg_scalars(*st, v, false);
-        st->first = 0;
-    }
+    cgcg_step_body(st, partials, G);
 }
 
 // r = b - q (q = A x0), r.r, r.z, b.b; initialises the scalar state.

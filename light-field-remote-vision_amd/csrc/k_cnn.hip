@@ -37,63 +37,93 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-constexpr int kCvBM = 64, kCvBN = 64, kCvBK = 32, kCvPitch = 40;  // LDS rows of 40 bf16 (80 B)
+constexpr int kCvBM = 128, kCvBK = 32, kCvPitch = 40;  // LDS rows of 40 bf16 (80 B, 16-byte aligned)
+
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
 enum : int { kEpiRelu = 1, kEpiSkip = 2, kEpiF32 = 4, kEpiAccum = 8, kEpiBias = 16 };
 
 // Y[p][co] = act(sum_{t,ci} W[co][t][ci] X[p + off(t)][ci] + bias[co] + skip[p][co]).
 // EPI: kEpiRelu, kEpiSkip (bf16 skip), kEpiBias, and the output kind: bf16,
 // or FP32 (kEpiF32), optionally added to what Y holds (kEpiAccum).
-template <int EPI>
+// A block owns 128 pixels x 32 WN output channels; its 4 waves are arranged
+// (4 / WN) x WN, each on (128 WN / 4) pixels x 32 channels, i.e. 2 WN x 2
+// tiles of v_mfma_f32_16x16x32_bf16 (gfx950).  Per K chunk of 32 (one tap,
+// 32 input channels) the next chunk's tiles are loaded into registers while
+// the current one is multiplied out of LDS, then stored into the other LDS
+// buffer: one barrier per chunk.  An MFMA operand's 8 K values per lane are
+// 8 consecutive input channels in both the pixel and the weight tile, so
+// any K order the instruction uses inside pairs them alike.
+template <int EPI, int WN>
 __global__ void __launch_bounds__(256)
 k_conv3x3(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
           const uint16_t *__restrict__ skip, void *__restrict__ Y, int B, int H, int W, int Cin, int Cout)
 {
-    __shared__ __align__(16) uint16_t As[kCvBM * kCvPitch];
-    __shared__ __align__(16) uint16_t Bs[kCvBN * kCvPitch];
+    constexpr int BN = 32 * WN, WM = 4 / WN, FM = kCvBM / WM / 16;
+    __shared__ __align__(16) uint16_t As[2][kCvBM * kCvPitch];
+    __shared__ __align__(16) uint16_t Bs[2][BN * kCvPitch];
     const int HW = H * W, P = B * HW;
-    const int p0 = blockIdx.x * kCvBM, co0 = blockIdx.y * kCvBN;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, wm = wv >> 1, wn = wv & 1;
-    const int sr = tid >> 2, sc = (tid & 3) * 8;  // staging row and its 8-element column chunk
-    const int sp = p0 + sr;
-    const bool prow = sp < P;
-    const int sb = prow ? sp / HW : 0, rem = prow ? sp - sb * HW : 0;
-    const int sy = rem / W, sx = rem - sy * W;
-    const int sco = co0 + sr;
-    f4v acc[2][2];
+    const int p0 = blockIdx.x * kCvBM, co0 = blockIdx.y * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, wm = wv / WN, wn = wv % WN;
+    // staging: A rows sr and sr + 64 (8 channels each), B row sr (threads < 4 BN)
+    const int sr = tid >> 2, sc = (tid & 3) * 8;
+    int sb[2], sy[2], sx[2];
+    bool prow[2];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int h = 0; h < 2; h++) {
+        const int sp = p0 + sr + 64 * h;
+        prow[h] = sp < P;
+        sb[h] = prow[h] ? sp / HW : 0;
+        const int rem = prow[h] ? sp - sb[h] * HW : 0;
+        sy[h] = rem / W;
+        sx[h] = rem - sy[h] * W;
+    }
+    const int sco = co0 + sr;
+    const bool bload = tid < 4 * BN && sco < Cout;
+    const uint4 zero = make_uint4(0, 0, 0, 0);
+    const int ncc = Cin / kCvBK, nk = 9 * ncc;
+    auto load = [&](int kc, uint4 (&av)[2], uint4 &bv) {
+        const int t = kc / ncc, ci0 = (kc - t * ncc) * kCvBK;
+        const int dy = t / 3 - 1, dx = t % 3 - 1;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int yy = sy[h] + dy, xx = sx[h] + dx;
+            const bool valid = prow[h] && yy >= 0 && yy < H && xx >= 0 && xx < W;
+            av[h] = valid ? *(const uint4 *)(X + ((size_t)(sb[h] * H + yy) * W + xx) * Cin + ci0 + sc) : zero;
+        }
+        bv = bload ? *(const uint4 *)(Wt + ((size_t)sco * 9 + t) * Cin + ci0 + sc) : zero;
+    };
+    auto store = [&](int buf, const uint4 (&av)[2], const uint4 &bv) {
+        *(uint4 *)(&As[buf][sr * kCvPitch + sc]) = av[0];
+        *(uint4 *)(&As[buf][(sr + 64) * kCvPitch + sc]) = av[1];
+        if (tid < 4 * BN) *(uint4 *)(&Bs[buf][sr * kCvPitch + sc]) = bv;
+    };
+    f4v acc[FM][2];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
-    const uint4 zero = make_uint4(0, 0, 0, 0);
-    for (int t = 0; t < 9; t++) {
-        const int yy = sy + t / 3 - 1, xx = sx + t % 3 - 1;
-        const bool valid = prow && yy >= 0 && yy < H && xx >= 0 && xx < W;
-        const uint16_t *xrow = X + ((size_t)(sb * H + (valid ? yy : 0)) * W + (valid ? xx : 0)) * Cin;
-        const uint16_t *wrow = Wt + ((size_t)(sco < Cout ? sco : 0) * 9 + t) * Cin;
-        for (int ci0 = 0; ci0 < Cin; ci0 += kCvBK) {
-            const uint4 av = valid ? *(const uint4 *)(xrow + ci0 + sc) : zero;
-            const uint4 bv = sco < Cout ? *(const uint4 *)(wrow + ci0 + sc) : zero;
-            __syncthreads();
-            *(uint4 *)(As + sr * kCvPitch + sc) = av;
-            *(uint4 *)(Bs + sr * kCvPitch + sc) = bv;
-            __syncthreads();
+    uint4 av[2], bv;
+    load(0, av, bv);
+    store(0, av, bv);
+    __syncthreads();
+    for (int kc = 0; kc < nk; kc++) {
+        const int buf = kc & 1;
+        if (kc + 1 < nk) load(kc + 1, av, bv);
+        bf8v a[FM], b[2];
 #pragma unroll
-            for (int kk = 0; kk < kCvBK; kk += 16) {
-                s4v a[2], b[2];
+        for (int i = 0; i < FM; i++)
+            a[i] = *(const bf8v *)(&As[buf][(wm * (kCvBM / WM) + i * 16 + (lane & 15)) * kCvPitch + 8 * (lane >> 4)]);
 #pragma unroll
-                for (int i = 0; i < 2; i++)
-                    a[i] = *(const s4v *)(As + (wm * 32 + i * 16 + (lane & 15)) * kCvPitch + kk + 4 * (lane >> 4));
+        for (int j = 0; j < 2; j++)
+            b[j] = *(const bf8v *)(&Bs[buf][(wn * 32 + j * 16 + (lane & 15)) * kCvPitch + 8 * (lane >> 4)]);
 #pragma unroll
-                for (int j = 0; j < 2; j++)
-                    b[j] = *(const s4v *)(Bs + (wn * 32 + j * 16 + (lane & 15)) * kCvPitch + kk + 4 * (lane >> 4));
+        for (int i = 0; i < FM; i++)
 #pragma unroll
-                for (int i = 0; i < 2; i++)
-#pragma unroll
-                    for (int j = 0; j < 2; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
-            }
-        }
+            for (int j = 0; j < 2; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (kc + 1 < nk) store(buf ^ 1, av, bv);
+        __syncthreads();
     }
     // lane holds rows 4 (lane / 16) + r, column lane % 16 of each 16 x 16 tile
 #pragma unroll
@@ -102,10 +132,10 @@ k_conv3x3(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wt, const
         if (co >= Cout) continue;
         const float bco = (EPI & kEpiBias) ? bias[co] : 0.0f;
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < FM; i++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int p = p0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+                const int p = p0 + wm * (kCvBM / WM) + i * 16 + 4 * (lane >> 4) + r;
                 if (p >= P) continue;
                 const size_t o = (size_t)p * Cout + co;
                 float v = acc[i][j][r] + bco;
@@ -399,8 +429,15 @@ hipError_t cnn_conv_fwd(const uint16_t *X, const uint16_t *Wt, const float *bias
                         int B, int H, int W, int Cin, int Cout, int epi, hipStream_t s)
 {
     if (Cin % kCvBK != 0) return hipErrorInvalidValue;
-    const dim3 grid((unsigned int)(((size_t)B * H * W + kCvBM - 1) / kCvBM), (unsigned int)((Cout + kCvBN - 1) / kCvBN));
-#define CV(E) k_conv3x3<E><<<grid, 256, 0, s>>>(X, Wt, bias, skip, Y, B, H, W, Cin, Cout)
+    const unsigned int gm = (unsigned int)(((size_t)B * H * W + kCvBM - 1) / kCvBM);
+    // 64-channel blocks when there are more than 32 output channels, else 32
+    const bool wide = Cout > 32;
+    const dim3 grid(gm, (unsigned int)((Cout + (wide ? 63 : 31)) / (wide ? 64 : 32)));
+#define CV(E)                                                                                       \
+    do {                                                                                            \
+        if (wide) k_conv3x3<E, 2><<<grid, 256, 0, s>>>(X, Wt, bias, skip, Y, B, H, W, Cin, Cout);   \
+        else k_conv3x3<E, 1><<<grid, 256, 0, s>>>(X, Wt, bias, skip, Y, B, H, W, Cin, Cout);        \
+    } while (0)
     switch (epi) {
     case kEpiBias | kEpiRelu: CV(kEpiBias | kEpiRelu); break;
     case kEpiBias | kEpiRelu | kEpiSkip: CV(kEpiBias | kEpiRelu | kEpiSkip); break;
