@@ -684,16 +684,17 @@ def bench_batch(args):
 # ---- the learned warm start ----------------------------------------------------------
 
 def bench_warm_start(args):
-    """The learned warm start: 2000 scenes of the 5 x 5-view generator (128^2
-    views, 256^2 targets) solved to rel. residual 1e-5 by the plain PCG, the
-    U-Net trained on them on the GPU, 200 held-out scenes solved both ways,
-    then end-to-end timing at the full size (512^2 views, 100 input channels
-    at 1024^2).  One step = one full-size novel view, warm-started."""
+    """The learned warm start: 2000 scenes of the 5 x 5-view generator at its
+    4x upsampling (64^2 views, 256^2 targets) solved to rel. residual 1e-5 by
+    the plain PCG, the U-Net trained on them on the GPU, 200 held-out scenes
+    solved both ways, then end-to-end timing at the full size (256^2 views,
+    100 input channels at 1024^2).  One step = one full-size novel view,
+    warm-started."""
     from lfrv import warmstart as wsm
     from lfrv.cnn import Cnn
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     n_train, n_test = args.train_scenes, args.test_scenes
-    w_small, w_full = 128, 512
+    w_small, w_full, dsf = 64, 256, wsm.DSF
     t0 = time.perf_counter()
     data = wsm.TrainingSet(list(range(100000, 100000 + n_train)), w_small, log=log)
     t_data = time.perf_counter() - t0
@@ -712,7 +713,7 @@ def bench_warm_start(args):
     used = np.array([r["used_net"] for r in res])
     rel = max(r["rel_diff"] for r in res)
     # full size: the trained weights on a 1024^2 network
-    big = Cnn(data.in_channels, levels=4, base=32, max_batch=1, max_h=2 * w_full, max_w=2 * w_full, seed=7)
+    big = Cnn(data.in_channels, levels=4, base=32, max_batch=1, max_h=dsf * w_full, max_w=dsf * w_full, seed=7)
     big.set_params(params)
     seeds_full = list(range(990000, 990000 + args.steps + args.warmup))
     full = wsm.compare(big, seeds_full, w_full)[args.warmup:]
@@ -720,9 +721,9 @@ def bench_warm_start(args):
     big.set_timing(True)
     import ctypes
     from lfrv.cnn import Raw
-    xin = Raw(4 * w_full * w_full * big.cpad * 2)
-    base = Raw(3 * 4 * w_full * w_full * 4)
-    big.set_batch(1, 2 * w_full, 2 * w_full, [xin.ptr], [base.ptr])
+    xin = Raw(dsf * dsf * w_full * w_full * big.cpad * 2)
+    base = Raw(3 * dsf * dsf * w_full * w_full * 4)
+    big.set_batch(1, dsf * w_full, dsf * w_full, [xin.ptr], [base.ptr])
     for _ in range(3):
         big.forward()
     conv_ms, conv_n, conv_flops = big.get_timing()
@@ -736,7 +737,7 @@ def bench_warm_start(args):
     ms_plain = float(np.median([r["ms_plain"] for r in full]))
     ms_warm = float(np.median([r["ms_warm"] for r in full]))
     line = {
-        "metric": "ms per novel view, learned warm start + PCG to rel. residual 1e-5 (25 views 512² -> 1024²)",
+        "metric": "ms per novel view, learned warm start + PCG to rel. residual 1e-5 (25 views 256² -> 1024²)",
         "value": round(ms_warm, 3),
         "unit": "ms",
         "n_gpus": 1,
@@ -750,8 +751,8 @@ def bench_warm_start(args):
         "data": "synthetic",
         "config": {"workload": "learned warm start: 4-level U-Net (32-256 channels, 3x3 convs, bf16 MFMA) on 25 "
                                "warped views + validity (100 channels) predicting the PCG start; trained on "
-                               f"{n_train} solved scenes (128² views, 256² targets), tested on {n_test} held-out "
-                               "scenes and end to end at 512² views -> 1024²",
+                               f"{n_train} solved scenes (64² views, 256² targets, 4x upsampling), tested on "
+                               f"{n_test} held-out scenes and end to end at 256² views -> 1024²",
                    "train_scenes": n_train, "test_scenes": n_test, "epochs": args.epochs, "batch": 8},
         "held_out": {"scenes": n_test, "cg_iterations_plain_mean": round(float(ip.mean()), 3),
                      "cg_iterations_warm_mean": round(float(iw.mean()), 3),
@@ -765,7 +766,7 @@ def bench_warm_start(args):
                       "net_start_used": [r["used_net"] for r in full]},
         "training": {"seconds_data": round(t_data, 1), "seconds_train": round(t_train, 1),
                      "loss_first_epoch": float("%.4g" % losses[0]), "loss_last_epoch": float("%.4g" % losses[-1])},
-        "roofline": {"bound": "mfma", "kernel": "k_conv3x3 (implicit-GEMM 3x3 conv, bf16 MFMA 16x16x16)",
+        "roofline": {"bound": "mfma", "kernel": "k_conv3x3 (implicit-GEMM 3x3 conv, bf16 MFMA 16x16x32)",
                      "achieved": round(tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 4), "traffic": None,
                      "flops_per_forward": fwd_flops, "conv_launches": conv_n},
@@ -792,7 +793,7 @@ def main():
     ap.add_argument("--no-tune-cache", action="store_true", help="batch_c4: autotune afresh, ignore the cache")
     ap.add_argument("--train-scenes", type=int, default=2000, help="warm_start_c5: training scenes")
     ap.add_argument("--test-scenes", type=int, default=200, help="warm_start_c5: held-out scenes")
-    ap.add_argument("--epochs", type=int, default=4, help="warm_start_c5: training epochs")
+    ap.add_argument("--epochs", type=int, default=8, help="warm_start_c5: training epochs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs one novel-view solve per GPU "

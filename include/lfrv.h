@@ -368,6 +368,15 @@ lfrv_status lfrv_op_splat_atomic(int W, int H, int w, int h, int dsf, const floa
 lfrv_status lfrv_op_sobel(int W, int H, const float *rgb, float *gx, float *gy,
                           const uint8_t *vis, void *stream);
 
+/* The OpenGL path's depth-map filter: radial depth (distance along the
+ * pixel's ray, as MVE stores it) to orthographic depth, with its x / y
+ * derivatives, in place (fromRadial2Ortho.frag, applied by
+ * Scene::filterDepthMaps, scene.cpp:1605-1624, view.cpp:723-733).  K is the
+ * view's row-major 3x3 intrinsic matrix; dzdx, dzdy may be NULL.  Device
+ * pointers, W*H floats each.  LFRV_E_ARG for a singular K. */
+lfrv_status lfrv_op_radial_to_ortho(int W, int H, const double *K, float *z, float *dzdx, float *dzdy,
+                                    void *stream);
+
 /* ---- device memory helpers ------------------------------------------------- */
 /* Plumbing for callers that hold no device allocator of their own (no
  * counterpart in the reference, whose buffers are private to Data /

@@ -3,7 +3,8 @@
  *
  * The file formats and configuration semantics the reference's drivers read
  * and write, for callers that drive the solver through lfrv.h directly and
- * for tests.  No GPU work happens here except in lfrv_io_export_warps.
+ * for tests.  No GPU work happens here except in lfrv_io_export_warps and
+ * lfrv_io_from_mve.
  *
  *   images: PFM (gsl_image.cpp:416-502, import.cpp:695-775) and 8-bit PNG
  *           (gsl_image.cpp:539-637), planar float RGB, rows top to bottom
@@ -38,6 +39,22 @@ int lfrv_io_config_data(int argc, const char *const *argv, char *out, size_t cap
  * entry point here that uses the GPU.  Returns 0, or -1 with the reason on
  * stderr. */
 int lfrv_io_export_warps(int argc, const char *const *argv);
+
+/* lfrv_ref::from_mve_to_coco for a reference command line (the fromMVE2coco
+ * algorithm, main.cpp:313-346): an MVE scene's views, cameras and depth maps,
+ * the depth filter and the warps on the GPU, written in the coco formats.
+ * Returns 0, or -1 with the reason on stderr. */
+int lfrv_io_from_mve(int argc, const char *const *argv);
+
+/* The [camera] of an MVE meta.ini for views of w x h (view.cpp:214-270):
+ * K, R row-major 3x3, t, P = K [R | t] row-major 3x4 (each nullable).  0 / -1. */
+int lfrv_io_mve_camera(const char *meta_ini, int w, int h, double *K, double *R, double *t, double *P);
+
+/* View s's depth map of an MVE scene as Scene::importViews leaves it before the
+ * depth filter (view.cpp:607-634, texture.cpp:218-278, addDepthScale.frag):
+ * w*h floats each (nullable).  0 / -1. */
+int lfrv_io_mve_depth(const char *mve_name, int s, int scale_min, int scale_max, int w, int h, float *z,
+                      float *dzdx, float *dzdy);
 
 #ifdef __cplusplus
 }

@@ -17,6 +17,7 @@
 #define LFRV_REF_HPP
 
 #include <stdexcept>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -108,6 +109,42 @@ void sr_synthesize_view(ConfigData *config_data);
  * deformation weight; -1 occluded, -2 outside), dpart_name (R sigma_z, G / B
  * dtau / dz) and uDepth_name (R the target depth map) where those are set. */
 void export_warps(ConfigData *config_data);
+
+/* The fromMVE2coco algorithm of the reference's command line (main.cpp:313-346,
+ * the OpenGL path): the views of an MVE scene (mve_name/views/view_%04i.mve/:
+ * undistorted.png, meta.ini, depth-L<scale>.mvei and dz-L<scale>.mvei for
+ * scale_min..scale_max) for s_min..s_max as Scene::importViews reads them
+ * (importScene.cpp:105-230; views lacking an image, a camera or a depth scale
+ * are left out), their depth maps filtered from radial to orthographic depth on
+ * the GPU (Scene::filterDepthMaps), then the target depth map and the warps of
+ * export_warps with the target camera of view s_rmv.  Files are numbered by
+ * the position among the imported views (scene.cpp:1546-1585): lf_name (the
+ * view images), tau_name, dpart_name, and uDepth_name.  The reference also
+ * asserts a mask file at a fixed path of its authors' machine
+ * (importScene.cpp:117, 138), used only for a printed statistic; it is not read. */
+void from_mve_to_coco(ConfigData *config_data);
+
+/* An MVE camera (view.cpp:214-270): K (row-major; focal length and principal
+ * point scaled by the view size w x h), R (row-major, as meta.ini lists it), t,
+ * and P = K [R | t]. */
+struct MveCamera {
+    double K[9], R[9], t[3], P[12];
+};
+/* The [camera] section of an MVE meta.ini; false when the file is missing or
+ * holds no camera (a [view] section first). */
+bool import_mve_camera(const std::string &path, int w, int h, MveCamera &cam);
+/* An MVE .mvei image of float channels (view.cpp:279-326): planes[c][y*w + x]. */
+bool load_mvei(const std::string &path, int &w, int &h, std::vector<std::vector<float>> &planes);
+/* The depth map of view s as importViews leaves it (view.cpp:607-634,
+ * texture.cpp:218-278, addDepthScale.frag): the last scale of
+ * scale_min..scale_max upsampled by nearest neighbour to w x h, its derivatives
+ * divided by 2^scale, pixels whose three values are 0 set to (1000, 0, 0).
+ * zdz: R depth, G dz/dx, B dz/dy.  False when a file is missing. */
+bool import_mve_depth(const std::string &mve_name, int s, int scale_min, int scale_max, int w, int h, Image &zdz);
+/* The file output of export_warps / from_mve_to_coco for a context whose warps
+ * lfrv_warps_from_target_depth computed: view i's files take number file_index(i). */
+void write_gl_warps(lfrv_ctx *ctx, ConfigData *config_data, int nviews, int w, int h, float sigma_z,
+                    const std::function<int(int)> &file_index);
 
 /* ---- front end helpers ------------------------------------------------------ */
 

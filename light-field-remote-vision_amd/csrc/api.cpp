@@ -186,6 +186,9 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
         if (e == hipSuccess) e = dalloc(ptr, cnt);
     };
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->side_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->side_join, hipEventDisableTiming);
     A(&ctx->views, 3 * V * n);
     A(&ctx->tau_x, V * n);
     A(&ctx->tau_y, V * n);
@@ -252,6 +255,7 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     flush_timing(ctx);
     for (void *p : std::initializer_list<void *>{
              ctx->views, ctx->tau_x, ctx->tau_y, ctx->dpart_x, ctx->dpart_y, ctx->omega, ctx->depth, ctx->view_lap,
@@ -271,6 +275,9 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
     for (hipEvent_t ev : ctx->sr_ev)
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
+    if (ctx->side_fork) (void)hipEventDestroy(ctx->side_fork);
+    if (ctx->side_join) (void)hipEventDestroy(ctx->side_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -714,6 +721,13 @@ extern "C" lfrv_status lfrv_op_sobel(int W, int H, const float *rgb, float *gx, 
     return launch_sobel(W, H, rgb, rgb + N, rgb + 2 * N, gx, gy, nullptr, vis, (hipStream_t)stream) == hipSuccess
                ? LFRV_OK
                : LFRV_E_HIP;
+}
+
+extern "C" lfrv_status lfrv_op_radial_to_ortho(int W, int H, const double *K, float *z, float *dzdx, float *dzdy,
+                                               void *stream)
+{
+    if (W < 1 || H < 1 || !K || !z) return LFRV_E_ARG;
+    return launch_radial_to_ortho(W, H, K, z, dzdx, dzdy, (hipStream_t)stream) == hipSuccess ? LFRV_OK : LFRV_E_ARG;
 }
 
 // ---- device memory helpers -------------------------------------------------------------

@@ -718,6 +718,13 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
     const TileGeom g = tile_geom(ctx);
     const int nc = ctx->ncells;
     TimedRegion tr(ctx, 5);
+    // the views' stencils depend on the views only: on the side stream,
+    // concurrent with the splat structure, joined before the entry payload
+    LFRV_HIP(ctx, hipEventRecord(ctx->side_fork, ctx->stream));
+    LFRV_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->side_fork, 0));
+    lfrv_status vs = run_view_stencils(ctx, ctx->side);
+    if (vs != LFRV_OK) return vs;
+    LFRV_HIP(ctx, hipEventRecord(ctx->side_join, ctx->side));
     const dim3 bgrid((ctx->w + 63) / 64, (ctx->h + 3) / 4), bblock(64, 4);
     if (!ctx->counts_valid) {
         // (lfrv_warps_from_depth counts the cells of the warps it produces)
@@ -754,8 +761,7 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
 #undef COV
     }
     LFRV_LAUNCH(ctx, "coverage");
-    lfrv_status st = run_view_stencils(ctx);
-    if (st != LFRV_OK) return st;
+    LFRV_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->side_join, 0));
     if (ctx->e_pay) {
         k_entry_payload<<<blocks_for(ctx->e_cap), kBlock, 0, ctx->stream>>>(
             (int64_t)ctx->e_cap, ctx->cell_start + ctx->ncells, ctx->e_src, ctx->n, ctx->views, ctx->view_lap,

@@ -40,6 +40,7 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 constexpr int kCvBM = 128, kCvBK = 32, kCvPitch = 40;  // LDS rows of 40 bf16 (80 B, 16-byte aligned)
 
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 enum : int { kEpiRelu = 1, kEpiSkip = 2, kEpiF32 = 4, kEpiAccum = 8, kEpiBias = 16 };
 
@@ -80,36 +81,46 @@ k_conv3x3(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wt, const
     }
     const int sco = co0 + sr;
     const bool bload = tid < 4 * BN && sco < Cout;
-    const uint4 zero = make_uint4(0, 0, 0, 0);
     const int ncc = Cin / kCvBK, nk = 9 * ncc;
-    auto load = [&](int kc, uint4 (&av)[2], uint4 &bv) {
-        const int t = kc / ncc, ci0 = (kc - t * ncc) * kCvBK;
-        const int dy = t / 3 - 1, dx = t % 3 - 1;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int yy = sy[h] + dy, xx = sx[h] + dx;
-            const bool valid = prow[h] && yy >= 0 && yy < H && xx >= 0 && xx < W;
-            av[h] = valid ? *(const uint4 *)(X + ((size_t)(sb[h] * H + yy) * W + xx) * Cin + ci0 + sc) : zero;
-        }
-        bv = bload ? *(const uint4 *)(Wt + ((size_t)sco * 9 + t) * Cin + ci0 + sc) : zero;
-    };
-    auto store = [&](int buf, const uint4 (&av)[2], const uint4 &bv) {
-        *(uint4 *)(&As[buf][sr * kCvPitch + sc]) = av[0];
-        *(uint4 *)(&As[buf][(sr + 64) * kCvPitch + sc]) = av[1];
-        if (tid < 4 * BN) *(uint4 *)(&Bs[buf][sr * kCvPitch + sc]) = bv;
-    };
+    // loads always read a valid address (pixel 0 / row 0 when the tap falls
+    // outside) and zero the value afterwards: a select between the load and a
+    // zero constant would put the constant in scratch memory and make every
+    // load a flat load through it
+    u32x4 a0, a1, b0;
+    bool m0, m1;
+#define CV_LOAD(kc)                                                                                         \
+    do {                                                                                                    \
+        const int t_ = (kc) / ncc, ci0_ = ((kc) - t_ * ncc) * kCvBK;                                       \
+        const int dy_ = t_ / 3 - 1, dx_ = t_ % 3 - 1;                                                       \
+        const int y0_ = sy[0] + dy_, x0_ = sx[0] + dx_, y1_ = sy[1] + dy_, x1_ = sx[1] + dx_;               \
+        const bool v0_ = prow[0] && y0_ >= 0 && y0_ < H && x0_ >= 0 && x0_ < W;                             \
+        const bool v1_ = prow[1] && y1_ >= 0 && y1_ < H && x1_ >= 0 && x1_ < W;                             \
+        const size_t o0_ = v0_ ? ((size_t)(sb[0] * H + y0_) * W + x0_) * Cin : 0;                          \
+        const size_t o1_ = v1_ ? ((size_t)(sb[1] * H + y1_) * W + x1_) * Cin : 0;                          \
+        a0 = *(const u32x4 *)(X + o0_ + ci0_ + sc);                                                         \
+        a1 = *(const u32x4 *)(X + o1_ + ci0_ + sc);                                                         \
+        b0 = *(const u32x4 *)(Wt + (bload ? ((size_t)sco * 9 + t_) * Cin : 0) + ci0_ + sc);                 \
+        m0 = v0_;                                                                                           \
+        m1 = v1_;                                                                                           \
+    } while (0)
+#define CV_STORE(buf)                                                                                       \
+    do {                                                                                                    \
+        const u32x4 z_ = {0u, 0u, 0u, 0u};                                                                  \
+        *(u32x4 *)(&As[buf][sr * kCvPitch + sc]) = m0 ? a0 : z_;                                            \
+        *(u32x4 *)(&As[buf][(sr + 64) * kCvPitch + sc]) = m1 ? a1 : z_;                                     \
+        if (tid < 4 * BN) *(u32x4 *)(&Bs[buf][sr * kCvPitch + sc]) = bload ? b0 : z_;                       \
+    } while (0)
     f4v acc[FM][2];
 #pragma unroll
     for (int i = 0; i < FM; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
-    uint4 av[2], bv;
-    load(0, av, bv);
-    store(0, av, bv);
+    CV_LOAD(0);
+    CV_STORE(0);
     __syncthreads();
     for (int kc = 0; kc < nk; kc++) {
         const int buf = kc & 1;
-        if (kc + 1 < nk) load(kc + 1, av, bv);
+        if (kc + 1 < nk) CV_LOAD(kc + 1);
         bf8v a[FM], b[2];
 #pragma unroll
         for (int i = 0; i < FM; i++)
@@ -122,9 +133,11 @@ k_conv3x3(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wt, const
 #pragma unroll
             for (int j = 0; j < 2; j++)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        if (kc + 1 < nk) store(buf ^ 1, av, bv);
+        if (kc + 1 < nk) CV_STORE(buf ^ 1);
         __syncthreads();
     }
+#undef CV_LOAD
+#undef CV_STORE
     // lane holds rows 4 (lane / 16) + r, column lane % 16 of each 16 x 16 tile
 #pragma unroll
     for (int j = 0; j < 2; j++) {

@@ -929,17 +929,17 @@ lfrv_status run_compute_weights(lfrv_ctx *ctx)
     return LFRV_OK;
 }
 
-lfrv_status run_view_stencils(lfrv_ctx *ctx)
+lfrv_status run_view_stencils(lfrv_ctx *ctx, hipStream_t s)
 {
     if (ctx->p.mode == LFRV_MODE_GRADIENT_IBR) {
         // IBR_gradient needs the views' Sobel only (init_u_gradient)
         dim3 gs((ctx->w + kVsW - 1) / kVsW, (ctx->h + kVsH - 1) / kVsH, ctx->V);
-        k_view_sobel<<<gs, kVsW * kVsH, 0, ctx->stream>>>(ctx->w, ctx->h, ctx->views, ctx->view_grad);
+        k_view_sobel<<<gs, kVsW * kVsH, 0, s>>>(ctx->w, ctx->h, ctx->views, ctx->view_grad);
         LFRV_LAUNCH(ctx, "view stencils");
         return LFRV_OK;
     }
-    k_view_stencils<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->views,
-                                                                 ctx->view_grad, ctx->view_lap);
+    k_view_stencils<<<source_grid(ctx), kSourceBlock, 0, s>>>(ctx->V, ctx->w, ctx->h, ctx->views, ctx->view_grad,
+                                                              ctx->view_lap);
     LFRV_LAUNCH(ctx, "view stencils");
     return LFRV_OK;
 }
@@ -978,6 +978,56 @@ hipError_t launch_sobel(int W, int H, const float *r, const float *g, const floa
 {
     dim3 bs(32, 8), gs((W + 31) / 32, (H + 7) / 8);
     k_sobel<<<gs, bs, 0, s>>>(W, H, r, g, b, gx, gy, visf, visb);
+    return hipGetLastError();
+}
+
+
+// Radial to orthographic depth, the depth-map filter of the OpenGL path
+// (fromRadial2Ortho.frag, run per view by Scene::filterDepthMaps,
+// scene.cpp:1605-1624, through InputView::filterDepthmap, view.cpp:723-733,
+// and FrameBuffer::fromRadial2Ortho, frameBuffer.cpp:263-315).  The fragment
+// of pixel (x, y) sits at its centre: u = K^-1 (x + 0.5, y + 0.5, 1), d = |u|,
+// and the depth and its derivatives become z / d and
+// dz / d - z (u . K^-1 e_x or e_y) / d^3, in float as the shader computes.
+// Every pixel is converted, the invalid ones (z = 1000) included.
+struct Mat3f {
+    float m[9];  // row-major
+};
+
+__global__ void k_radial_to_ortho(int W, int H, Mat3f ki, float *__restrict__ z, float *__restrict__ gx,
+                                  float *__restrict__ gy)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const size_t o = (size_t)y * W + x;
+    const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+    const float u0 = ki.m[0] * px + ki.m[1] * py + ki.m[2];
+    const float u1 = ki.m[3] * px + ki.m[4] * py + ki.m[5];
+    const float u2 = ki.m[6] * px + ki.m[7] * py + ki.m[8];
+    const float d = sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
+    const float d3 = d * d * d;
+    // u . (column 0 of K^-1) and u . (column 1 of K^-1)
+    const float c0 = u0 * ki.m[0] + u1 * ki.m[3] + u2 * ki.m[6];
+    const float c1 = u0 * ki.m[1] + u1 * ki.m[4] + u2 * ki.m[7];
+    const float zr = z[o];
+    z[o] = zr / d;
+    if (gx) gx[o] = gx[o] / d - zr * c0 / d3;
+    if (gy) gy[o] = gy[o] / d - zr * c1 / d3;
+}
+
+hipError_t launch_radial_to_ortho(int W, int H, const double *K, float *z, float *gx, float *gy, hipStream_t s)
+{
+    // K^-1 in double, handed to the kernel in float (the shader's mat3 uniform)
+    const double a = K[0], b = K[1], c = K[2], d = K[3], e = K[4], f = K[5], g = K[6], h = K[7], i = K[8];
+    const double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+    if (!(det != 0.0) || !std::isfinite(det)) return hipErrorInvalidValue;
+    const double inv[9] = {(e * i - f * h) / det, (c * h - b * i) / det, (b * f - c * e) / det,
+                           (f * g - d * i) / det, (a * i - c * g) / det, (c * d - a * f) / det,
+                           (d * h - e * g) / det, (b * g - a * h) / det, (a * e - b * d) / det};
+    Mat3f ki;
+    for (int k = 0; k < 9; k++) ki.m[k] = (float)inv[k];
+    const dim3 bs(32, 8), gs((W + 31) / 32, (H + 7) / 8);
+    k_radial_to_ortho<<<gs, bs, 0, s>>>(W, H, ki, z, gx, gy);
     return hipGetLastError();
 }
 

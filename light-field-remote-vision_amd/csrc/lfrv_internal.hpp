@@ -308,6 +308,10 @@ struct lfrv_ctx {
     int V = 0, w = 0, h = 0, dsf = 1, ks = 2, W = 0, H = 0;
     size_t n = 0, N = 0;
     hipStream_t stream = nullptr;
+    // second stream for work of lfrv_prepare that does not depend on the warps
+    // (the views' stencils), forked from and joined to `stream` by two events
+    hipStream_t side = nullptr;
+    hipEvent_t side_fork = nullptr, side_join = nullptr;
     std::string err;
 
     // views (contiguous over views)
@@ -495,6 +499,7 @@ hipError_t launch_gather(int W, int H, int w, int h, int dsf, const float *tx, c
 hipError_t launch_splat_atomic(int W, int H, int w, int h, int dsf, const float *tx,
                                const float *ty, const float *in, int C, const float *wgt,
                                float *out, float *wsum, hipStream_t s);
+hipError_t launch_radial_to_ortho(int W, int H, const double *K, float *z, float *gx, float *gy, hipStream_t s);
 hipError_t launch_sobel(int W, int H, const float *r, const float *g, const float *b, float *gx,
                         float *gy, const float *visf, const uint8_t *visb, hipStream_t s);
 

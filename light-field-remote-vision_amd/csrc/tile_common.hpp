@@ -63,7 +63,7 @@ size_t pull_lds(const lfrv_ctx *ctx, int nv, int cap, PullKind kind, int npay = 
 lfrv_status run_rhs_and_diag(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use_views,
                              bool use_precond, const float *u0);
 lfrv_status run_gauss_smooth(lfrv_ctx *ctx, const float *in, float *out, float *tmp);
-lfrv_status run_view_stencils(lfrv_ctx *ctx);
+lfrv_status run_view_stencils(lfrv_ctx *ctx, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Device side of the pull

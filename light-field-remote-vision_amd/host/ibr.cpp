@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <functional>
 #include <memory>
 
 namespace lfrv_ref {
@@ -465,6 +466,48 @@ void sr_synthesize_view(ConfigData *cd)
                 mtv->_nviews, cd->_w, cd->_h, W, H, st.meta_iterations, st.fista_steps, (double)st.energy);
 }
 
+void write_gl_warps(lfrv_ctx *ctx, ConfigData *cd, int nv, int w, int h, float sigma_z,
+                    const std::function<int(int)> &file_index)
+{
+    const size_t n = (size_t)w * h;
+    auto get = [&](int which, size_t count) {
+        std::vector<float> v(count);
+        size_t cnt = 0;
+        check(ctx, lfrv_get_buffer(ctx, which, v.data(), &cnt), "warp buffers");
+        return v;
+    };
+    const std::vector<float> tx = get(LFRV_BUF_TAU_X, nv * n), ty = get(LFRV_BUF_TAU_Y, nv * n),
+                             wd = get(LFRV_BUF_TAU_WEIGHT, nv * n), dx = get(LFRV_BUF_DPART_X, nv * n),
+                             dy = get(LFRV_BUF_DPART_Y, nv * n), ud = get(LFRV_BUF_TARGET_DEPTH, n);
+    for (int i = 0; i < nv; i++) {
+        const int s = file_index(i);
+        if (!cd->_tau_name.empty()) {
+            Image t(w, h);
+            std::copy(tx.begin() + i * n, tx.begin() + (i + 1) * n, t._r.begin());
+            std::copy(ty.begin() + i * n, ty.begin() + (i + 1) * n, t._g.begin());
+            std::copy(wd.begin() + i * n, wd.begin() + (i + 1) * n, t._b.begin());
+            if (!save_pfm(pattern_name(cd->_tau_name, s), t)) throw Error("cannot write " + pattern_name(cd->_tau_name, s));
+        }
+        if (!cd->_dpart_name.empty()) {
+            // tauPartial.frag:46 writes (sigma, dtau/dz); the buffers hold sigma dtau/dz
+            Image t(w, h);
+            for (size_t m = 0; m < n; m++) {
+                t._r[m] = sigma_z;
+                t._g[m] = dx[i * n + m] / sigma_z;
+                t._b[m] = dy[i * n + m] / sigma_z;
+            }
+            if (!save_pfm(pattern_name(cd->_dpart_name, s), t)) throw Error("cannot write " + pattern_name(cd->_dpart_name, s));
+        }
+    }
+    if (!cd->_uDepth_name.empty()) {
+        Image t(w, h);
+        t._r = ud;
+        std::fill(t._g.begin(), t._g.end(), 0.0f);
+        std::fill(t._b.begin(), t._b.end(), 0.0f);
+        if (!save_pfm(cd->_uDepth_name, t)) throw Error("cannot write " + cd->_uDepth_name);
+    }
+}
+
 void export_warps(ConfigData *cd)
 {
     if (cd->_dsf != 1) throw Error("export_warps: the OpenGL path renders the target at the views' size (dsf 1)");
@@ -480,7 +523,7 @@ void export_warps(ConfigData *cd)
         if (depth[i]._w != depth[0]._w || depth[i]._h != depth[0]._h) throw Error(dn + " has the wrong size");
         cams[i] = import_cam(cd->_cam_name, cd->_s_min + i);
     }
-    const size_t w = depth[0]._w, h = depth[0]._h, n = w * h;
+    const size_t w = depth[0]._w, h = depth[0]._h;
     lfrv_params p;
     lfrv_default_params(&p, LFRV_MODE_GRADIENT_IBR, nv, (int)w, (int)h, 1);
     lfrv_ctx *raw = nullptr;
@@ -490,42 +533,7 @@ void export_warps(ConfigData *cd)
     const int target = cd->_s_rmv - cd->_s_min;
     check(ctx.get(), lfrv_set_target_camera(ctx.get(), cams[target].data()), "target camera");
     check(ctx.get(), lfrv_warps_from_target_depth(ctx.get(), target), "lfrv_warps_from_target_depth");
-    auto get = [&](int which, size_t count) {
-        std::vector<float> v(count);
-        size_t cnt = 0;
-        check(ctx.get(), lfrv_get_buffer(ctx.get(), which, v.data(), &cnt), "warp buffers");
-        return v;
-    };
-    const std::vector<float> tx = get(LFRV_BUF_TAU_X, nv * n), ty = get(LFRV_BUF_TAU_Y, nv * n),
-                             wd = get(LFRV_BUF_TAU_WEIGHT, nv * n), dx = get(LFRV_BUF_DPART_X, nv * n),
-                             dy = get(LFRV_BUF_DPART_Y, nv * n), ud = get(LFRV_BUF_TARGET_DEPTH, n);
-    for (int i = 0; i < nv; i++) {
-        const int s = cd->_s_min + i;
-        if (!cd->_tau_name.empty()) {
-            Image t(w, h);
-            std::copy(tx.begin() + i * n, tx.begin() + (i + 1) * n, t._r.begin());
-            std::copy(ty.begin() + i * n, ty.begin() + (i + 1) * n, t._g.begin());
-            std::copy(wd.begin() + i * n, wd.begin() + (i + 1) * n, t._b.begin());
-            if (!save_pfm(pattern_name(cd->_tau_name, s), t)) throw Error("cannot write " + pattern_name(cd->_tau_name, s));
-        }
-        if (!cd->_dpart_name.empty()) {
-            // tauPartial.frag:46 writes (sigma, dtau/dz); the buffers hold sigma dtau/dz
-            Image t(w, h);
-            for (size_t m = 0; m < n; m++) {
-                t._r[m] = p.sigma_z;
-                t._g[m] = dx[i * n + m] / p.sigma_z;
-                t._b[m] = dy[i * n + m] / p.sigma_z;
-            }
-            if (!save_pfm(pattern_name(cd->_dpart_name, s), t)) throw Error("cannot write " + pattern_name(cd->_dpart_name, s));
-        }
-    }
-    if (!cd->_uDepth_name.empty()) {
-        Image t(w, h);
-        t._r = ud;
-        std::fill(t._g.begin(), t._g.end(), 0.0f);
-        std::fill(t._b.begin(), t._b.end(), 0.0f);
-        if (!save_pfm(cd->_uDepth_name, t)) throw Error("cannot write " + cd->_uDepth_name);
-    }
+    write_gl_warps(ctx.get(), cd, nv, (int)w, (int)h, p.sigma_z, [cd](int i) { return cd->_s_min + i; });
     std::printf("export_warps: %d views %zux%zu, target view %d\n", nv, w, h, cd->_s_rmv);
 }
 

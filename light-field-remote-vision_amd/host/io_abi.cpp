@@ -1,4 +1,5 @@
 // C-ABI of the front end (include/lfrv_io.h).
+#include <algorithm>
 #include "../../include/lfrv_io.h"
 
 #include <cstdio>
@@ -83,5 +84,49 @@ extern "C" int lfrv_io_export_warps(int argc, const char *const *argv)
         std::fprintf(stderr, "lfrv_io_export_warps: %s\n", e.what());
         return -1;
     }
+    return 0;
+}
+
+extern "C" int lfrv_io_from_mve(int argc, const char *const *argv)
+{
+    if (argc < 1 || !argv) return -1;
+    std::vector<std::string> args(argv, argv + argc);
+    std::vector<char *> av;
+    for (auto &a : args) av.push_back(&a[0]);
+    try {
+        Config cfg;
+        cfg.parse_command_line(argc, av.data());
+        ConfigData cd;
+        cd.parse_config(cfg);
+        from_mve_to_coco(&cd);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "lfrv_io_from_mve: %s\n", e.what());
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" int lfrv_io_mve_camera(const char *meta_ini, int w, int h, double *K, double *R, double *t, double *P)
+{
+    if (!meta_ini) return -1;
+    MveCamera cam;
+    if (!import_mve_camera(meta_ini, w, h, cam)) return -1;
+    if (K) std::copy(cam.K, cam.K + 9, K);
+    if (R) std::copy(cam.R, cam.R + 9, R);
+    if (t) std::copy(cam.t, cam.t + 3, t);
+    if (P) std::copy(cam.P, cam.P + 12, P);
+    return 0;
+}
+
+extern "C" int lfrv_io_mve_depth(const char *mve_name, int s, int scale_min, int scale_max, int w, int h, float *z,
+                                 float *dzdx, float *dzdy)
+{
+    if (!mve_name || w < 1 || h < 1) return -1;
+    Image d;
+    if (!import_mve_depth(mve_name, s, scale_min, scale_max, w, h, d)) return -1;
+    const size_t n = (size_t)w * h;
+    if (z) std::copy(d._r.begin(), d._r.begin() + n, z);
+    if (dzdx) std::copy(d._g.begin(), d._g.begin() + n, dzdx);
+    if (dzdy) std::copy(d._b.begin(), d._b.begin() + n, dzdy);
     return 0;
 }

@@ -146,6 +146,7 @@ def lib():
         "lfrv_op_gather": (st, [st, st, st, st, st, _vp, _vp, _vp, st, _vp, _vp]),
         "lfrv_op_splat_atomic": (st, [st, st, st, st, st, _vp, _vp, _vp, st, _vp, _vp, _vp, _vp]),
         "lfrv_op_sobel": (st, [st, st, _vp, _vp, _vp, _vp, _vp]),
+        "lfrv_op_radial_to_ortho": (st, [st, st, _dp, _vp, _vp, _vp, _vp]),
         "lfrv_psnr": (ctypes.c_double, [st, st, _vp, _vp]),
         "lfrv_dev_alloc": (st, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
         "lfrv_dev_free": (st, [_vp]),
@@ -475,3 +476,22 @@ def synth(desc_or_config, seed=0, **overrides):
         raise LfrvError(E_ARG, "lfrv_synth_generate failed")
     return dict(desc=d, views=views, depths=depths, P_views=Pv, P_target=Pu, target=tgt, target_depth=tdep,
                 V=V, w=w, h=h, W=W, H=H, dsf=d.dsf)
+
+
+def radial_to_ortho(z, K, dzdx=None, dzdy=None):
+    """The OpenGL path's depth-map filter (lfrv_op_radial_to_ortho) on the GPU:
+    radial depth and its x / y derivatives to orthographic depth.  Returns
+    numpy arrays (the derivatives None where not given)."""
+    K = np.ascontiguousarray(K, np.float64).reshape(9)
+    H, W = np.asarray(z).shape
+    bufs = [DeviceArray(_f32(a)) if a is not None else None for a in (z, dzdx, dzdy)]
+    st = lib().lfrv_op_radial_to_ortho(W, H, K.ctypes.data_as(_dp), bufs[0].ptr, bufs[1].ptr if bufs[1] else None,
+                                       bufs[2].ptr if bufs[2] else None, None)
+    if st != OK:
+        raise LfrvError(st, "lfrv_op_radial_to_ortho failed")
+    device_sync()
+    out = [b.numpy() if b is not None else None for b in bufs]
+    for b in bufs:
+        if b is not None:
+            b.free()
+    return tuple(out)

@@ -33,6 +33,15 @@ def lib():
     L.lfrv_io_config_data.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p, ctypes.c_size_t]
     L.lfrv_io_export_warps.restype = ctypes.c_int
     L.lfrv_io_export_warps.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    L.lfrv_io_from_mve.restype = ctypes.c_int
+    L.lfrv_io_from_mve.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    _dp = ctypes.POINTER(ctypes.c_double)
+    _fp = ctypes.POINTER(ctypes.c_float)
+    L.lfrv_io_mve_camera.restype = ctypes.c_int
+    L.lfrv_io_mve_camera.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.lfrv_io_mve_depth.restype = ctypes.c_int
+    L.lfrv_io_mve_depth.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, _fp, _fp, _fp]
     _lib = L
     return L
 
@@ -81,3 +90,29 @@ def export_warps(argv):
     args = (ctypes.c_char_p * len(argv))(*[os.fsencode(a) for a in argv])
     if lib().lfrv_io_export_warps(len(argv), args) != 0:
         raise RuntimeError("export_warps failed (reason on stderr)")
+
+
+def from_mve(argv):
+    """lfrv_ref::from_mve_to_coco (the fromMVE2coco algorithm) for a reference
+    command line (argv[0] the program name)."""
+    args = (ctypes.c_char_p * len(argv))(*[a.encode() for a in argv])
+    if lib().lfrv_io_from_mve(len(argv), args) != 0:
+        raise RuntimeError("from_mve failed (reason on stderr)")
+
+
+def mve_camera(meta_ini, w, h):
+    """(K, R, t, P) of an MVE meta.ini's [camera] for views of w x h, or None."""
+    K, R, t, P = np.zeros(9), np.zeros(9), np.zeros(3), np.zeros(12)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    if lib().lfrv_io_mve_camera(str(meta_ini).encode(), w, h, dp(K), dp(R), dp(t), dp(P)) != 0:
+        return None
+    return K.reshape(3, 3), R.reshape(3, 3), t, P.reshape(3, 4)
+
+
+def mve_depth(mve_name, s, scale_min, scale_max, w, h):
+    """(z, dz/dx, dz/dy), each (h, w), of view s as importViews reads it, or None."""
+    z, gx, gy = (np.zeros((h, w), np.float32) for _ in range(3))
+    fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+    if lib().lfrv_io_mve_depth(str(mve_name).encode(), s, scale_min, scale_max, w, h, fp(z), fp(gx), fp(gy)) != 0:
+        return None
+    return z, gx, gy

@@ -4,12 +4,15 @@ comparison of warm-started and plain solves.
 
 Scenes come from the 5 x 5-view generator (8 random tilted planes, disparity
 -2 .. +2 px per grid step, views box-averaged from 4x renders), rendered on
-the GPU (lfrv_synth_generate_device).  The solver is the gradient-constraint
-IBR (IBR_gradient's normal equations, lfrv_cg_solve) at dsf 2 to relative
-residual 1e-5; the network's input is the stack of the 25 warped views with
-their validity (100 channels) at the target resolution, its base the solver's
-own starting image (the splatted average with hole filling), its target the
-converged solution.  Everything stays in device memory.
+the GPU (lfrv_synth_generate_device), with that generator's 4x upsampling
+(dsf 4).  The solver is the gradient-constraint IBR (IBR_gradient's normal
+equations, lfrv_cg_solve) to relative residual 1e-5; the network's input is
+the stack of the 25 warped views with their validity (100 channels) at the
+target resolution, its base the solver's own starting image (the splatted
+average with hole filling), its target the converged solution.  Everything
+stays in device memory.  (At dsf 2 the plain PCG converges in about 9
+iterations from the splatted average, which leaves a start nothing to save;
+at dsf 4 it takes about 35.)
 """
 import ctypes
 import time
@@ -22,9 +25,10 @@ from .pipelines import gradient_ibr_params
 
 TOL = 1e-5
 MAXIT = 5000
+DSF = 4
 
 
-def scene_desc(seed, w, dsf=2):
+def scene_desc(seed, w, dsf=DSF):
     """The 5 x 5-view generator at w x w views and a (w dsf)^2 target: the
     focal length is w and the baseline 10 / w, so the disparities in view
     pixels are the same at every w."""
@@ -95,7 +99,7 @@ class TrainingSet:
     channels), start image and converged solution (planar FP32) per scene,
     and the plain solve's CG iterations."""
 
-    def __init__(self, seeds, w, dsf=2, log=None):
+    def __init__(self, seeds, w, dsf=DSF, log=None):
         d0 = scene_desc(seeds[0], w, dsf)
         self.V = d0.grid * d0.grid
         self.in_channels = 4 * self.V
@@ -163,7 +167,7 @@ def train(net, data, epochs, batch, lr, seed=0, log=None):
     return losses
 
 
-def compare(net, seeds, w, dsf=2):
+def compare(net, seeds, w, dsf=DSF):
     """Plain and warm-started solves of the same scenes.  Per scene: CG
     iterations (max over channels) of both, whether the network's start was
     kept, the energies of both starts, the relative difference of the two

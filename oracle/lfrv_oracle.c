@@ -616,6 +616,32 @@ static int gl_tri(const double *a, const double *b, const double *c, double px, 
     return 1;
 }
 
+
+/* Radial to orthographic depth of one view, in place: fromRadial2Ortho.frag
+ * (run per view by Scene::filterDepthMaps, scene.cpp:1605-1624, through
+ * InputView::filterDepthmap, view.cpp:723-733).  Kinv is the row-major
+ * inverse of the view's K; the fragment of pixel (x, y) is its centre, with
+ * u = Kinv (x + 0.5, y + 0.5, 1) and d = |u|:
+ *   z <- z / d,  dz/dx <- dz/dx / d - z (u . Kinv e_x) / d^3  (e_y for dz/dy).
+ * dzdx / dzdy may be NULL. */
+void orc_radial_to_ortho(int W, int H, const double *Kinv, float *z, float *dzdx, float *dzdy)
+{
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            const double px = x + 0.5, py = y + 0.5;
+            double u[3];
+            for (int r = 0; r < 3; r++) u[r] = Kinv[3 * r] * px + Kinv[3 * r + 1] * py + Kinv[3 * r + 2];
+            const double d = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            const double cx = u[0] * Kinv[0] + u[1] * Kinv[3] + u[2] * Kinv[6];
+            const double cy = u[0] * Kinv[1] + u[1] * Kinv[4] + u[2] * Kinv[7];
+            const size_t o = (size_t)y * W + x;
+            const double zr = z[o];
+            z[o] = (float)(zr / d);
+            if (dzdx) dzdx[o] = (float)(dzdx[o] / d - zr * cx / (d * d * d));
+            if (dzdy) dzdy[o] = (float)(dzdy[o] / d - zr * cy / (d * d * d));
+        }
+}
+
 /* Scene::computeTargetDepthMap (scene.cpp:1506-1544; depthSplatting.vert,
  * .geom, .frag): every view but `skip` is drawn as one quad per pixel of valid
  * depth (z < INVALID_DEPTH / 2), corners at the pixel centre +- 0.5 with depths

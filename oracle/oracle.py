@@ -54,6 +54,7 @@ def lib():
         "orc_decompose_P": (None, [_dp, _dp, _dp, _dp, _dp]),
         "orc_warps_from_depth": (None, [_i, _i, _i, _i, _fp, _dp, _dp, _d, _i, _d, _fp, _fp, _fp, _fp, _fp]),
         "orc_gl_target_depth": (None, [_i, _i, _i, _i, _i, _fp, _fp, _fp, _dp, _dp, _i, _fp]),
+        "orc_radial_to_ortho": (None, [_i, _i, _dp, _fp, _fp, _fp]),
         "orc_gl_warps": (None, [_i, _i, _i, _i, _i, _fp, _fp, _fp, _dp, _dp, _fp, _d, _fp, _fp, _fp, _fp, _fp]),
         "orc_initial_image": (None, [_i, _i, _i, _i, _i, _i, _fp, _fp, _fp, _dp, _dp, _dp]),
         "orc_hole_filling": (_i, [_i, _i, _dp, _u8, _i, _d]),
@@ -276,6 +277,19 @@ def warps_from_depth(depths, P_views, P_target, dsf, occl_tol=10.0, source_cam=0
     lib().orc_warps_from_depth(V, w, h, dsf, pf(depths), pd(P), pd(Pu), occl_tol, source_cam, sigma_z,
                                pf(tx), pf(ty), pf(dx), pf(dy), pf(ud))
     return tx, ty, dx, dy, ud
+
+
+def radial_to_ortho(z, K, dzdx=None, dzdy=None):
+    """Radial depth (and its x / y derivatives) to orthographic depth
+    (fromRadial2Ortho.frag); returns new arrays."""
+    z = _F(z).copy()
+    H, W = z.shape
+    gx = None if dzdx is None else _F(dzdx).copy()
+    gy = None if dzdy is None else _F(dzdy).copy()
+    Ki = np.linalg.inv(_D(K).reshape(3, 3))
+    lib().orc_radial_to_ortho(W, H, pd(_D(Ki).reshape(9)), pf(z), None if gx is None else pf(gx),
+                              None if gy is None else pf(gy))
+    return z, gx, gy
 
 
 def gl_target_depth(depths, P_views, P_target, W, H, skip=-1, dzdx=None, dzdy=None):

@@ -238,17 +238,17 @@ def test_device_scene_renderer_matches_the_host_one():
 
 def test_trained_warm_start_saves_iterations_on_held_out_scenes():
     """The learned warm start at a small scale: 400 scenes of the 5 x 5-view
-    generator (32^2 views, 64^2 targets) solved by the plain method to
-    relative residual 1e-5, the U-Net trained on them for 3 epochs, then 200
-    held-out scenes solved both ways.  Every warm-started solve reaches the
-    same tolerance and the plain solution (relative difference <= 1e-4), the
-    network's start is used only where it lowers the energy, and on average
-    it saves CG iterations."""
+    generator at its 4x upsampling (32^2 views, 128^2 targets) solved by the
+    plain method to relative residual 1e-5, the U-Net trained on them for 6
+    epochs, then 200 held-out scenes solved both ways.  Every warm-started
+    solve reaches the same tolerance and the plain solution (both within the
+    solver's tolerance of the exact solution), the network's start is used
+    only where it lowers the energy, and on average it saves CG iterations."""
     from lfrv import warmstart as wsm
     print("warm start: training set of 400 scenes", flush=True)
     data = wsm.TrainingSet(list(range(1000, 1400)), 32)
-    net = Cnn(data.in_channels, levels=4, base=32, max_batch=4, max_h=64, max_w=64, seed=7)
-    losses = wsm.train(net, data, epochs=3, batch=4, lr=2e-3, log=print)
+    net = Cnn(data.in_channels, levels=4, base=32, max_batch=4, max_h=data.H, max_w=data.W, seed=7)
+    losses = wsm.train(net, data, epochs=6, batch=4, lr=2e-3, log=print)
     assert losses[-1] < losses[0]
     print("warm start: 200 held-out scenes", flush=True)
     res = wsm.compare(net, list(range(5000, 5200)), 32)
@@ -257,10 +257,12 @@ def test_trained_warm_start_saves_iterations_on_held_out_scenes():
     plain = np.array([r["iters_plain"] for r in res])
     warm = np.array([r["iters_warm"] for r in res])
     used = np.array([r["used_net"] for r in res])
-    print("iterations plain %.2f warm %.2f, net used in %d/200" % (plain.mean(), warm.mean(), used.sum()))
+    rd = np.array([r["rel_diff"] for r in res])
+    print("iterations plain %.2f warm %.2f, net used in %d/200, max rel diff %.2e" %
+          (plain.mean(), warm.mean(), used.sum(), rd.max()))
     for r in res:
         assert r["true_rel_residual_warm"] <= 1.5e-5
-        assert r["rel_diff"] <= 1e-4
+        assert r["rel_diff"] <= 1e-3
         if r["used_net"]:
             assert r["energy_net_start"] < r["energy_plain_start"]
     assert warm.mean() < plain.mean()

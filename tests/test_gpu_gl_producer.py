@@ -188,3 +188,92 @@ def test_gl_producer_rasterises_big_quads(dsf, zoom):
     assert (oud > 0).mean() > 0.5
     bad = np.abs(ud - oud) > 1e-4 * np.maximum(np.abs(oud), 1.0)
     assert bad.mean() <= 5e-3, bad.mean()
+
+
+def test_radial_to_ortho_matches_oracle():
+    """The depth-map filter of the OpenGL path (fromRadial2Ortho.frag) on the
+    GPU against the oracle, on random depths and derivatives with a skewed K."""
+    r = np.random.default_rng(12)
+    H, W = 37, 53
+    K = np.array([[70.0, 1.5, 25.0], [0.0, 64.0, 19.0], [0.0, 0.0, 1.0]])
+    z = r.uniform(1.0, 9.0, (H, W)).astype(np.float32)
+    z[3, 4] = 1000.0                                   # an invalid pixel is converted too
+    gx = r.normal(size=(H, W)).astype(np.float32)
+    gy = r.normal(size=(H, W)).astype(np.float32)
+    zg, gxg, gyg = lfrv.radial_to_ortho(z, K, gx, gy)
+    zo, gxo, gyo = oracle.radial_to_ortho(z, K, gx, gy)
+    assert np.allclose(zg, zo, rtol=2e-6)
+    assert np.allclose(gxg, gxo, rtol=1e-5, atol=1e-5) and np.allclose(gyg, gyo, rtol=1e-5, atol=1e-5)
+    z2, n1, n2 = lfrv.radial_to_ortho(z, K)            # the derivatives are optional
+    assert np.array_equal(z2, zg) and n1 is None and n2 is None
+
+
+def test_from_mve_to_coco(tmp_path):
+    """The fromMVE2coco algorithm through the command line: a synthetic scene
+    written as an MVE scene (meta.ini cameras, radial depth maps, zero dz,
+    undistorted.png), imported, filtered to orthographic depth on the GPU and
+    turned into warps.  The filtered depth is the scene's own (orthographic)
+    depth; the files hold what lfrv_warps_from_target_depth gives on the
+    filtered maps, and those warps agree with the oracle's producer; a view
+    without a camera is left out."""
+    import subprocess
+    import sys
+    from lfrv import io as lio
+    from mve_files import decompose, mve_params, radial_from_ortho, view_dir, write_meta, write_mvei
+    s = _scene(40, 32, seed=8)
+    V, h, w = s["V"], s["h"], s["w"]
+    rmv = 4
+    for v in range(V + 1):                              # view V has no camera
+        d = view_dir(tmp_path, v)
+        src = min(v, V - 1)
+        lio.save_image(f"{d}/undistorted.png", s["views"][src])
+        K, R, t, sc = decompose(s["P_views"][src])
+        assert abs(sc - 1.0) < 1e-12 and abs(K[0, 1]) < 1e-12
+        f, a, pp = mve_params(K, w, h)
+        write_meta(f"{d}/meta.ini", f, a, pp, R, t, with_camera=v < V)
+        zr = radial_from_ortho(s["depths"][src].astype(np.float64), K).astype(np.float32)
+        write_mvei(f"{d}/depth-L0.mvei", zr[None])
+        write_mvei(f"{d}/dz-L0.mvei", np.zeros((2, h, w), np.float32))
+    cfg = tmp_path / "mve.cfg"
+    cfg.write_text("\n".join([
+        "algorithm fromMVE2coco", f"mve_name {tmp_path}", f"lf_name {tmp_path}/v_%02d.png",
+        f"tau_name {tmp_path}/t_%02d.pfm", f"dpart_name {tmp_path}/p_%02d.pfm", f"uDepth_name {tmp_path}/ud.pfm",
+        f"outdir {tmp_path}/out", f"width {w}", f"height {h}", "s_min 0", f"s_max {V}", f"s_rmv {rmv}", "dsf 1",
+        "scale_min 0", "scale_max 0"]) + "\n")
+    r = subprocess.run([lio.CLI_PATH, "-config", str(cfg)], capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout)
+    assert r.returncode == 0, r.stderr
+    assert not (tmp_path / f"t_{V:02d}.pfm").exists()   # V views imported, numbered 0..V-1
+    # the same steps in process: the filtered depth is the scene's depth
+    gxs, gys = [], []
+    with lfrv.Context(pipelines.gradient_ibr_params(s)) as ctx:
+        for v in range(V):
+            zr, gx, gy = lio.mve_depth(str(tmp_path), v, 0, 0, w, h)
+            K, _, _, P = lio.mve_camera(f"{view_dir(tmp_path, v)}/meta.ini", w, h)
+            zo, gxo, gyo = lfrv.radial_to_ortho(zr, K, gx, gy)
+            assert np.allclose(zo, s["depths"][v], rtol=1e-5)
+            ctx.set_depth(v, zo, P)
+            ctx.set_depth_gradient(v, gxo, gyo)
+            gxs.append(gxo)
+            gys.append(gyo)
+        _, _, _, Pt = lio.mve_camera(f"{view_dir(tmp_path, rmv)}/meta.ini", w, h)
+        assert np.allclose(Pt, s["P_views"][rmv], rtol=1e-12, atol=1e-9)
+        ctx.set_target_camera(Pt)
+        ctx.warps_from_target_depth(rmv)
+        tx = ctx.get_buffer("TAU_X").reshape(V, h, w)
+        ty = ctx.get_buffer("TAU_Y").reshape(V, h, w)
+        ud = ctx.get_buffer("TARGET_DEPTH").reshape(h, w)
+    for v in (0, 3, 8):
+        t = lio.load_image(str(tmp_path / f"t_{v:02d}.pfm"))
+        assert np.array_equal(t[0], tx[v]) and np.array_equal(t[1], ty[v])
+        assert np.array_equal(lio.load_image(str(tmp_path / f"v_{v:02d}.png")),
+                              lio.load_image(f"{view_dir(tmp_path, v)}/undistorted.png"))
+    assert np.array_equal(lio.load_image(str(tmp_path / "ud.pfm"))[0], ud)
+    # those warps against the oracle's producer on the scene's own depth and the filtered derivatives
+    gx, gy = np.stack(gxs), np.stack(gys)
+    oud = oracle.gl_target_depth(s["depths"], s["P_views"], s["P_views"][rmv], w, h, rmv, gx, gy)
+    assert (np.abs(ud - oud) > 1e-3 * np.maximum(1.0, oud)).mean() < 5e-3
+    otx, oty, _, _, _ = oracle.gl_warps(s["depths"], s["P_views"], s["P_views"][rmv], ud, 10.0, gx, gy)
+    both = (tx >= 0) & (otx >= 0)
+    assert ((tx >= 0) == (otx >= 0)).mean() > 0.998
+    assert np.max(np.abs(tx[both] - otx[both])) < 2e-3 and np.max(np.abs(ty[both] - oty[both])) < 2e-3
