@@ -774,6 +774,30 @@ def bench_warm_start(args):
     print(json.dumps(line))
 
 
+def c2_step_bytes(N, Vn, e, Nc, iters):
+    """Algorithmic HBM bytes of every kernel of one dsf = 1 gradient-IBR view
+    (warps from depth to the PCG solution), each as its data must move once:
+    inputs read, outputs written, per unit counts (N target pixels, Vn source
+    pixels of all views, e binned entries, Nc footprint cells)."""
+    return {
+        "k_fill_key + k_key_to_depth": 8 * N + 12 * N,          # z-buffer keys set, read, depth written
+        "k_tau_from_depth": 20 * Vn,                             # depth read; tau, sigma dtau/dz written
+        "k_tau_visibility": 8 * Vn + 4 * N,                      # tau read, target depth read; counts
+        "scan (3 kernels)": 12 * Nc,                             # counts read, starts and cursors written
+        "k_cell_fill": 8 * Vn + 4 * e + 4 * Nc,                  # tau read, entry indices written, cursors
+        "k_cell_sort": 4 * e + 12 * e + 16 * Nc,                 # indices read; sorted indices and tau written; tap sums
+        "k_coverage_from_taps": 16 * Nc + 4 * N,
+        "k_view_sobel": 20 * Vn,                                 # RGB read, gradients written
+        "k_visibility": 8 * N,
+        "k_weights<2,true>": 12 * Vn,                            # tau read, omega written
+        "k_gibr_setup_st2": 24 * e + 84 * N,                     # entries (index, tau, omega) + view RGB / Sobel; 21 planes out
+        "k_rhs_diag_cached": 32 * N,
+        "k_cgcg_begin_st2 + k_cgcg_begin": 100 * N,
+        "k_cgcg_st2": 144 * N * iters,                           # per PCG iteration
+        "k_st2_residual": 48 * N,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -793,7 +817,7 @@ def main():
     ap.add_argument("--no-tune-cache", action="store_true", help="batch_c4: autotune afresh, ignore the cache")
     ap.add_argument("--train-scenes", type=int, default=2000, help="warm_start_c5: training scenes")
     ap.add_argument("--test-scenes", type=int, default=200, help="warm_start_c5: held-out scenes")
-    ap.add_argument("--epochs", type=int, default=8, help="warm_start_c5: training epochs")
+    ap.add_argument("--epochs", type=int, default=16, help="warm_start_c5: training epochs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs one novel-view solve per GPU "
@@ -909,6 +933,17 @@ def main():
         "warp_producers": producers,
         "hole_filling": holes,
     }
+    if scene["dsf"] == 1:
+        # the whole view against the HBM roofline: the algorithmic bytes of all
+        # its kernels over the measured time per view
+        sb = c2_step_bytes(N, scene["V"] * scene["w"] * scene["h"], e_binned, (scene["W"] + 1) * (scene["H"] + 1),
+                           max(st_last["iterations"]))
+        tot = sum(sb.values())
+        gbs = tot / (ms_per_view * 1e-3) / 1e9
+        line["step_roofline"] = {"bound": "hbm", "algorithmic_bytes": tot, "ms": round(ms_per_view, 3),
+                                 "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                 "bytes_by_kernel": {k: int(v) for k, v in sb.items()}}
     if up_n:
         line["kernels"]["cg_update"] = {"avg_launch_us": round(up_avg_s * 1e6, 2),
                                         "achieved_gbs": round(up_bytes / up_avg_s / 1e9, 1), "launches": up_n}

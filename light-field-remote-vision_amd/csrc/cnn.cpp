@@ -32,8 +32,11 @@ hipError_t cnn_sq_loss(const float *out, const float *base, const float *target,
 hipError_t cnn_add_prediction(const float *out, const float *base, float *pred, int B, int HW, int co, hipStream_t s);
 hipError_t cnn_adam(float *w, float *m, float *v, const float *g, uint16_t *wb, size_t n, float lr, float b1,
                     float b2, float eps, int step, hipStream_t s);
-hipError_t cnn_stack_pack(const float *acc, const float *wsum, uint16_t *out, size_t N, int cpad, int chan0,
-                          hipStream_t s);
+int cnn_stack_group();
+size_t cnn_stack_anchor_floats(int W, int H, int dsf);
+hipError_t cnn_stack_group_pass(int dsf, int w, int h, int W, int H, const float *tx, const float *ty,
+                                const float *views, int v0, int nv, float *anchors, uint16_t *out, int cpad,
+                                int zero_from, hipStream_t s);
 
 constexpr int kEpiRelu = 1, kEpiSkip = 2, kEpiF32 = 4, kEpiAccum = 8, kEpiBias = 16;
 constexpr int kHeadCout = 32;     // the head's 3 channels, computed as 32
@@ -588,16 +591,22 @@ extern "C" lfrv_status lfrv_warped_stack(lfrv_ctx *ctx, void *out, int cpad)
 {
     if (!ctx || !out) return LFRV_E_ARG;
     if (cpad < 4 * ctx->V || cpad % 32) return fail(ctx, LFRV_E_ARG, "cpad must hold 4 channels per view and be a multiple of 32");
-    const size_t N = ctx->N, n = ctx->n;
-    LFRV_HIP(ctx, hipMemsetAsync(out, 0, N * cpad * 2, ctx->stream));
-    for (int v = 0; v < ctx->V; v++) {
-        LFRV_HIP(ctx, hipMemsetAsync(ctx->tmp3N, 0, 3 * N * 4, ctx->stream));
-        LFRV_HIP(ctx, hipMemsetAsync(ctx->tmpN, 0, N * 4, ctx->stream));
-        LFRV_HIP(ctx, launch_splat_atomic(ctx->W, ctx->H, ctx->w, ctx->h, ctx->dsf, ctx->tau_x + v * n,
-                                          ctx->tau_y + v * n, ctx->views + (size_t)v * 3 * n, 3, nullptr, ctx->tmp3N,
-                                          ctx->tmpN, ctx->stream));
-        LFRV_HIP(ctx, cnn_stack_pack(ctx->tmp3N, ctx->tmpN, (uint16_t *)out, N, cpad, 4 * v, ctx->stream));
+    if (ctx->dsf > 4) return fail(ctx, LFRV_E_ARG, "the warped stack is built for dsf <= 4");
+    float *anchors = nullptr;
+    LFRV_HIP(ctx, hipMallocAsync(reinterpret_cast<void **>(&anchors),
+                                 cnn_stack_anchor_floats(ctx->W, ctx->H, ctx->dsf) * sizeof(float), ctx->stream));
+    const int G = cnn_stack_group();
+    hipError_t e = hipSuccess;
+    for (int v0 = 0; v0 < ctx->V && e == hipSuccess; v0 += G) {
+        const int nv = std::min(G, ctx->V - v0);
+        const bool last = v0 + G >= ctx->V;
+        // the last group's pass also clears the channels beyond its own 4 G
+        e = cnn_stack_group_pass(ctx->dsf, ctx->w, ctx->h, ctx->W, ctx->H, ctx->tau_x, ctx->tau_y, ctx->views, v0, nv,
+                                 anchors, (uint16_t *)out, cpad, last ? 4 * (v0 + G) : -1, ctx->stream);
     }
+    const hipError_t ef = hipFreeAsync(anchors, ctx->stream);
+    LFRV_HIP(ctx, e);
+    LFRV_HIP(ctx, ef);
     return LFRV_OK;
 }
 

@@ -21,6 +21,7 @@
 // transposed in channels; the weight gradient is a second MFMA kernel whose
 // K runs over pixels.
 #include "lfrv_internal.hpp"
+#include "tile_common.hpp"
 
 namespace lfrv {
 
@@ -417,21 +418,106 @@ __global__ void k_adam(float *__restrict__ w, float *__restrict__ m, float *__re
     }
 }
 
-// One view of the warped stack: channels chan0 .. chan0+3 of out (bf16 NHWC
-// [N][cpad]) = the splatted colour sums normalised by the splatted weight,
-// and min(1, weight); 0 where no footprint reached the pixel.
-__global__ void k_stack_pack(const float *__restrict__ acc, const float *__restrict__ wsum, uint16_t *__restrict__ out,
-                             size_t N, int cpad, int chan0)
+// The warped stack (the network's input): per view, the view's colours
+// splatted into the target with the solver's footprint weights and
+// normalised by the splatted weight, and min(1, weight); 0 where no footprint
+// reached the pixel; channels 4 v .. 4 v + 3 of a bf16 NHWC [N][cpad] tensor,
+// the channels beyond 4 V zero.
+//
+// A footprint splat is a 2 x 2 splat of the entry onto box starts (anchors)
+// followed by a D x D box sum (D = dsf, box_weights): per source pixel 4
+// anchor taps instead of (D + 1)^2, which is what the atomics cost.  Views go
+// in groups of kStackGroup: their anchors (r, g, b, weight sums on a
+// (W + D + 1) x (H + D + 1) grid, anchor x0 at column x0 + D) by float
+// atomics, then one pass per target tile that box-sums them from LDS and
+// writes the group's 4 kStackGroup channels of each pixel in two 16-byte
+// stores.
+constexpr int kStackGroup = 4;
+constexpr int kStackTW = 32, kStackTH = 8;
+
+template <int KS>
+__global__ void k_stack_anchor(int w, int h, int W, int H, int WA, int HA, const float *__restrict__ tx,
+                               const float *__restrict__ ty, const float *__restrict__ views, int v0, int nv,
+                               float *__restrict__ anchors)
 {
-    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < N; o += (size_t)gridDim.x * blockDim.x) {
-        const float w = wsum[o];
-        const float iw = w > 0.0f ? 1.0f / w : 0.0f;
-        uint16_t *q = out + o * cpad + chan0;
-        q[0] = f2bf(acc[o] * iw);
-        q[1] = f2bf(acc[N + o] * iw);
-        q[2] = f2bf(acc[2 * N + o] * iw);
-        q[3] = f2bf(fminf(w, 1.0f));
+    constexpr int D = KS - 1;
+    const size_t n = (size_t)w * h, NA = (size_t)WA * HA;
+    const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = blockIdx.y;
+    if (m >= n || g >= nv) return;
+    const size_t k = (size_t)(v0 + g) * n + m;
+    const float t0 = tx[k], t1 = ty[k];
+    if (!cell_valid(t0, t1, W, H)) return;
+    int x0, y0;
+    float ax[2], ay[2];
+    if (!box_weights<KS>(t0, t1, W, H, x0, y0, ax, ay)) return;
+    const float *vp = views + (size_t)(v0 + g) * 3 * n + m;
+    const float val[4] = {vp[0], vp[n], vp[2 * n], 1.0f};
+    float *A = anchors + (size_t)g * 4 * NA;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const size_t o = (size_t)(y0 + i + D) * WA + (x0 + j + D);
+            const float a = ay[i] * ax[j];
+#pragma unroll
+            for (int q = 0; q < 4; q++) atomicAdd(A + q * NA + o, a * val[q]);
+        }
+}
+
+template <int D>
+__global__ void __launch_bounds__(kStackTW * kStackTH)
+k_stack_box_pack(int W, int H, int WA, int HA, const float *__restrict__ anchors, int nv, uint16_t *__restrict__ out,
+                 int cpad, int chan0, int zero_from)
+{
+    constexpr int LW = kStackTW + D - 1, LH = kStackTH + D - 1, LP = LW * LH, NPL = 4 * kStackGroup;
+    __shared__ float win[NPL * LP];
+    const size_t NA = (size_t)WA * HA;
+    const int x0 = blockIdx.x * kStackTW, y0 = blockIdx.y * kStackTH;
+    // pixel (x, y) sums the anchors of box starts x - D + 1 .. x, i.e. anchor
+    // columns x + 1 .. x + D; the window starts at column x0 + 1, row y0 + 1
+    for (int i = threadIdx.x; i < NPL * LP; i += blockDim.x) {
+        const int pl = i / LP, r = i - pl * LP, ly = r / LW, lx = r - ly * LW;
+        const int ax = x0 + 1 + lx, ay = y0 + 1 + ly;
+        win[i] = (pl < 4 * nv && ax < WA && ay < HA) ? anchors[(size_t)pl * NA + (size_t)ay * WA + ax] : 0.0f;
     }
+    __syncthreads();
+    const int lx = threadIdx.x % kStackTW, ly = threadIdx.x / kStackTW;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= W || y >= H) return;
+    uint16_t ch[NPL];
+#pragma unroll
+    for (int g = 0; g < kStackGroup; g++) {
+        float sum[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float *pw = win + (4 * g + q) * LP;
+            float acc = 0.0f;
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; j < D; j++) acc += pw[(ly + i) * LW + lx + j];
+            sum[q] = acc;
+        }
+        const float iw = sum[3] > 0.0f ? 1.0f / sum[3] : 0.0f;
+        const bool live = g < nv;
+        ch[4 * g + 0] = live ? f2bf(sum[0] * iw) : 0;
+        ch[4 * g + 1] = live ? f2bf(sum[1] * iw) : 0;
+        ch[4 * g + 2] = live ? f2bf(sum[2] * iw) : 0;
+        ch[4 * g + 3] = live ? f2bf(fminf(sum[3], 1.0f)) : 0;
+    }
+    uint16_t *q = out + ((size_t)y * W + x) * cpad + chan0;
+    u32x4 lo, hi;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        lo[k] = (uint32_t)ch[2 * k] | ((uint32_t)ch[2 * k + 1] << 16);
+        hi[k] = (uint32_t)ch[8 + 2 * k] | ((uint32_t)ch[8 + 2 * k + 1] << 16);
+    }
+    *(u32x4 *)q = lo;
+    *(u32x4 *)(q + 8) = hi;
+    // after the last group: the channels beyond the views, zero
+    if (zero_from >= 0)
+        for (int c = zero_from; c < cpad; c += 8) *(u32x4 *)(out + ((size_t)y * W + x) * cpad + c) = u32x4{0, 0, 0, 0};
 }
 
 // ---- launchers --------------------------------------------------------------
@@ -545,10 +631,35 @@ hipError_t cnn_add_prediction(const float *out, const float *base, float *pred, 
     return hipGetLastError();
 }
 
-hipError_t cnn_stack_pack(const float *acc, const float *wsum, uint16_t *out, size_t N, int cpad, int chan0,
-                          hipStream_t s)
+int cnn_stack_group() { return kStackGroup; }
+
+size_t cnn_stack_anchor_floats(int W, int H, int dsf) { return (size_t)kStackGroup * 4 * (W + dsf + 1) * (H + dsf + 1); }
+
+// One group of views of the warped stack: anchors (cleared here) then the box
+// pass writing channels 4 v0 .. 4 v0 + 4 kStackGroup - 1 (zero for the views
+// beyond nv); zero_from >= 0 also clears channels zero_from .. cpad - 1.
+hipError_t cnn_stack_group_pass(int dsf, int w, int h, int W, int H, const float *tx, const float *ty,
+                                const float *views, int v0, int nv, float *anchors, uint16_t *out, int cpad,
+                                int zero_from, hipStream_t s)
 {
-    k_stack_pack<<<ew_blocks(N), 256, 0, s>>>(acc, wsum, out, N, cpad, chan0);
+    const int WA = W + dsf + 1, HA = H + dsf + 1;
+    if (nv < 1 || nv > kStackGroup || (4 * v0) % 16 || cpad % 8) return hipErrorInvalidValue;
+    hipError_t e = hipMemsetAsync(anchors, 0, cnn_stack_anchor_floats(W, H, dsf) * sizeof(float), s);
+    if (e != hipSuccess) return e;
+    const size_t n = (size_t)w * h;
+    const dim3 ga((unsigned int)((n + 255) / 256), (unsigned int)nv);
+    const dim3 gb((W + kStackTW - 1) / kStackTW, (H + kStackTH - 1) / kStackTH);
+    switch (dsf) {
+#define ST(D)                                                                                                   \
+    case D:                                                                                                     \
+        k_stack_anchor<D + 1><<<ga, 256, 0, s>>>(w, h, W, H, WA, HA, tx, ty, views, v0, nv, anchors);          \
+        k_stack_box_pack<D><<<gb, kStackTW * kStackTH, 0, s>>>(W, H, WA, HA, anchors, nv, out, cpad, 4 * v0,    \
+                                                                zero_from);                                     \
+        break;
+        ST(1) ST(2) ST(3) ST(4)
+#undef ST
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

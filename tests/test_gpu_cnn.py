@@ -140,8 +140,18 @@ def test_training_lowers_the_loss():
     net.close()
 
 
-def test_warped_stack_matches_the_splat():
-    s = lfrv.synth(0, seed=3)
+@pytest.mark.parametrize("preset,dsf", [(0, 1), (0, 2), (0, 4), (3, 4)])
+def test_warped_stack_matches_the_splat(preset, dsf):
+    """The stack (anchor splat + box sum per group of 4 views) against the
+    oracle's footprint splat of each view, normalised; channels beyond 4 V
+    zero.  9 views (groups of 4, 4, 1) or the 25-view generator."""
+    d = lfrv.synth_desc(preset, 3)
+    d.dsf = d.render_scale = dsf
+    if preset == 3:
+        d.w = d.h = 32
+        d.focal, d.baseline = 32.0, 10.0 / 32
+    s = lfrv.synth(d)
+    print(f"warped stack dsf={s['dsf']} V={s['V']}", flush=True)
     V, h, w, H, W = s["V"], s["h"], s["w"], s["H"], s["W"]
     L = cnn_lib()
     cpad = (4 * V + 31) // 32 * 32
