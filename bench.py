@@ -683,6 +683,37 @@ def bench_batch(args):
 
 # ---- the learned warm start ----------------------------------------------------------
 
+def conv_layer_rates(side, reps=20):
+    """MFMA rate of the convolution kernel (lfrv_op_conv3x3, FP32 output) on
+    the U-Net's layer shapes at a side x side input: 20 back-to-back launches
+    per shape on random bf16 data, timed by the host around a device sync."""
+    from lfrv.cnn import Raw, cnn_lib
+    L = cnn_lib()
+    rng = np.random.default_rng(5)
+    shapes = [("enc0a", 0, 128, 32), ("enc0b", 0, 32, 32), ("enc1a", 1, 32, 64), ("enc1b", 1, 64, 64),
+              ("enc2a", 2, 64, 128), ("enc2b", 2, 128, 128), ("enc3a", 3, 128, 256), ("enc3b", 3, 256, 256),
+              ("dec2a", 2, 256, 128), ("dec1a", 1, 128, 64), ("dec0a", 0, 64, 32)]
+    out = {}
+    for name, lvl, ci, co in shapes:
+        h = side >> lvl
+        X = Raw((rng.integers(0x3f00, 0x3f80, h * h * ci, dtype=np.uint16)).astype(np.uint16))
+        Wt = Raw((rng.integers(0x3c00, 0x3c80, co * 9 * ci, dtype=np.uint16)).astype(np.uint16))
+        Y = Raw(h * h * co * 4)
+        L.lfrv_op_conv3x3(1, h, h, ci, co, X.ptr, Wt.ptr, None, Y.ptr, None)
+        lfrv.device_sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            L.lfrv_op_conv3x3(1, h, h, ci, co, X.ptr, Wt.ptr, None, Y.ptr, None)
+        lfrv.device_sync()
+        us = 1e6 * (time.perf_counter() - t0) / reps
+        fl = 2.0 * 9 * ci * co * h * h
+        out[name] = {"input": [h, h, ci], "cout": co, "us": round(us, 1), "tflops": round(fl / us * 1e-6, 1),
+                     "frac": round(fl / us * 1e-6 / 2500.0, 4)}
+        for b in (X, Wt, Y):
+            b.free()
+    return out
+
+
 def bench_warm_start(args):
     """The learned warm start: 2000 scenes of the 5 x 5-view generator at its
     4x upsampling (64^2 views, 256^2 targets) solved to rel. residual 1e-5 by
@@ -734,6 +765,7 @@ def bench_warm_start(args):
     big.close()
     del ctypes
     tflops = conv_flops / (conv_ms * 1e-3) / 1e12
+    layers = conv_layer_rates(dsf * w_full)
     ms_plain = float(np.median([r["ms_plain"] for r in full]))
     ms_warm = float(np.median([r["ms_warm"] for r in full]))
     line = {
@@ -770,6 +802,7 @@ def bench_warm_start(args):
                      "achieved": round(tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 4), "traffic": None,
                      "flops_per_forward": fwd_flops, "conv_launches": conv_n},
+        "conv_layers": layers,
     }
     print(json.dumps(line))
 

@@ -879,20 +879,21 @@ lfrv_status run_warps_from_target_depth(lfrv_ctx *ctx, int skip)
 }
 
 template <int KS>
-static void launch_weights_ks(lfrv_ctx *ctx, const float *gxs, const float *gys, bool zero_grad)
+static void launch_weights_ks(lfrv_ctx *ctx, const float *gxs, const float *gys, bool zero_grad, hipStream_t st)
 {
     if (zero_grad)
-        k_weights<KS, true><<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
+        k_weights<KS, true><<<source_grid(ctx), kSourceBlock, 0, st>>>(
             ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, gxs, gys, ctx->dpart_x,
             ctx->dpart_y, ctx->p.sigma_sensor, ctx->p.ugrad_threshold, ctx->omega);
     else
-        k_weights<KS, false><<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
+        k_weights<KS, false><<<source_grid(ctx), kSourceBlock, 0, st>>>(
             ctx->V, ctx->W, ctx->H, ctx->w, ctx->h, ctx->tau_x, ctx->tau_y, gxs, gys, ctx->dpart_x,
             ctx->dpart_y, ctx->p.sigma_sensor, ctx->p.ugrad_threshold, ctx->omega);
 }
 
-lfrv_status run_compute_weights(lfrv_ctx *ctx)
+lfrv_status run_compute_weights(lfrv_ctx *ctx, hipStream_t st)
 {
+    if (!st) st = ctx->stream;
     const float *gxs, *gys;
     dim3 bs(32, 8), gs((ctx->W + 31) / 32, (ctx->H + 7) / 8);
     const size_t N = ctx->N;
@@ -900,7 +901,7 @@ lfrv_status run_compute_weights(lfrv_ctx *ctx)
         // vtv_sr_compute_gradient_device writes the gated Sobel of U into the
         // dual variable of channel 0 (optimization.cu:1090-1091); kept as is.
         float *px0 = ctx->PX, *py0 = ctx->PY;  // the current dual buffers
-        k_sobel<<<gs, bs, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->U + N, ctx->U + 2 * N, px0, py0,
+        k_sobel<<<gs, bs, 0, st>>>(ctx->W, ctx->H, ctx->U, ctx->U + N, ctx->U + 2 * N, px0, py0,
                                             ctx->visibility, nullptr);
         gxs = px0;
         gys = py0;
@@ -910,17 +911,17 @@ lfrv_status run_compute_weights(lfrv_ctx *ctx)
         // Sobel of the cleared image is zero everywhere, so is its warp: the
         // weights then take the zero-gradient form and need no Sobel.
         if (!ctx->u_zero)
-            k_sobel<<<gs, bs, 0, ctx->stream>>>(ctx->W, ctx->H, ctx->U, ctx->U + N, ctx->U + 2 * N,
+            k_sobel<<<gs, bs, 0, st>>>(ctx->W, ctx->H, ctx->U, ctx->U + N, ctx->U + 2 * N,
                                                 ctx->u_grad_x, ctx->u_grad_y, nullptr, nullptr);
         gxs = ctx->u_grad_x;
         gys = ctx->u_grad_y;
     }
     const bool zero_grad = ctx->p.mode == LFRV_MODE_GRADIENT_IBR && ctx->u_zero;
     switch (ctx->ks) {
-    case 2: launch_weights_ks<2>(ctx, gxs, gys, zero_grad); break;
-    case 3: launch_weights_ks<3>(ctx, gxs, gys, zero_grad); break;
-    case 4: launch_weights_ks<4>(ctx, gxs, gys, zero_grad); break;
-    case 5: launch_weights_ks<5>(ctx, gxs, gys, zero_grad); break;
+    case 2: launch_weights_ks<2>(ctx, gxs, gys, zero_grad, st); break;
+    case 3: launch_weights_ks<3>(ctx, gxs, gys, zero_grad, st); break;
+    case 4: launch_weights_ks<4>(ctx, gxs, gys, zero_grad, st); break;
+    case 5: launch_weights_ks<5>(ctx, gxs, gys, zero_grad, st); break;
     default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
     }
     LFRV_LAUNCH(ctx, "compute_weights kernels");

@@ -462,7 +462,8 @@ lfrv_status run_warps_from_target_depth(lfrv_ctx *ctx, int skip);
 lfrv_status run_prepare(lfrv_ctx *ctx);
 lfrv_status run_refresh_entry_weights(lfrv_ctx *ctx, float scale);
 lfrv_status run_visibility(lfrv_ctx *ctx);
-lfrv_status run_compute_weights(lfrv_ctx *ctx);
+// s: the stream to launch on (nullptr: the context's own)
+lfrv_status run_compute_weights(lfrv_ctx *ctx, hipStream_t s = nullptr);
 lfrv_status run_initial_image(lfrv_ctx *ctx);
 lfrv_status run_hole_filling(lfrv_ctx *ctx, int iters, double eps, int *ran);
 lfrv_status run_push_pull(lfrv_ctx *ctx, int levels);
