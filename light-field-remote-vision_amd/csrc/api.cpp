@@ -223,6 +223,10 @@ extern "C" lfrv_status lfrv_create(const lfrv_params *p, lfrv_ctx **out)
     if (ctx->ks == 2) {
         A(&ctx->gram, 5 * N);
         A(&ctx->cg_extra, 9 * N);
+    } else if (p->mode == LFRV_MODE_GRADIENT_IBR) {
+        // the box operator's coefficients, and the second r, s, w buffers and u
+        A(&ctx->box_ops, 5 * (size_t)(ctx->W + ctx->ks - 1) * (ctx->H + ctx->ks - 1));
+        A(&ctx->cg_extra, 12 * N);
     }
     ctx->pp_cap = 3 * (N + N / 2) + 3 * N + N / 4 + 64;
     A(&ctx->pp_scratch, ctx->pp_cap);
@@ -265,7 +269,7 @@ extern "C" void lfrv_destroy(lfrv_ctx *ctx)
              ctx->u_grad_y, ctx->target_mask, ctx->visibility, ctx->target_depth, ctx->tmpN, ctx->cg_minv,
              ctx->hole_list, ctx->pp_scratch, ctx->cg_state, ctx->counters, ctx->dcams, ctx->scal, ctx->partials,
              ctx->cache_num, ctx->cache_sq, ctx->gram, ctx->cg_extra, ctx->sr_loop, ctx->sr_prev, ctx->depth_grad,
-             ctx->tau_weight, ctx->gl_big})
+             ctx->tau_weight, ctx->gl_big, ctx->box_ops})
         if (p) (void)hipFree(p);
     if (ctx->cg_state_host) (void)hipHostFree(ctx->cg_state_host);
     if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);

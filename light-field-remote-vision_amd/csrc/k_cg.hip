@@ -903,6 +903,337 @@ unsigned int reduce_blocks(size_t N)
     return (unsigned int)(b < 2048 ? (b == 0 ? 1 : b) : 2048);
 }
 
+
+// ---------------------------------------------------------------------------
+// dsf >= 2: the data operator sum_e omega_e S_e^T S_e in assembled form.
+//
+// With D = dsf, an entry's footprint weights are the 2 x 2 anchor taps t_e of
+// its box start followed by a D x D box (box_weights).  So
+//   S^T diag(omega) S u = Box^T G Box u,   G = sum_e omega_e t_e t_e^T,
+// Box u at an anchor the sum of u over its D x D box, and G couples each
+// anchor with its 8 neighbours (5 planes by symmetry: centre, E, S, SE, SW).
+// G is summed once per set of weights (k_gram_box, a pull over the entries);
+// a PCG operator application is then a tiled stencil (box_apply_tile): the
+// box sums of u in LDS (rows, then columns), G at the anchors, the box sums
+// back onto the pixels, plus c0 u and the Neumann Laplacian of u.  Anchor
+// plane index a holds the cell a + 1 of the splat structure, whose box covers
+// pixels a - D + 1 .. a.
+// ---------------------------------------------------------------------------
+template <int KS>
+__global__ void __launch_bounds__(kPullMaxThreads) k_gram_box(TileGeom g, TileEntries te, int cap, size_t NA,
+                                                              float *__restrict__ ops)
+{
+    extern __shared__ int smem[];
+    constexpr int NC = 10;
+    const int t = xcd_tile(blockIdx.x, g.ntiles);
+    const int tx0 = (t % g.ntx) * g.tw, ty0 = (t / g.ntx) * g.th;
+    const int ww = g.tw + 1, wh = g.th + 1, ncw = ww * wh;
+    float acc[NC];
+    cell_accumulate<KS, NC, false, true, 0, 1>(
+        g, te, t, smem, cap, acc,
+        [&](int, int, int, const float(&ax)[2], const float(&ay)[2], float om, float(&a)[NC]) {
+            const float w[4] = {ay[0] * ax[0], ay[0] * ax[1], ay[1] * ax[0], ay[1] * ax[1]};  // tl tr bl br
+            a[0] += om * w[0] * w[0];
+            a[1] += om * w[1] * w[1];
+            a[2] += om * w[2] * w[2];
+            a[3] += om * w[3] * w[3];
+            a[4] += om * w[0] * w[1];  // tl - tr (E of tl)
+            a[5] += om * w[2] * w[3];  // bl - br (E of bl)
+            a[6] += om * w[0] * w[2];  // tl - bl (S of tl)
+            a[7] += om * w[1] * w[3];  // tr - br (S of tr)
+            a[8] += om * w[0] * w[3];  // tl - br (SE of tl)
+            a[9] += om * w[1] * w[2];  // tr - bl (SW of tr)
+        });
+    float *part = reinterpret_cast<float *>(smem + 2 * wh + 2);
+    const int c = threadIdx.x;
+    if (c < ncw) {
+#pragma unroll
+        for (int k = 0; k < NC; k++) part[k * ncw + c] = acc[k];
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < g.tw * g.th; p += blockDim.x) {
+        const int lx = p % g.tw, ly = p / g.tw;
+        const int x = tx0 + lx, y = ty0 + ly;
+        if (x >= g.Wc || y >= g.Hc) continue;
+        // the anchor is the tl tap of its own cell, tr of the left cell, bl of
+        // the upper cell and br of the upper-left cell
+        const int c00 = (ly + 1) * ww + lx + 1, c10 = c00 - 1, c01 = c00 - ww, c11 = c00 - ww - 1;
+        const size_t o = (size_t)y * g.Wc + x;
+        ops[o] = part[0 * ncw + c00] + part[1 * ncw + c10] + part[2 * ncw + c01] + part[3 * ncw + c11];
+        ops[NA + o] = part[4 * ncw + c00] + part[5 * ncw + c01];
+        ops[2 * NA + o] = part[6 * ncw + c00] + part[7 * ncw + c10];
+        ops[3 * NA + o] = part[8 * ncw + c00];
+        ops[4 * NA + o] = part[9 * ncw + c10];
+    }
+}
+
+constexpr int kBoxTW = 32, kBoxTH = 16;
+
+template <int KS>
+struct BoxDims {
+    static constexpr int D = KS - 1, TW = kBoxTW, TH = kBoxTH, NT = TW * TH;
+    static constexpr int LW = TW + 2 * D, LH = TH + 2 * D;  // u from (tx0 - D, ty0 - D)
+    static constexpr int QW = TW + D - 1, QH = TH + D - 1;  // anchors tx0 .. tx0 + QW - 1
+    static constexpr int BW = QW + 2, BH = QH + 2;          // box sums at anchors tx0 - 1 + bx
+    static constexpr int NU = (LW * LH + NT - 1) / NT, NQ = (QW * QH + NT - 1) / NT;
+};
+
+// Coefficients of anchor (xq, yq): own centre, E, S, SE, SW and the
+// neighbours' couplings to it (zero beyond the anchor grid).
+__device__ __forceinline__ void box_coeffs(const float *__restrict__ ops, size_t NA, int Wc, int Hc, int xq, int yq,
+                                           float (&f)[9])
+{
+    const bool in = xq >= 0 && yq >= 0 && xq < Wc && yq < Hc;
+    const int xc = min(max(xq, 0), Wc - 1), yc = min(max(yq, 0), Hc - 1);
+    const size_t o = (size_t)yc * Wc + xc;
+    const bool xm = xc > 0, ym = yc > 0, xp = xc < Wc - 1;
+    const size_t oE = xm ? o - 1 : o, oS = ym ? o - Wc : o, oSE = (xm && ym) ? o - Wc - 1 : o,
+                 oSW = (xp && ym) ? o - Wc + 1 : o;
+    f[0] = ops[o];
+    f[1] = ops[NA + o];
+    f[2] = ops[2 * NA + o];
+    f[3] = ops[3 * NA + o];
+    f[4] = ops[4 * NA + o];
+    const float e = ops[NA + oE], sn = ops[2 * NA + oS], se = ops[3 * NA + oSE], sw = ops[4 * NA + oSW];
+    f[5] = xm ? e : 0.0f;
+    f[6] = ym ? sn : 0.0f;
+    f[7] = (xm && ym) ? se : 0.0f;
+    f[8] = (xp && ym) ? sw : 0.0f;
+    if (!in)
+#pragma unroll
+        for (int q = 0; q < 9; q++) f[q] = 0.0f;
+}
+
+// A u at this thread's pixel of the tile (tx0, ty0): wscale Box^T G Box u +
+// c0 u + beta_reg (Neumann Laplacian of u).  All threads of the block call it.
+// uc receives u at the pixel.  The caller provides the LDS.
+template <int KS>
+__device__ __forceinline__ void box_apply_tile(int W, int H, int Wc, int Hc, int tx0, int ty0, float wscale, float c0,
+                                               float beta_reg, const float *__restrict__ ops, size_t NA,
+                                               const float *__restrict__ u, float *su, float *sh, float *sb,
+                                               float *sq, float (&out)[3], float (&uc)[3])
+{
+    using S = BoxDims<KS>;
+    constexpr int D = S::D, TW = S::TW, NT = S::NT, LW = S::LW, LH = S::LH, QW = S::QW, QH = S::QH, BW = S::BW,
+                  BH = S::BH;
+    const size_t N = (size_t)W * H;
+    const int tid = threadIdx.x;
+    // global loads first: u on the tile and its D-ring, the anchors' coefficients
+    float uv[S::NU][3];
+#pragma unroll
+    for (int k = 0; k < S::NU; k++) {
+        const int i = tid + k * NT;
+        const int x = tx0 - D + i % LW, y = ty0 - D + i / LW;
+        const bool in = i < LW * LH && x >= 0 && x < W && y >= 0 && y < H;
+        const size_t o = in ? (size_t)y * W + x : 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) uv[k][c] = in ? u[c * N + o] : 0.0f;
+    }
+    float cf[S::NQ][9];
+#pragma unroll
+    for (int k = 0; k < S::NQ; k++) {
+        const int i = tid + k * NT;
+        box_coeffs(ops, NA, Wc, Hc, i < QW * QH ? tx0 + i % QW : -1, ty0 + i / QW, cf[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < S::NU; k++) {
+        const int i = tid + k * NT;
+        if (i < LW * LH)
+#pragma unroll
+            for (int c = 0; c < 3; c++) su[c * LW * LH + i] = uv[k][c];
+    }
+    __syncthreads();
+    // box sums of u: rows (anchor column bx covers tile columns bx .. bx + D - 1), then columns
+    for (int i = tid; i < LH * BW; i += NT) {
+        const int ly = i / BW, bx = i - ly * BW;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float sm = 0.0f;
+#pragma unroll
+            for (int l = 0; l < D; l++) sm += su[c * LW * LH + ly * LW + bx + l];
+            sh[c * LH * BW + i] = sm;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < BH * BW; i += NT) {
+        const int by = i / BW, bx = i - by * BW;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float sm = 0.0f;
+#pragma unroll
+            for (int l = 0; l < D; l++) sm += sh[c * LH * BW + (by + l) * BW + bx];
+            sb[c * BH * BW + i] = sm;
+        }
+    }
+    __syncthreads();
+    // G at the anchors
+#pragma unroll
+    for (int k = 0; k < S::NQ; k++) {
+        const int i = tid + k * NT;
+        if (i >= QW * QH) continue;
+        const int qy = i / QW, qx = i - qy * QW;
+        const int b = (qy + 1) * BW + qx + 1;
+        const float(&f)[9] = cf[k];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float *B = sb + c * BH * BW;
+            sq[c * QW * QH + i] = f[0] * B[b] + f[1] * B[b + 1] + f[5] * B[b - 1] + f[2] * B[b + BW] +
+                                  f[6] * B[b - BW] + f[3] * B[b + BW + 1] + f[7] * B[b - BW - 1] +
+                                  f[4] * B[b + BW - 1] + f[8] * B[b - BW + 1];
+        }
+    }
+    __syncthreads();
+    // box sums back: pixel lx takes anchors lx .. lx + D - 1; rows into sh, then columns
+    for (int i = tid; i < QH * TW; i += NT) {
+        const int qy = i / TW, lx = i - qy * TW;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float sm = 0.0f;
+#pragma unroll
+            for (int l = 0; l < D; l++) sm += sq[c * QW * QH + qy * QW + lx + l];
+            sh[c * QH * TW + i] = sm;
+        }
+    }
+    __syncthreads();
+    const int lx = tid % TW, ly = tid / TW;
+    const int x = tx0 + lx, y = ty0 + ly;
+    const int li = (ly + D) * LW + lx + D;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int l = 0; l < D; l++) acc += sh[c * QH * TW + (ly + l) * TW + lx];
+        const float *dp = su + c * LW * LH;
+        const float dv = dp[li];
+        float lap = 0.0f;
+        if (W > 1) {
+            if (x == 0) lap += dv - dp[li + 1];
+            else if (x < W - 1) lap += 2.0f * dv - dp[li - 1] - dp[li + 1];
+            else lap += dv - dp[li - 1];
+        }
+        if (H > 1) {
+            if (y == 0) lap += dv - dp[li + LW];
+            else if (y < H - 1) lap += 2.0f * dv - dp[li - LW] - dp[li + LW];
+            else lap += dv - dp[li - LW];
+        }
+        out[c] = wscale * acc + c0 * dv + beta_reg * lap;
+        uc[c] = dv;
+    }
+    __syncthreads();  // the LDS is free again
+}
+
+// LDS floats of box_apply_tile: u, the row / column box sums, the anchor values
+template <int KS>
+constexpr int box_lds_floats()
+{
+    using S = BoxDims<KS>;
+    return 3 * (S::LW * S::LH + S::LH * S::BW + S::BH * S::BW + S::QW * S::QH);
+}
+
+enum : int { kBoxResInit = 0, kBoxBegin = 1, kBoxOp = 2, kBoxResid = 3 };
+
+// One box-operator pass over the tiles.
+//   kBoxResInit: r = b - A x, u = Minv r (no sums)
+//   kBoxBegin:   w = A u; 12 sums r.r, r.u, b.b, w.u (the begin pass)
+//   kBoxOp:      w = A u; 9 sums r.u, w.u, r.r (a PCG iteration)
+//   kBoxResid:   3 sums |b - A x|^2 (the true residual)
+// `in` is x (ResInit, Resid) or u (Begin, Op).  A solve already done returns.
+template <int KS, int MODE>
+__global__ void __launch_bounds__(kBoxTW * kBoxTH) k_box_pass(int W, int H, int Wc, int Hc, int ntx, int ntiles,
+                                                                float wscale, float c0, float beta_reg,
+                                                                const float *__restrict__ ops, size_t NA,
+                                                                const float *__restrict__ in,
+                                                                const float *__restrict__ b,
+                                                                const float *__restrict__ r,
+                                                                const float *__restrict__ minv,
+                                                                float *__restrict__ out, float *__restrict__ out2,
+                                                                const CgState *st, double *partials)
+{
+    constexpr int NL = box_lds_floats<KS>();
+    __shared__ float lds[NL];
+    __shared__ double red[12 * 16];
+    using S = BoxDims<KS>;
+    if (MODE == kBoxOp && st->done) return;
+    float *su = lds, *sh = su + 3 * S::LW * S::LH, *sb = sh + 3 * S::LH * S::BW, *sq = sb + 3 * S::BH * S::BW;
+    const int t = xcd_tile(blockIdx.x, ntiles);
+    const int tx0 = (t % ntx) * S::TW, ty0 = (t / ntx) * S::TH;
+    const int x = tx0 + threadIdx.x % S::TW, y = ty0 + threadIdx.x / S::TW;
+    const bool pin = x < W && y < H;
+    const size_t N = (size_t)W * H, o = pin ? (size_t)y * W + x : 0;
+    float av[3], uv[3];
+    box_apply_tile<KS>(W, H, Wc, Hc, tx0, ty0, wscale, c0, beta_reg, ops, NA, in, su, sh, sb, sq, av, uv);
+    constexpr int NQS = MODE == kBoxBegin ? 12 : MODE == kBoxOp ? 9 : 3;
+    double v[NQS];
+#pragma unroll
+    for (int q = 0; q < NQS; q++) v[q] = 0.0;
+    if (pin) {
+        const float mi = (MODE == kBoxResInit) ? minv[o] : 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const size_t k = c * N + o;
+            if (MODE == kBoxResInit) {
+                const float rv = b[k] - av[c];
+                out[k] = rv;
+                out2[k] = mi * rv;
+            } else if (MODE == kBoxResid) {
+                const double rv = (double)b[k] - (double)av[c];
+                v[c] += rv * rv;
+            } else {
+                out[k] = av[c];
+                const double rr = r[k], uu = uv[c], ww = av[c];
+                if (MODE == kBoxBegin) {
+                    const double bb = b[k];
+                    v[c] += rr * rr;
+                    v[3 + c] += rr * uu;
+                    v[6 + c] += bb * bb;
+                    v[9 + c] += ww * uu;
+                } else {
+                    v[c] += rr * uu;
+                    v[3 + c] += ww * uu;
+                    v[6 + c] += rr * rr;
+                }
+            }
+        }
+    }
+    if (MODE != kBoxResInit) block_partials<NQS>(v, partials, red);
+}
+
+// The vector updates of a PCG iteration from the scalars of the state:
+// s_i = w_i + beta s_{i-1}, p_i = Minv r_i + beta p_{i-1}, x += alpha p_i,
+// r_{i+1} = r_i - alpha s_i, u_{i+1} = Minv r_{i+1} (the operand of the next
+// box pass), per pixel and channel (the same updates as k_cgcg_st2's).
+__global__ void __launch_bounds__(256) k_box_update(size_t N, const CgState *st, const float *__restrict__ minv,
+                                                    const float *__restrict__ r_in, const float *__restrict__ w_in,
+                                                    const float *__restrict__ s_in, float *__restrict__ r_out,
+                                                    float *__restrict__ s_out, float *__restrict__ p,
+                                                    float *__restrict__ x, float *__restrict__ u)
+{
+    const int done = st->done, first = st->first;
+    float al[3], be[3];
+    for (int c = 0; c < 3; c++) {
+        al[c] = (float)st->alpha[c];
+        be[c] = (float)st->beta[c];
+    }
+    if (done) return;
+    for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < N; o += (size_t)gridDim.x * blockDim.x) {
+        const float mi = minv[o];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const size_t k = c * N + o;
+            const float rv = r_in[k];
+            const float s = first ? w_in[k] : w_in[k] + be[c] * s_in[k];
+            const float uo = mi * rv;
+            const float pn = first ? uo : uo + be[c] * p[k];
+            const float rn = rv - al[c] * s;
+            p[k] = pn;
+            x[k] += al[c] * pn;
+            s_out[k] = s;
+            r_out[k] = rn;
+            u[k] = mi * rn;
+        }
+    }
+}
+
 }  // namespace
 
 static lfrv_status cg_finish_stats(lfrv_ctx *ctx, lfrv_cg_stats *st, std::chrono::steady_clock::time_point t_start);
@@ -952,6 +1283,30 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         ctx->gram_epoch = ctx->epoch;
     }
     const int op_blocks = st2 ? st2_tiles : ctx->ntiles;
+    // dsf >= 2 with views: the assembled box operator
+    const bool boxp = !st2 && ctx->ks >= 3 && use_views && wscale != 0.0f && ctx->box_ops != nullptr;
+    const int Wc = ctx->W + ctx->ks - 1, Hc = ctx->H + ctx->ks - 1;
+    const size_t NA = (size_t)Wc * Hc;
+    const int box_ntx = (ctx->W + kBoxTW - 1) / kBoxTW, box_tiles = box_ntx * ((ctx->H + kBoxTH - 1) / kBoxTH);
+    if (boxp && ctx->box_epoch != ctx->epoch) {
+        TimedRegion tr(ctx, 5);
+        TileGeom ga = g;
+        ga.ntx = (Wc + g.tw - 1) / g.tw;
+        ga.ntiles = ga.ntx * ((Hc + g.th - 1) / g.th);
+        const int wh = g.th + 1, ncw = (g.tw + 1) * wh;
+        const int gcap = std::min(1024, kStagePer * ctx->pull_bs);
+        const size_t glds = sizeof(int) * (2 * wh + 2) + sizeof(float) * std::max<size_t>(10 * (size_t)ncw, 3 * (size_t)gcap);
+#define GB(K) k_gram_box<K><<<ga.ntiles, ctx->pull_bs, glds, ctx->stream>>>(ga, te, gcap, NA, ctx->box_ops)
+        switch (ctx->ks) {
+        case 3: GB(3); break;
+        case 4: GB(4); break;
+        case 5: GB(5); break;
+        default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
+        }
+#undef GB
+        LFRV_LAUNCH(ctx, "box operator assembly");
+        ctx->box_epoch = ctx->epoch;
+    }
 
 #define OPK(K, MODE, DIN, DOUT)                                                                          \
     k_cg_op<K><<<ctx->ntiles, ctx->pull_bs, lds, ctx->stream>>>(g, te, cap, wscale, (float)c0, (float)beta, \
@@ -975,6 +1330,17 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
                 ctx->cg_q, part);
         }
         k_cgcg_begin<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, st2_tiles, tol * tol, maxit);
+    } else if (boxp) {
+        // r_0 = b - A x_0 and u_0 = Minv r_0, then w_0 = A u_0 with the start sums
+        float *u = ctx->cg_extra + 9 * N;
+        TimedRegion tr(ctx, 5);
+#define BP(K, M, IN, R, OUT, OUT2) k_box_pass<K, M><<<box_tiles, kBoxTW * kBoxTH, 0, ctx->stream>>>(ctx->W, ctx->H, Wc, Hc, box_ntx, box_tiles, wscale, (float)c0, (float)beta, ctx->box_ops, NA, IN, ctx->cg_b, R, minv, OUT, OUT2, stt, part)
+        switch (ctx->ks) {
+        case 3: BP(3, kBoxResInit, x, nullptr, r, u); BP(3, kBoxBegin, u, r, ctx->cg_q, nullptr); break;
+        case 4: BP(4, kBoxResInit, x, nullptr, r, u); BP(4, kBoxBegin, u, r, ctx->cg_q, nullptr); break;
+        case 5: BP(5, kBoxResInit, x, nullptr, r, u); BP(5, kBoxBegin, u, r, ctx->cg_q, nullptr); break;
+        }
+        k_cgcg_begin<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, box_tiles, tol * tol, maxit);
     } else {
         {
             TimedRegion tr(ctx, 5);
@@ -1060,6 +1426,69 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
         }
         done = true;
     }
+    if (boxp) {
+        // single-reduction iterations on the box operator: the vector updates,
+        // then w = A u with the iteration's sums, then the scalar step; r, s, w
+        // double-buffered as in the dsf = 1 stencil path
+        float *rbuf[2] = {ctx->cg_r, ctx->cg_extra};
+        float *sbuf[2] = {ctx->cg_extra + 3 * N, ctx->tmp3N};
+        float *wbuf[2] = {ctx->cg_q, ctx->cg_extra + 6 * N};
+        float *u = ctx->cg_extra + 9 * N;
+        const unsigned int ub = (unsigned int)std::min<size_t>((N + 255) / 256, 8192);
+        auto box_poll = [&]() -> lfrv_status {
+            {
+                TimedRegion tr(ctx, 5);
+                switch (ctx->ks) {
+                case 3: BP(3, kBoxResid, x, nullptr, nullptr, nullptr); break;
+                case 4: BP(4, kBoxResid, x, nullptr, nullptr, nullptr); break;
+                case 5: BP(5, kBoxResid, x, nullptr, nullptr, nullptr); break;
+                }
+            }
+            k_reduce_out<3><<<1, kFinalizeThreads, 0, ctx->stream>>>(part, box_tiles, stt->resid);
+            LFRV_LAUNCH(ctx, "cg residual");
+            LFRV_HIP(ctx, hipMemcpyAsync(ctx->cg_state_host, stt, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+            LFRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            final_ready = true;
+            return LFRV_OK;
+        };
+        LFRV_LAUNCH(ctx, "cg start");
+        int next_chunk = std::max(1, ctx->cg_iter_hint);
+        while (it < maxit && !done) {
+            const int nk = std::min(next_chunk, maxit - it);
+            next_chunk = 4;
+            for (int k = 0; k < nk; k++) {
+                {
+                    TimedRegion tr(ctx, 1);
+                    k_box_update<<<ub, 256, 0, ctx->stream>>>(N, stt, minv, rbuf[cur], wbuf[cur], sbuf[cur],
+                                                              rbuf[cur ^ 1], sbuf[cur ^ 1], ctx->cg_d, x, u);
+                }
+                {
+                    TimedRegion tr(ctx, 0);
+                    switch (ctx->ks) {
+                    case 3: BP(3, kBoxOp, u, rbuf[cur ^ 1], wbuf[cur ^ 1], nullptr); break;
+                    case 4: BP(4, kBoxOp, u, rbuf[cur ^ 1], wbuf[cur ^ 1], nullptr); break;
+                    case 5: BP(5, kBoxOp, u, rbuf[cur ^ 1], wbuf[cur ^ 1], nullptr); break;
+                    }
+                }
+                k_cgcg_step<<<1, kFinalizeThreads, 0, ctx->stream>>>(stt, part, box_tiles);
+                cur ^= 1;
+            }
+            LFRV_LAUNCH(ctx, "cg iteration");
+            it += nk;
+            lfrv_status ps = box_poll();
+            if (ps != LFRV_OK) return ps;
+            done = ctx->cg_state_host->done != 0;
+        }
+        if (final_ready) {
+            const CgState &hs = *ctx->cg_state_host;
+            ctx->cg_iter_hint = std::max(hs.iter[0], std::max(hs.iter[1], hs.iter[2]));
+        } else {
+            lfrv_status ps = box_poll();  // maxit == 0
+            if (ps != LFRV_OK) return ps;
+        }
+        done = true;
+    }
+#undef BP
     while (it < maxit && !done) {
         const int nk = std::min(chunk, maxit - it);
         for (int k = 0; k < nk; k++) {

@@ -353,6 +353,8 @@ struct lfrv_ctx {
     float *gram = nullptr;     // dsf = 1: 5N raw stencil coefficients of sum_i S_i^T omega_i S_i
     float *cg_extra = nullptr; // dsf = 1: 9N, the second r, s and w buffers of the single-reduction PCG
     uint64_t gram_epoch = 0;
+    float *box_ops = nullptr;  // IBR_gradient, dsf >= 2: 5 anchor planes of sum_e omega_e t_e t_e^T
+    uint64_t box_epoch = 0;
     int cg_iter_hint = 8;      // iterations of the last single-reduction solve (first look at the flag)
     int pull_budget = 48 * 1024;  // LDS bytes of a pull's entry cache
 
