@@ -383,6 +383,26 @@ def bench_color_sr(args, workload):
     pd_ms, pd_n = ctx.get_timing("pd_step")
     ctx.set_timing(False)
     producers = warp_producer_times(ctx, scene)
+    convergence = None
+    if not nmeta:
+        # convergence of the fixed-step solve, after the timed region: the
+        # primal energy (FP64) and the PSNR to the ground truth at checkpoints
+        ctx.clear_solution()
+        pipelines.color_sr_setup(ctx)
+        ctx.sr_init_regularizer_weight()
+        e0, _ = ctx.sr_primal_energy()
+        traj = [dict(step=0, energy=round(e0, 6), psnr_db=round(lfrv.psnr(ctx.get_solution(), scene["target"]), 3))]
+        done = 0
+        for k in sorted({max(1, steps // 10), steps // 4, steps // 2, steps}):
+            ctx.sr_iterate(k - done)
+            done = k
+            ek, _ = ctx.sr_primal_energy()
+            traj.append(dict(step=k, energy=round(ek, 6),
+                             psnr_db=round(lfrv.psnr(ctx.get_solution(), scene["target"]), 3)))
+        emin = min(t["energy"] for t in traj)
+        convergence = dict(trajectory=traj, rel_energy_decrease=round((e0 - traj[-1]["energy"]) / e0, 6),
+                           final_over_min_energy=round(traj[-1]["energy"] / emin, 6),
+                           same_as_timed=bool(np.array_equal(ctx.get_solution(), U)))
     host_loop = None
     if nmeta:
         # the same view with the loop driven from the host, for comparison
@@ -471,6 +491,8 @@ def bench_color_sr(args, workload):
         "kernels": dict(kern, binned_entries=e_binned, tiles=ntiles),
         "warp_producers": producers,
     }
+    if convergence:
+        line["convergence"] = convergence
     if nmeta:
         line["irls"] = dict(meta_iterations=last["meta_iterations"], fista_steps=last["fista_steps"],
                             last_stop=last["last_stop"], converged=last["converged"],
