@@ -69,10 +69,10 @@ def test_breakdown_of_the_stencil_pcg(small_scene):
         assert ei.value.status == lfrv.E_BREAKDOWN
 
 
-def test_breakdown_of_the_entry_walk_pcg():
-    """dsf 2 (3 x 3 footprints): the operator is applied by walking the
-    entries (k_cg_op) and alpha taken by its own finalize (k_cg_alpha);
-    dt_alpha < 0 without the gradient term makes d.q negative on the first
+def test_breakdown_of_the_anchor_grid_pcg():
+    """dsf 2 (3 x 3 footprints): the operator is applied on the assembled
+    anchor grid (k_box_pass) and the first alpha taken by the begin step;
+    dt_alpha < 0 without the gradient term makes w.u negative on the first
     direction."""
     s = lfrv.synth(2, seed=5, w=48, h=40, dsf=2)
     print("IBR_gradient dsf 2 with dt_alpha = -1, dt_beta = 0", flush=True)
@@ -123,12 +123,16 @@ def test_iteration_cap_is_reported(small_scene):
         assert all(st["converged"]) and max(st["true_rel_residual"]) < 1e-5
 
 
-def test_zero_iteration_cap(small_scene):
+@pytest.mark.parametrize("dsf", [1, 2])
+def test_zero_iteration_cap(small_scene, dsf):
     """maxit = 0 runs no iteration: the starting point comes back unchanged
     with its own residual (recursive and true agree: r_0 = b - A x_0), the
     status says it did not converge, and the next solve in the context is
-    the same as in a fresh one."""
-    print("IBR_gradient with maxit 0 after a full solve", flush=True)
+    the same as in a fresh one.  dsf 1: the stencil path; dsf 2: the anchor
+    grid."""
+    print(f"IBR_gradient with maxit 0 after a full solve, dsf {dsf}", flush=True)
+    if dsf == 2:
+        small_scene = lfrv.synth(2, seed=5, w=48, h=40, dsf=2)
     with _ctx(small_scene) as ctx:
         ctx.ibr_gradient(1e-5, 2000)           # the context has a previous solve behind it
         ctx.clear_solution()
