@@ -38,6 +38,9 @@ TOL = 1e-5
 # launch measured with FETCH_SIZE / WRITE_SIZE (copy-calibrated)
 PROFILE = os.path.join(ROOT, "profiles", "k_cgcg_st2_gradient_ibr_9views_1024.md")
 PROFILE_C3 = os.path.join(ROOT, "profiles", "k_sr_step_st_color_sr_25views_2048.md")
+PROFILE_C1 = os.path.join(ROOT, "profiles", "k_sr_step_st_color_sr_9views_256.md")
+# the IBR_color workloads whose step kernel has a PMC record: workload -> (profile, kernel)
+COLOR_TRAFFIC = {"color_sr_c3": (PROFILE_C3, "k_sr_step_st<5>"), "color_sr_c1": (PROFILE_C1, "k_sr_step_st<3>")}
 
 
 def measured_traffic(path=PROFILE):
@@ -480,8 +483,8 @@ def bench_color_sr(args, workload):
         "roofline": {"bound": "hbm", "kernel": dom,
                      "achieved": k["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(k["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(PROFILE_C3) if (workload == "color_sr_c3" and
-                                                                 dom == f"k_sr_step_st<{ks}>") else None,
+                     "traffic": (measured_traffic(COLOR_TRAFFIC[workload][0])
+                                 if COLOR_TRAFFIC.get(workload, (None, None))[1] == dom else None),
                      "bytes_per_launch": k["bytes_per_launch"], "avg_launch_us": k["avg_launch_us"],
                      "timed_launches": k["launches"], "timing_stride": TIMING_STRIDE},
         "ms_per_step_median": round(float(np.median(per_view)), 3),
