@@ -823,7 +823,7 @@ def bench_warm_start(args):
                       "net_start_used": [r["used_net"] for r in full]},
         "training": {"seconds_data": round(t_data, 1), "seconds_train": round(t_train, 1),
                      "loss_first_epoch": float("%.4g" % losses[0]), "loss_last_epoch": float("%.4g" % losses[-1])},
-        "roofline": {"bound": "mfma", "kernel": "k_conv3x3 (implicit-GEMM 3x3 conv, bf16 MFMA 16x16x32)",
+        "roofline": {"bound": "mfma", "kernel": "k_conv3x3_halo / k_conv3x3 (implicit-GEMM 3x3 conv, bf16 MFMA 16x16x32)",
                      "achieved": round(tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 4), "traffic": None,
                      "flops_per_forward": fwd_flops, "conv_launches": conv_n},

@@ -13,10 +13,11 @@
 //
 // The convolution is an implicit GEMM: C[pixel][cout] = sum_k A[pixel][k]
 // W[cout][k] with A[p][(t, ci)] = X[p + offset(t)][ci] (0 outside the
-// image).  A block of 256 threads (4 waves) owns 64 pixels x 64 output
-// channels; per K chunk of 32 it stages the A and W tiles in LDS (one 16-byte
-// load per thread each) and each wave runs 2 x 2 v_mfma_f32_16x16x16_bf16 on
-// its 32 x 32 sub-tile.  The backward pass with respect to the input is the
+// image).  k_conv3x3 walks K by (tap, 32-channel chunk) over 128 pixels in
+// row order; k_conv3x3_halo (64 input channels and more) by 32-channel chunk
+// over a 2-D pixel tile whose halo serves all 9 taps.  Both run
+// v_mfma_f32_16x16x32_bf16 on 16-pixel x 16-channel tiles.  The backward
+// pass with respect to the input is the
 // same kernel on the output gradient with the weights flipped in space and
 // transposed in channels; the weight gradient is a second MFMA kernel whose
 // K runs over pixels.
@@ -162,6 +163,136 @@ k_conv3x3(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wt, const
                     ((uint16_t *)Y)[o] = f2bf(v);
                 }
             }
+    }
+}
+
+// The same convolution for layers with 64 or more input channels, staged by
+// chunk of 32 input channels instead of by (tap, chunk): a block owns a
+// TH x 16 pixel tile of one image and 32 WN output channels.  Per chunk the
+// (TH + 2) x 18 halo of the tile (32 channels per pixel) and the weights of
+// all 9 taps go to LDS once; the 9 taps then read the halo at their offsets,
+// so a pixel's input is fetched (TH + 2) 18 / (16 TH) times per chunk instead
+// of 9, and a wave runs 9 FM 2 MFMAs between barriers instead of FM 2.  The
+// next chunk's halo and weights are loaded into registers before the 9 taps
+// are multiplied out and stored after them (two barriers: the weights have
+// one LDS buffer, the halo two).  With a single chunk (32 input channels)
+// nothing is left to overlap, and k_conv3x3 is the faster form.
+template <int EPI, int WN, int TH>
+__global__ void __launch_bounds__(256)
+k_conv3x3_halo(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
+             const uint16_t *__restrict__ skip, void *__restrict__ Y, int B, int H, int W, int Cin, int Cout)
+{
+    constexpr int TW = 16, HX = TW + 2, HP = HX * (TH + 2);
+    constexpr int BN = 32 * WN, WM = 4 / WN, FM = TH / WM;
+    static_assert(TH % WM == 0, "tile rows split evenly over the waves");
+    constexpr int NA = (HP * 4 + 255) / 256, NB = (36 * BN + 255) / 256;
+    __shared__ __align__(16) uint16_t As[2][HP * kCvPitch];
+    __shared__ __align__(16) uint16_t Bs[9 * BN * kCvPitch];
+    const int ntx = (W + TW - 1) / TW, nty = (H + TH - 1) / TH;
+    const int img = blockIdx.x / (ntx * nty), trem = blockIdx.x - img * ntx * nty;
+    const int ty = trem / ntx, tx = trem - ty * ntx;
+    const int y0 = ty * TH, x0 = tx * TW, co0 = blockIdx.y * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, wm = wv / WN, wn = wv % WN;
+    const int ncc = Cin / kCvBK;
+    size_t aoff[NA];
+    int alds[NA];
+    bool am[NA];
+#pragma unroll
+    for (int j = 0; j < NA; j++) {
+        const int idx = tid + 256 * j, hp = idx >> 2, q = idx & 3;
+        const bool slot = idx < HP * 4;
+        const int hy = hp / HX, hx = hp - hy * HX, y = y0 - 1 + hy, x = x0 - 1 + hx;
+        am[j] = slot && y >= 0 && y < H && x >= 0 && x < W;
+        aoff[j] = am[j] ? ((size_t)(img * H + y) * W + x) * Cin + 8 * q : 0;
+        alds[j] = slot ? hp * kCvPitch + 8 * q : -1;
+    }
+    // weight slots: idx -> tap idx / (4 BN), channel row (idx / 4) % BN, 8-channel group idx % 4
+    size_t boff[NB];
+    int blds[NB];
+    bool bm[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        const int idx = tid + 256 * j, t = idx / (4 * BN), r = (idx >> 2) - t * BN, q = idx & 3;
+        const bool slot = idx < 36 * BN;
+        bm[j] = slot && co0 + r < Cout;
+        boff[j] = bm[j] ? ((size_t)(co0 + r) * 9 + t) * Cin + 8 * q : 0;
+        blds[j] = slot ? (t * BN + r) * kCvPitch + 8 * q : -1;
+    }
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    u32x4 ar[NA], br[NB];
+    auto load = [&](int cc) {
+#pragma unroll
+        for (int j = 0; j < NA; j++) ar[j] = *(const u32x4 *)(X + aoff[j] + cc * kCvBK);
+#pragma unroll
+        for (int j = 0; j < NB; j++) br[j] = *(const u32x4 *)(Wt + boff[j] + cc * kCvBK);
+    };
+    auto store = [&](int abuf) {
+#pragma unroll
+        for (int j = 0; j < NA; j++)
+            if (alds[j] >= 0) *(u32x4 *)(&As[abuf][alds[j]]) = am[j] ? ar[j] : z;
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            if (blds[j] >= 0) *(u32x4 *)(&Bs[blds[j]]) = bm[j] ? br[j] : z;
+    };
+    f4v acc[FM][2];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int cc = 0; cc < ncc; cc++) {
+        const int abuf = cc & 1;
+        if (cc + 1 < ncc) load(cc + 1);
+#pragma unroll 3
+        for (int t = 0; t < 9; t++) {
+            const int dy = t / 3, dx = t - 3 * dy;
+            bf8v a[FM], b[2];
+#pragma unroll
+            for (int i = 0; i < FM; i++) {
+                const int hp = (wm * FM + i + dy) * HX + (lane & 15) + dx;
+                a[i] = *(const bf8v *)(&As[abuf][hp * kCvPitch + 8 * (lane >> 4)]);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                b[j] = *(const bf8v *)(&Bs[(t * BN + wn * 32 + j * 16 + (lane & 15)) * kCvPitch + 8 * (lane >> 4)]);
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (cc + 1 < ncc) {
+            __syncthreads();  // every wave is done with the weights of this chunk
+            store(abuf ^ 1);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int co = co0 + wn * 32 + j * 16 + (lane & 15);
+        if (co >= Cout) continue;
+        const float bco = (EPI & kEpiBias) ? bias[co] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < FM; i++) {
+            const int y = y0 + wm * FM + i;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int x = x0 + 4 * (lane >> 4) + r;
+                if (y >= H || x >= W) continue;
+                const size_t o = ((size_t)(img * H + y) * W + x) * Cout + co;
+                float v = acc[i][j][r] + bco;
+                if (EPI & kEpiSkip) v += bf2f(skip[o]);
+                if (EPI & kEpiRelu) v = fmaxf(v, 0.0f);
+                if (EPI & kEpiF32) {
+                    float *yf = (float *)Y;
+                    yf[o] = (EPI & kEpiAccum) ? yf[o] + v : v;
+                } else {
+                    ((uint16_t *)Y)[o] = f2bf(v);
+                }
+            }
+        }
     }
 }
 
@@ -531,11 +662,17 @@ hipError_t cnn_conv_fwd(const uint16_t *X, const uint16_t *Wt, const float *bias
     const unsigned int gm = (unsigned int)(((size_t)B * H * W + kCvBM - 1) / kCvBM);
     // 64-channel blocks when there are more than 32 output channels, else 32
     const bool wide = Cout > 32;
-    const dim3 grid(gm, (unsigned int)((Cout + (wide ? 63 : 31)) / (wide ? 64 : 32)));
-#define CV(E)                                                                                       \
-    do {                                                                                            \
-        if (wide) k_conv3x3<E, 2><<<grid, 256, 0, s>>>(X, Wt, bias, skip, Y, B, H, W, Cin, Cout);   \
-        else k_conv3x3<E, 1><<<grid, 256, 0, s>>>(X, Wt, bias, skip, Y, B, H, W, Cin, Cout);        \
+    const unsigned int gn = (unsigned int)((Cout + (wide ? 63 : 31)) / (wide ? 64 : 32));
+    const dim3 grid(gm, gn);
+    // halo tiles of 8 x 16 pixels from 64 input channels up
+    const bool halo = Cin >= 2 * kCvBK;
+    const dim3 hgrid((unsigned int)((size_t)B * ((W + 15) / 16) * ((H + 7) / 8)), gn);
+#define CV(E)                                                                                                 \
+    do {                                                                                                      \
+        if (halo && wide) k_conv3x3_halo<E, 2, 8><<<hgrid, 256, 0, s>>>(X, Wt, bias, skip, Y, B, H, W, Cin, Cout); \
+        else if (halo) k_conv3x3_halo<E, 1, 8><<<hgrid, 256, 0, s>>>(X, Wt, bias, skip, Y, B, H, W, Cin, Cout); \
+        else if (wide) k_conv3x3<E, 2><<<grid, 256, 0, s>>>(X, Wt, bias, skip, Y, B, H, W, Cin, Cout);        \
+        else k_conv3x3<E, 1><<<grid, 256, 0, s>>>(X, Wt, bias, skip, Y, B, H, W, Cin, Cout);                  \
     } while (0)
     switch (epi) {
     case kEpiBias | kEpiRelu: CV(kEpiBias | kEpiRelu); break;
