@@ -277,7 +277,6 @@ __device__ __forceinline__ void st2_coeffs(const St2Coeffs &g, int W, size_t o, 
 // s_{i-1} (never written).  All global loads of the pass (the pixel's
 // vectors, its halo cell's, the stencil coefficients) are issued before the
 // one barrier.
-__device__ void cgcg_scalars(CgState &s, const double (&pv)[9], bool after_start);
 
 // The scalar state is moved through LDS by one-block kernels: its words are
 // loaded by as many threads at once, together with the partial sums, and
@@ -285,6 +284,51 @@ __device__ void cgcg_scalars(CgState &s, const double (&pv)[9], bool after_start
 // the memory latency once per field it reads after a write).
 constexpr int kCgStateWords = (int)(sizeof(CgState) / 4);
 static_assert(sizeof(CgState) % 4 == 0 && kCgStateWords <= kFinalizeThreads, "CgState moved by words");
+
+// Channel c of the scalar step between two passes (thread c of the finalize;
+// the caller sets the done flag from every channel's active flag).  After the
+// start pass: alpha_0 = gamma_0 / delta_0.  After an iteration: convergence of
+// the new residual (rho), then beta and alpha of the next iteration.
+// `pv` = gamma[3], delta[3], rho[3].
+__device__ __forceinline__ void cgcg_scalars_channel(CgState &s, const double (&pv)[9], bool after_start, int c)
+{
+    const double gam = pv[c], del = pv[3 + c];
+    if (after_start) {
+        s.rz[c] = gam;
+        s.dq[c] = del;
+        s.beta[c] = 0.0;
+        s.alpha[c] = 0.0;
+        if (s.active[c]) {
+            if (!(del > 0.0)) {
+                s.breakdown[c] = 1;
+                s.active[c] = 0;
+            } else {
+                s.alpha[c] = gam / del;
+            }
+        }
+    } else if (s.active[c]) {
+        s.iter[c] += 1;
+        s.rr[c] = pv[6 + c];
+        if (pv[6 + c] <= s.tol2 * s.bb[c] || s.iter[c] >= s.maxit) {
+            s.active[c] = 0;
+            s.alpha[c] = 0.0;
+        } else {
+            const double bt = gam / s.rz[c];
+            const double den = del - bt * gam / s.alpha[c];
+            if (!(den > 0.0)) {
+                s.breakdown[c] = 1;
+                s.active[c] = 0;
+                s.alpha[c] = 0.0;
+            } else {
+                s.beta[c] = bt;
+                s.alpha[c] = gam / den;
+                s.rz[c] = gam;
+                s.dq[c] = del;
+            }
+        }
+    }
+}
+
 
 // Scalars after iteration i: convergence on r_{i+1}.r_{i+1}, then beta and
 // alpha of the next iteration (a solve already done is left as it is).
@@ -299,8 +343,10 @@ __device__ __forceinline__ void cgcg_step_body(CgState *st, const double *partia
     if (t < kCgStateWords) reinterpret_cast<uint32_t *>(&sst)[t] = word;
     __syncthreads();
     if (sst.done) return;
+    if (t < 3) cgcg_scalars_channel(sst, v, false, t);
+    __syncthreads();
     if (t == 0) {
-        cgcg_scalars(sst, v, false);
+        sst.done = !(sst.active[0] | sst.active[1] | sst.active[2]);
         sst.first = 0;
     }
     __syncthreads();
@@ -643,78 +689,35 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_cgcg_begin(CgState *st, co
     reduce_partials<12>(partials, G, v, red);
     if (t < kCgStateWords) reinterpret_cast<uint32_t *>(&sst)[t] = word;
     __syncthreads();
-    if (t == 0) {
+    // thread c sets channel c (every thread holds the sums)
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 3; c++) any |= (v[c] > tol2 * v[6 + c]) && maxit > 0;
+    if (t < 3) {
         CgState &s = sst;
-        int any = 0;
-        for (int c = 0; c < 3; c++) {
-            s.rr[c] = v[c];
-            s.rz[c] = v[3 + c];
-            s.bb[c] = v[6 + c];
-            s.alpha[c] = 0.0;
-            s.beta[c] = 0.0;
-            s.iter[c] = 0;
-            s.breakdown[c] = 0;
-            s.active[c] = (v[c] > tol2 * v[6 + c]) && maxit > 0;
-            any |= s.active[c];
+        const int c = t;
+        s.rr[c] = v[c];
+        s.rz[c] = v[3 + c];
+        s.bb[c] = v[6 + c];
+        s.alpha[c] = 0.0;
+        s.beta[c] = 0.0;
+        s.iter[c] = 0;
+        s.breakdown[c] = 0;
+        s.active[c] = (v[c] > tol2 * v[6 + c]) && maxit > 0;
+        if (any) {
+            const double pv[9] = {v[3], v[4], v[5], v[9], v[10], v[11], v[0], v[1], v[2]};
+            cgcg_scalars_channel(s, pv, true, c);
         }
-        s.tol2 = tol2;
-        s.maxit = maxit;
-        s.first = 1;
-        s.done = !any;
-        const double pv[9] = {v[3], v[4], v[5], v[9], v[10], v[11], v[0], v[1], v[2]};
-        cgcg_scalars(s, pv, true);
+    }
+    if (t == 0) {
+        sst.tol2 = tol2;
+        sst.maxit = maxit;
+        sst.first = 1;
     }
     __syncthreads();
+    if (t == 0) sst.done = !(sst.active[0] | sst.active[1] | sst.active[2]);
+    __syncthreads();
     if (t < kCgStateWords) reinterpret_cast<uint32_t *>(st)[t] = reinterpret_cast<const uint32_t *>(&sst)[t];
-}
-
-// The scalar step between two passes (one thread).  After the start pass:
-// alpha_0 = gamma_0 / delta_0.  After an iteration: convergence of the new
-// residual (rho), then beta and alpha of the next iteration.
-// `pv` = gamma[3], delta[3], rho[3].
-__device__ void cgcg_scalars(CgState &s, const double (&pv)[9], bool after_start)
-{
-    if (s.done) return;
-    int any = 0;
-    for (int c = 0; c < 3; c++) {
-        const double gam = pv[c], del = pv[3 + c];
-        if (after_start) {
-            s.rz[c] = gam;
-            s.dq[c] = del;
-            s.beta[c] = 0.0;
-            s.alpha[c] = 0.0;
-            if (s.active[c]) {
-                if (!(del > 0.0)) {
-                    s.breakdown[c] = 1;
-                    s.active[c] = 0;
-                } else {
-                    s.alpha[c] = gam / del;
-                }
-            }
-        } else if (s.active[c]) {
-            s.iter[c] += 1;
-            s.rr[c] = pv[6 + c];
-            if (pv[6 + c] <= s.tol2 * s.bb[c] || s.iter[c] >= s.maxit) {
-                s.active[c] = 0;
-                s.alpha[c] = 0.0;
-            } else {
-                const double bt = gam / s.rz[c];
-                const double den = del - bt * gam / s.alpha[c];
-                if (!(den > 0.0)) {
-                    s.breakdown[c] = 1;
-                    s.active[c] = 0;
-                    s.alpha[c] = 0.0;
-                } else {
-                    s.beta[c] = bt;
-                    s.alpha[c] = gam / den;
-                    s.rz[c] = gam;
-                    s.dq[c] = del;
-                }
-            }
-        }
-        any |= s.active[c];
-    }
-    s.done = !any;
 }
 
 // The scalar step of one solve between two passes.

@@ -100,15 +100,37 @@ __device__ __forceinline__ bool footprint(float tx, float ty, int W, int H, int 
     return true;
 }
 
-// Wave-level FP64 sum (wave64) followed by a block-level sum through LDS.
-// Every thread must call it; the result is valid in thread 0.
+// v of the lane `SHIFT` below in the same row of 16 lanes, 0 for the first
+// SHIFT lanes of a row (DPP row_shr on both halves of the double).
+template <int SHIFT>
+__device__ __forceinline__ double dpp_row_shr(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x110 + SHIFT, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x110 + SHIFT, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Wave-level FP64 sum (wave64), fixed order.  With every lane active the rows
+// of 16 lanes are summed by DPP shifts (lane 15 of a row ends with the row's
+// sum) and the four row sums added in every lane; a partial wave (a block
+// size that is not a multiple of 64) falls back to shuffles.  The sum is valid
+// in lane 0 at least.
 __device__ __forceinline__ double wave_sum(double v)
 {
+    if (__builtin_amdgcn_read_exec() == ~0ull) {
+        v += dpp_row_shr<1>(v);
+        v += dpp_row_shr<2>(v);
+        v += dpp_row_shr<4>(v);
+        v += dpp_row_shr<8>(v);
+        return (__shfl(v, 15, 64) + __shfl(v, 31, 64)) + (__shfl(v, 47, 64) + __shfl(v, 63, 64));
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     return v;
 }
 
+// Block-level sum through LDS: every thread must call it; on return every
+// thread holds the sums (per-wave sums added by thread q in wave order).
 template <int NQ>
 __device__ __forceinline__ void block_sum(double (&v)[NQ], double *lds /* >= NQ * 16 */)
 {
@@ -120,11 +142,15 @@ __device__ __forceinline__ void block_sum(double (&v)[NQ], double *lds /* >= NQ 
         if (lane == 0) lds[q * 16 + wid] = s;
     }
     __syncthreads();
-    if (wid == 0) {
-        // the per-wave sums, summed by wave 0 (fixed order: deterministic)
-#pragma unroll
-        for (int q = 0; q < NQ; q++) v[q] = wave_sum(lane < nw ? lds[q * 16 + lane] : 0.0);
+    if (threadIdx.x < NQ) {
+        const int q = threadIdx.x;
+        double s = 0.0;
+        for (int k = 0; k < nw; k++) s += lds[q * 16 + k];
+        lds[q * 16] = s;
     }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; q++) v[q] = lds[q * 16];
     __syncthreads();
 }
 
@@ -219,7 +245,10 @@ __device__ __forceinline__ void reduce_partials(const double *partials, int G, d
     block_sum<NQ>(out, lds);
 }
 
-constexpr int kFinalizeThreads = 1024;  // threads of the one-block finalize kernels
+// Threads of the one-block finalize kernels.  A finalize after a full-GPU pass
+// is latency-bound (an empty one-block launch there takes 4.5 us on MI355X):
+// 256 threads with four strided loads each beat 1024 with one or two.
+constexpr int kFinalizeThreads = 256;
 constexpr int kSrOpsPlanes = 16;        // planes of the assembled IBR_color data term
 
 // Finalize kernel of the plain sums.
