@@ -205,11 +205,11 @@ __device__ __forceinline__ void block_partials_at(double (&v)[NQ], double *parti
 }
 
 // Deterministic sum of G block partials (layout [q][block]) by one block;
-// the result is valid in thread 0.  Up to 2 x blocks-size partials, all
+// every thread holds the result.  Up to 2 x blocks-size partials, all
 // loads of a thread are independent and in flight at once (the partials come
 // from other XCDs' L2s, so a dependent load chain would pay the full memory
-// latency per element); beyond that, four independent loads per thread and
-// value at a time.
+// latency per element); beyond that, four blocks' worth of every value per
+// round of loads.
 template <int NQ>
 __device__ __forceinline__ void reduce_partials(const double *partials, int G, double (&out)[NQ], double *lds)
 {
@@ -227,20 +227,22 @@ __device__ __forceinline__ void reduce_partials(const double *partials, int G, d
 #pragma unroll
         for (int q = 0; q < NQ; q++) out[q] += hi[q];
     } else {
-#pragma unroll 1
-        for (int q = 0; q < NQ; q++) {
-            const double *pq = partials + (size_t)q * G;
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            int b = threadIdx.x;
-            for (; b + 3 * bs < G; b += 4 * bs) {
-                a0 += pq[b];
-                a1 += pq[b + bs];
-                a2 += pq[b + 2 * bs];
-                a3 += pq[b + 3 * bs];
-            }
-            for (; b < G; b += bs) a0 += pq[b];
-            out[q] = (a0 + a1) + (a2 + a3);
+        // four blocks' worth of every quantity in flight at once
+#pragma unroll
+        for (int q = 0; q < NQ; q++) out[q] = 0.0;
+        int b = t;
+        for (; b + 3 * bs < G; b += 4 * bs) {
+            double x[4][NQ];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int q = 0; q < NQ; q++) x[k][q] = partials[(size_t)q * G + b + k * bs];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) out[q] += (x[0][q] + x[1][q]) + (x[2][q] + x[3][q]);
         }
+        for (; b < G; b += bs)
+#pragma unroll
+            for (int q = 0; q < NQ; q++) out[q] += partials[(size_t)q * G + b];
     }
     block_sum<NQ>(out, lds);
 }
