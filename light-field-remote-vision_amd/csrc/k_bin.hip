@@ -38,12 +38,17 @@ __global__ void k_cell_count(int V, int w, int h, int W, int H, int Wc, const fl
     }
 }
 
+// The fill's blocks are numbered so that each XCD (blocks are dealt to the 8
+// XCDs round-robin) takes a contiguous band of source rows: the cells its
+// entries land in are then mostly its own, and the partial cache lines of
+// the scattered entry writes are completed in one L2 instead of several.
 template <int KS>
 __global__ void k_cell_fill(int V, int w, int h, int W, int H, int Wc, const float *__restrict__ tx,
-                            const float *__restrict__ ty, const int *__restrict__ start,
-                            int *__restrict__ cursor, int *__restrict__ e_src)
+                            const float *__restrict__ ty, int *__restrict__ cursor, int *__restrict__ e_src)
 {
-    const int mx = blockIdx.x * blockDim.x + threadIdx.x, my = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nbx = (w + blockDim.x - 1) / blockDim.x, nb = nbx * ((h + blockDim.y - 1) / blockDim.y);
+    const int lb = xcd_tile(blockIdx.x, nb), bx = lb % nbx, by = lb / nbx;
+    const int mx = bx * blockDim.x + threadIdx.x, my = by * blockDim.y + threadIdx.y;
     if (mx >= w || my >= h) return;
     const size_t n = (size_t)w * h, m = (size_t)my * w + mx;
     for (int v0 = 0; v0 < V; v0 += kViewBatch) {
@@ -59,11 +64,11 @@ __global__ void k_cell_fill(int V, int w, int h, int W, int H, int Wc, const flo
         for (int b = 0; b < kViewBatch; b++) {
             const bool ok = v0 + b < V && cell_valid(t0[b], t1[b], W, H);
             c[b] = ok ? cell_of<KS>(t0[b], t1[b], Wc) : -1;
-            pos[b] = ok ? atomicAdd(cursor + c[b], 1) : 0;
+            pos[b] = ok ? atomicAdd(cursor + c[b], 1) : 0;  // the cursors start at the cell starts
         }
 #pragma unroll
         for (int b = 0; b < kViewBatch; b++)
-            if (c[b] >= 0) e_src[start[c[b]] + pos[b]] = (int)((size_t)(v0 + b) * n + m);
+            if (c[b] >= 0) e_src[pos[b]] = (int)((size_t)(v0 + b) * n + m);
     }
 }
 
@@ -177,67 +182,72 @@ __global__ void k_coverage_from_taps(int W, int H, int Wc, int ncells, const flo
     coverage[(size_t)y * W + x] = s;
 }
 
-// Device-wide exclusive scan of `in` (n ints) into `out` (n + 1 ints):
-// per-block scan with block sums, a single-block scan of the sums, offsets.
-constexpr int kScanPer = 8;  // elements per thread
+// Device-wide exclusive scan of the cell counts `in` (n ints) into the cell
+// starts `out` (n + 1 ints) in two launches: the sums of the blocks of
+// kScanBlock counts, then per block the sum of the blocks before it (at most
+// a few hundred ints, read by every block) and the scan of its own counts.
+// The fill cursors start at the cell starts (the fill's atomicAdd then
+// returns the entry's slot itself).
+constexpr int kScanPer = 8;                  // counts per thread
+constexpr int kScanBlock = 256 * kScanPer;  // counts per block
 
-__global__ void k_scan_blocks(int n, const int *__restrict__ in, int *__restrict__ out, int *__restrict__ bsum)
+__device__ __forceinline__ int wave_incl_scan(int v)
 {
-    __shared__ int tmp[256];
-    const int base = blockIdx.x * 256 * kScanPer + threadIdx.x * kScanPer;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_scan_reduce(int n, const int *__restrict__ in, int *__restrict__ bsum)
+{
+    __shared__ int ws[4];
+    const int base = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanPer; i++) s += (base + i < n) ? in[base + i] : 0;
+    s = wave_incl_scan(s);
+    if ((threadIdx.x & 63) == 63) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+__global__ void __launch_bounds__(256) k_scan_apply(int n, const int *__restrict__ in, const int *__restrict__ bsum,
+                                                    int *__restrict__ out, int *__restrict__ cursor)
+{
+    __shared__ int ws[4], wo[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // this block's offset: the sums of the blocks before it
+    int o = 0;
+    for (int i = threadIdx.x; i < (int)blockIdx.x; i += 256) o += bsum[i];
+    const int base = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
     int v[kScanPer], s = 0;
+#pragma unroll
     for (int i = 0; i < kScanPer; i++) {
         v[i] = (base + i < n) ? in[base + i] : 0;
         s += v[i];
     }
-    tmp[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const int t = (threadIdx.x >= off) ? tmp[threadIdx.x - off] : 0;
-        __syncthreads();
-        tmp[threadIdx.x] += t;
-        __syncthreads();
+    o = wave_incl_scan(o);
+    const int incl = wave_incl_scan(s);
+    if (lane == 63) {
+        wo[wid] = o;
+        ws[wid] = incl;
     }
-    int run = tmp[threadIdx.x] - s;
+    __syncthreads();
+    int run = (wo[0] + wo[1]) + (wo[2] + wo[3]) + incl - s;
+    for (int w = 0; w < wid; w++) run += ws[w];
+#pragma unroll
     for (int i = 0; i < kScanPer; i++) {
-        if (base + i < n) out[base + i] = run;
+        if (base + i < n) {
+            out[base + i] = run;
+            cursor[base + i] = run;
+        }
         run += v[i];
     }
-    if (threadIdx.x == 255) bsum[blockIdx.x] = tmp[255];
-}
-
-__global__ void k_scan_sums(int nb, int *__restrict__ bsum, int *__restrict__ total_out)
-{
-    __shared__ int tmp[1024];
-    const int per = (nb + 1023) / 1024;
-    const int b0 = threadIdx.x * per, b1 = min(b0 + per, nb);
-    int s = 0;
-    for (int i = b0; i < b1; i++) s += bsum[i];
-    tmp[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int t = (threadIdx.x >= off) ? tmp[threadIdx.x - off] : 0;
-        __syncthreads();
-        tmp[threadIdx.x] += t;
-        __syncthreads();
-    }
-    int run = tmp[threadIdx.x] - s;
-    for (int i = b0; i < b1; i++) {
-        const int c = bsum[i];
-        bsum[i] = run;
-        run += c;
-    }
-    if (threadIdx.x == 1023) *total_out = tmp[1023];
-}
-
-// Adds the block offsets; also clears the fill cursors of the n cells.
-__global__ void k_scan_add(int n, int *__restrict__ out, const int *__restrict__ bsum, int *__restrict__ cursor)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        out[i] += bsum[i / (256 * kScanPer)];
-        cursor[i] = 0;
-    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) out[n] = run;
 }
 
 // The payload of the IBR_color data term for the first *total entries: the
@@ -734,11 +744,10 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
 #undef CC
         ctx->counts_valid = true;
     }
-    const int nb = (nc + 256 * kScanPer - 1) / (256 * kScanPer);
-    k_scan_blocks<<<nb, 256, 0, ctx->stream>>>(nc, ctx->cell_count, ctx->cell_start, ctx->scan_tmp);
-    k_scan_sums<<<1, 1024, 0, ctx->stream>>>(nb, ctx->scan_tmp, ctx->cell_start + nc);
-    k_scan_add<<<blocks_for(nc), kBlock, 0, ctx->stream>>>(nc, ctx->cell_start, ctx->scan_tmp, ctx->cell_cursor);
-#define CF(K) k_cell_fill<K><<<bgrid, bblock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_start, ctx->cell_cursor, ctx->e_src)
+    const int nb = (nc + kScanBlock - 1) / kScanBlock;
+    k_scan_reduce<<<nb, 256, 0, ctx->stream>>>(nc, ctx->cell_count, ctx->scan_tmp);
+    k_scan_apply<<<nb, 256, 0, ctx->stream>>>(nc, ctx->cell_count, ctx->scan_tmp, ctx->cell_start, ctx->cell_cursor);
+#define CF(K) k_cell_fill<K><<<bgrid.x * bgrid.y, bblock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, g.Wc, ctx->tau_x, ctx->tau_y, ctx->cell_cursor, ctx->e_src)
     LFRV_KS_DISPATCH(ctx->ks, CF);
 #undef CF
     // dsf = 1: the sort also forms the per-cell tap sums of the coverage
