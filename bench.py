@@ -839,9 +839,9 @@ def c2_step_bytes(N, Vn, e, Nc, iters):
     pixels of all views, e binned entries, Nc footprint cells)."""
     return {
         "k_fill_key + k_key_to_depth": 8 * N + 12 * N,          # z-buffer keys set, read, depth written
-        "k_tau_from_depth": 20 * Vn,                             # depth read; tau, sigma dtau/dz written
+        "k_tau_from_depth": 12 * Vn,                             # depth read; tau written (dtau/dz formed when read)
         "k_tau_visibility": 8 * Vn + 4 * N,                      # tau read, target depth read; counts
-        "scan (3 kernels)": 12 * Nc,                             # counts read, starts and cursors written
+        "scan (2 kernels)": 12 * Nc,                             # counts read, starts and cursors written
         "k_cell_fill": 8 * Vn + 4 * e + 4 * Nc,                  # tau read, entry indices written, cursors
         "k_cell_sort": 4 * e + 12 * e + 16 * Nc,                 # indices read; sorted indices and tau written; tap sums
         "k_coverage_from_taps": 16 * Nc + 4 * N,

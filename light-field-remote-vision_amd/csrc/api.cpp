@@ -344,7 +344,8 @@ extern "C" lfrv_status lfrv_set_partial_tau(lfrv_ctx *ctx, int i, const float *s
     if (!ctx || i < 0 || i >= ctx->V) return ctx ? fail(ctx, LFRV_E_ARG, "view index out of range") : LFRV_E_ARG;
     const size_t n = ctx->n;
     float *dx = ctx->dpart_x + (size_t)i * n, *dy = ctx->dpart_y + (size_t)i * n;
-    lfrv_status st = copy_in(ctx, ctx->tmpN, sigma_z, n);
+    lfrv_status st = materialize_dpart(ctx);  // the other views keep theirs
+    if (st == LFRV_OK) st = copy_in(ctx, ctx->tmpN, sigma_z, n);
     if (st == LFRV_OK) st = copy_in(ctx, dx, dtdz_x, n);
     if (st == LFRV_OK) st = copy_in(ctx, dy, dtdz_y, n);
     if (st != LFRV_OK) return st;
@@ -358,7 +359,9 @@ extern "C" lfrv_status lfrv_set_partial_tau(lfrv_ctx *ctx, int i, const float *s
 extern "C" lfrv_status lfrv_set_depth(lfrv_ctx *ctx, int i, const float *depth, const double *P)
 {
     if (!ctx || i < 0 || i >= ctx->V || !P) return ctx ? fail(ctx, LFRV_E_ARG, "bad lfrv_set_depth arguments") : LFRV_E_ARG;
-    lfrv_status st = copy_in(ctx, ctx->depth + (size_t)i * ctx->n, depth, ctx->n);
+    // dpart still owed for the current warps is formed from the depth they came from
+    lfrv_status st = materialize_dpart(ctx);
+    if (st == LFRV_OK) st = copy_in(ctx, ctx->depth + (size_t)i * ctx->n, depth, ctx->n);
     if (st != LFRV_OK) return st;
     std::memcpy(ctx->P_views.data() + 12 * i, P, 12 * sizeof(double));
     ctx->have_depth[i] = 1;
@@ -627,6 +630,10 @@ static float *buffer_ptr(lfrv_ctx *ctx, int which, size_t *count)
 extern "C" lfrv_status lfrv_get_buffer(lfrv_ctx *ctx, int which, float *out, size_t *count)
 {
     if (!ctx || !count) return LFRV_E_ARG;
+    if (which == LFRV_BUF_DPART_X || which == LFRV_BUF_DPART_Y) {
+        const lfrv_status ds = materialize_dpart(ctx);
+        if (ds != LFRV_OK) return ds;
+    }
     float *p = buffer_ptr(ctx, which, count);
     if (!p) return fail(ctx, LFRV_E_ARG, "unknown buffer");
     if (!out) return LFRV_OK;
@@ -638,6 +645,10 @@ extern "C" lfrv_status lfrv_get_buffer(lfrv_ctx *ctx, int which, float *out, siz
 extern "C" lfrv_status lfrv_set_buffer(lfrv_ctx *ctx, int which, const float *in)
 {
     if (!ctx) return LFRV_E_ARG;
+    if (which == LFRV_BUF_DPART_X || which == LFRV_BUF_DPART_Y) {
+        const lfrv_status ds = materialize_dpart(ctx);  // the other plane stays valid
+        if (ds != LFRV_OK) return ds;
+    }
     size_t count = 0;
     float *p = buffer_ptr(ctx, which, &count);
     if (!p) return fail(ctx, LFRV_E_ARG, "unknown buffer");

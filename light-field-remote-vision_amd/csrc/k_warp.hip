@@ -57,8 +57,10 @@ __global__ void k_tau_from_depth(int V, int w, int h, int dsf, const float *__re
         if (d < 1.0f / kInvalidDepth || kInvalidDepth < d) {
             tx[k] = -(float)w;
             ty[k] = -(float)h;
-            dpx[k] = 0.0f;
-            dpy[k] = 0.0f;
+            if (dpx) {
+                dpx[k] = 0.0f;
+                dpy[k] = 0.0f;
+            }
         } else {
             const float xx = (float)mx + 0.5f, yy = (float)my + 0.5f;
             const float Mx0 = cam.M[0] * xx + cam.M[1] * yy + cam.M[2];
@@ -69,8 +71,10 @@ __global__ void k_tau_from_depth(int V, int w, int h, int dsf, const float *__re
             tuy = X1 / X2;
             tx[k] = tux;
             ty[k] = tuy;
-            dpx[k] = sigma_z * (Mx0 - tux * Mx2) / X2;
-            dpy[k] = sigma_z * (Mx1 - tuy * Mx2) / X2;
+            if (dpx) {
+                dpx[k] = sigma_z * (Mx0 - tux * Mx2) / X2;
+                dpy[k] = sigma_z * (Mx1 - tuy * Mx2) / X2;
+            }
             key = f2key(X2);
             live = 1;
         }
@@ -97,6 +101,34 @@ __global__ void k_tau_from_depth(int V, int w, int h, int dsf, const float *__re
             if (0 <= py && py <= H - 1) atomicMin(zkey + (size_t)py * W + px, gk);
         }
     }
+}
+
+// sigma_z dtau/dz of the import warps (the dpart k_tau_from_depth leaves out
+// when no reader needs them yet): the same arithmetic on the same depth and
+// cameras, so the values are those it would have written.
+__global__ void k_dpart_from_depth(int V, int w, int h, const float *__restrict__ depth,
+                                   const WarpCam *__restrict__ cams, float sigma_z, float *__restrict__ dpx,
+                                   float *__restrict__ dpy)
+{
+    const int mx = blockIdx.x * blockDim.x + threadIdx.x, my = blockIdx.y * blockDim.y + threadIdx.y;
+    const int v = blockIdx.z;
+    if (mx >= w || my >= h) return;
+    const size_t k = (size_t)v * w * h + (size_t)my * w + mx;
+    const WarpCam &cam = cams[v];
+    const float d = depth[k];
+    if (d < 1.0f / kInvalidDepth || kInvalidDepth < d) {
+        dpx[k] = 0.0f;
+        dpy[k] = 0.0f;
+        return;
+    }
+    const float xx = (float)mx + 0.5f, yy = (float)my + 0.5f;
+    const float Mx0 = cam.M[0] * xx + cam.M[1] * yy + cam.M[2];
+    const float Mx1 = cam.M[3] * xx + cam.M[4] * yy + cam.M[5];
+    const float Mx2 = cam.M[6] * xx + cam.M[7] * yy + cam.M[8];
+    const float X0 = d * Mx0 + cam.c[0], X1 = d * Mx1 + cam.c[1], X2 = d * Mx2 + cam.c[2];
+    const float tux = X0 / X2, tuy = X1 / X2;
+    dpx[k] = sigma_z * (Mx0 - tux * Mx2) / X2;
+    dpy[k] = sigma_z * (Mx1 - tuy * Mx2) / X2;
 }
 
 // Fills the z-buffer keys; also clears the ncnt cell counters.
@@ -821,9 +853,13 @@ lfrv_status run_warps_from_depth(lfrv_ctx *ctx)
     const size_t ncnt = (size_t)ctx->ncells + 1;
     k_fill_key<<<blocks_for(std::max(ctx->N, ncnt)), kBlock, 0, ctx->stream>>>(zkey, ctx->N, inv_key, ctx->cell_count,
                                                                               ncnt);
+    // IBR_gradient's first weights take the zero-gradient form, which does not
+    // read dpart: it is formed later, when something reads it
+    const bool lazy = ctx->p.mode == LFRV_MODE_GRADIENT_IBR;
     k_tau_from_depth<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
         ctx->V, ctx->w, ctx->h, ctx->dsf, ctx->depth, dcams, ctx->p.sigma_z, ctx->tau_x, ctx->tau_y,
-        ctx->dpart_x, ctx->dpart_y, zkey);
+        lazy ? nullptr : ctx->dpart_x, lazy ? nullptr : ctx->dpart_y, zkey);
+    ctx->dpart_pending = lazy;
     k_key_to_depth<<<blocks_for(ctx->N), kBlock, 0, ctx->stream>>>(zkey, ctx->target_depth, ctx->N);
 #define TV(K) k_tau_visibility<K><<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(ctx->V, ctx->w, ctx->h, ctx->target_depth, dcams, ctx->p.occlusion_tol_px, ctx->tau_x, ctx->tau_y, ctx->cell_count)
     switch (ctx->ks) {
@@ -872,9 +908,20 @@ lfrv_status run_warps_from_target_depth(lfrv_ctx *ctx, int skip)
         ctx->V, ctx->w, ctx->h, ctx->W, ctx->H, ctx->depth, ctx->depth_grad, ctx->dcams, ctx->target_depth,
         ctx->p.sigma_z, ctx->tau_x, ctx->tau_y, ctx->tau_weight, ctx->dpart_x, ctx->dpart_y);
     LFRV_LAUNCH(ctx, "warps_from_target_depth kernels");
+    ctx->dpart_pending = false;
     ctx->prepared = false;
     ctx->epoch++;
     ctx->counts_valid = false;  // the splat structure counts its cells itself
+    return LFRV_OK;
+}
+
+lfrv_status materialize_dpart(lfrv_ctx *ctx)
+{
+    if (!ctx->dpart_pending) return LFRV_OK;
+    k_dpart_from_depth<<<source_grid(ctx), kSourceBlock, 0, ctx->stream>>>(
+        ctx->V, ctx->w, ctx->h, ctx->depth, ctx->dcams, ctx->p.sigma_z, ctx->dpart_x, ctx->dpart_y);
+    LFRV_LAUNCH(ctx, "dpart from depth");
+    ctx->dpart_pending = false;
     return LFRV_OK;
 }
 
@@ -917,6 +964,10 @@ lfrv_status run_compute_weights(lfrv_ctx *ctx, hipStream_t st)
         gys = ctx->u_grad_y;
     }
     const bool zero_grad = ctx->p.mode == LFRV_MODE_GRADIENT_IBR && ctx->u_zero;
+    if (!zero_grad) {
+        const lfrv_status ds = materialize_dpart(ctx);
+        if (ds != LFRV_OK) return ds;
+    }
     switch (ctx->ks) {
     case 2: launch_weights_ks<2>(ctx, gxs, gys, zero_grad, st); break;
     case 3: launch_weights_ks<3>(ctx, gxs, gys, zero_grad, st); break;

@@ -378,6 +378,10 @@ struct lfrv_ctx {
     bool prepared = false;
     bool entry_weights_valid = false;
     bool counts_valid = false;  // cell_count holds the cell counts of the current warps
+    // dpart_x / dpart_y not yet written for the current import warps: IBR_gradient's
+    // first weights do not read them, so they are formed from the depth when a
+    // reader asks (materialize_dpart)
+    bool dpart_pending = false;
     uint64_t epoch = 1;        // bumped whenever entries, weights or views change
     uint64_t setup_epoch = 0;  // epoch of the last gradient-IBR set-up pull
     float *cache_num = nullptr, *cache_sq = nullptr;  // its raw sums: 3N and N
@@ -497,6 +501,7 @@ lfrv_status run_refresh_entry_weights(lfrv_ctx *ctx, float scale);
 lfrv_status run_visibility(lfrv_ctx *ctx);
 // s: the stream to launch on (nullptr: the context's own)
 lfrv_status run_compute_weights(lfrv_ctx *ctx, hipStream_t s = nullptr);
+lfrv_status materialize_dpart(lfrv_ctx *ctx);
 lfrv_status run_initial_image(lfrv_ctx *ctx);
 lfrv_status run_hole_filling(lfrv_ctx *ctx, int iters, double eps, int *ran);
 lfrv_status run_push_pull(lfrv_ctx *ctx, int levels);
