@@ -380,9 +380,35 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
         }
         __syncthreads();
     }
-    // D x D box sums, separably: the row sums of U (rows 1.. of the U tile) and
-    // lap U (rows 0..) into sb, then the column sums in registers, written
-    // over them once every thread has read its column
+    // D x D box sums.  D <= 2: each box point reads its D x D values directly
+    // (as many reads as the separable form, one barrier instead of three; the
+    // sums are added in the separable form's order).  Larger D: separably,
+    // the row sums of U (rows 1.. of the U tile) and lap U (rows 0..) into sb,
+    // then the column sums in registers, written over them once every thread
+    // has read its column.
+    if constexpr (D <= 2) {
+        for (int i = tid; i < BH * P; i += NT) {
+            if (i % P >= BW) continue;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float su = 0.0f, sl = 0.0f;
+#pragma unroll
+                for (int l = 0; l < D; l++) {
+                    float ru = 0.0f, rl = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < D; k++) {
+                        ru += us[c * SH * P + i + (l + 1) * P + k + 1];
+                        if (grad) rl += ls[c * LH * P + i + l * P + k];
+                    }
+                    su += ru;
+                    sl += rl;
+                }
+                bus[c * BH * P + i] = su;
+                if (grad) bls[c * BH * P + i] = sl;
+            }
+        }
+        __syncthreads();
+    } else {
     constexpr int HR = S::HR, NBT = (BH * P + NT - 1) / NT;
     float *hsu = sb, *hsl = sb + 3 * HR * P;
     for (int i = tid; i < HR * P; i += NT) {
@@ -427,6 +453,7 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
         }
     }
     __syncthreads();
+    }
 
     // ---- anchor residuals ----
 #pragma unroll
@@ -445,7 +472,9 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
         }
     }
     __syncthreads();
-    // rows: r(x, qy) = sum_k q(x + k, qy), stored with pitch TW for the pixel pass
+    // rows: r(x, qy) = sum_k q(x + k, qy), stored with pitch TW for the pixel
+    // pass (D <= 2: the pixel pass reads its D x D residuals directly)
+    if constexpr (D > 2) {
     for (int i = tid; i < QH * P; i += NT) {
         const int lx = i % P, qy = i / P;
         if (lx >= TW) continue;
@@ -459,6 +488,7 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
         }
     }
     __syncthreads();
+    }
 
     // ---- pixel: columns of the row sums, then the step of vtv_sr_ista_step_unstructured ----
     const int lx = tid % TW, ly = tid / TW;
@@ -466,9 +496,20 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
 #pragma unroll
     for (int c = 0; c < 6; c++) {
         float sm = 0.0f;
-        if (c < 3 || grad)
+        if (c < 3 || grad) {
+            if constexpr (D <= 2) {
 #pragma unroll
-            for (int l = 0; l < D; l++) sm += rs[(c * QH + ly + l) * TW + lx];
+                for (int l = 0; l < D; l++) {
+                    float row = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < D; k++) row += qs[c * NQP + (ly + l) * P + lx + k];
+                    sm += row;
+                }
+            } else {
+#pragma unroll
+                for (int l = 0; l < D; l++) sm += rs[(c * QH + ly + l) * TW + lx];
+            }
+        }
         acc[c] = sm;
     }
 #pragma unroll
