@@ -266,8 +266,9 @@ struct StDims {
 };
 
 // The data term of one 32 x 16 tile.  Every thread of the block calls it; on
-// return (the LDS free again) each thread holds F (the clamped gradient step)
-// and U at its pixel, and the pixel's coverage.  With ur, the U values of
+// return each thread holds F (the clamped gradient step) and U at its pixel,
+// and the pixel's coverage; sb is free for the caller at once, sa only after
+// a barrier.  With ur, the U values of
 // [tx0-1, tx0+TW] x [ty0-1, ty0+TH] (3 planes, pitch TW+2) are also left there.
 template <int KS>
 __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, int tx0, int ty0, const DataTermArgs &da,
@@ -533,7 +534,8 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
         G += up[c];
         F[c] = fminf(1.0f, fmaxf(0.0f, G));
     }
-    __syncthreads();  // the LDS is the caller's again
+    // no barrier here: sb is free since the residual phase's barrier, while
+    // other threads may still read sa
 }
 
 template <int KS>
@@ -590,7 +592,7 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_step_st(int W, int H, int 
     constexpr int PW = TW + 1, PH = TH + 1;  // p region [x0-1, x0+TW-1] x [y0-1, y0+TH-1]
     constexpr int NPP = PH * UW;             // p slots with the U region's pitch (x < PW used)
     constexpr int NPT = (NPP + NT - 1) / NT;
-    static_assert(6 * NPP <= StDims<KS>::AREA, "primal-dual tiles exceed the data-term LDS");
+    static_assert(6 * NPP <= StDims<KS>::BAREA, "primal-dual tiles exceed the data term's second LDS region");
     static_assert(UW == kStTW + 2 && UH == kStTH + 2, "the ring the data term leaves");
     __shared__ float sa[StDims<KS>::AREA];
     __shared__ float sb[StDims<KS>::BAREA];
@@ -619,7 +621,7 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_step_st(int W, int H, int 
     sr_dataterm_tile<KS>(W, H, Wc, Hc, tx0, ty0, da, U, ops, NA, coverage, sa, sb, F, up, cov, us);
 
     // ---- the primal-dual iteration (vtv_rof.cu:118-147, 280-321; cuda_kernels.cu:504-532) ----
-    float *ps = sa;
+    float *ps = sb;  // free once the data term returns (sa may still be read)
 #pragma unroll
     for (int k = 0; k < NPT; k++) {
         const int i = tid + k * NT;
