@@ -240,11 +240,14 @@ def host_driven_sr_loop(ctx, W, H, niter, nmeta):
     return steps
 
 
-def oracle_sr_view(scene, prm, steps, taus=None, budget_s=None):
+def oracle_sr_view(scene, prm, steps, taus=None, budget_s=None, vis_ties=None):
     """One IBR_color view by the FP64 oracle: warps (its own, or `taus` =
     (tx, ty, dpx, dpy)), weights, visibility, push-pull initial image, TV
     weight, then FISTA steps.  With `budget_s` it stops after the first two
-    steps when the rest would not fit in that many seconds."""
+    steps when the rest would not fit in that many seconds.  `vis_ties` (the
+    GPU's visibility) decides the pixels whose coverage is 1 to within 1e-5:
+    the rule coverage >= 1 is a tie there, which FP32 and FP64 sums of the
+    same footprint weights break either way."""
     oracle = _oracle()
     s = scene
     dsf, W, H = s["dsf"], s["W"], s["H"]
@@ -260,6 +263,9 @@ def oracle_sr_view(scene, prm, steps, taus=None, budget_s=None):
     st.PX[0], st.PY[0] = gx, gy
     cov = oracle.coverage(W, H, dsf, tx, ty)
     vis = (cov >= 1.0).astype(np.uint8)
+    ties = np.abs(cov - 1.0) < 1e-5
+    if vis_ties is not None:
+        vis = np.where(ties, vis_ties.astype(np.uint8), vis)
     st.norm_mask += cov
     U0, st.norm_mask = oracle.initial_image(W, H, dsf, tx, ty, s["views"], om)
     st.U = oracle.push_pull(U0, vis)
@@ -280,7 +286,7 @@ def oracle_sr_view(scene, prm, steps, taus=None, budget_s=None):
     t_steps = time.perf_counter() - t1
     e, _ = oracle.sr_energy(W, H, dsf, tx, ty, s["views"], om, sig, prm.lambda_, prm.dt_alpha, prm.dt_beta,
                             prm.gw_type, vis, st.U)
-    return dict(U=st.U, energy=e, steps=done, t_setup=t_setup, t_steps=t_steps)
+    return dict(U=st.U, energy=e, steps=done, t_setup=t_setup, t_steps=t_steps, ties=int(ties.sum()))
 
 
 def cpu_baseline_sr(scene, prm, steps, threads, budget_s=30.0):
@@ -304,19 +310,20 @@ def cpu_baseline_sr(scene, prm, steps, threads, budget_s=30.0):
                 cpu=cpu_model(), nproc=os.cpu_count())
 
 
-def error_vs_oracle_sr(scene, prm, taus, steps, U_gpu, e_gpu, psnr_gpu, threads):
+def error_vs_oracle_sr(scene, prm, taus, steps, U_gpu, e_gpu, psnr_gpu, threads, vis_gpu):
     """The GPU view after `steps` FISTA steps against the FP64 oracle's, on
-    the GPU's warps."""
+    the GPU's warps (and its visibility where coverage ties with 1)."""
     oracle = _oracle()
     oracle.set_threads(threads)
-    r = oracle_sr_view(scene, prm, steps, taus=taus)
+    r = oracle_sr_view(scene, prm, steps, taus=taus, vis_ties=vis_gpu)
     U = U_gpu.astype(np.float64)
     p_orc = oracle.psnr(r["U"], scene["target"])
     return dict(steps_compared=steps,
                 rel_err_vs_oracle=float("%.3g" % (np.linalg.norm(U - r["U"]) / np.linalg.norm(r["U"]))),
                 max_abs_err_vs_oracle=float("%.3g" % np.max(np.abs(U - r["U"]))),
                 rel_energy_err=float("%.3g" % (abs(e_gpu - r["energy"]) / abs(r["energy"]))),
-                dpsnr_vs_oracle_db=round(psnr_gpu - p_orc, 5), oracle_psnr_db=round(p_orc, 3))
+                dpsnr_vs_oracle_db=round(psnr_gpu - p_orc, 5), oracle_psnr_db=round(p_orc, 3),
+                visibility_ties=r["ties"])
 
 
 def calib_copy(args):
@@ -516,9 +523,10 @@ def bench_color_sr(args, workload):
             eg, _ = ctx.sr_primal_energy()
             shp = (scene["V"], scene["h"], scene["w"])
             taus = tuple(ctx.get_buffer(b).reshape(shp) for b in ("TAU_X", "TAU_Y", "DPART_X", "DPART_Y"))
+            vis_g = ctx.get_buffer("VISIBILITY").reshape(scene["H"], scene["W"])
             ctx.close()
             line.update(error_vs_oracle_sr(scene, prm, taus, cmp_steps, Ug, eg, lfrv.psnr(Ug, scene["target"]),
-                                           threads))
+                                           threads, vis_g))
         line["cpu_baseline"] = cpu_baseline_sr(scene, prm, irls[-1]["fista_steps"] if nmeta else steps, threads)
     print(json.dumps(line))
 

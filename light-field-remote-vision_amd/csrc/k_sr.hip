@@ -257,10 +257,14 @@ struct StDims {
     static constexpr int LW = BW + D - 1, LH = BH + D - 1;  // pixels tx0 - D + lx (box sums' support)
     static constexpr int SW = LW + 2, SH = LH + 2;          // U from tx0 - D - 1 (the Laplacian's ring)
     static constexpr int P = SW;                            // the common row pitch
-    static constexpr int A1 = 3 * SH * P + 3 * LH * P, A2 = 6 * QH * P + 6 * QH * TW;
-    static constexpr int AREA = A1 > A2 ? A1 : A2;  // sa: U and lap U, later the anchor residuals and their row sums
-    static constexpr int HR = BH + D - 1;           // rows of the horizontal box sums
-    static constexpr int BAREA = 6 * HR * P;        // sb: row sums of U, lap U, then BU (3) and BL (3)
+    // The two LDS regions take turns: sa holds U and lap U, sb their row sums
+    // (D > 2); then the box sums BU, BL go to the region nobody reads any more
+    // (sa for D > 2, sb for D <= 2), the anchor residuals to the other one, and
+    // (D > 2) the residuals' row sums to sa again.
+    static constexpr int A1 = 3 * SH * P + 3 * LH * P, A2 = 6 * QH * P;
+    static constexpr int AREA = A1 > A2 ? A1 : A2;
+    static constexpr int HR = BH + D - 1;     // rows of the horizontal box sums
+    static constexpr int BAREA = 6 * HR * P;  // sb
     static constexpr int NQP = QH * P;              // anchor slots (qx < QW used)
     static constexpr int NQT = (NQP + NT - 1) / NT, NST = (SH * P + NT - 1) / NT;
 };
@@ -279,9 +283,14 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
     using S = StDims<KS>;
     constexpr int D = S::D, TW = S::TW, NT = S::NT, QW = S::QW, QH = S::QH, BW = S::BW, BH = S::BH, LW = S::LW,
                   LH = S::LH, SH = S::SH, P = S::P, NQP = S::NQP, NQT = S::NQT, NST = S::NST;
+    constexpr bool SEP = D > 2;  // separable box sums
+    static_assert(6 * BH * P <= (SEP ? S::AREA : S::BAREA) && 6 * NQP <= (SEP ? S::BAREA : S::AREA) &&
+                      6 * QH * TW <= S::AREA,
+                  "the box sums, residuals and row sums fit their LDS regions");
     float *us = sa, *ls = sa + 3 * SH * P;
-    float *qs = sa, *rs = sa + 6 * NQP;
-    float *bus = sb, *bls = sb + 3 * BH * P;
+    float *bus = SEP ? sa : sb, *bls = bus + 3 * BH * P;
+    float *qs = SEP ? sb : sa;
+    float *rs = sa;  // D > 2: the residuals' row sums, once BU and BL are read
     const size_t N = (size_t)W * H;
     const bool grad = da.beta > 0.0f;
     const float *g9 = da.gw_type ? ops : ops + 8 * NA;  // couplings of the gradient term
@@ -385,8 +394,7 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
     // (as many reads as the separable form, one barrier instead of three; the
     // sums are added in the separable form's order).  Larger D: separably,
     // the row sums of U (rows 1.. of the U tile) and lap U (rows 0..) into sb,
-    // then the column sums in registers, written over them once every thread
-    // has read its column.
+    // then their column sums into sa, which nothing reads any more.
     if constexpr (D <= 2) {
         for (int i = tid; i < BH * P; i += NT) {
             if (i % P >= BW) continue;
@@ -442,7 +450,7 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
                 if (grad) bsum[k][3 + c] += hsl[c * HR * P + i + l * P];
             }
     }
-    __syncthreads();
+    // BU and BL go to sa, which the row sums no longer read: no barrier here
 #pragma unroll
     for (int k = 0; k < NBT; k++) {
         const int i = tid + k * NT;
