@@ -35,6 +35,43 @@ def test_warps_from_depth_matches_oracle(small_scene):
     assert np.mean(np.abs(zb[fin] - oz[fin]) > 1e-3) < 2e-3
 
 
+def test_depth_partials_formed_when_read(small_scene):
+    """IBR_gradient's import warps leave sigma dtau/dz to be formed when read:
+    the planes read back are those of the warps' own depth even after a view's
+    depth changes (formed before the change), equal to an IBR_color context's,
+    which writes them with the warps; the weights that read them (a non-zero
+    solution) match the eager context's too."""
+    s = small_scene
+    V, h, w = s["V"], s["h"], s["w"]
+    print("depth partials: lazy (IBR_gradient) against eager (IBR_color)", flush=True)
+    with _ctx(s) as ctx:
+        ctx.warps_from_depth()
+        ctx.set_depth(0, s["depths"][0] * 1.5, s["P_views"][0])
+        dpx = ctx.get_buffer("DPART_X").reshape(V, h, w)
+        dpy = ctx.get_buffer("DPART_Y").reshape(V, h, w)
+    prm = pipelines.color_sr_params(s)
+    with lfrv.Context(prm) as ctx:
+        pipelines.load_scene(ctx, s)
+        ctx.warps_from_depth()
+        ex = ctx.get_buffer("DPART_X").reshape(V, h, w)
+        ey = ctx.get_buffer("DPART_Y").reshape(V, h, w)
+    assert np.array_equal(dpx, ex) and np.array_equal(dpy, ey)
+    assert np.abs(ex).max() > 0.0
+    # the weights of a non-zero solution read the planes: formed on that call
+    U = np.clip(s["target"] + 0.05, 0.0, 1.0).astype(np.float32)
+    om = []
+    for lazy in (True, False):
+        with _ctx(s) as ctx:
+            ctx.warps_from_depth()
+            if not lazy:
+                ctx.get_buffer("DPART_X")  # formed before the weights ask
+            ctx.prepare()
+            ctx.set_solution(U)
+            ctx.compute_weights()
+            om.append(ctx.get_buffer("OMEGA"))
+    assert np.array_equal(om[0], om[1])
+
+
 @pytest.fixture(scope="module", params=["dsf1", "dsf2", "dsf3", "dsf4"])
 def stage_scene(request, small_scene):
     if request.param == "dsf1":
