@@ -132,7 +132,11 @@ lfrv_status lfrv_set_depth_gradient(lfrv_ctx *ctx, int i, const float *dzdx, con
 /* tau and sigma_z*dtau/dz of every view from its depth map and the cameras,
  * with target z-buffer visibility.  Replaces import_warps_from_depth
  * (import.cpp:243-450) plus the partial-tau producer (import.cpp:658-680,
- * tauPartial.frag:23-47); runs on the GPU. */
+ * tauPartial.frag:23-47); runs on the GPU.  In LFRV_MODE_GRADIENT_IBR the
+ * sigma_z*dtau/dz planes are formed from the same depth and cameras when first
+ * read (lfrv_get_buffer, weights of a non-zero solution) or before an input they
+ * depend on changes (lfrv_set_depth, lfrv_set_partial_tau, lfrv_set_buffer):
+ * the values are the same, the first weights of a solve do not read them. */
 lfrv_status lfrv_warps_from_depth(lfrv_ctx *ctx);
 
 /* The OpenGL warp producer (the reference's fromMVE2coco path, dsf = 1 there),
