@@ -73,13 +73,24 @@ __global__ void k_cell_fill(int V, int w, int h, int W, int H, int Wc, const flo
 }
 
 // Order every cell's entries by source index and gather their tau.  A block
-// owns kSortCells consecutive cells, whose entries are one contiguous run:
-// the run is staged in LDS with coalesced loads, each thread insertion-sorts
-// its cell's segment there (cells hold a handful of entries), and the sorted
-// run is written back coalesced.  Runs longer than kSortCap are sorted in
-// place in global memory.
+// of kSortCells threads owns `cpb` (<= kSortCells) consecutive cells, whose
+// entries are one contiguous run: the run is staged in LDS with coalesced
+// loads, each owner thread insertion-sorts its cell's segment there (cells
+// hold a handful of entries), and the sorted run is written back coalesced.
+// The host picks cpb so that the expected run (cpb times the mean entries per
+// cell, with a margin) fits kSortCap: with many views per cell (dsf 1, 25
+// views: 25 entries per cell) a block of 256 cells would overflow the stage.
+// Runs longer than kSortCap are sorted in place in global memory.
 constexpr int kSortCells = 256;
 constexpr int kSortCap = 4096;
+
+// Cells per sort block for a mean of `mean` entries per cell.
+inline int sort_cells_per_block(double mean)
+{
+    const double fit = (double)kSortCap / (1.5 * std::max(mean, 1.0));
+    const int c = (int)(fit / 32.0) * 32;
+    return std::max(32, std::min(kSortCells, c));
+}
 
 // Per-cell footprint tap sums sum_e a_e(i, j) of the cell's sorted entries
 // (the coverage pull's per-cell phase), written as tap planes [KS*KS][ncells].
@@ -106,14 +117,14 @@ __device__ __forceinline__ void cell_tap_sums(int c, int a, int b, const int *ke
 }
 
 template <int KS>
-__global__ void __launch_bounds__(kSortCells) k_cell_sort(int ncells, int W, int H, const int *__restrict__ start,
+__global__ void __launch_bounds__(kSortCells) k_cell_sort(int ncells, int cpb, int W, int H, const int *__restrict__ start,
                                                          int *__restrict__ e_src, const float *__restrict__ tx,
                                                          const float *__restrict__ ty, float *__restrict__ e_tx,
                                                          float *__restrict__ e_ty, float *__restrict__ cell_tap)
 {
     __shared__ int keys[kSortCap];
-    const int c0 = blockIdx.x * kSortCells;
-    const int c1 = min(c0 + kSortCells, ncells);
+    const int c0 = blockIdx.x * cpb;
+    const int c1 = min(c0 + cpb, ncells);
     const int base = start[c0], total = start[c1] - base;
     const int c = c0 + (int)threadIdx.x;
     if (total <= kSortCap) {
@@ -753,8 +764,10 @@ lfrv_status run_prepare(lfrv_ctx *ctx)
     // dsf = 1: the sort also forms the per-cell tap sums of the coverage
     // (scratch: the single-reduction CG buffers, idle here)
     float *cell_tap = (ctx->ks == 2 && ctx->cg_extra) ? ctx->cg_extra : nullptr;
-    const int sort_blocks = (nc + kSortCells - 1) / kSortCells;
-#define CS(K) k_cell_sort<K><<<sort_blocks, kSortCells, 0, ctx->stream>>>(nc, ctx->W, ctx->H, ctx->cell_start, ctx->e_src, ctx->tau_x, ctx->tau_y, ctx->e_tx, ctx->e_ty, cell_tap)
+    // the mean entries per cell, from the source pixels (invalid ones drop out)
+    const int cpb = sort_cells_per_block((double)ctx->V * (double)ctx->n / (double)nc);
+    const int sort_blocks = (nc + cpb - 1) / cpb;
+#define CS(K) k_cell_sort<K><<<sort_blocks, kSortCells, 0, ctx->stream>>>(nc, cpb, ctx->W, ctx->H, ctx->cell_start, ctx->e_src, ctx->tau_x, ctx->tau_y, ctx->e_tx, ctx->e_ty, cell_tap)
     LFRV_KS_DISPATCH(ctx->ks, CS);
 #undef CS
     LFRV_LAUNCH(ctx, "splat structure");
