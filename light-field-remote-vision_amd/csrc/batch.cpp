@@ -40,6 +40,12 @@ struct lfrv_batch {
     hipGraphExec_t exec = nullptr;
     int graph_blocks = -1, graph_items = -1;
     std::vector<hipEvent_t> events;
+    // the IBR_color steps of all batchable solves: one launch per step and footprint
+    hipStream_t pd_stream = nullptr;
+    SrBatchItem *d_sr_items = nullptr;
+    int2 *d_sr_work = nullptr;
+    float *d_theta = nullptr;
+    size_t cap_sr_items = 0, cap_sr_work = 0, cap_theta = 0;
 };
 
 namespace {
@@ -93,6 +99,11 @@ extern "C" lfrv_status lfrv_batch_create(const lfrv_batch_params *p, lfrv_batch 
         delete b;
         return LFRV_E_HIP;
     }
+    if (hipStreamCreate(&b->pd_stream) != hipSuccess) {
+        (void)hipStreamDestroy(b->stream);
+        delete b;
+        return LFRV_E_HIP;
+    }
     *out = b;
     return LFRV_OK;
 }
@@ -101,13 +112,16 @@ extern "C" void lfrv_batch_destroy(lfrv_batch *b)
 {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->pd_stream) (void)hipStreamSynchronize(b->pd_stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (void *q : {(void *)b->d_items, (void *)b->d_work, (void *)b->d_done})
+    for (void *q : {(void *)b->d_items, (void *)b->d_work, (void *)b->d_done, (void *)b->d_sr_items,
+                    (void *)b->d_sr_work, (void *)b->d_theta})
         if (q) (void)hipFree(q);
     if (b->h_done) (void)hipHostFree(b->h_done);
     for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->pd_stream) (void)hipStreamDestroy(b->pd_stream);
     delete b;
 }
 
@@ -138,6 +152,13 @@ extern "C" lfrv_status lfrv_batch_run(lfrv_batch *b, lfrv_batch_result *res)
     std::vector<lfrv_batch_result> out(ne);
     std::vector<int> cg_index;  // entry of each CG item
     std::vector<CgBatchItem> items;
+    std::vector<int> sr_index;  // entry of each batched IBR_color item
+    std::vector<SrBatchItem> sr_items;
+    std::vector<float> theta_host;
+    std::vector<size_t> sorted_theta;
+    // uploaded with asynchronous copies: they live until the batch has run
+    std::vector<SrBatchItem> sr_sorted;
+    std::vector<int2> sr_work;
     // 1. every solve's set-up on its own stream, in two rounds over the solves:
     // the hole filling of an IBR_gradient solve waits on the host for its
     // visibility pass's hole count, so every solve's warps, splat structure,
@@ -171,12 +192,89 @@ extern "C" lfrv_status lfrv_batch_run(lfrv_batch *b, lfrv_batch_result *res)
             cg_index.push_back((int)i);
         } else {
             B_CHECK(b, ctx, lfrv_sr_init_regularizer_weight(ctx));
-            B_CHECK(b, ctx, lfrv_sr_iterate(ctx, e.steps));
+            if (sr_batchable(ctx) && e.steps > 0) {
+                SrBatchItem it{};
+                theta_host.resize(theta_host.size() + e.steps);
+                B_CHECK(b, ctx, sr_batch_begin(ctx, e.steps, &it, theta_host.data() + theta_host.size() - e.steps));
+                sr_items.push_back(it);
+                sr_index.push_back((int)i);
+            } else {
+                B_CHECK(b, ctx, lfrv_sr_iterate(ctx, e.steps));
+            }
             out[i].iterations = e.steps;
         }
         hipEvent_t ev = event_of(b, i);
         if (!ev) return bfail(b, LFRV_E_HIP, "event creation failed");
         B_HIP(b, hipEventRecord(ev, ctx->stream));
+    }
+    // the batched IBR_color steps: after every such solve's set-up, on their
+    // own stream beside the PCG iterations; step s of all solves of one
+    // footprint in one launch.  Each solve's stream then waits for them, and
+    // its event marks their end.
+    if (!sr_items.empty()) {
+        const int ns = (int)sr_items.size();
+        for (int k = 0; k < ns; k++) B_HIP(b, hipStreamWaitEvent(b->pd_stream, b->events[sr_index[k]], 0));
+        // items grouped by footprint, each group's work list (item, local block)
+        std::vector<SrBatchItem> &items_sorted = sr_sorted;
+        std::vector<int2> &work = sr_work;
+        struct Group { int ks; bool grad; int block0, nblocks; };
+        std::vector<Group> groups;
+        int max_steps = 0;
+        size_t toff = 0;
+        std::vector<size_t> theta_off(ns);
+        for (int k = 0; k < ns; k++) {
+            theta_off[k] = toff;
+            toff += (size_t)sr_items[k].steps;
+        }
+        for (int gk = 0; gk < 6; gk++) {
+            const int ks = 3 + gk / 2;
+            const bool grad = (gk & 1) != 0;
+            const int b0 = (int)work.size();
+            for (int k = 0; k < ns; k++) {
+                if (b->entries[sr_index[k]].ctx->ks != ks || (sr_items[k].da.beta > 0.0f) != grad) continue;
+                const int idx = (int)items_sorted.size();
+                items_sorted.push_back(sr_items[k]);
+                items_sorted.back().theta = nullptr;  // set below, once the device array exists
+                items_sorted.back().NA = sr_items[k].NA;
+                for (int j = 0; j < sr_items[k].ntiles; j++) work.push_back(make_int2(idx, j));
+                max_steps = std::max(max_steps, sr_items[k].steps);
+                sorted_theta.push_back(theta_off[k]);
+            }
+            if ((int)work.size() > b0) groups.push_back({ks, grad, b0, (int)work.size() - b0});
+        }
+        const size_t nit = items_sorted.size();
+        if (nit > b->cap_sr_items) {
+            if (b->d_sr_items) B_HIP(b, hipFree(b->d_sr_items));
+            B_HIP(b, hipMalloc((void **)&b->d_sr_items, nit * sizeof(SrBatchItem)));
+            b->cap_sr_items = nit;
+        }
+        if (work.size() > b->cap_sr_work) {
+            if (b->d_sr_work) B_HIP(b, hipFree(b->d_sr_work));
+            B_HIP(b, hipMalloc((void **)&b->d_sr_work, work.size() * sizeof(int2)));
+            b->cap_sr_work = work.size();
+        }
+        if (toff > b->cap_theta) {
+            if (b->d_theta) B_HIP(b, hipFree(b->d_theta));
+            B_HIP(b, hipMalloc((void **)&b->d_theta, toff * sizeof(float)));
+            b->cap_theta = toff;
+        }
+        for (size_t k = 0; k < nit; k++) items_sorted[k].theta = b->d_theta + sorted_theta[k];
+        B_HIP(b, hipMemcpyAsync(b->d_theta, theta_host.data(), toff * sizeof(float), hipMemcpyHostToDevice,
+                                b->pd_stream));
+        B_HIP(b, hipMemcpyAsync(b->d_sr_items, items_sorted.data(), nit * sizeof(SrBatchItem), hipMemcpyHostToDevice,
+                                b->pd_stream));
+        B_HIP(b, hipMemcpyAsync(b->d_sr_work, work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice,
+                                b->pd_stream));
+        for (int st = 0; st < max_steps; st++)
+            for (const Group &gr : groups)
+                B_HIP(b, launch_sr_batch_step(gr.ks, gr.grad, b->d_sr_items, b->d_sr_work + gr.block0, gr.nblocks,
+                                              st, b->pd_stream));
+        for (int k = 0; k < ns; k++) {
+            lfrv_ctx *ctx = b->entries[sr_index[k]].ctx;
+            B_HIP(b, hipEventRecord(b->events[sr_index[k]], b->pd_stream));
+            B_HIP(b, hipStreamWaitEvent(ctx->stream, b->events[sr_index[k]], 0));
+            sr_batch_end(ctx, sr_items[k].steps);
+        }
     }
     const int ni = (int)items.size();
     std::vector<char> seen(ne, 0);

@@ -20,10 +20,6 @@ lfrv_status run_pd_steps(lfrv_ctx *ctx, int iters, const float *F, bool overrela
 
 namespace {
 
-struct DataTermArgs {
-    float alpha, beta, sigma2, grad_step, dsf2, prox_factor;  // prox_factor = -1 / (lambda L)
-    int gw_type;
-};
 
 // Data-term gradient and the clamped gradient step for one target tile
 // (vtv_sr_ista_step_unstructured, optimization.cu:1775-1812):
@@ -249,9 +245,12 @@ constexpr int kStTW = 32, kStTH = 16;
 // that pitch (idle where x is past the pass's width), so a wave's accesses are
 // contiguous and free of bank conflicts; a pitch per plane made them jump at
 // each row wrap.
-template <int KS>
+// GRAD: the gradient-constraint term is on (dt_beta > 0): lap U, its box
+// sums and residuals are formed too (NC = 6 planes instead of 3).
+template <int KS, bool GRAD = true>
 struct StDims {
     static constexpr int D = KS - 1, TW = kStTW, TH = kStTH, NT = TW * TH;
+    static constexpr int NC = GRAD ? 6 : 3;
     static constexpr int QW = TW + D - 1, QH = TH + D - 1;  // anchor points x' = tx0 + qx
     static constexpr int BW = QW + 2, BH = QH + 2;          // box sums at x' = tx0 - 1 + bx
     static constexpr int LW = BW + D - 1, LH = BH + D - 1;  // pixels tx0 - D + lx (box sums' support)
@@ -260,12 +259,14 @@ struct StDims {
     // The two LDS regions take turns: sa holds U and lap U, sb their row sums
     // (D > 2); then the box sums BU, BL go to the region nobody reads any more
     // (sa for D > 2, sb for D <= 2), the anchor residuals to the other one, and
-    // (D > 2) the residuals' row sums to sa again.
-    static constexpr int A1 = 3 * SH * P + 3 * LH * P, A2 = 6 * QH * P;
+    // (D > 2) the residuals' row sums to sa again; sb ends with the duals of
+    // the primal-dual step (6 planes of (TH + 1) x (TW + 2)).
+    static constexpr int A1 = 3 * SH * P + (GRAD ? 3 * LH * P : 0), A2 = NC * QH * P;
     static constexpr int AREA = A1 > A2 ? A1 : A2;
-    static constexpr int HR = BH + D - 1;     // rows of the horizontal box sums
-    static constexpr int BAREA = 6 * HR * P;  // sb
-    static constexpr int NQP = QH * P;              // anchor slots (qx < QW used)
+    static constexpr int HR = BH + D - 1;  // rows of the horizontal box sums
+    static constexpr int B1 = NC * HR * P, B2 = 6 * (TH + 1) * (TW + 2);
+    static constexpr int BAREA = B1 > B2 ? B1 : B2;  // sb
+    static constexpr int NQP = QH * P;               // anchor slots (qx < QW used)
     static constexpr int NQT = (NQP + NT - 1) / NT, NST = (SH * P + NT - 1) / NT;
 };
 
@@ -274,25 +275,25 @@ struct StDims {
 // and the pixel's coverage; sb is free for the caller at once, sa only after
 // a barrier.  With ur, the U values of
 // [tx0-1, tx0+TW] x [ty0-1, ty0+TH] (3 planes, pitch TW+2) are also left there.
-template <int KS>
+template <int KS, bool GRAD>
 __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, int tx0, int ty0, const DataTermArgs &da,
                                                  const float *__restrict__ U, const float *__restrict__ ops, size_t NA,
                                                  const float *__restrict__ coverage, float *sa, float *sb,
                                                  float (&F)[3], float (&up)[3], float &cov, float *ur = nullptr)
 {
-    using S = StDims<KS>;
+    using S = StDims<KS, GRAD>;
     constexpr int D = S::D, TW = S::TW, NT = S::NT, QW = S::QW, QH = S::QH, BW = S::BW, BH = S::BH, LW = S::LW,
                   LH = S::LH, SH = S::SH, P = S::P, NQP = S::NQP, NQT = S::NQT, NST = S::NST;
     constexpr bool SEP = D > 2;  // separable box sums
-    static_assert(6 * BH * P <= (SEP ? S::AREA : S::BAREA) && 6 * NQP <= (SEP ? S::BAREA : S::AREA) &&
-                      6 * QH * TW <= S::AREA,
+    static_assert(S::NC * BH * P <= (SEP ? S::AREA : S::BAREA) && S::NC * NQP <= (SEP ? S::BAREA : S::AREA) &&
+                      S::NC * QH * TW <= S::AREA,
                   "the box sums, residuals and row sums fit their LDS regions");
     float *us = sa, *ls = sa + 3 * SH * P;
     float *bus = SEP ? sa : sb, *bls = bus + 3 * BH * P;
     float *qs = SEP ? sb : sa;
     float *rs = sa;  // D > 2: the residuals' row sums, once BU and BL are read
     const size_t N = (size_t)W * H;
-    const bool grad = da.beta > 0.0f;
+    constexpr bool grad = GRAD;  // da.beta > 0
     const float *g9 = da.gw_type ? ops : ops + 8 * NA;  // couplings of the gradient term
     const int tid = threadIdx.x;
 
@@ -458,7 +459,7 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             bus[c * BH * P + i] = bsum[k][c];
-            bls[c * BH * P + i] = bsum[k][3 + c];
+            if (grad) bls[c * BH * P + i] = bsum[k][3 + c];  // (no room for BL without the gradient term)
         }
     }
     __syncthreads();
@@ -546,7 +547,7 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
     // other threads may still read sa
 }
 
-template <int KS>
+template <int KS, bool GRAD>
 __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, int Wc, int Hc, int ntx, int ntiles,
                                                                    DataTermArgs da, const float *__restrict__ U,
                                                                    const float *__restrict__ ops, size_t NA,
@@ -555,12 +556,12 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
                                                                    float *__restrict__ norm_mask, const int *halt)
 {
     if (halt && *halt) return;  // a step enqueued behind a stopping rule that fired
-    __shared__ float sa[StDims<KS>::AREA];
-    __shared__ float sb[StDims<KS>::BAREA];
+    __shared__ float sa[StDims<KS, GRAD>::AREA];
+    __shared__ float sb[StDims<KS, GRAD>::BAREA];
     const int t = xcd_tile(blockIdx.x, ntiles);
     const int tx0 = (t % ntx) * kStTW, ty0 = (t / ntx) * kStTH;
     float F[3], up[3], cov;
-    sr_dataterm_tile<KS>(W, H, Wc, Hc, tx0, ty0, da, U, ops, NA, coverage, sa, sb, F, up, cov);
+    sr_dataterm_tile<KS, GRAD>(W, H, Wc, Hc, tx0, ty0, da, U, ops, NA, coverage, sa, sb, F, up, cov);
     const int px = tx0 + threadIdx.x % kStTW, py = ty0 + threadIdx.x / kStTW;
     if (px >= W || py >= H) return;
     const size_t N = (size_t)W * H, po = (size_t)py * W + px;
@@ -572,9 +573,6 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
     if (da.alpha > 0.0f || da.beta > 0.0f) norm_mask[po] = cov;
 }
 
-struct PdArgs {
-    float tau, lam, sigma, theta;
-};
 
 // One outer step for dsf >= 2 in one pass: the data term of the tile, then
 // the first primal-dual iteration on it (k_pd_step's update, with Uq = U as the
@@ -582,30 +580,28 @@ struct PdArgs {
 // goes to unew.  With uq_relax (the step's only inner iteration) the
 // over-relaxation of vtv_rof.cu:28-45 is fused too: uq_relax <- U + theta (new -
 // U).  Gout (nullable) receives F for further inner iterations.
-template <int KS>
-__global__ void __launch_bounds__(kStTW * kStTH) k_sr_step_st(int W, int H, int Wc, int Hc, int ntx, int ntiles,
-                                                               DataTermArgs da, PdArgs pa, const float *__restrict__ U,
-                                                               const float *__restrict__ ops, size_t NA,
-                                                               const float *__restrict__ coverage,
-                                                               const float *__restrict__ pxin,
-                                                               const float *__restrict__ pyin,
-                                                               const float *__restrict__ g, float *__restrict__ pxout,
-                                                               float *__restrict__ pyout, float *__restrict__ unew,
-                                                               float *__restrict__ uq_relax, float *__restrict__ Gout,
-                                                               float *__restrict__ norm_mask, const int *halt)
+// The step on tile t (of the W x H target's ntx-wide tile grid); every
+// thread of the block calls it.
+template <int KS, bool GRAD>
+__device__ __forceinline__ void sr_step_tile(int t, int W, int H, int Wc, int Hc, int ntx, const DataTermArgs &da,
+                                             const PdArgs &pa, const float *__restrict__ U,
+                                             const float *__restrict__ ops, size_t NA,
+                                             const float *__restrict__ coverage, const float *__restrict__ pxin,
+                                             const float *__restrict__ pyin, const float *__restrict__ g,
+                                             float *__restrict__ pxout, float *__restrict__ pyout,
+                                             float *__restrict__ unew, float *__restrict__ uq_relax,
+                                             float *__restrict__ Gout, float *__restrict__ norm_mask)
 {
-    if (halt && *halt) return;  // a step enqueued behind a stopping rule that fired
     constexpr int TW = kStTW, TH = kStTH, NT = TW * TH;
     constexpr int UW = TW + 2, UH = TH + 2;  // U region [x0-1, x0+TW] x [y0-1, y0+TH]
     constexpr int PW = TW + 1, PH = TH + 1;  // p region [x0-1, x0+TW-1] x [y0-1, y0+TH-1]
     constexpr int NPP = PH * UW;             // p slots with the U region's pitch (x < PW used)
     constexpr int NPT = (NPP + NT - 1) / NT;
-    static_assert(6 * NPP <= StDims<KS>::BAREA, "primal-dual tiles exceed the data term's second LDS region");
+    static_assert(6 * NPP <= StDims<KS, GRAD>::BAREA, "primal-dual tiles exceed the data term's second LDS region");
     static_assert(UW == kStTW + 2 && UH == kStTH + 2, "the ring the data term leaves");
-    __shared__ float sa[StDims<KS>::AREA];
-    __shared__ float sb[StDims<KS>::BAREA];
+    __shared__ float sa[StDims<KS, GRAD>::AREA];
+    __shared__ float sb[StDims<KS, GRAD>::BAREA];
     __shared__ float us[3 * UH * UW];  // U on the primal-dual step's ring, left by the data term's staging
-    const int t = xcd_tile(blockIdx.x, ntiles);
     const int tx0 = (t % ntx) * TW, ty0 = (t / ntx) * TH;
     const size_t N = (size_t)W * H;
     const int tid = threadIdx.x;
@@ -626,7 +622,7 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_step_st(int W, int H, int 
         gr[k] = (in && g) ? g[o] : 1.0f;
     }
     float F[3], up[3], cov;
-    sr_dataterm_tile<KS>(W, H, Wc, Hc, tx0, ty0, da, U, ops, NA, coverage, sa, sb, F, up, cov, us);
+    sr_dataterm_tile<KS, GRAD>(W, H, Wc, Hc, tx0, ty0, da, U, ops, NA, coverage, sa, sb, F, up, cov, us);
 
     // ---- the primal-dual iteration (vtv_rof.cu:118-147, 280-321; cuda_kernels.cu:504-532) ----
     float *ps = sb;  // free once the data term returns (sa may still be read)
@@ -688,6 +684,42 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_step_st(int W, int H, int 
         if (Gout) Gout[c * N + o] = F[c];
     }
     if (da.alpha > 0.0f || da.beta > 0.0f) norm_mask[o] = cov;
+}
+
+template <int KS, bool GRAD>
+__global__ void __launch_bounds__(kStTW * kStTH) k_sr_step_st(int W, int H, int Wc, int Hc, int ntx, int ntiles,
+                                                               DataTermArgs da, PdArgs pa, const float *__restrict__ U,
+                                                               const float *__restrict__ ops, size_t NA,
+                                                               const float *__restrict__ coverage,
+                                                               const float *__restrict__ pxin,
+                                                               const float *__restrict__ pyin,
+                                                               const float *__restrict__ g, float *__restrict__ pxout,
+                                                               float *__restrict__ pyout, float *__restrict__ unew,
+                                                               float *__restrict__ uq_relax, float *__restrict__ Gout,
+                                                               float *__restrict__ norm_mask, const int *halt)
+{
+    if (halt && *halt) return;  // a step enqueued behind a stopping rule that fired
+    sr_step_tile<KS, GRAD>(xcd_tile(blockIdx.x, ntiles), W, H, Wc, Hc, ntx, da, pa, U, ops, NA, coverage, pxin, pyin, g,
+                     pxout, pyout, unew, uq_relax, Gout, norm_mask);
+}
+
+// Step s of every IBR_color solve of a batch in one launch: block b runs tile
+// xcd_tile(j, ntiles) of solve k, (k, j) = work[b], exactly as block j of
+// that solve's own k_sr_step_st launch (the same tile, the same arithmetic),
+// on the buffers of step parity s & 1; a solve with fewer steps returns.
+template <int KS, bool GRAD>
+__global__ void __launch_bounds__(kStTW * kStTH) k_sr_step_st_batch(const SrBatchItem *__restrict__ items,
+                                                                     const int2 *__restrict__ work, int s)
+{
+    const int2 wk = work[blockIdx.x];
+    const SrBatchItem &it = items[wk.x];
+    if (s >= it.steps) return;
+    const int p = s & 1;
+    PdArgs pa = it.pa;
+    pa.theta = it.theta[s];
+    sr_step_tile<KS, GRAD>(xcd_tile(wk.y, it.ntiles), it.W, it.H, it.Wc, it.Hc, it.ntx, it.da, pa, it.U[p], it.ops, it.NA,
+                     it.coverage, it.PX[p], it.PY[p], it.g, it.PX[p ^ 1], it.PY[p ^ 1], it.U[p ^ 1], it.uq,
+                     nullptr, it.norm_mask);
 }
 
 // theta over-relaxation (vtv_rof.cu:28-45): uq <- u + theta (uq - u); u <- uq_old
@@ -919,15 +951,8 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_sr_loop_step(SrLoop *L, co
 
 namespace {
 
-// One FISTA step on the stream; with `halt` every launch of the step is a
-// no-op once a stopping rule has fired on the device.
-lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
+DataTermArgs data_term_args(const lfrv_ctx *ctx)
 {
-    if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "FISTA step before lfrv_prepare");
-    lfrv_status st = run_refresh_entry_weights(ctx, 1.0f);
-    if (st != LFRV_OK) return st;
-    const TileGeom g = tile_geom(ctx);
-    const TileEntries te = tile_entries(ctx);
     DataTermArgs da;
     da.alpha = ctx->p.dt_alpha;
     da.beta = ctx->p.dt_beta;
@@ -937,11 +962,73 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
     const float L = (float)(1.0 / (double)ctx->p.lambda);  // optimization.cu:1741
     da.prox_factor = (float)(-1.0 / ((double)ctx->p.lambda * (double)L));
     da.gw_type = ctx->p.gw_type;
-    // FISTA weights (optimization.cu:1744-1746): alpha_new is a cuflt (float)
-    // there, theta the float of the double quotient; the over-relaxation is
-    // fused into the last primal-dual step
-    const float an = (float)(0.5 * (1.0 + std::sqrt(1.0 + 4.0 * std::pow(ctx->fista_alpha, 2.0))));
-    const float theta = (float)((ctx->fista_alpha - 1.0) / (double)an);
+    return da;
+}
+
+// FISTA weights (optimization.cu:1744-1746): alpha_new is a cuflt (float)
+// there, theta the float of the double quotient.  Returns theta of the step
+// that starts at alpha; `an` receives alpha_new.
+float fista_theta(double alpha, float &an)
+{
+    an = (float)(0.5 * (1.0 + std::sqrt(1.0 + 4.0 * std::pow(alpha, 2.0))));
+    return (float)((alpha - 1.0) / (double)an);
+}
+
+// The primal-dual constants of the step fused with the data term
+// (vtv_rof.cu:118-147), theta left for the caller.
+PdArgs pd_args(const lfrv_ctx *ctx)
+{
+    PdArgs pa;
+    pa.tau = pa.sigma = (float)(0.3 / std::sqrt(8.0));
+    const float Lp = (float)(1.0 / (double)ctx->p.lambda);
+    pa.lam = (float)(1.0 / (double)Lp);
+    pa.theta = 0.0f;
+    return pa;
+}
+
+// The assembled anchor-grid operator of the data term (dsf >= 2), rebuilt
+// when the weights or the entries changed.
+lfrv_status ensure_sr_ops(lfrv_ctx *ctx)
+{
+    if (ctx->sr_ops_epoch == ctx->epoch) return LFRV_OK;
+    const TileGeom g = tile_geom(ctx);
+    const TileEntries te = tile_entries(ctx);
+    const int Wc = ctx->W + ctx->ks - 1, Hc = ctx->H + ctx->ks - 1;
+    const size_t NA = (size_t)Wc * Hc;
+    const int grad = ctx->p.dt_beta > 0.0f ? 1 : 0;
+    TileGeom ga = g;
+    ga.ntx = (Wc + g.tw - 1) / g.tw;
+    ga.ntiles = ga.ntx * ((Hc + g.th - 1) / g.th);
+    const int wh = g.th + 1, ncw = (g.tw + 1) * wh;
+    const int cap = std::min(1024, kStagePer * ctx->pull_bs);
+    const size_t lds = sizeof(int) * (2 * wh + 2) + sizeof(float) * std::max<size_t>(44 * (size_t)ncw, 9 * (size_t)cap);
+    TimedRegion tr(ctx, 5);
+#define AS(K) k_sr_assemble<K><<<ga.ntiles, ctx->pull_bs, lds, ctx->stream>>>(ga, te, cap, grad, ctx->p.gw_type != 0, NA, ctx->sr_ops)
+    switch (ctx->ks) {
+    case 3: AS(3); break;
+    case 4: AS(4); break;
+    case 5: AS(5); break;
+    default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
+    }
+#undef AS
+    LFRV_LAUNCH(ctx, "data-term assembly");
+    ctx->sr_ops_epoch = ctx->epoch;
+    return LFRV_OK;
+}
+
+// One FISTA step on the stream; with `halt` every launch of the step is a
+// no-op once a stopping rule has fired on the device.
+lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
+{
+    if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "FISTA step before lfrv_prepare");
+    lfrv_status st = run_refresh_entry_weights(ctx, 1.0f);
+    if (st != LFRV_OK) return st;
+    const TileGeom g = tile_geom(ctx);
+    const TileEntries te = tile_entries(ctx);
+    const DataTermArgs da = data_term_args(ctx);
+    // the over-relaxation is fused into the last primal-dual step
+    float an;
+    const float theta = fista_theta(ctx->fista_alpha, an);
     const int inner = ctx->p.inner_iterations;
     bool pd_done = false;  // the first primal-dual iteration ran with the data term
     const int HAL = ctx->ks - 1;
@@ -963,41 +1050,21 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
         // the entries changed; the step itself is a stencil
         const int Wc = ctx->W + ctx->ks - 1, Hc = ctx->H + ctx->ks - 1;
         const size_t NA = (size_t)Wc * Hc;
-        const int grad = ctx->p.dt_beta > 0.0f ? 1 : 0;
-        if (ctx->sr_ops_epoch != ctx->epoch) {
-            TileGeom ga = g;
-            ga.ntx = (Wc + g.tw - 1) / g.tw;
-            ga.ntiles = ga.ntx * ((Hc + g.th - 1) / g.th);
-            const int wh = g.th + 1, ncw = (g.tw + 1) * wh;
-            const int cap = std::min(1024, kStagePer * ctx->pull_bs);
-            const size_t lds = sizeof(int) * (2 * wh + 2) + sizeof(float) * std::max<size_t>(44 * (size_t)ncw, 9 * (size_t)cap);
-            TimedRegion tr(ctx, 5);
-#define AS(K) k_sr_assemble<K><<<ga.ntiles, ctx->pull_bs, lds, ctx->stream>>>(ga, te, cap, grad, ctx->p.gw_type != 0, NA, ctx->sr_ops)
-            switch (ctx->ks) {
-            case 3: AS(3); break;
-            case 4: AS(4); break;
-            case 5: AS(5); break;
-            default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
-            }
-#undef AS
-            LFRV_LAUNCH(ctx, "data-term assembly");
-            ctx->sr_ops_epoch = ctx->epoch;
-        }
+        st = ensure_sr_ops(ctx);
+        if (st != LFRV_OK) return st;
         const int ntx = (ctx->W + kStTW - 1) / kStTW, ntiles = ntx * ((ctx->H + kStTH - 1) / kStTH);
         const KernelEvents ev = kernel_events(ctx, 3);  // stamped by the dispatch itself
         if (inner > 0) {
             // the data term and the first primal-dual iteration in one pass: U is
             // read, the new primal goes to U2 (with one inner iteration the
             // over-relaxed one to Uq), the duals to P2X / P2Y
-            PdArgs pa;
-            pa.tau = pa.sigma = (float)(0.3 / std::sqrt(8.0));
-            const float Lp = (float)(1.0 / (double)ctx->p.lambda);
-            pa.lam = (float)(1.0 / (double)Lp);
+            PdArgs pa = pd_args(ctx);
             pa.theta = theta;
             const float *gw = ctx->has_reg_g ? ctx->reg_g : nullptr;
             float *uq_relax = inner == 1 ? ctx->Uq : nullptr;
             float *gout = inner > 1 ? ctx->G : nullptr;
-#define ST(K) hipExtLaunchKernelGGL(k_sr_step_st<K>, dim3(ntiles), dim3(kStTW * kStTH), 0, ctx->stream, ev.a, ev.b, 0, ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, pa, (const float *)ctx->U, (const float *)ctx->sr_ops, NA, (const float *)ctx->coverage, (const float *)ctx->PX, (const float *)ctx->PY, gw, ctx->P2X, ctx->P2Y, ctx->U2, uq_relax, gout, ctx->norm_mask, halt)
+#define ST1(K, GR) hipExtLaunchKernelGGL(k_sr_step_st<K, GR>, dim3(ntiles), dim3(kStTW * kStTH), 0, ctx->stream, ev.a, ev.b, 0, ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, pa, (const float *)ctx->U, (const float *)ctx->sr_ops, NA, (const float *)ctx->coverage, (const float *)ctx->PX, (const float *)ctx->PY, gw, ctx->P2X, ctx->P2Y, ctx->U2, uq_relax, gout, ctx->norm_mask, halt)
+#define ST(K) do { if (da.beta > 0.0f) ST1(K, true); else ST1(K, false); } while (0)
             switch (ctx->ks) {
             case 3: ST(3); break;
             case 4: ST(4); break;
@@ -1005,6 +1072,7 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
             default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
             }
 #undef ST
+#undef ST1
             record_kernel_events(ctx, 3, ev);
             LFRV_LAUNCH(ctx, "data term + primal-dual step");
             std::swap(ctx->PX, ctx->P2X);
@@ -1015,7 +1083,8 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
                 std::swap(ctx->Uq, ctx->U2);  // the remaining iterations start from Uq
             pd_done = true;
         } else {
-#define DT(K) hipExtLaunchKernelGGL(k_sr_dataterm_st<K>, dim3(ntiles), dim3(kStTW * kStTH), 0, ctx->stream, ev.a, ev.b, 0, ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, (const float *)ctx->U, (const float *)ctx->sr_ops, NA, (const float *)ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask, halt)
+#define DT1(K, GR) hipExtLaunchKernelGGL(k_sr_dataterm_st<K, GR>, dim3(ntiles), dim3(kStTW * kStTH), 0, ctx->stream, ev.a, ev.b, 0, ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, (const float *)ctx->U, (const float *)ctx->sr_ops, NA, (const float *)ctx->coverage, ctx->G, ctx->Uq, ctx->norm_mask, halt)
+#define DT(K) do { if (da.beta > 0.0f) DT1(K, true); else DT1(K, false); } while (0)
             switch (ctx->ks) {
             case 3: DT(3); break;
             case 4: DT(4); break;
@@ -1023,6 +1092,7 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
             default: return fail(ctx, LFRV_E_ARG, "dsf out of range");
             }
 #undef DT
+#undef DT1
             record_kernel_events(ctx, 3, ev);
             LFRV_LAUNCH(ctx, "data term");
         }
@@ -1078,6 +1148,84 @@ EnergyCoef energy_coef(const lfrv_ctx *ctx)
 lfrv_status run_fista(lfrv_ctx *ctx)
 {
     return enqueue_fista(ctx, nullptr);
+}
+
+bool sr_batchable(const lfrv_ctx *ctx)
+{
+    return ctx->p.mode == LFRV_MODE_COLOR_SR && ctx->ks >= 3 && ctx->ks <= 5 && ctx->p.inner_iterations == 1;
+}
+
+lfrv_status sr_batch_begin(lfrv_ctx *ctx, int steps, SrBatchItem *it, float *theta_host)
+{
+    if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "FISTA step before lfrv_prepare");
+    if (!sr_batchable(ctx)) return fail(ctx, LFRV_E_ARG, "batched FISTA steps need dsf >= 2 and one inner iteration");
+    lfrv_status st = run_refresh_entry_weights(ctx, 1.0f);
+    if (st == LFRV_OK) st = ensure_sr_ops(ctx);
+    if (st != LFRV_OK) return st;
+    const int Wc = ctx->W + ctx->ks - 1, Hc = ctx->H + ctx->ks - 1;
+    it->W = ctx->W;
+    it->H = ctx->H;
+    it->Wc = Wc;
+    it->Hc = Hc;
+    it->ntx = (ctx->W + kStTW - 1) / kStTW;
+    it->ntiles = it->ntx * ((ctx->H + kStTH - 1) / kStTH);
+    it->steps = steps;
+    it->da = data_term_args(ctx);
+    it->pa = pd_args(ctx);
+    it->theta = nullptr;
+    it->ops = ctx->sr_ops;
+    it->NA = (size_t)Wc * Hc;
+    it->coverage = ctx->coverage;
+    it->g = ctx->has_reg_g ? ctx->reg_g : nullptr;
+    it->U[0] = ctx->U;
+    it->U[1] = ctx->U2;
+    it->PX[0] = ctx->PX;
+    it->PX[1] = ctx->P2X;
+    it->PY[0] = ctx->PY;
+    it->PY[1] = ctx->P2Y;
+    it->uq = ctx->Uq;
+    it->norm_mask = ctx->norm_mask;
+    double a = ctx->fista_alpha;
+    for (int k = 0; k < steps; k++) {
+        float an;
+        theta_host[k] = fista_theta(a, an);
+        a = an;
+    }
+    ctx->u_zero = false;
+    return LFRV_OK;
+}
+
+void sr_batch_end(lfrv_ctx *ctx, int steps)
+{
+    double a = ctx->fista_alpha;
+    for (int k = 0; k < steps; k++) {
+        float an;
+        (void)fista_theta(a, an);
+        a = an;
+    }
+    ctx->fista_alpha = a;
+    if (steps & 1) {  // the solve's own steps swap after every step
+        std::swap(ctx->U, ctx->U2);
+        std::swap(ctx->PX, ctx->P2X);
+        std::swap(ctx->PY, ctx->P2Y);
+    }
+}
+
+hipError_t launch_sr_batch_step(int ks, bool grad, const SrBatchItem *items, const int2 *work, int nblocks, int s,
+                                hipStream_t stream)
+{
+#define SB(K, G) k_sr_step_st_batch<K, G><<<nblocks, kStTW * kStTH, 0, stream>>>(items, work, s)
+    switch (ks * 2 + (grad ? 1 : 0)) {
+    case 6: SB(3, false); break;
+    case 7: SB(3, true); break;
+    case 8: SB(4, false); break;
+    case 9: SB(4, true); break;
+    case 10: SB(5, false); break;
+    case 11: SB(5, true); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef SB
+    return hipGetLastError();
 }
 
 lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts)

@@ -502,6 +502,43 @@ lfrv_status run_visibility(lfrv_ctx *ctx);
 // s: the stream to launch on (nullptr: the context's own)
 lfrv_status run_compute_weights(lfrv_ctx *ctx, hipStream_t s = nullptr);
 lfrv_status materialize_dpart(lfrv_ctx *ctx);
+
+// ---- IBR_color steps of a batch of solves (k_sr.hip) ----
+struct DataTermArgs {
+    float alpha, beta, sigma2, grad_step, dsf2, prox_factor;  // prox_factor = -1 / (lambda L)
+    int gw_type;
+};
+struct PdArgs {
+    float tau, lam, sigma, theta;
+};
+// One solve's FISTA steps inside a batched launch: step s reads U[s & 1],
+// PX[s & 1], PY[s & 1] and writes the other parity (the pointer swaps of
+// the solve's own steps), with theta[s] of its FISTA sequence.
+struct SrBatchItem {
+    int W, H, Wc, Hc, ntx, ntiles, steps;
+    DataTermArgs da;
+    PdArgs pa;
+    const float *theta;  // device, `steps` entries
+    const float *ops;
+    size_t NA;
+    const float *coverage, *g;
+    float *U[2], *PX[2], *PY[2];
+    float *uq, *norm_mask;
+};
+// Whether a context's FISTA steps can run in a batched launch (dsf >= 2, one
+// primal-dual iteration per step).
+bool sr_batchable(const lfrv_ctx *ctx);
+// Readies `steps` steps of ctx for a batched launch on ctx's stream (the
+// assembled operator, the entry weights) and fills `it` (all but `theta`)
+// and theta_host[0 .. steps).
+lfrv_status sr_batch_begin(lfrv_ctx *ctx, int steps, SrBatchItem *it, float *theta_host);
+// After the batched steps: the context's buffers and FISTA state as after
+// `steps` steps of its own.
+void sr_batch_end(lfrv_ctx *ctx, int steps);
+// Step s of the batch (items of footprint ks, with the gradient term or
+// without, work list of (item, local block)).
+hipError_t launch_sr_batch_step(int ks, bool grad, const SrBatchItem *items, const int2 *work, int nblocks, int s,
+                                hipStream_t stream);
 lfrv_status run_initial_image(lfrv_ctx *ctx);
 lfrv_status run_hole_filling(lfrv_ctx *ctx, int iters, double eps, int *ran);
 lfrv_status run_push_pull(lfrv_ctx *ctx, int levels);
