@@ -14,12 +14,15 @@
  *     launch of the scalar steps of all solves, `chunk` iterations captured
  *     once into a hipGraph and replayed until every solve has converged (a
  *     converged solve's blocks return at once);
- *   - the FISTA steps of every IBR_color solve queued on its own stream,
- *     concurrent with the graph.
+ *   - step s of every dsf >= 2 IBR_color solve with one primal-dual
+ *     iteration per step in one launch (block -> (solve, tile) from a work
+ *     list, each solve's buffers of step parity s & 1 and its own FISTA
+ *     theta), on a stream of the batch concurrent with the graph; other
+ *     IBR_color solves run their steps on their own streams.
  * The results equal the single-solve entry points' bit for bit
- * (lfrv_cg_solve: every block of the batched launch does what the same block
- * of the solve's own launch does, partial sums in the same slots;
- * lfrv_sr_iterate: the same calls).
+ * (lfrv_cg_solve, lfrv_sr_iterate: every block of a batched launch does what
+ * the same block of the solve's own launch does, partial sums in the same
+ * slots).
  */
 #ifndef LFRV_BATCH_H
 #define LFRV_BATCH_H

@@ -1,6 +1,7 @@
 """Batches of solves (lfrv_batch) against the same solves made one at a time:
 PCG solves of uneven sizes and view counts iterated together through one
-captured graph, and IBR_color solves on their own streams beside them."""
+captured graph, and the IBR_color solves' steps batched one launch per step
+beside them."""
 import numpy as np
 import pytest
 
