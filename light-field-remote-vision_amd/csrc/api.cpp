@@ -564,7 +564,7 @@ extern "C" lfrv_status lfrv_sr_iterate(lfrv_ctx *ctx, int steps)
     if (steps < 0) return fail(ctx, LFRV_E_ARG, "negative step count");
     ctx->u_zero = false;
     for (int k = 0; k < steps; k++) {
-        const lfrv_status st = run_fista(ctx);
+        const lfrv_status st = run_fista(ctx, k == steps - 1);
         if (st != LFRV_OK) return st;
     }
     return LFRV_OK;

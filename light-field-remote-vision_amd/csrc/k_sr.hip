@@ -579,7 +579,10 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_dataterm_st(int W, int H, 
 // step sets it, optimization.cu:1765-1767).  U is only read; the new primal
 // goes to unew.  With uq_relax (the step's only inner iteration) the
 // over-relaxation of vtv_rof.cu:28-45 is fused too: uq_relax <- U + theta (new -
-// U).  Gout (nullable) receives F for further inner iterations.
+// U).  Gout (nullable) receives F for further inner iterations.  uq_relax and
+// norm_mask are nullable: a run of fixed steps reads neither between its steps
+// (the next step starts from the new primal, and the mask is the coverage
+// again), so only its last step writes them.
 // The step on tile t (of the W x H target's ntx-wide tile grid); every
 // thread of the block calls it.
 template <int KS, bool GRAD>
@@ -683,7 +686,7 @@ __device__ __forceinline__ void sr_step_tile(int t, int W, int H, int Wc, int Hc
         if (uq_relax) uq_relax[c * N + o] = up[c] + pa.theta * (v - up[c]);
         if (Gout) Gout[c * N + o] = F[c];
     }
-    if (da.alpha > 0.0f || da.beta > 0.0f) norm_mask[o] = cov;
+    if (norm_mask && (da.alpha > 0.0f || da.beta > 0.0f)) norm_mask[o] = cov;
 }
 
 template <int KS, bool GRAD>
@@ -707,6 +710,7 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_step_st(int W, int H, int 
 // xcd_tile(j, ntiles) of solve k, (k, j) = work[b], exactly as block j of
 // that solve's own k_sr_step_st launch (the same tile, the same arithmetic),
 // on the buffers of step parity s & 1; a solve with fewer steps returns.
+// Only a solve's last step writes Uq and the norm mask, as in its own run.
 template <int KS, bool GRAD>
 __global__ void __launch_bounds__(kStTW * kStTH) k_sr_step_st_batch(const SrBatchItem *__restrict__ items,
                                                                      const int2 *__restrict__ work, int s)
@@ -717,9 +721,10 @@ __global__ void __launch_bounds__(kStTW * kStTH) k_sr_step_st_batch(const SrBatc
     const int p = s & 1;
     PdArgs pa = it.pa;
     pa.theta = it.theta[s];
+    const bool last = s == it.steps - 1;
     sr_step_tile<KS, GRAD>(xcd_tile(wk.y, it.ntiles), it.W, it.H, it.Wc, it.Hc, it.ntx, it.da, pa, it.U[p], it.ops, it.NA,
-                     it.coverage, it.PX[p], it.PY[p], it.g, it.PX[p ^ 1], it.PY[p ^ 1], it.U[p ^ 1], it.uq,
-                     nullptr, it.norm_mask);
+                     it.coverage, it.PX[p], it.PY[p], it.g, it.PX[p ^ 1], it.PY[p ^ 1], it.U[p ^ 1],
+                     last ? it.uq : nullptr, nullptr, last ? it.norm_mask : nullptr);
 }
 
 // theta over-relaxation (vtv_rof.cu:28-45): uq <- u + theta (uq - u); u <- uq_old
@@ -1017,8 +1022,10 @@ lfrv_status ensure_sr_ops(lfrv_ctx *ctx)
 }
 
 // One FISTA step on the stream; with `halt` every launch of the step is a
-// no-op once a stopping rule has fired on the device.
-lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
+// no-op once a stopping rule has fired on the device.  A step that is not the
+// last of a fixed run (last = false) leaves Uq and the norm mask to the run's
+// last step where one pass does the whole step (dsf >= 2, one inner iteration).
+lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt, bool last = true)
 {
     if (!ctx->prepared) return fail(ctx, LFRV_E_STATE, "FISTA step before lfrv_prepare");
     lfrv_status st = run_refresh_entry_weights(ctx, 1.0f);
@@ -1061,9 +1068,10 @@ lfrv_status enqueue_fista(lfrv_ctx *ctx, const int *halt)
             PdArgs pa = pd_args(ctx);
             pa.theta = theta;
             const float *gw = ctx->has_reg_g ? ctx->reg_g : nullptr;
-            float *uq_relax = inner == 1 ? ctx->Uq : nullptr;
+            float *uq_relax = (inner == 1 && last) ? ctx->Uq : nullptr;
             float *gout = inner > 1 ? ctx->G : nullptr;
-#define ST1(K, GR) hipExtLaunchKernelGGL(k_sr_step_st<K, GR>, dim3(ntiles), dim3(kStTW * kStTH), 0, ctx->stream, ev.a, ev.b, 0, ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, pa, (const float *)ctx->U, (const float *)ctx->sr_ops, NA, (const float *)ctx->coverage, (const float *)ctx->PX, (const float *)ctx->PY, gw, ctx->P2X, ctx->P2Y, ctx->U2, uq_relax, gout, ctx->norm_mask, halt)
+            float *nmask = (inner > 1 || last) ? ctx->norm_mask : nullptr;
+#define ST1(K, GR) hipExtLaunchKernelGGL(k_sr_step_st<K, GR>, dim3(ntiles), dim3(kStTW * kStTH), 0, ctx->stream, ev.a, ev.b, 0, ctx->W, ctx->H, Wc, Hc, ntx, ntiles, da, pa, (const float *)ctx->U, (const float *)ctx->sr_ops, NA, (const float *)ctx->coverage, (const float *)ctx->PX, (const float *)ctx->PY, gw, ctx->P2X, ctx->P2Y, ctx->U2, uq_relax, gout, nmask, halt)
 #define ST(K) do { if (da.beta > 0.0f) ST1(K, true); else ST1(K, false); } while (0)
             switch (ctx->ks) {
             case 3: ST(3); break;
@@ -1145,9 +1153,9 @@ EnergyCoef energy_coef(const lfrv_ctx *ctx)
 
 }  // namespace
 
-lfrv_status run_fista(lfrv_ctx *ctx)
+lfrv_status run_fista(lfrv_ctx *ctx, bool last)
 {
-    return enqueue_fista(ctx, nullptr);
+    return enqueue_fista(ctx, nullptr, last);
 }
 
 bool sr_batchable(const lfrv_ctx *ctx)

@@ -547,7 +547,9 @@ lfrv_status run_cg(lfrv_ctx *ctx, double alpha, double c0, double beta, bool use
                    bool use_precond, bool poisson_rhs, double tol, int maxit, lfrv_cg_stats *st);
 lfrv_status run_gibr_energy(lfrv_ctx *ctx, double *e);
 lfrv_status run_regularizer_weight(lfrv_ctx *ctx);
-lfrv_status run_fista(lfrv_ctx *ctx);
+// last = false: a step followed by another of the same run (Uq and the norm
+// mask are then left to the last one)
+lfrv_status run_fista(lfrv_ctx *ctx, bool last = true);
 lfrv_status run_sr_energy(lfrv_ctx *ctx, double *e, double *parts);
 lfrv_status run_count_moving(lfrv_ctx *ctx, const float *prev_dev, float max_diff, int64_t *moving);
 lfrv_status run_sr_solve(lfrv_ctx *ctx, int niter, int nmeta, lfrv_sr_stats *st);

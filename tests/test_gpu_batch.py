@@ -59,14 +59,14 @@ def test_batch_equals_single_solves(use_graph, chunk):
     for s in cg_scenes:
         with _cg_ctx(s) as ctx:
             st = ctx.ibr_gradient(1e-5, 5000)
-            want.append((ctx.get_solution(), max(st["iterations"])))
+            want.append((ctx.get_solution(), max(st["iterations"]), None))
     for s, c in zip(pd_scenes, PD_CASES):
         with _pd_ctx(s) as ctx:
             ctx.clear_solution()
             pipelines.color_sr_setup(ctx)
             ctx.sr_init_regularizer_weight()
             ctx.sr_iterate(c[2])
-            want.append((ctx.get_solution(), c[2]))
+            want.append((ctx.get_solution(), c[2], (ctx.get_buffer("UQ"), ctx.get_buffer("NORM_MASK"))))
     # together, interleaved kinds, run twice (the second run replays the same graph)
     ctxs = []
     b = Batch(chunk=chunk, use_graph=use_graph)
@@ -85,12 +85,15 @@ def test_batch_equals_single_solves(use_graph, chunk):
     for rep in range(2):
         res = b.run()
         for j, (ctx, r) in enumerate(zip(ctxs, res)):
-            U, iters = want[order[j]]
+            U, iters, pd_out = want[order[j]]
             assert r["status"] == lfrv.OK
             assert r["iterations"] == iters, (j, r, iters)
             if r["kind"] == 0:
                 assert r["true_rel_residual"] <= 1e-5
             assert np.array_equal(ctx.get_solution(), U), (rep, j)
+            if pd_out is not None:  # the over-relaxed solution and the normalisation buffer too
+                assert np.array_equal(ctx.get_buffer("UQ"), pd_out[0]), (rep, j)
+                assert np.array_equal(ctx.get_buffer("NORM_MASK"), pd_out[1]), (rep, j)
             assert r["done_ms"] > 0
     b.close()
     for ctx in ctxs:

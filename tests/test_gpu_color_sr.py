@@ -143,11 +143,13 @@ def test_sr_count_moving():
         assert ctx.sr_count_moving(prev, 1e-3) == 100
 
 
-@pytest.mark.parametrize("dsf,inner", [(2, 1), (1, 3)])
-def test_sr_iterate_equals_single_steps(dsf, inner):
-    """lfrv_sr_iterate(k) gives the solution and duals of k lfrv_sr_iteration_fista calls, bit for bit."""
+@pytest.mark.parametrize("dsf,inner,beta", [(2, 1, 0.3), (3, 1, 0.0), (1, 3, 0.3)])
+def test_sr_iterate_equals_single_steps(dsf, inner, beta):
+    """lfrv_sr_iterate(k) gives the solution, duals, over-relaxed solution and
+    normalisation buffer of k lfrv_sr_iteration_fista calls, bit for bit (a
+    fixed run writes the last two only in its last step)."""
     s = _scene(dsf=dsf, kind=2 if dsf == 1 else 1)
-    prm = pipelines.color_sr_params(s, inner_iterations=inner, dt_beta=0.3, gradient_step=1.0)
+    prm = pipelines.color_sr_params(s, inner_iterations=inner, dt_beta=beta, gradient_step=1.0)
     out = []
     for batched in (False, True):
         with lfrv.Context(prm) as ctx:
@@ -160,6 +162,8 @@ def test_sr_iterate_equals_single_steps(dsf, inner):
             else:
                 for _ in range(7):
                     ctx.sr_iteration_fista()
-            out.append((ctx.get_solution(), ctx.get_buffer("PX"), ctx.get_buffer("UQ")))
+            out.append((ctx.get_solution(), ctx.get_buffer("PX"), ctx.get_buffer("PY"), ctx.get_buffer("UQ"),
+                        ctx.get_buffer("NORM_MASK")))
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+    assert np.abs(out[0][3].ravel() - out[0][0].ravel()).max() > 0  # the over-relaxed solution is not the solution
