@@ -439,14 +439,15 @@ def bench_color_sr(args, workload):
         # one pass per outer step: the assembled data term and the primal-dual
         # iteration with the over-relaxation.  Per anchor (W + dsf) x (H + dsf)
         # the couplings G9 (5 planes) and bI (3), with the gradient term G9' (5,
-        # unweighted) and bG (3); per target pixel U, coverage, p (6 planes), g
+        # unweighted) and one right-hand side b (3) for both terms instead of bI;
+        # per target pixel U, coverage, p (6 planes), g
         # read (44 B), p and the new U written (36 B).  The over-relaxed Uq and
         # norm_mask (16 B more) are written by the run's last step only, which the
         # sampled timing (launches 0, 8, 16, ... counted over all views) never
         # stamps at 300 or 500 steps per view (the last step's index is 3 or 7 mod 8)
         NA = (scene["W"] + ks - 1) * (scene["H"] + ks - 1)
         dt_kernel = f"k_sr_step_st<{ks}>"
-        dt_bytes = 4 * NA * (8 + (8 if beta > 0 else 0)) + 80 * N
+        dt_bytes = 4 * NA * (8 + (5 if beta > 0 else 0)) + 80 * N
     # fused PD step: Uq, px, py, g, F read; px, py, new primal written (88 B
     # per pixel), with the outer step's over-relaxation fused in: U read, and
     # U and the over-relaxed Uq written (+24 B)

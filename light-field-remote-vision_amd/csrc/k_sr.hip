@@ -154,16 +154,22 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_sr_dataterm(TileGeom g, Til
 // without omega (gw_type 0), bI = sum omega t v, bG = sum t lap v [* omega].
 // They are assembled once per set of weights (k_sr_assemble); every outer
 // step is then a tiled stencil (k_sr_dataterm_st): box sums, the 9-point
-// couplings, and the box sums back onto the pixels.
+// couplings, and the box sums back onto the pixels.  The step weighs the two
+// terms' back-projections by constants of the parameters before adding them
+// per pixel (cI = alpha sigma^2, cG = -beta gradient_step [sigma^2]), and the
+// back-projection is linear, so with the gradient term the step back-projects
+// one residual, cI (G9 BU - bI) + cG (dsf^2 G9' BL - bG), and the assembly
+// stores the constant part as one plane set b = cI bI + cG bG.
 //
 // Anchor grid: point x' in [0, W + dsf) has box start x' + 1 - dsf; it is the
 // top-left tap of the cell x' + 1 (cells hold box start cx - dsf).  Pixel x
 // lies in the boxes of the points x .. x + dsf - 1.
-// (kSrOpsPlanes planes: G9 (5), bI (3), G9' (5), bG (3))
+// (kSrOpsPlanes planes: G9 (5), bI (3) or b (3) with the gradient term, G9' (5))
 
 template <int KS>
 __global__ void __launch_bounds__(kPullMaxThreads) k_sr_assemble(TileGeom g, TileEntries te, int cap, int grad,
-                                                                 int gw, size_t NA, float *__restrict__ ops)
+                                                                 int gw, float ci, float cg, size_t NA,
+                                                                 float *__restrict__ ops)
 {
     extern __shared__ int smem[];
     constexpr int NC = 44;  // 10 weighted couplings, 12 bI taps, 12 bG taps, 10 unweighted couplings
@@ -216,19 +222,17 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_sr_assemble(TileGeom g, Til
             ops[(size_t)(plane + 3) * NA + o] = part[(base + 8) * ncw + c00];
             ops[(size_t)(plane + 4) * NA + o] = part[(base + 9) * ncw + c10];
         };
-        auto taps = [&](int base, int plane) {
-#pragma unroll
-            for (int ch = 0; ch < 3; ch++)
-                ops[(size_t)(plane + ch) * NA + o] =
-                    part[(base + 0 * 3 + ch) * ncw + c00] + part[(base + 1 * 3 + ch) * ncw + c10] +
-                    part[(base + 2 * 3 + ch) * ncw + c01] + part[(base + 3 * 3 + ch) * ncw + c11];
+        auto tap = [&](int base, int ch) {
+            return part[(base + 0 * 3 + ch) * ncw + c00] + part[(base + 1 * 3 + ch) * ncw + c10] +
+                   part[(base + 2 * 3 + ch) * ncw + c01] + part[(base + 3 * 3 + ch) * ncw + c11];
         };
         couple(0, 0);
-        taps(10, 5);
-        if (grad) {
-            couple(34, 8);
-            taps(22, 13);
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            const float bI = tap(10, ch);
+            ops[(size_t)(5 + ch) * NA + o] = grad ? ci * bI + cg * tap(22, ch) : bI;
         }
+        if (grad) couple(34, 8);
     }
 }
 
@@ -236,8 +240,9 @@ __global__ void __launch_bounds__(kPullMaxThreads) k_sr_assemble(TileGeom g, Til
 // operator.  Every global load of the block (U with its halo, the couplings
 // and right-hand sides of the tile's anchors, coverage) is issued first; the
 // rest runs in LDS: lap U, the box sums BU / BL at the anchors (+1 ring), the
-// anchor residuals QI = G9 BU - bI and QG = dsf^2 G9' BL - bG, then the box
-// sums back onto the pixels, split into rows and columns.
+// anchor residuals QI = G9 BU - bI (with the gradient term the weighted sum
+// cI G9 BU + cG dsf^2 G9' BL - b), then the box sums back onto the pixels,
+// split into rows and columns.
 constexpr int kStTW = 32, kStTH = 16;
 
 // Every LDS plane of the tile has the row pitch P of the widest one (U with
@@ -261,7 +266,7 @@ struct StDims {
     // (sa for D > 2, sb for D <= 2), the anchor residuals to the other one, and
     // (D > 2) the residuals' row sums to sa again; sb ends with the duals of
     // the primal-dual step (6 planes of (TH + 1) x (TW + 2)).
-    static constexpr int A1 = 3 * SH * P + (GRAD ? 3 * LH * P : 0), A2 = NC * QH * P;
+    static constexpr int A1 = 3 * SH * P + (GRAD ? 3 * LH * P : 0), A2 = 3 * QH * P;  // (3 residual planes)
     static constexpr int AREA = A1 > A2 ? A1 : A2;
     static constexpr int HR = BH + D - 1;  // rows of the horizontal box sums
     static constexpr int B1 = NC * HR * P, B2 = 6 * (TH + 1) * (TW + 2);
@@ -285,8 +290,8 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
     constexpr int D = S::D, TW = S::TW, NT = S::NT, QW = S::QW, QH = S::QH, BW = S::BW, BH = S::BH, LW = S::LW,
                   LH = S::LH, SH = S::SH, P = S::P, NQP = S::NQP, NQT = S::NQT, NST = S::NST;
     constexpr bool SEP = D > 2;  // separable box sums
-    static_assert(S::NC * BH * P <= (SEP ? S::AREA : S::BAREA) && S::NC * NQP <= (SEP ? S::BAREA : S::AREA) &&
-                      S::NC * QH * TW <= S::AREA,
+    static_assert(S::NC * BH * P <= (SEP ? S::AREA : S::BAREA) && 3 * NQP <= (SEP ? S::BAREA : S::AREA) &&
+                      3 * QH * TW <= S::AREA,
                   "the box sums, residuals and row sums fit their LDS regions");
     float *us = sa, *ls = sa + 3 * SH * P;
     float *bus = SEP ? sa : sb, *bls = bus + 3 * BH * P;
@@ -295,6 +300,7 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
     const size_t N = (size_t)W * H;
     constexpr bool grad = GRAD;  // da.beta > 0
     const float *g9 = da.gw_type ? ops : ops + 8 * NA;  // couplings of the gradient term
+    const float ci = dt_intensity_weight(da), cgd = dt_gradient_weight(da) * da.dsf2;
     const int tid = threadIdx.x;
 
     // ---- all global loads ----
@@ -308,7 +314,7 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
 #pragma unroll
         for (int c = 0; c < 3; c++) uv[k][c] = in ? U[c * N + o] : 0.0f;
     }
-    float cf[NQT][9], cg[NQT][9], bi[NQT][3], bg[NQT][3];
+    float cf[NQT][9], cg[NQT][9], bi[NQT][3];  // bi: bI, or b with the gradient term
 #pragma unroll
     for (int k = 0; k < NQT; k++) {
         const int i = tid + k * NT;
@@ -337,11 +343,7 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
         load9(ops, cf[k]);
 #pragma unroll
         for (int c = 0; c < 3; c++) bi[k][c] = in ? ops[(size_t)(5 + c) * NA + o] : 0.0f;
-        if (grad) {
-            load9(g9, cg[k]);
-#pragma unroll
-            for (int c = 0; c < 3; c++) bg[k][c] = in ? ops[(size_t)(13 + c) * NA + o] : 0.0f;
-        }
+        if (grad) load9(g9, cg[k]);
     }
     const int px = tx0 + tid % TW, py = ty0 + tid / TW;
     const bool pin = px < W && py < H;
@@ -477,8 +479,10 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
         };
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            qs[c * NQP + i] = st9(bus + c * BH * P, cf[k]) - bi[k][c];
-            if (grad) qs[(3 + c) * NQP + i] = da.dsf2 * st9(bls + c * BH * P, cg[k]) - bg[k][c];
+            if (grad)
+                qs[c * NQP + i] = ci * st9(bus + c * BH * P, cf[k]) + cgd * st9(bls + c * BH * P, cg[k]) - bi[k][c];
+            else
+                qs[c * NQP + i] = st9(bus + c * BH * P, cf[k]) - bi[k][c];
         }
     }
     __syncthreads();
@@ -489,8 +493,7 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
         const int lx = i % P, qy = i / P;
         if (lx >= TW) continue;
 #pragma unroll
-        for (int c = 0; c < 6; c++) {
-            if (c >= 3 && !grad) break;
+        for (int c = 0; c < 3; c++) {
             float sm = 0.0f;
 #pragma unroll
             for (int k = 0; k < D; k++) sm += qs[c * NQP + i + k];
@@ -502,41 +505,35 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
 
     // ---- pixel: columns of the row sums, then the step of vtv_sr_ista_step_unstructured ----
     const int lx = tid % TW, ly = tid / TW;
-    float acc[6];
+    float acc[3];
 #pragma unroll
-    for (int c = 0; c < 6; c++) {
+    for (int c = 0; c < 3; c++) {
         float sm = 0.0f;
-        if (c < 3 || grad) {
-            if constexpr (D <= 2) {
+        if constexpr (D <= 2) {
 #pragma unroll
-                for (int l = 0; l < D; l++) {
-                    float row = 0.0f;
+            for (int l = 0; l < D; l++) {
+                float row = 0.0f;
 #pragma unroll
-                    for (int k = 0; k < D; k++) row += qs[c * NQP + (ly + l) * P + lx + k];
-                    sm += row;
-                }
-            } else {
-#pragma unroll
-                for (int l = 0; l < D; l++) sm += rs[(c * QH + ly + l) * TW + lx];
+                for (int k = 0; k < D; k++) row += qs[c * NQP + (ly + l) * P + lx + k];
+                sm += row;
             }
+        } else {
+#pragma unroll
+            for (int l = 0; l < D; l++) sm += rs[(c * QH + ly + l) * TW + lx];
         }
         acc[c] = sm;
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         float G = 0.0f;
-        if (da.alpha > 0.0f) {
+        if (grad) {  // both terms, weighted in the residual
+            G = acc[c];
+            if (cov != 0.0f) G /= cov;
+        } else if (da.alpha > 0.0f) {
             float gi = acc[c];
             if (cov != 0.0f) gi /= cov;
             gi *= da.sigma2;
             G += da.alpha * gi;
-        }
-        if (grad) {
-            float gg = acc[3 + c];
-            if (cov != 0.0f) gg /= cov;
-            if (da.gw_type != 0) gg *= da.sigma2;
-            gg *= -da.grad_step;
-            G += da.beta * gg;
         }
         G = fminf(1.0f, fmaxf(-1.0f, G));
         G *= da.prox_factor;
@@ -1001,6 +998,7 @@ lfrv_status ensure_sr_ops(lfrv_ctx *ctx)
     const int Wc = ctx->W + ctx->ks - 1, Hc = ctx->H + ctx->ks - 1;
     const size_t NA = (size_t)Wc * Hc;
     const int grad = ctx->p.dt_beta > 0.0f ? 1 : 0;
+    const DataTermArgs da = data_term_args(ctx);
     TileGeom ga = g;
     ga.ntx = (Wc + g.tw - 1) / g.tw;
     ga.ntiles = ga.ntx * ((Hc + g.th - 1) / g.th);
@@ -1008,7 +1006,7 @@ lfrv_status ensure_sr_ops(lfrv_ctx *ctx)
     const int cap = std::min(1024, kStagePer * ctx->pull_bs);
     const size_t lds = sizeof(int) * (2 * wh + 2) + sizeof(float) * std::max<size_t>(44 * (size_t)ncw, 9 * (size_t)cap);
     TimedRegion tr(ctx, 5);
-#define AS(K) k_sr_assemble<K><<<ga.ntiles, ctx->pull_bs, lds, ctx->stream>>>(ga, te, cap, grad, ctx->p.gw_type != 0, NA, ctx->sr_ops)
+#define AS(K) k_sr_assemble<K><<<ga.ntiles, ctx->pull_bs, lds, ctx->stream>>>(ga, te, cap, grad, ctx->p.gw_type != 0, dt_intensity_weight(da), dt_gradient_weight(da), NA, ctx->sr_ops)
     switch (ctx->ks) {
     case 3: AS(3); break;
     case 4: AS(4); break;

@@ -251,7 +251,7 @@ __device__ __forceinline__ void reduce_partials(const double *partials, int G, d
 // is latency-bound (an empty one-block launch there takes 4.5 us on MI355X):
 // 256 threads with four strided loads each beat 1024 with one or two.
 constexpr int kFinalizeThreads = 256;
-constexpr int kSrOpsPlanes = 16;        // planes of the assembled IBR_color data term
+constexpr int kSrOpsPlanes = 13;        // planes of the assembled IBR_color data term
 
 // Finalize kernel of the plain sums.
 template <int NQ>
@@ -508,6 +508,18 @@ struct DataTermArgs {
     float alpha, beta, sigma2, grad_step, dsf2, prox_factor;  // prox_factor = -1 / (lambda L)
     int gw_type;
 };
+// The weights of the two data terms' back-projections in the gradient step
+// (optimization.cu:1835-2116): alpha sigma^2 for the intensity
+// term (0 when alpha <= 0), -beta gradient_step [sigma^2 with gw_type] for the
+// gradient term.
+__host__ __device__ __forceinline__ float dt_intensity_weight(const DataTermArgs &da)
+{
+    return da.alpha > 0.0f ? da.alpha * da.sigma2 : 0.0f;
+}
+__host__ __device__ __forceinline__ float dt_gradient_weight(const DataTermArgs &da)
+{
+    return da.beta * -da.grad_step * (da.gw_type != 0 ? da.sigma2 : 1.0f);
+}
 struct PdArgs {
     float tau, lam, sigma, theta;
 };
