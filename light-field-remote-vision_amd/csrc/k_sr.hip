@@ -269,7 +269,10 @@ struct StDims {
     static constexpr int A1 = 3 * SH * P + (GRAD ? 3 * LH * P : 0), A2 = 3 * QH * P;  // (3 residual planes)
     static constexpr int AREA = A1 > A2 ? A1 : A2;
     static constexpr int HR = BH + D - 1;  // rows of the horizontal box sums
-    static constexpr int B1 = NC * HR * P, B2 = 6 * (TH + 1) * (TW + 2);
+    // D > 2: the row sums keep only the BW columns the column sums read (pitch
+    // BW: a wave's active lanes then write and read consecutive words);
+    // D <= 2: the box sums themselves
+    static constexpr int B1 = D > 2 ? NC * HR * BW : NC * BH * P, B2 = 6 * (TH + 1) * (TW + 2);
     static constexpr int BAREA = B1 > B2 ? B1 : B2;  // sb
     static constexpr int NQP = QH * P;               // anchor slots (qx < QW used)
     static constexpr int NQT = (NQP + NT - 1) / NT, NST = (SH * P + NT - 1) / NT;
@@ -422,7 +425,7 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
         __syncthreads();
     } else {
     constexpr int HR = S::HR, NBT = (BH * P + NT - 1) / NT;
-    float *hsu = sb, *hsl = sb + 3 * HR * P;
+    float *hsu = sb, *hsl = sb + 3 * HR * BW;  // pitch BW
     for (int i = tid; i < HR * P; i += NT) {
         if (i % P >= BW) continue;
 #pragma unroll
@@ -433,8 +436,9 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
                 su += us[c * SH * P + i + P + k + 1];
                 if (grad) sl += ls[c * LH * P + i + k];
             }
-            hsu[c * HR * P + i] = su;
-            if (grad) hsl[c * HR * P + i] = sl;
+            const int h = i / P * BW + i % P;
+            hsu[c * HR * BW + h] = su;
+            if (grad) hsl[c * HR * BW + h] = sl;
         }
     }
     __syncthreads();
@@ -445,12 +449,13 @@ __device__ __forceinline__ void sr_dataterm_tile(int W, int H, int Wc, int Hc, i
 #pragma unroll
         for (int c = 0; c < 6; c++) bsum[k][c] = 0.0f;
         if (i >= BH * P || i % P >= BW) continue;
+        const int h = i / P * BW + i % P;
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
             for (int l = 0; l < D; l++) {
-                bsum[k][c] += hsu[c * HR * P + i + l * P];
-                if (grad) bsum[k][3 + c] += hsl[c * HR * P + i + l * P];
+                bsum[k][c] += hsu[c * HR * BW + h + l * BW];
+                if (grad) bsum[k][3 + c] += hsl[c * HR * BW + h + l * BW];
             }
     }
     // BU and BL go to sa, which the row sums no longer read: no barrier here
