@@ -40,7 +40,8 @@ PROFILE = os.path.join(ROOT, "profiles", "k_cgcg_st2_gradient_ibr_9views_1024.md
 PROFILE_C3 = os.path.join(ROOT, "profiles", "k_sr_step_st_color_sr_25views_2048.md")
 PROFILE_C1 = os.path.join(ROOT, "profiles", "k_sr_step_st_color_sr_9views_256.md")
 # the IBR_color workloads whose step kernel has a PMC record: workload -> (profile, kernel)
-COLOR_TRAFFIC = {"color_sr_c3": (PROFILE_C3, "k_sr_step_st<5>"), "color_sr_c1": (PROFILE_C1, "k_sr_step_st<3>")}
+COLOR_TRAFFIC = {"color_sr_c3": (PROFILE_C3, "k_sr_step_st<5, true>"),
+                 "color_sr_c1": (PROFILE_C1, "k_sr_step_st<3, false>")}
 
 
 def measured_traffic(path=PROFILE):
@@ -446,7 +447,7 @@ def bench_color_sr(args, workload):
         # sampled timing (launches 0, 8, 16, ... counted over all views) never
         # stamps at 300 or 500 steps per view (the last step's index is 3 or 7 mod 8)
         NA = (scene["W"] + ks - 1) * (scene["H"] + ks - 1)
-        dt_kernel = f"k_sr_step_st<{ks}>"
+        dt_kernel = f"k_sr_step_st<{ks}, {'true' if beta > 0 else 'false'}>"  # <dsf + 1, gradient term>
         dt_bytes = 4 * NA * (8 + (5 if beta > 0 else 0)) + 80 * N
     # fused PD step: Uq, px, py, g, F read; px, py, new primal written (88 B
     # per pixel), with the outer step's over-relaxation fused in: U read, and
